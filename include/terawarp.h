@@ -1,0 +1,200 @@
+/*
+ * terawarp.h -- C ABI of the MI355X warp path (libterawarp.so).
+ *
+ * Plain C types only: POD descriptors, caller-owned buffers, int status.
+ * Every entry point replaces one group of the reference's `extern "C"` GPU
+ * wrappers (declared at q_imgwarp_tps_quicksmallmemory.cpp:23-40 and
+ * q_warp_affine_tps.cpp:21-26), which took newmat Matrix& and
+ * DisplaceFieldF3D*** arguments, allocated and copied on every call, ended
+ * with cudaDeviceReset() and always returned success.  Here:
+ *   - a context owns the device, a HIP stream and cached device workspaces;
+ *     nothing resets the device;
+ *   - a buffer is host memory (the library stages it through the device,
+ *     like the reference wrappers did) or device memory (no copies);
+ *   - errors come back as negative status codes, with a message available
+ *     from tw_last_error();
+ *   - calls on one context are serialised by the caller (one context per
+ *     host thread); the calls that take a context are asynchronous on the
+ *     context's stream only when every buffer is in device memory, and
+ *     synchronous otherwise.
+ *
+ * Coordinates: voxel (x, y, z) of a volume with origin (x0, y0, z0) is at
+ * global position (x0 + x, y0 + y, z0 + z).  Volumes are [c][z][y][x] with x
+ * fastest.  All transforms are backward maps: output voxel p reads the
+ * subject volume at T(p), T mapping target space to subject space.
+ */
+#ifndef TERAWARP_H
+#define TERAWARP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define TW_OK              0
+#define TW_E_ARG          -1  /* invalid argument / shape mismatch */
+#define TW_E_SINGULAR     -2  /* singular TPS system or degenerate affine fit */
+#define TW_E_NOMEM        -3  /* device or host allocation failed */
+#define TW_E_DEVICE       -4  /* HIP runtime error / no device */
+#define TW_E_IO           -5  /* file I/O or format error */
+#define TW_E_UNSUPPORTED  -6  /* valid request this build does not implement */
+#define TW_E_NOCONVERGE   -7  /* iterative refinement did not reach FP64 accuracy */
+
+/* ---- enums ------------------------------------------------------------ */
+#define TW_U8   1
+#define TW_U16  2
+
+#define TW_MEM_HOST    0
+#define TW_MEM_DEVICE  1
+
+/* Precision forms.
+ * TW_EXACT : FP64 throughout, in the reference's operation order (parity).
+ * TW_PLAIN : FP64 TPS solve; FP32 grid field; FP32 B-spline upsample;
+ *            FP64 trilinear in the reference's order.
+ * TW_MIXED : FP32 LU (MFMA trailing updates) + FP64 iterative refinement;
+ *            FP32 compensated grid field stored as offset-scaled FP16;
+ *            FP32 upsample and trilinear. */
+#define TW_EXACT  2
+#define TW_PLAIN  0
+#define TW_MIXED  1
+
+#define TW_INTERP_LINEAR  0
+#define TW_INTERP_NN      1
+
+#define TW_DF_F32  0
+#define TW_DF_F16  1
+
+typedef struct tw_ctx tw_ctx;
+
+/* A volume or a window of one.  full_* is the extent of the whole volume
+ * the window belongs to (0 = the window itself); it sets the out-of-bounds
+ * test of the resamplers (reference: sz_img_ori in q_imgblockwarp_CYF,
+ * q_imgwarp_tps_quicksmallmemory.cpp:2208-2304). */
+typedef struct {
+    void*   data;
+    int64_t nx, ny, nz, nc;
+    int64_t x0, y0, z0;
+    int64_t full_nx, full_ny, full_nz;
+    int32_t dtype;  /* TW_U8 / TW_U16 */
+    int32_t mem;    /* TW_MEM_HOST / TW_MEM_DEVICE */
+} tw_volume;
+
+/* Coarse displacement grid (reference: Vol3DSimple<DisplaceFieldF3D>,
+ * q_littleQuickWarp_common.h:42-70, 16-byte AoS records).  Here three
+ * planes sx, sy, sz of gx*gy*gz values, x fastest.  Node (i, j, k) sits at
+ * ((i-1)*f + ox, (j-1)*f + oy, (k-1)*f + oz)
+ * (q_imgwarp_tps_quicksmallmemory.cpp:628-630, 1126-1143).
+ * TW_DF_F16: value = offset[b][c] + scale[b][c] * half, where b indexes
+ * blocks of 8x8x8 nodes (nbx = ceil(gx/8), ...); the library allocates and
+ * owns the per-block tables of a grid it fills (tw_grid_release). */
+typedef struct {
+    void*   data;
+    int64_t gx, gy, gz;
+    int32_t f;                 /* node spacing = B-spline block size */
+    int32_t storage;           /* TW_DF_F32 / TW_DF_F16 */
+    int64_t ox, oy, oz;
+    int32_t mem;               /* where data lives */
+    int32_t reserved;
+    float*  qtab;              /* F16: device table, per node block 6 floats
+                                  (offset xyz, scale xyz); owned by library */
+} tw_grid;
+
+/* Per-call timings of the last call on a context, from HIP events on the
+ * context's stream (milliseconds; 0 when a stage did not run). */
+typedef struct {
+    double fit_ms, grid_ms, quant_ms, resample_ms, h2d_ms, d2h_ms, total_ms;
+    int32_t refine_iters;      /* TW_MIXED solve: refinement sweeps used */
+    int32_t reserved;
+    double  fit_residual;      /* relative FP64 residual of the solved system */
+} tw_timings;
+
+/* ---- context ------------------------------------------------------------ */
+int         tw_create(int device, tw_ctx** out);
+int         tw_destroy(tw_ctx* ctx);
+/* Use an existing hipStream_t (as void*); NULL restores the context's own. */
+int         tw_set_stream(tw_ctx* ctx, void* hip_stream);
+int         tw_sync(tw_ctx* ctx);
+/* 1: record per-stage HIP events on every call (tw_get_timings). */
+int         tw_set_timing(tw_ctx* ctx, int enable);
+int         tw_get_timings(tw_ctx* ctx, tw_timings* out);
+const char* tw_last_error(tw_ctx* ctx);
+const char* tw_version(void);
+
+/* ---- affine / rigid ---------------------------------------------------- */
+/* DLT fit with Hartley normalisation, pts_b ~= M pts_a; M is 4x4 row-major.
+ * Replaces q_affine_compute_affinmatrix_3D (q_warp_affine_tps.cpp:649-731);
+ * at the plugin call site pts_a = target markers, pts_b = subject markers,
+ * so M maps target space to subject space (q_warp_affine_tps.cpp:1340-1351).
+ * Host-side math (3N x 13 SVD).  n >= 4. */
+int tw_affine_fit(const float* pts_a, const float* pts_b, int n, double* M);
+
+/* Matrix of the README-only modes (README.md:85-100), target -> subject,
+ * 4x4 row-major, for a subject volume of size (nx, ny, nz):
+ *   mode 0 rotation   : angles para (deg) about x, then y, then z, about the
+ *                       image centre ((n-1)/2 per axis)
+ *   mode 1 translation: output(p) = subject(p - para)
+ *   mode 2 scaling    : output(p) = subject(p / para), about the origin
+ * para[3]: per-axis parameters. */
+int tw_mode_matrix(int mode, const double* para, int64_t nx, int64_t ny, int64_t nz, double* M);
+
+/* Trilinear (or nearest) affine resample, FP64 in the reference's order,
+ * truncating like the reference's affine path (q_warp_affine_tps.cpp:1433).
+ * Replaces gpu_interpolation_affine (aff.cu:539-619) and the tera-block
+ * gpu_interpolation_new (aff.cu:449-536).  src and dst share dtype and nc. */
+int tw_resample_affine(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const double* M, int interp);
+
+/* ---- TPS ---------------------------------------------------------------- */
+/* Regularised 3-D STPS fit, kernel -|r|: solves the bordered system
+ *   [[K + lambda I, X], [X', 0]] [c; d] = [Y; 0],  X = [1 centres], Y = [1 targets]
+ * whose solution equals the reference's null-space computation
+ * (q_imgwarp_tps_quicksmallmemory.cpp:449-586: gpu_QR, gpu_extendornormal,
+ * matrixMultiply, gpu_A_i_new; lambda = 0.2 at 581).  centres are the
+ * kernel centres (target space; the file-target markers at the plugin call
+ * site, plugin_Bigimagewarp.cpp:197), targets the subject-space points.
+ * c: K x 4 row-major, d: 4 x 4 row-major, host memory (the reference's
+ * xnx4_c, x4x4_d).  precision: TW_PLAIN/TW_EXACT (FP64 LU) or TW_MIXED
+ * (FP32 LU + FP64 refinement). */
+int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda,
+               int precision, double* c, double* d);
+
+/* Evaluate the TPS displacement at every grid node.  Replaces
+ * gpu_computedistance (weiyijisuan.cu:118-219).  grid->gx..gz, f, ox..oz,
+ * storage and mem are inputs; grid->data must hold 3*gx*gy*gz values of
+ * the storage type.  precision TW_EXACT: FP64 in the reference's order;
+ * otherwise FP32 with compensated summation. */
+int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const double* c, const double* d,
+                     int precision, tw_grid* grid);
+
+/* Release library-owned tables of a grid (qtab). */
+int tw_grid_release(tw_ctx* ctx, tw_grid* grid);
+
+/* Fused cubic-B-spline upsample of the grid + trilinear/NN resample of src
+ * into dst (TPS rounding: +0.5 then truncate).  Replaces
+ * gpu_interpolation_stps (chazhi.cu:844-1040) and the tera-block
+ * gpu_interpolation (chazhi.cu:557-663).  precision: TW_EXACT (FP64
+ * 64-term upsample in the reference's order), TW_PLAIN, TW_MIXED. */
+int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid,
+                    int interp, int precision);
+
+/* Dense per-voxel displacement of the upsample over dst's window: three
+ * float planes of nx*ny*nz (device or host per mem).  Stage check only. */
+int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int64_t ny, int64_t nz,
+                   int64_t x0, int64_t y0, int64_t z0, int precision, float* out, int mem);
+
+/* Whole in-memory chain of imgwarp_smallmemory_image's GPU branch
+ * (q_imgwarp_tps_quicksmallmemory.cpp:1997-2162): fit, grid, resample.
+ * Grid spacing f (the reference uses 4), lambda 0.2. */
+int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
+                const float* targets, int K, int f, double lambda, int interp, int precision);
+
+/* Grid node count per axis for a volume extent n and spacing f:
+ * ceil(n / f) + 3 (q_imgwarp_tps_quicksmallmemory.cpp:592). */
+int64_t tw_grid_dim(int64_t n, int f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TERAWARP_H */

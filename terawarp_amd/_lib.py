@@ -1,0 +1,106 @@
+"""ctypes declarations of libterawarp.so (include/terawarp.h).
+
+The library is built in-tree by terawarp_amd.build; loading never compiles.
+A missing library raises, there is no fallback path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libterawarp.so")
+
+TW_OK = 0
+TW_E_ARG = -1
+TW_E_SINGULAR = -2
+TW_E_NOMEM = -3
+TW_E_DEVICE = -4
+TW_E_IO = -5
+TW_E_UNSUPPORTED = -6
+TW_E_NOCONVERGE = -7
+
+TW_U8 = 1
+TW_U16 = 2
+TW_MEM_HOST = 0
+TW_MEM_DEVICE = 1
+TW_PLAIN = 0
+TW_MIXED = 1
+TW_EXACT = 2
+TW_INTERP_LINEAR = 0
+TW_INTERP_NN = 1
+TW_DF_F32 = 0
+TW_DF_F16 = 1
+
+# Every entry point include/terawarp.h declares.
+EXPORTS = (
+    "tw_create", "tw_destroy", "tw_set_stream", "tw_sync", "tw_set_timing", "tw_get_timings", "tw_last_error",
+    "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
+    "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_grid_dim",
+)
+
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_P = ctypes.c_void_p
+
+
+class TwVolume(ctypes.Structure):
+    _fields_ = [("data", _P), ("nx", _i64), ("ny", _i64), ("nz", _i64), ("nc", _i64),
+                ("x0", _i64), ("y0", _i64), ("z0", _i64),
+                ("full_nx", _i64), ("full_ny", _i64), ("full_nz", _i64),
+                ("dtype", _i32), ("mem", _i32)]
+
+
+class TwGrid(ctypes.Structure):
+    _fields_ = [("data", _P), ("gx", _i64), ("gy", _i64), ("gz", _i64), ("f", _i32), ("storage", _i32),
+                ("ox", _i64), ("oy", _i64), ("oz", _i64), ("mem", _i32), ("reserved", _i32),
+                ("qtab", _P)]
+
+
+class TwTimings(ctypes.Structure):
+    _fields_ = [("fit_ms", ctypes.c_double), ("grid_ms", ctypes.c_double), ("quant_ms", ctypes.c_double),
+                ("resample_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double), ("d2h_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("refine_iters", _i32), ("reserved", _i32),
+                ("fit_residual", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libterawarp.so and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m terawarp_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    V, G, T = ctypes.POINTER(TwVolume), ctypes.POINTER(TwGrid), ctypes.POINTER(TwTimings)
+    dp = ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
+    sig = {
+        "tw_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+        "tw_destroy": (ctypes.c_int, [_P]),
+        "tw_set_stream": (ctypes.c_int, [_P, _P]),
+        "tw_sync": (ctypes.c_int, [_P]),
+        "tw_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+        "tw_get_timings": (ctypes.c_int, [_P, T]),
+        "tw_last_error": (ctypes.c_char_p, [_P]),
+        "tw_version": (ctypes.c_char_p, []),
+        "tw_affine_fit": (ctypes.c_int, [fp, fp, ctypes.c_int, dp]),
+        "tw_mode_matrix": (ctypes.c_int, [ctypes.c_int, dp, _i64, _i64, _i64, dp]),
+        "tw_resample_affine": (ctypes.c_int, [_P, V, V, dp, ctypes.c_int]),
+        "tw_tps_fit": (ctypes.c_int, [_P, fp, fp, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]),
+        "tw_tps_grid_eval": (ctypes.c_int, [_P, fp, ctypes.c_int, dp, dp, ctypes.c_int, G]),
+        "tw_grid_release": (ctypes.c_int, [_P, G]),
+        "tw_resample_tps": (ctypes.c_int, [_P, V, V, G, ctypes.c_int, ctypes.c_int]),
+        "tw_upsample_df": (ctypes.c_int, [_P, G, _i64, _i64, _i64, _i64, _i64, _i64, ctypes.c_int, fp,
+                                          ctypes.c_int]),
+        "tw_warp_tps": (ctypes.c_int, [_P, V, V, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int]),
+        "tw_grid_dim": (_i64, [_i64, ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

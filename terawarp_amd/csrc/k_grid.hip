@@ -1,0 +1,270 @@
+// k_grid.hip -- TPS displacement at the coarse grid nodes.
+//
+// Reference: CPU loop of compute_df_stps_subsampled_volume_4bspline
+// (q_imgwarp_tps_quicksmallmemory.cpp:588-649; block variant with offsets
+// 1084-1152) and its CUDA port get_cd / gpu_computedistance
+// (weiyijisuan.cu:91-219).
+//
+// Node (i, j, k) sits at P = ((i-1)f + ox, (j-1)f + oy, (k-1)f + oz).
+// x_stps = [1 P] d + sum_n -|P - centre_n| c_n and DF = x_stps - P.
+// The kernel distance takes float differences of the integer node
+// coordinate and the float centre, as the reference does.
+//
+// TW_EXACT: FP64, the reference's summation order (affine part, then the
+//   landmark sum in ascending n, then their sum), one thread per node.
+// Otherwise FP32: the affine part is evaluated in FP64 as
+//   d0 + P (d_lin - I) so that the large identity term never enters FP32,
+//   and the landmark sum is FP32 with TwoSum error compensation.
+// Landmarks are read with wave-uniform addresses (scalar loads).
+//
+// Roofline: compute-bound; ~28 VALU ops per (node, landmark) pair in FP32,
+// so nodes*K*28 / 3.9e13 lane-ops/s on MI355X.
+#include "tw_internal.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+struct GridArgs {
+    const float4* pts;   // K centres (x, y, z, 0)
+    const double* c64;   // K x 3 coefficients (cols 1..3), TW_EXACT
+    const float4* c32;   // K coefficients (c1, c2, c3, 0), FP32 path
+    int K;
+    int f;
+    i64 gx, gy, gz, ox, oy, oz;
+    double d[16];        // affine block, row-major 4x4
+    float* out;          // 3 planes gx*gy*gz
+};
+
+__device__ __forceinline__ void two_sum(float& s, float& comp, float y) {
+    const float t = s + y;
+    const float bp = t - s;
+    const float err = (s - (t - bp)) + (y - bp);
+    s = t;
+    comp += err;
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
+    const i64 plane = a.gx * a.gy * a.gz;
+    const i64 idx = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= plane) return;
+    const i64 i = idx % a.gx;
+    const i64 j = (idx / a.gx) % a.gy;
+    const i64 k = idx / (a.gx * a.gy);
+    const i64 X = (i - 1) * a.f + a.ox, Y = (j - 1) * a.f + a.oy, Z = (k - 1) * a.f + a.oz;
+    const float Xf = (float)X, Yf = (float)Y, Zf = (float)Z;
+    if (EXACT) {
+        const double xo[4] = {1.0, (double)X, (double)Y, (double)Z};
+        double b1 = 0.0, b2 = 0.0, b3 = 0.0;
+        for (int n = 0; n < a.K; n++) {
+            const float4 p = a.pts[n];
+            const double dx = (float)(Xf - p.x), dy = (float)(Yf - p.y), dz = (float)(Zf - p.z);
+            const double kr = -sqrt(dx * dx + dy * dy + dz * dz);
+            b1 += kr * a.c64[3 * n + 0];
+            b2 += kr * a.c64[3 * n + 1];
+            b3 += kr * a.c64[3 * n + 2];
+        }
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int m = 0; m < 4; m++) {
+            a1 += xo[m] * a.d[m * 4 + 1];
+            a2 += xo[m] * a.d[m * 4 + 2];
+            a3 += xo[m] * a.d[m * 4 + 3];
+        }
+        a.out[idx] = (float)((a1 + b1) - (double)X);
+        a.out[plane + idx] = (float)((a2 + b2) - (double)Y);
+        a.out[2 * plane + idx] = (float)((a3 + b3) - (double)Z);
+    } else {
+        float s1 = 0.f, s2 = 0.f, s3 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+        for (int n = 0; n < a.K; n++) {
+            const float4 p = a.pts[n];
+            const float4 c = a.c32[n];
+            const float dx = Xf - p.x, dy = Yf - p.y, dz = Zf - p.z;
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const float r = __builtin_amdgcn_sqrtf(r2);
+            two_sum(s1, e1, -r * c.x);
+            two_sum(s2, e2, -r * c.y);
+            two_sum(s3, e3, -r * c.z);
+        }
+        // Affine part minus identity, in FP64.
+        const double Xd = (double)X, Yd = (double)Y, Zd = (double)Z;
+        const double l1 = a.d[1] + Xd * (a.d[5] - 1.0) + Yd * a.d[9] + Zd * a.d[13];
+        const double l2 = a.d[2] + Xd * a.d[6] + Yd * (a.d[10] - 1.0) + Zd * a.d[14];
+        const double l3 = a.d[3] + Xd * a.d[7] + Yd * a.d[11] + Zd * (a.d[15] - 1.0);
+        a.out[idx] = (float)(l1 + ((double)s1 + (double)e1));
+        a.out[plane + idx] = (float)(l2 + ((double)s2 + (double)e2));
+        a.out[2 * plane + idx] = (float)(l3 + ((double)s3 + (double)e3));
+    }
+}
+
+// FP16 offset-scaled storage: per block of 8x8x8 nodes and component,
+// value = offset + scale * h with h in [-1, 1].
+constexpr int QB = 8;
+
+__global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ df, __half* __restrict__ out,
+                                                  float* __restrict__ qtab, i64 gx, i64 gy, i64 gz, i64 qbx,
+                                                  i64 qby) {
+    __shared__ float red[2][3][256];
+    __shared__ float prm[6];
+    const i64 plane = gx * gy * gz;
+    const i64 b = blockIdx.x;
+    const i64 bx = b % qbx, by = (b / qbx) % qby, bz = b / (qbx * qby);
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int t = threadIdx.x; t < QB * QB * QB; t += 256) {
+        const i64 i = bx * QB + (t % QB), j = by * QB + (t / QB) % QB, k = bz * QB + t / (QB * QB);
+        if (i >= gx || j >= gy || k >= gz) continue;
+        const i64 idx = (k * gy + j) * gx + i;
+        for (int c = 0; c < 3; c++) {
+            const float v = df[c * plane + idx];
+            mn[c] = fminf(mn[c], v);
+            mx[c] = fmaxf(mx[c], v);
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        red[0][c][threadIdx.x] = mn[c];
+        red[1][c][threadIdx.x] = mx[c];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) {
+                red[0][c][threadIdx.x] = fminf(red[0][c][threadIdx.x], red[0][c][threadIdx.x + s]);
+                red[1][c][threadIdx.x] = fmaxf(red[1][c][threadIdx.x], red[1][c][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        const float lo = red[0][c][0], hi = red[1][c][0];
+        const float off = 0.5f * (lo + hi);
+        float sc = fmaxf(hi - off, off - lo);
+        if (!(sc > 0.f)) sc = 1.f;
+        prm[c] = off;
+        prm[3 + c] = sc;
+        qtab[b * 6 + c] = off;
+        qtab[b * 6 + 3 + c] = sc;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < QB * QB * QB; t += 256) {
+        const i64 i = bx * QB + (t % QB), j = by * QB + (t / QB) % QB, k = bz * QB + t / (QB * QB);
+        if (i >= gx || j >= gy || k >= gz) continue;
+        const i64 idx = (k * gy + j) * gx + i;
+        for (int c = 0; c < 3; c++) {
+            const float h = (df[c * plane + idx] - prm[c]) / prm[3 + c];
+            out[c * plane + idx] = __float2half_rn(fminf(1.f, fmaxf(-1.f, h)));
+        }
+    }
+}
+
+}  // namespace
+
+int tw_grid_quantize_f16(tw_ctx* ctx, const float* df32, tw_grid* grid) {
+    const i64 qbx = (grid->gx + QB - 1) / QB, qby = (grid->gy + QB - 1) / QB, qbz = (grid->gz + QB - 1) / QB;
+    const i64 nb = qbx * qby * qbz;
+    if (!grid->qtab) {
+        float* t = nullptr;
+        TW_HIP(ctx, hipMalloc(&t, (size_t)nb * 6 * sizeof(float)));
+        grid->qtab = t;
+    }
+    tw_stage_begin(ctx, ST_QUANT);
+    hipLaunchKernelGGL(k_quantize, dim3((unsigned)nb), dim3(256), 0, ctx->stream, df32, (__half*)grid->data,
+                       grid->qtab, grid->gx, grid->gy, grid->gz, qbx, qby);
+    TW_LAUNCH_CHECK(ctx);
+    tw_stage_end(ctx, ST_QUANT);
+    return TW_OK;
+}
+
+extern "C" int tw_grid_release(tw_ctx* ctx, tw_grid* grid) {
+    if (!grid) return TW_E_ARG;
+    if (grid->qtab) {
+        if (ctx) {
+            (void)hipSetDevice(ctx->device);
+            (void)hipStreamSynchronize(ctx->stream);
+        }
+        (void)hipFree(grid->qtab);
+        grid->qtab = nullptr;
+    }
+    return TW_OK;
+}
+
+extern "C" int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const double* c, const double* d,
+                                int precision, tw_grid* grid) {
+    if (!ctx) return TW_E_ARG;
+    if (!centres || !c || !d || !grid || !grid->data) return tw_fail(ctx, TW_E_ARG, "null argument");
+    if (K < 1) return tw_fail(ctx, TW_E_ARG, "K must be >= 1");
+    if (grid->gx < 1 || grid->gy < 1 || grid->gz < 1 || grid->f < 1)
+        return tw_fail(ctx, TW_E_ARG, "grid extent and spacing must be positive");
+    if (grid->storage != TW_DF_F32 && grid->storage != TW_DF_F16)
+        return tw_fail(ctx, TW_E_ARG, "grid storage must be TW_DF_F32 or TW_DF_F16");
+    if (grid->storage == TW_DF_F16 && precision == TW_EXACT)
+        return tw_fail(ctx, TW_E_ARG, "TW_EXACT needs an FP32 grid");
+    int rc;
+    if ((rc = tw_bind(ctx))) return rc;
+    tw_stages_reset(ctx);
+    tw_stage_begin(ctx, ST_TOTAL);
+    const bool exact = precision == TW_EXACT;
+    const i64 plane = grid->gx * grid->gy * grid->gz;
+
+    // Landmark packs: centres (float4), coefficients (FP64 x3 or float4).
+    const size_t pts_bytes = (size_t)K * sizeof(float4);
+    const size_t coef_bytes = exact ? (size_t)K * 3 * sizeof(double) : (size_t)K * sizeof(float4);
+    void* pack = nullptr;
+    if ((rc = tw_ws(ctx, WS_PTS, pts_bytes + coef_bytes, &pack))) return rc;
+    std::vector<uint8_t> host(pts_bytes + coef_bytes);
+    float4* hp = (float4*)host.data();
+    for (int n = 0; n < K; n++) hp[n] = make_float4(centres[3 * n], centres[3 * n + 1], centres[3 * n + 2], 0.f);
+    if (exact) {
+        double* hc = (double*)(host.data() + pts_bytes);
+        for (int n = 0; n < K; n++)
+            for (int q = 0; q < 3; q++) hc[3 * n + q] = c[4 * n + 1 + q];
+    } else {
+        float4* hc = (float4*)(host.data() + pts_bytes);
+        for (int n = 0; n < K; n++)
+            hc[n] = make_float4((float)c[4 * n + 1], (float)c[4 * n + 2], (float)c[4 * n + 3], 0.f);
+    }
+    TW_HIP(ctx, hipMemcpyAsync(pack, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+
+    GridArgs a;
+    memset(&a, 0, sizeof a);
+    a.pts = (const float4*)pack;
+    a.c64 = exact ? (const double*)((uint8_t*)pack + pts_bytes) : nullptr;
+    a.c32 = exact ? nullptr : (const float4*)((uint8_t*)pack + pts_bytes);
+    a.K = K;
+    a.f = grid->f;
+    a.gx = grid->gx; a.gy = grid->gy; a.gz = grid->gz;
+    a.ox = grid->ox; a.oy = grid->oy; a.oz = grid->oz;
+    for (int i = 0; i < 16; i++) a.d[i] = d[i];
+
+    // FP32 output: the grid itself (device F32), else a workspace.
+    float* out32 = nullptr;
+    const bool direct = grid->storage == TW_DF_F32 && grid->mem == TW_MEM_DEVICE;
+    if (direct) {
+        out32 = (float*)grid->data;
+    } else {
+        void* p = nullptr;
+        if ((rc = tw_ws(ctx, WS_GRID, (size_t)plane * 3 * sizeof(float), &p))) return rc;
+        out32 = (float*)p;
+    }
+    a.out = out32;
+    tw_stage_begin(ctx, ST_GRID);
+    const unsigned nblk = (unsigned)((plane + 255) / 256);
+    if (exact)
+        hipLaunchKernelGGL(k_grid_eval<true>, dim3(nblk), dim3(256), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL(k_grid_eval<false>, dim3(nblk), dim3(256), 0, ctx->stream, a);
+    TW_LAUNCH_CHECK(ctx);
+    tw_stage_end(ctx, ST_GRID);
+
+    if (grid->storage == TW_DF_F16) {
+        if (grid->mem != TW_MEM_DEVICE) return tw_fail(ctx, TW_E_ARG, "an FP16 grid must live in device memory");
+        if ((rc = tw_grid_quantize_f16(ctx, out32, grid))) return rc;
+    } else if (!direct) {
+        TW_HIP(ctx, hipMemcpyAsync(grid->data, out32, (size_t)plane * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    }
+    tw_stage_end(ctx, ST_TOTAL);
+    // The host pack must outlive the copy; host grids need the result now.
+    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TW_OK;
+}

@@ -1,0 +1,529 @@
+// k_tps.hip -- fused cubic-B-spline upsample of the grid DF + resample.
+//
+// Reference: in-memory CPU loop q_dfblcokinterp_bspline + q_imgblockwarp
+// (q_imgwarp_tps_quicksmallmemory.cpp:85-203, driver 2119-2133) and its
+// CUDA port get_Displacement_brightness_stps (chazhi.cu:226-338).
+//
+// Semantics kept from the reference:
+//  * voxel X uses grid nodes b..b+3 with b = (X-o)/f and phase t = (X-o)%f;
+//    the upsample is the tensor product of the cubic B-spline weights
+//    TB[t][k] (uniform, approximating);
+//  * the displacement is stored as float, and the sample position is the
+//    FLOAT sum of the integer coordinate and the float displacement
+//    (pos_warp + sx is long + float in the reference, 106-108);
+//  * strict bounds test against the full subject extent, floor/ceil
+//    neighbours, weights 1-frac, the t/b, l/r, u/d blend tree, +0.5 and
+//    truncation (135-160); nearest neighbour rounds pos+0.5 and clamps.
+//
+// Variants:
+//  TW_EXACT  one thread per voxel, the 64-term FP64 basis product in the
+//            reference's order (nodes z->x->y, basis (TB_z*TB_x)*TB_y),
+//            rounded to float: bit-identical to the FP64 CPU path.
+//  TW_PLAIN  FP32 separable upsample (x, then y, then z), FP64 blend tree
+//            in the reference's order.
+//  TW_MIXED  as PLAIN with an FP32 blend (7 lerps) and optional
+//            offset-scaled FP16 nodes.
+//
+// Fused kernel layout (PLAIN/MIXED): a work-group of 64x4 threads owns a
+// tile of 64 (x) by 4 (y) by TZ (z) output voxels.  It stages the tile's
+// grid nodes in LDS, collapses them along x for each of its 64 columns
+// (XC, in LDS), and then every thread walks its (x, y) column along z,
+// collapsing y per node layer and keeping a sliding window of four layers
+// in registers, so a voxel costs 12 FMAs for the z combination plus a few
+// amortised ones.  Output rows are written coalesced along x.
+//
+// Roofline: B_alg = 2*bpv + 3*s_df*nodes/voxels bytes per voxel (subject
+// read + output write + grid read); HBM-bound when the per-voxel VALU work
+// stays under ~33 ops at 60% of the 8 TB/s peak.
+#include "tw_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+struct TpsArgs {
+    const void* src;
+    void* dst;
+    i64 snx, sny, snz, sx0, sy0, sz0;  // source window
+    i64 fnx, fny, fnz;                 // full subject extent
+    i64 dnx, dny, dnz, dx0, dy0, dz0;  // destination window (global origin)
+    i64 nc;
+    const void* grid;                  // 3 planes, float or half
+    i64 gx, gy, gz, ox, oy, oz;
+    int f;
+    int tz;                            // voxels per work-group along z
+    const float* qtab;                 // FP16: per 8^3 node block (off xyz, scale xyz)
+    i64 qbx, qby;
+    const float* w32;                  // f x 4 FP32 weights
+    const double* w64;                 // f x 4 FP64 weights TB
+    float* dfout;                      // WRITE_DF: 3 planes of dnx*dny*dnz
+    i64 zbase;                         // first z block of this launch (exact kernel)
+};
+
+constexpr int TX = 64, TY = 4;
+
+// ---------------------------------------------------------------------------
+// Shared per-voxel resample tail.
+
+template <typename T>
+__device__ __forceinline__ double ld(const T* s, i64 i) {
+    return (double)s[i];
+}
+
+// Reference blend in FP64 (q_imgwarp_tps_quicksmallmemory.cpp:133-160).
+template <typename T, int INTERP>
+__device__ __forceinline__ void resample_fp64(const TpsArgs& a, i64 X, i64 Y, i64 Z, float d0, float d1, float d2,
+                                              i64 o, i64 dvol) {
+    const T* __restrict__ src = (const T*)a.src;
+    T* __restrict__ dst = (T*)a.dst;
+    const double p0 = (float)X + d0, p1 = (float)Y + d1, p2 = (float)Z + d2;
+    if (p0 < 0 || p0 > (double)(a.fnx - 1) || p1 < 0 || p1 > (double)(a.fny - 1) || p2 < 0 ||
+        p2 > (double)(a.fnz - 1)) {
+        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;
+        return;
+    }
+    const i64 sxy = a.snx * a.sny, svol = sxy * a.snz;
+    if (INTERP == TW_INTERP_NN) {
+        i64 qx = (i64)(p0 + 0.5), qy = (i64)(p1 + 0.5), qz = (i64)(p2 + 0.5);
+        qx = (qx < a.fnx ? qx : a.fnx - 1) - a.sx0;
+        qy = (qy < a.fny ? qy : a.fny - 1) - a.sy0;
+        qz = (qz < a.fnz ? qz : a.fnz - 1) - a.sz0;
+        const bool in = qx >= 0 && qx < a.snx && qy >= 0 && qy < a.sny && qz >= 0 && qz < a.snz;
+        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = in ? src[c * svol + qz * sxy + qy * a.snx + qx] : (T)0;
+        return;
+    }
+    const i64 xs = (i64)floor(p0), xb = (i64)ceil(p0);
+    const i64 ys = (i64)floor(p1), yb = (i64)ceil(p1);
+    const i64 zs = (i64)floor(p2), zb = (i64)ceil(p2);
+    const double lw = 1.0 - (p0 - (double)xs), rw = 1.0 - lw;
+    const double tw = 1.0 - (p1 - (double)ys), bw = 1.0 - tw;
+    const double uw = 1.0 - (p2 - (double)zs), dw = 1.0 - uw;
+    const i64 lxs = xs - a.sx0, lxb = xb - a.sx0, lys = ys - a.sy0, lyb = yb - a.sy0, lzs = zs - a.sz0,
+              lzb = zb - a.sz0;
+    if (lxs < 0 || lxb >= a.snx || lys < 0 || lyb >= a.sny || lzs < 0 || lzb >= a.snz) {
+        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;
+        return;
+    }
+    const i64 r00 = lzs * sxy + lys * a.snx, r01 = lzs * sxy + lyb * a.snx;
+    const i64 r10 = lzb * sxy + lys * a.snx, r11 = lzb * sxy + lyb * a.snx;
+    for (i64 c = 0; c < a.nc; c++) {
+        const T* s = src + c * svol;
+        const double hi = tw * (lw * ld(s, r00 + lxs) + rw * ld(s, r00 + lxb)) +
+                          bw * (lw * ld(s, r01 + lxs) + rw * ld(s, r01 + lxb));
+        const double lo = tw * (lw * ld(s, r10 + lxs) + rw * ld(s, r10 + lxb)) +
+                          bw * (lw * ld(s, r11 + lxs) + rw * ld(s, r11 + lxb));
+        const double iv = uw * hi + dw * lo + 0.5;
+        dst[c * dvol + o] = (T)iv;
+    }
+}
+
+// FP32 blend: float position (same rounding as the reference), exact
+// fractional part, seven lerps, +0.5, truncation.
+template <typename T, int INTERP>
+__device__ __forceinline__ void resample_fp32(const TpsArgs& a, i64 X, i64 Y, i64 Z, float d0, float d1, float d2,
+                                              i64 o, i64 dvol) {
+    if (INTERP == TW_INTERP_NN) {
+        resample_fp64<T, INTERP>(a, X, Y, Z, d0, d1, d2, o, dvol);
+        return;
+    }
+    const T* __restrict__ src = (const T*)a.src;
+    T* __restrict__ dst = (T*)a.dst;
+    const float p0 = (float)X + d0, p1 = (float)Y + d1, p2 = (float)Z + d2;
+    if (p0 < 0.f || p0 > (float)(a.fnx - 1) || p1 < 0.f || p1 > (float)(a.fny - 1) || p2 < 0.f ||
+        p2 > (float)(a.fnz - 1)) {
+        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;
+        return;
+    }
+    const float fx = floorf(p0), fy = floorf(p1), fz = floorf(p2);
+    const float wx = p0 - fx, wy = p1 - fy, wz = p2 - fz;
+    const i64 lxs = (i64)fx - a.sx0, lys = (i64)fy - a.sy0, lzs = (i64)fz - a.sz0;
+    const i64 lxb = lxs + (wx > 0.f), lyb = lys + (wy > 0.f), lzb = lzs + (wz > 0.f);
+    if (lxs < 0 || lxb >= a.snx || lys < 0 || lyb >= a.sny || lzs < 0 || lzb >= a.snz) {
+        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;
+        return;
+    }
+    const i64 sxy = a.snx * a.sny, svol = sxy * a.snz;
+    const i64 r00 = lzs * sxy + lys * a.snx, r01 = lzs * sxy + lyb * a.snx;
+    const i64 r10 = lzb * sxy + lys * a.snx, r11 = lzb * sxy + lyb * a.snx;
+    for (i64 c = 0; c < a.nc; c++) {
+        const T* s = src + c * svol;
+        const float v000 = (float)s[r00 + lxs], v100 = (float)s[r00 + lxb];
+        const float v010 = (float)s[r01 + lxs], v110 = (float)s[r01 + lxb];
+        const float v001 = (float)s[r10 + lxs], v101 = (float)s[r10 + lxb];
+        const float v011 = (float)s[r11 + lxs], v111 = (float)s[r11 + lxb];
+        const float a00 = __builtin_fmaf(wx, v100 - v000, v000);
+        const float a10 = __builtin_fmaf(wx, v110 - v010, v010);
+        const float a01 = __builtin_fmaf(wx, v101 - v001, v001);
+        const float a11 = __builtin_fmaf(wx, v111 - v011, v011);
+        const float b0 = __builtin_fmaf(wy, a10 - a00, a00);
+        const float b1 = __builtin_fmaf(wy, a11 - a01, a01);
+        const float v = __builtin_fmaf(wz, b1 - b0, b0) + 0.5f;
+        dst[c * dvol + o] = (T)v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TW_EXACT: one voxel per thread, reference-order FP64 upsample.
+
+template <typename T, int INTERP, bool WRITE_DF>
+__global__ __launch_bounds__(256) void k_tps_exact(const TpsArgs a) {
+    const i64 lx = (i64)blockIdx.x * TX + threadIdx.x;
+    const i64 ly = (i64)blockIdx.y * TY + threadIdx.y;
+    const i64 lz = a.zbase + blockIdx.z;
+    if (lx >= a.dnx || ly >= a.dny || lz >= a.dnz) return;
+    const i64 X = a.dx0 + lx, Y = a.dy0 + ly, Z = a.dz0 + lz;
+    const int f = a.f;
+    const i64 rx = X - a.ox, ry = Y - a.oy, rz = Z - a.oz;
+    const i64 bx = rx / f, by = ry / f, bz = rz / f;
+    const int tx = (int)(rx - bx * f), ty = (int)(ry - by * f), tz = (int)(rz - bz * f);
+    const float* g = (const float*)a.grid;
+    const i64 plane = a.gx * a.gy * a.gz;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int dz = 0; dz < 4; dz++)
+        for (int dx = 0; dx < 4; dx++)
+            for (int dy = 0; dy < 4; dy++) {
+                const double w = (a.w64[tz * 4 + dz] * a.w64[tx * 4 + dx]) * a.w64[ty * 4 + dy];
+                const i64 idx = ((bz + dz) * a.gy + (by + dy)) * a.gx + (bx + dx);
+                s0 += w * (double)g[idx];
+                s1 += w * (double)g[plane + idx];
+                s2 += w * (double)g[2 * plane + idx];
+            }
+    const float d0 = (float)s0, d1 = (float)s1, d2 = (float)s2;
+    const i64 dvol = a.dnx * a.dny * a.dnz;
+    const i64 o = (lz * a.dny + ly) * a.dnx + lx;
+    if (WRITE_DF) {
+        a.dfout[o] = d0;
+        a.dfout[dvol + o] = d1;
+        a.dfout[2 * dvol + o] = d2;
+    } else {
+        resample_fp64<T, INTERP>(a, X, Y, Z, d0, d1, d2, o, dvol);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused separable kernel (TW_PLAIN / TW_MIXED).
+
+struct LdsPlan {
+    int nxn, nyn, nzn;  // maxima of the node tile extents
+    size_t bytes;
+};
+
+__host__ __device__ inline int node_span(int tile, int f) { return (tile - 1) / f + 5; }
+
+template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF>
+__global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a) {
+    extern __shared__ float lds[];
+    const int f = a.f;
+    const int NXM = node_span(TX, f), NYM = node_span(TY, f), NZM = node_span(a.tz, f);
+    float* wsh = lds;                        // f x 4 weights
+    float* nodes = wsh + 4 * f;              // [3][NZM][NYM][NXM]
+    float* xc = nodes + 3 * NZM * NYM * NXM; // [3][NZM][NYM][TX]
+
+    const int lxi = threadIdx.x, lyi = threadIdx.y, tid = lyi * TX + lxi;
+    const i64 tx0 = (i64)blockIdx.x * TX, ty0 = (i64)blockIdx.y * TY, tz0 = (i64)blockIdx.z * a.tz;
+    const i64 tx1 = min(tx0 + TX, a.dnx) - 1, ty1 = min(ty0 + TY, a.dny) - 1, tz1 = min(tz0 + a.tz, a.dnz) - 1;
+    // Node tile: global node indices [n?0, n?0 + extent).
+    const i64 nx0 = (a.dx0 + tx0 - a.ox) / f, ny0 = (a.dy0 + ty0 - a.oy) / f, nz0 = (a.dz0 + tz0 - a.oz) / f;
+    const int nxn = (int)((a.dx0 + tx1 - a.ox) / f + 4 - nx0);
+    const int nyn = (int)((a.dy0 + ty1 - a.oy) / f + 4 - ny0);
+    const int nzn = (int)((a.dz0 + tz1 - a.oz) / f + 4 - nz0);
+
+    for (int i = tid; i < 4 * f; i += 256) wsh[i] = a.w32[i];
+    // Stage the node tile (3 planes) in LDS.
+    const i64 plane = a.gx * a.gy * a.gz;
+    const int ntile = nxn * nyn * nzn;
+    for (int i = tid; i < ntile; i += 256) {
+        const int ix = i % nxn, iy = (i / nxn) % nyn, iz = i / (nxn * nyn);
+        const i64 gi = ((nz0 + iz) * a.gy + (ny0 + iy)) * a.gx + (nx0 + ix);
+        const int li = (iz * NYM + iy) * NXM + ix;
+        if (F16) {
+            const __half* g = (const __half*)a.grid;
+            const i64 qb = (((nz0 + iz) >> 3) * a.qby + ((ny0 + iy) >> 3)) * a.qbx + ((nx0 + ix) >> 3);
+            const float* q = a.qtab + qb * 6;
+            nodes[li] = __builtin_fmaf(q[3], __half2float(g[gi]), q[0]);
+            nodes[NZM * NYM * NXM + li] = __builtin_fmaf(q[4], __half2float(g[plane + gi]), q[1]);
+            nodes[2 * NZM * NYM * NXM + li] = __builtin_fmaf(q[5], __half2float(g[2 * plane + gi]), q[2]);
+        } else {
+            const float* g = (const float*)a.grid;
+            nodes[li] = g[gi];
+            nodes[NZM * NYM * NXM + li] = g[plane + gi];
+            nodes[2 * NZM * NYM * NXM + li] = g[2 * plane + gi];
+        }
+    }
+    __syncthreads();
+    // Collapse along x for each of the tile's 64 columns.
+    {
+        const i64 X = a.dx0 + tx0 + lxi;
+        const i64 rx = X - a.ox;
+        const int bxl = (int)(rx / f - nx0);
+        const int txp = (int)(rx % f);
+        const bool live = tx0 + lxi <= tx1;
+        const float w0 = wsh[txp * 4 + 0], w1 = wsh[txp * 4 + 1], w2 = wsh[txp * 4 + 2], w3 = wsh[txp * 4 + 3];
+        for (int r = lyi; r < nzn * nyn; r += TY) {
+            const int iy = r % nyn, iz = r / nyn;
+            for (int c = 0; c < 3; c++) {
+                float v = 0.f;
+                if (live) {
+                    const float* nrow = nodes + c * NZM * NYM * NXM + (iz * NYM + iy) * NXM + bxl;
+                    v = __builtin_fmaf(w3, nrow[3], __builtin_fmaf(w2, nrow[2], __builtin_fmaf(w1, nrow[1], w0 * nrow[0])));
+                }
+                xc[((c * NZM + iz) * NYM + iy) * TX + lxi] = v;
+            }
+        }
+    }
+    __syncthreads();
+    if (tx0 + lxi > tx1 || ty0 + lyi > ty1) return;
+
+    const i64 X = a.dx0 + tx0 + lxi, Y = a.dy0 + ty0 + lyi;
+    const i64 ry = Y - a.oy;
+    const int byl = (int)(ry / f - ny0);
+    const int typ = (int)(ry % f);
+    const float wy0 = wsh[typ * 4 + 0], wy1 = wsh[typ * 4 + 1], wy2 = wsh[typ * 4 + 2], wy3 = wsh[typ * 4 + 3];
+    const i64 dvol = a.dnx * a.dny * a.dnz;
+
+    auto layer = [&](int iz, float* L) {
+        #pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float* col = xc + ((c * NZM + iz) * NYM + byl) * TX + lxi;
+            L[c] = __builtin_fmaf(wy3, col[3 * TX], __builtin_fmaf(wy2, col[2 * TX],
+                                  __builtin_fmaf(wy1, col[TX], wy0 * col[0])));
+        }
+    };
+    float L0[3], L1[3], L2[3], L3[3];
+    int cur = -1;
+    for (i64 lz = tz0; lz <= tz1; lz++) {
+        const i64 Z = a.dz0 + lz;
+        const i64 rz = Z - a.oz;
+        const int bzl = (int)(rz / f - nz0);
+        const int tzp = (int)(rz % f);
+        if (bzl != cur) {
+            if (cur >= 0 && bzl == cur + 1) {
+                #pragma unroll
+                for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
+                layer(bzl + 3, L3);
+            } else {
+                layer(bzl, L0); layer(bzl + 1, L1); layer(bzl + 2, L2); layer(bzl + 3, L3);
+            }
+            cur = bzl;
+        }
+        const float wz0 = wsh[tzp * 4 + 0], wz1 = wsh[tzp * 4 + 1], wz2 = wsh[tzp * 4 + 2], wz3 = wsh[tzp * 4 + 3];
+        float d[3];
+        #pragma unroll
+        for (int c = 0; c < 3; c++)
+            d[c] = __builtin_fmaf(wz3, L3[c], __builtin_fmaf(wz2, L2[c], __builtin_fmaf(wz1, L1[c], wz0 * L0[c])));
+        const i64 o = (lz * a.dny + (ty0 + lyi)) * a.dnx + (tx0 + lxi);
+        if (WRITE_DF) {
+            a.dfout[o] = d[0];
+            a.dfout[dvol + o] = d[1];
+            a.dfout[2 * dvol + o] = d[2];
+        } else if (MODE == TW_PLAIN) {
+            resample_fp64<T, INTERP>(a, X, Y, Z, d[0], d[1], d[2], o, dvol);
+        } else {
+            resample_fp32<T, INTERP>(a, X, Y, Z, d[0], d[1], d[2], o, dvol);
+        }
+    }
+}
+
+int pick_tz(int f) {
+    int tz = 4 * f;
+    if (tz < 16) tz = 16;
+    if (tz > 64) tz = 64;
+    return tz;
+}
+
+size_t fused_lds_bytes(int f, int tz) {
+    const int nxm = node_span(TX, f), nym = node_span(TY, f), nzm = node_span(tz, f);
+    return sizeof(float) * (size_t)(4 * f + 3 * nzm * nym * nxm + 3 * nzm * nym * TX);
+}
+
+template <typename T, bool WRITE_DF>
+int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
+    const dim3 block(TX, TY, 1);
+    if (precision == TW_EXACT) {
+        const i64 zmax = 65535;
+        for (i64 z0 = 0; z0 < a.dnz; z0 += zmax) {
+            a.zbase = z0;
+            const i64 nz = std::min(zmax, a.dnz - z0);
+            const dim3 grid((unsigned)((a.dnx + TX - 1) / TX), (unsigned)((a.dny + TY - 1) / TY), (unsigned)nz);
+            if (interp == TW_INTERP_NN)
+                hipLaunchKernelGGL((k_tps_exact<T, TW_INTERP_NN, WRITE_DF>), grid, block, 0, ctx->stream, a);
+            else
+                hipLaunchKernelGGL((k_tps_exact<T, TW_INTERP_LINEAR, WRITE_DF>), grid, block, 0, ctx->stream, a);
+            TW_LAUNCH_CHECK(ctx);
+        }
+        return TW_OK;
+    }
+    a.tz = pick_tz(a.f);
+    const size_t lds = fused_lds_bytes(a.f, a.tz);
+    if (lds > 160 * 1024) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d needs %zu B of LDS", a.f, lds);
+    const i64 nbz = (a.dnz + a.tz - 1) / a.tz;
+    if (nbz > 65535) return tw_fail(ctx, TW_E_UNSUPPORTED, "volume too deep for one launch");
+    const dim3 grid((unsigned)((a.dnx + TX - 1) / TX), (unsigned)((a.dny + TY - 1) / TY), (unsigned)nbz);
+#define TW_FUSED(M, H, I) \
+    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF>), grid, block, lds, ctx->stream, a)
+    if (precision == TW_PLAIN) {
+        if (f16) {
+            if (interp == TW_INTERP_NN) TW_FUSED(TW_PLAIN, true, TW_INTERP_NN); else TW_FUSED(TW_PLAIN, true, TW_INTERP_LINEAR);
+        } else {
+            if (interp == TW_INTERP_NN) TW_FUSED(TW_PLAIN, false, TW_INTERP_NN); else TW_FUSED(TW_PLAIN, false, TW_INTERP_LINEAR);
+        }
+    } else {
+        if (f16) {
+            if (interp == TW_INTERP_NN) TW_FUSED(TW_MIXED, true, TW_INTERP_NN); else TW_FUSED(TW_MIXED, true, TW_INTERP_LINEAR);
+        } else {
+            if (interp == TW_INTERP_NN) TW_FUSED(TW_MIXED, false, TW_INTERP_NN); else TW_FUSED(TW_MIXED, false, TW_INTERP_LINEAR);
+        }
+    }
+#undef TW_FUSED
+    TW_LAUNCH_CHECK(ctx);
+    return TW_OK;
+}
+
+int upload_weights(tw_ctx* ctx, int f, const float** w32, const double** w64) {
+    std::vector<double> tb((size_t)f * 4);
+    tw_bspline_tb(f, tb.data());
+    const size_t b64 = tb.size() * sizeof(double), b32 = tb.size() * sizeof(float);
+    void* p = nullptr;
+    int rc = tw_ws(ctx, WS_WEIGHTS, b64 + b32, &p);
+    if (rc) return rc;
+    std::vector<uint8_t> host(b64 + b32);
+    memcpy(host.data(), tb.data(), b64);
+    float* h32 = (float*)(host.data() + b64);
+    for (size_t i = 0; i < tb.size(); i++) h32[i] = (float)tb[i];
+    TW_HIP(ctx, hipMemcpyAsync(p, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *w64 = (const double*)p;
+    *w32 = (const float*)((uint8_t*)p + b64);
+    return TW_OK;
+}
+
+int check_grid_cover(tw_ctx* ctx, const tw_grid* g, i64 x0, i64 y0, i64 z0, i64 nx, i64 ny, i64 nz) {
+    if (!g || !g->data) return tw_fail(ctx, TW_E_ARG, "null grid");
+    if (g->f < 1) return tw_fail(ctx, TW_E_ARG, "grid spacing must be >= 1");
+    if (g->mem != TW_MEM_DEVICE && g->mem != TW_MEM_HOST) return tw_fail(ctx, TW_E_ARG, "bad grid mem");
+    if (g->storage == TW_DF_F16 && (g->mem != TW_MEM_DEVICE || !g->qtab))
+        return tw_fail(ctx, TW_E_ARG, "an FP16 grid must be device-resident with its block table");
+    if (x0 < g->ox || y0 < g->oy || z0 < g->oz)
+        return tw_fail(ctx, TW_E_ARG, "output window starts before the grid origin");
+    const i64 f = g->f;
+    if ((x0 + nx - 1 - g->ox) / f + 3 >= g->gx || (y0 + ny - 1 - g->oy) / f + 3 >= g->gy ||
+        (z0 + nz - 1 - g->oz) / f + 3 >= g->gz)
+        return tw_fail(ctx, TW_E_ARG, "grid (%lld x %lld x %lld, f=%d) does not cover the output window",
+                       (long long)g->gx, (long long)g->gy, (long long)g->gz, g->f);
+    return TW_OK;
+}
+
+int stage_grid(tw_ctx* ctx, const tw_grid* g, const void** dev) {
+    if (g->mem == TW_MEM_DEVICE) {
+        *dev = g->data;
+        return TW_OK;
+    }
+    const size_t bytes = (size_t)g->gx * g->gy * g->gz * 3 * sizeof(float);
+    void* p = nullptr;
+    int rc = tw_ws(ctx, WS_GRID, bytes, &p);
+    if (rc) return rc;
+    TW_HIP(ctx, hipMemcpyAsync(p, g->data, bytes, hipMemcpyHostToDevice, ctx->stream));
+    *dev = p;
+    return TW_OK;
+}
+
+void fill_grid_args(TpsArgs& a, const tw_grid* g, const void* gdev) {
+    a.grid = gdev;
+    a.gx = g->gx; a.gy = g->gy; a.gz = g->gz;
+    a.ox = g->ox; a.oy = g->oy; a.oz = g->oz;
+    a.f = g->f;
+    a.qtab = g->qtab;
+    a.qbx = (g->gx + 7) / 8;
+    a.qby = (g->gy + 7) / 8;
+}
+
+}  // namespace
+
+extern "C" int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
+                               int precision) {
+    if (!ctx) return TW_E_ARG;
+    int rc;
+    if ((rc = tw_check_volume(ctx, src, "src")) || (rc = tw_check_volume(ctx, dst, "dst"))) return rc;
+    if (src->dtype != dst->dtype || src->nc != dst->nc)
+        return tw_fail(ctx, TW_E_ARG, "src and dst must share dtype and channel count");
+    if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
+        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN");
+    if (precision != TW_EXACT && precision != TW_PLAIN && precision != TW_MIXED)
+        return tw_fail(ctx, TW_E_ARG, "unknown precision %d", precision);
+    if ((rc = check_grid_cover(ctx, grid, dst->x0, dst->y0, dst->z0, dst->nx, dst->ny, dst->nz))) return rc;
+    if (precision == TW_EXACT && grid->storage != TW_DF_F32)
+        return tw_fail(ctx, TW_E_ARG, "TW_EXACT needs an FP32 grid");
+    if ((rc = tw_bind(ctx))) return rc;
+    tw_stages_reset(ctx);
+    tw_stage_begin(ctx, ST_TOTAL);
+
+    TpsArgs a;
+    memset(&a, 0, sizeof a);
+    const void* sdev = nullptr;
+    void* ddev = nullptr;
+    const void* gdev = nullptr;
+    if ((rc = tw_stage_in(ctx, src, WS_SRC, &sdev))) return rc;
+    if ((rc = tw_stage_out_buffer(ctx, dst, WS_DST, &ddev))) return rc;
+    if ((rc = stage_grid(ctx, grid, &gdev))) return rc;
+    if ((rc = upload_weights(ctx, grid->f, &a.w32, &a.w64))) return rc;
+    a.src = sdev;
+    a.dst = ddev;
+    a.snx = src->nx; a.sny = src->ny; a.snz = src->nz;
+    a.sx0 = src->x0; a.sy0 = src->y0; a.sz0 = src->z0;
+    a.fnx = src->full_nx ? src->full_nx : src->x0 + src->nx;
+    a.fny = src->full_ny ? src->full_ny : src->y0 + src->ny;
+    a.fnz = src->full_nz ? src->full_nz : src->z0 + src->nz;
+    a.dnx = dst->nx; a.dny = dst->ny; a.dnz = dst->nz;
+    a.dx0 = dst->x0; a.dy0 = dst->y0; a.dz0 = dst->z0;
+    a.nc = src->nc;
+    fill_grid_args(a, grid, gdev);
+
+    const bool f16 = grid->storage == TW_DF_F16;
+    tw_stage_begin(ctx, ST_RESAMPLE);
+    rc = src->dtype == TW_U8 ? launch_tps<uint8_t, false>(ctx, a, interp, precision, f16)
+                             : launch_tps<uint16_t, false>(ctx, a, interp, precision, f16);
+    if (rc) return rc;
+    tw_stage_end(ctx, ST_RESAMPLE);
+    if ((rc = tw_stage_out_finish(ctx, dst, ddev))) return rc;
+    tw_stage_end(ctx, ST_TOTAL);
+    if (src->mem == TW_MEM_HOST || grid->mem == TW_MEM_HOST) TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TW_OK;
+}
+
+extern "C" int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int64_t ny, int64_t nz, int64_t x0,
+                              int64_t y0, int64_t z0, int precision, float* out, int mem) {
+    if (!ctx) return TW_E_ARG;
+    if (!out || nx <= 0 || ny <= 0 || nz <= 0) return tw_fail(ctx, TW_E_ARG, "bad output window");
+    int rc;
+    if ((rc = check_grid_cover(ctx, grid, x0, y0, z0, nx, ny, nz))) return rc;
+    if (precision == TW_EXACT && grid->storage != TW_DF_F32)
+        return tw_fail(ctx, TW_E_ARG, "TW_EXACT needs an FP32 grid");
+    if ((rc = tw_bind(ctx))) return rc;
+    tw_stages_reset(ctx);
+    TpsArgs a;
+    memset(&a, 0, sizeof a);
+    const void* gdev = nullptr;
+    if ((rc = stage_grid(ctx, grid, &gdev))) return rc;
+    if ((rc = upload_weights(ctx, grid->f, &a.w32, &a.w64))) return rc;
+    const size_t bytes = (size_t)nx * ny * nz * 3 * sizeof(float);
+    float* dout = out;
+    if (mem == TW_MEM_HOST) {
+        void* p = nullptr;
+        if ((rc = tw_ws(ctx, WS_DF, bytes, &p))) return rc;
+        dout = (float*)p;
+    }
+    a.dnx = nx; a.dny = ny; a.dnz = nz;
+    a.dx0 = x0; a.dy0 = y0; a.dz0 = z0;
+    a.dfout = dout;
+    a.nc = 1;
+    a.fnx = a.fny = a.fnz = 1;
+    fill_grid_args(a, grid, gdev);
+    rc = launch_tps<uint8_t, true>(ctx, a, TW_INTERP_LINEAR, precision, grid->storage == TW_DF_F16);
+    if (rc) return rc;
+    if (mem == TW_MEM_HOST) {
+        TW_HIP(ctx, hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TW_OK;
+}

@@ -1,0 +1,95 @@
+// tw_internal.h -- shared definitions of libterawarp (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <string>
+#include <vector>
+
+#include "../../include/terawarp.h"
+
+typedef int64_t i64;
+
+// Workspace slots cached on a context (grown on demand, freed at destroy).
+enum TwSlot {
+    WS_SRC = 0,     // staged source volume
+    WS_DST,         // staged destination volume
+    WS_GRID,        // staged / internal displacement grid
+    WS_QTAB,        // FP16 grid block tables
+    WS_SOLVE_A,     // TPS system matrix
+    WS_SOLVE_B,     // TPS right-hand side / solution
+    WS_SOLVE_AUX,   // pivots, landmark packs
+    WS_SOLVE_AUX2,  // refinement vectors
+    WS_PTS,         // landmark / coefficient packs for the grid kernel
+    WS_DF,          // dense DF staging
+    WS_WEIGHTS,     // B-spline weight tables
+    WS_COUNT
+};
+
+enum TwStage { ST_FIT = 0, ST_GRID, ST_QUANT, ST_RESAMPLE, ST_H2D, ST_D2H, ST_TOTAL, ST_COUNT };
+
+struct tw_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::string err;
+    void* ws[WS_COUNT] = {};
+    size_t ws_cap[WS_COUNT] = {};
+    hipEvent_t ev_begin[ST_COUNT] = {};
+    hipEvent_t ev_end[ST_COUNT] = {};
+    bool ev_used[ST_COUNT] = {};
+    int refine_iters = 0;
+    double fit_residual = 0.0;
+    // Device properties used for launch shaping.
+    int num_cus = 256;
+};
+
+// Set the context error string and return code.
+int tw_fail(tw_ctx* ctx, int code, const char* fmt, ...);
+
+#define TW_HIP(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return tw_fail((ctx), TW_E_DEVICE, "%s failed: %s (%s:%d)", #call,         \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define TW_LAUNCH_CHECK(ctx) TW_HIP(ctx, hipGetLastError())
+
+// Grow-only device workspace of a context.
+int tw_ws(tw_ctx* ctx, int slot, size_t bytes, void** out);
+
+// Stage timing (no-ops unless timing is enabled on the context).
+void tw_stage_begin(tw_ctx* ctx, int stage);
+void tw_stage_end(tw_ctx* ctx, int stage);
+void tw_stages_reset(tw_ctx* ctx);
+
+// Make ctx's device current.
+int tw_bind(tw_ctx* ctx);
+
+inline int dtype_size(int dt) { return dt == TW_U8 ? 1 : (dt == TW_U16 ? 2 : 0); }
+
+inline i64 vol_elems(const tw_volume* v) { return v->nx * v->ny * v->nz * v->nc; }
+
+// Validate a volume descriptor; returns TW_OK or TW_E_ARG with message.
+int tw_check_volume(tw_ctx* ctx, const tw_volume* v, const char* name);
+
+// Bring a (possibly host) volume to the device: *dev points at device data.
+int tw_stage_in(tw_ctx* ctx, const tw_volume* v, int slot, const void** dev);
+// Device buffer for an output volume: v->data itself or a workspace slot.
+int tw_stage_out_buffer(tw_ctx* ctx, const tw_volume* v, int slot, void** dev);
+// Copy a staged output back to host memory (synchronises the stream).
+int tw_stage_out_finish(tw_ctx* ctx, const tw_volume* v, const void* dev);
+
+// Host-side FP64 cubic B-spline weights TB (n x 4), the reference's
+// q_nonrigid_ini_bsplinebasis_3D T*B product (q_imgwarp_tps_quicksmallmemory.cpp:663-691).
+void tw_bspline_tb(int n, double* TB);
+
+// Internal launchers shared between translation units.
+int tw_grid_quantize_f16(tw_ctx* ctx, const float* df32, tw_grid* grid);

@@ -1,0 +1,284 @@
+"""Python view of the C ABI (include/terawarp.h).
+
+Volumes are numpy arrays (host memory: the library stages them through the
+GPU) or torch tensors on a ROCm device (device memory: no copies), shaped
+(c, z, y, x) or (z, y, x), dtype uint8 or uint16 (torch.int16 is accepted as
+uint16 storage).  Points are (K, 3) float32 xyz.
+
+Mirrors the reference's call structure: `tps_fit` + `grid_eval` +
+`resample_tps` are the three stages of imgwarp_smallmemory_image's GPU
+branch (q_imgwarp_tps_quicksmallmemory.cpp:2066, 2143), `resample_affine` is
+q_imagewarp_affine_image's (q_warp_affine_tps.cpp:1453).
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+PLAIN, MIXED, EXACT = L.TW_PLAIN, L.TW_MIXED, L.TW_EXACT
+LINEAR, NN = L.TW_INTERP_LINEAR, L.TW_INTERP_NN
+F32, F16 = L.TW_DF_F32, L.TW_DF_F16
+
+
+class TeraWarpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"terawarp error {code}: {msg}")
+        self.code = code
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _pts(p):
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    if p.ndim != 2 or p.shape[1] != 3:
+        raise ValueError("points must be (K, 3)")
+    return p
+
+
+def _vol(a, origin=(0, 0, 0), full=None):
+    """TwVolume for a numpy array or a torch device tensor."""
+    if _is_torch(a):
+        torch = _torch()
+        if not a.is_cuda:
+            raise ValueError("torch volumes must live on the GPU; pass numpy arrays for host memory")
+        if not a.is_contiguous():
+            raise ValueError("volume tensor must be contiguous")
+        if a.dtype == torch.uint8:
+            dt = L.TW_U8
+        elif a.dtype in (torch.int16, getattr(torch, "uint16", torch.int16)):
+            dt = L.TW_U16
+        else:
+            raise ValueError(f"unsupported volume dtype {a.dtype}")
+        ptr, mem, shape = a.data_ptr(), L.TW_MEM_DEVICE, tuple(a.shape)
+    else:
+        if not (isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"]):
+            raise ValueError("numpy volume must be C-contiguous")
+        if a.dtype == np.uint8:
+            dt = L.TW_U8
+        elif a.dtype == np.uint16:
+            dt = L.TW_U16
+        else:
+            raise ValueError(f"unsupported volume dtype {a.dtype}")
+        ptr, mem, shape = a.ctypes.data, L.TW_MEM_HOST, a.shape
+    if len(shape) == 3:
+        shape = (1,) + shape
+    if len(shape) != 4:
+        raise ValueError("volume must be (z, y, x) or (c, z, y, x)")
+    nc, nz, ny, nx = shape
+    v = L.TwVolume()
+    v.data = ptr
+    v.nx, v.ny, v.nz, v.nc = nx, ny, nz, nc
+    v.x0, v.y0, v.z0 = origin
+    full = full or (0, 0, 0)
+    v.full_nx, v.full_ny, v.full_nz = full
+    v.dtype, v.mem = dt, mem
+    return v
+
+
+def grid_dim(n, f):
+    """ceil(n / f) + 3 nodes (q_imgwarp_tps_quicksmallmemory.cpp:592)."""
+    return int(math.ceil(n / f)) + 3
+
+
+def affine_fit(pts_a, pts_b):
+    """4x4 M with pts_b ~= M pts_a (DLT with Hartley normalisation)."""
+    a, b = _pts(pts_a), _pts(pts_b)
+    M = np.zeros((4, 4), np.float64)
+    rc = L.load().tw_affine_fit(_fptr(a), _fptr(b), a.shape[0], _dptr(M))
+    if rc != L.TW_OK:
+        raise TeraWarpError(rc, "affine fit failed (fewer than 4 points or degenerate)")
+    return M
+
+
+ROTATION, TRANSLATION, SCALING = 0, 1, 2
+
+
+def mode_matrix(mode, para, size_xyz):
+    """Target->subject matrix of the README's rotation/translation/scaling modes."""
+    p = np.ascontiguousarray(para, np.float64)
+    M = np.zeros((4, 4), np.float64)
+    rc = L.load().tw_mode_matrix(int(mode), _dptr(p), int(size_xyz[0]), int(size_xyz[1]), int(size_xyz[2]),
+                                 _dptr(M))
+    if rc != L.TW_OK:
+        raise TeraWarpError(rc, f"invalid mode {mode} / parameters {list(para)}")
+    return M
+
+
+class Grid:
+    """A displacement grid (3 planes) in host (numpy) or device (torch) memory."""
+
+    def __init__(self, warper, data, gshape_xyz, f, origin, storage):
+        self._w = warper
+        self.data = data
+        self.g = L.TwGrid()
+        gx, gy, gz = gshape_xyz
+        self.g.gx, self.g.gy, self.g.gz = gx, gy, gz
+        self.g.f = f
+        self.g.ox, self.g.oy, self.g.oz = origin
+        self.g.storage = storage
+        if _is_torch(data):
+            self.g.data = data.data_ptr()
+            self.g.mem = L.TW_MEM_DEVICE
+        else:
+            self.g.data = data.ctypes.data
+            self.g.mem = L.TW_MEM_HOST
+        self.g.qtab = None
+
+    def release(self):
+        if self.g.qtab:
+            L.load().tw_grid_release(self._w.ctx, ctypes.byref(self.g))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Warper:
+    """One library context (device, HIP stream, cached workspaces)."""
+
+    def __init__(self, device=0, timing=False):
+        self.lib = L.load()
+        ctx = ctypes.c_void_p()
+        rc = self.lib.tw_create(device, ctypes.byref(ctx))
+        if rc != L.TW_OK:
+            raise TeraWarpError(rc, "tw_create failed (no usable HIP device?)")
+        self.ctx = ctx
+        if timing:
+            self.lib.tw_set_timing(self.ctx, 1)
+
+    def close(self):
+        if self.ctx:
+            self.lib.tw_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != L.TW_OK:
+            raise TeraWarpError(rc, self.lib.tw_last_error(self.ctx).decode(errors="replace"))
+
+    def set_stream(self, stream):
+        """Launch on a torch.cuda.Stream (or a raw hipStream_t int); None = own stream."""
+        ptr = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+        self._check(self.lib.tw_set_stream(self.ctx, ptr))
+
+    def set_timing(self, on=True):
+        self._check(self.lib.tw_set_timing(self.ctx, 1 if on else 0))
+
+    def sync(self):
+        self._check(self.lib.tw_sync(self.ctx))
+
+    def timings(self):
+        t = L.TwTimings()
+        self._check(self.lib.tw_get_timings(self.ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in L.TwTimings._fields_ if k != "reserved"}
+
+    # ---- affine --------------------------------------------------------
+    def resample_affine(self, src, M, dst=None, out_size_xyz=None, interp=LINEAR, src_origin=(0, 0, 0),
+                        dst_origin=(0, 0, 0), full=None):
+        M = np.ascontiguousarray(M, np.float64).reshape(-1)[:12].copy()
+        if dst is None:
+            if _is_torch(src):
+                raise ValueError("pass dst for device volumes")
+            s4 = src if src.ndim == 4 else src[None]
+            nx, ny, nz = out_size_xyz or (s4.shape[3], s4.shape[2], s4.shape[1])
+            dst = np.zeros((s4.shape[0], nz, ny, nx), src.dtype)
+            if src.ndim == 3:
+                dst = dst[0]
+        vs, vd = _vol(src, src_origin, full), _vol(dst, dst_origin)
+        self._check(self.lib.tw_resample_affine(self.ctx, ctypes.byref(vs), ctypes.byref(vd), _dptr(M), interp))
+        return dst
+
+    # ---- TPS -------------------------------------------------------------
+    def tps_fit(self, centres, targets, lam=0.2, precision=PLAIN):
+        ce, ta = _pts(centres), _pts(targets)
+        if ce.shape != ta.shape:
+            raise ValueError("centres and targets differ in shape")
+        K = ce.shape[0]
+        c = np.zeros((K, 4), np.float64)
+        d = np.zeros((4, 4), np.float64)
+        self._check(self.lib.tw_tps_fit(self.ctx, _fptr(ce), _fptr(ta), K, lam, precision, _dptr(c), _dptr(d)))
+        return c, d
+
+    def new_grid(self, gshape_xyz, f, origin=(0, 0, 0), storage=F32, device=True):
+        gx, gy, gz = gshape_xyz
+        if device:
+            torch = _torch()
+            dt = torch.float32 if storage == F32 else torch.float16
+            data = torch.empty((3, gz, gy, gx), dtype=dt, device="cuda")
+        else:
+            if storage != F32:
+                raise ValueError("host grids are FP32")
+            data = np.zeros((3, gz, gy, gx), np.float32)
+        return Grid(self, data, gshape_xyz, f, origin, storage)
+
+    def grid_eval(self, centres, c, d, grid, precision=PLAIN):
+        ce = _pts(centres)
+        c = np.ascontiguousarray(c, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        self._check(self.lib.tw_tps_grid_eval(self.ctx, _fptr(ce), ce.shape[0], _dptr(c), _dptr(d), precision,
+                                              ctypes.byref(grid.g)))
+        return grid
+
+    def grid_from_array(self, df, f, origin=(0, 0, 0)):
+        """Wrap an existing (3, gz, gy, gx) float32 numpy array or device tensor."""
+        shape = tuple(df.shape)
+        return Grid(self, df, (shape[3], shape[2], shape[1]), f, origin, F32)
+
+    def resample_tps(self, src, grid, dst=None, interp=LINEAR, precision=PLAIN, src_origin=(0, 0, 0),
+                     dst_origin=(0, 0, 0), full=None):
+        if dst is None:
+            if _is_torch(src):
+                raise ValueError("pass dst for device volumes")
+            dst = np.zeros_like(src)
+        vs, vd = _vol(src, src_origin, full), _vol(dst, dst_origin)
+        self._check(self.lib.tw_resample_tps(self.ctx, ctypes.byref(vs), ctypes.byref(vd), ctypes.byref(grid.g),
+                                             interp, precision))
+        return dst
+
+    def upsample_df(self, grid, size_xyz, origin=(0, 0, 0), precision=PLAIN):
+        nx, ny, nz = size_xyz
+        out = np.zeros((3, nz, ny, nx), np.float32)
+        self._check(self.lib.tw_upsample_df(self.ctx, ctypes.byref(grid.g), nx, ny, nz, origin[0], origin[1],
+                                            origin[2], precision, _fptr(out), L.TW_MEM_HOST))
+        return out
+
+    def warp_tps(self, src, centres, targets, f=4, lam=0.2, interp=LINEAR, precision=PLAIN, dst=None):
+        ce, ta = _pts(centres), _pts(targets)
+        if dst is None:
+            if _is_torch(src):
+                raise ValueError("pass dst for device volumes")
+            dst = np.zeros_like(src)
+        vs, vd = _vol(src), _vol(dst)
+        self._check(self.lib.tw_warp_tps(self.ctx, ctypes.byref(vs), ctypes.byref(vd), _fptr(ce), _fptr(ta),
+                                         ce.shape[0], f, lam, interp, precision))
+        return dst

@@ -1,0 +1,212 @@
+"""GPU TPS path vs the FP64 oracle.
+
+Bars (stated per test):
+  TW_EXACT grid / upsample / resample  bit-identical to the oracle
+  TW_PLAIN resample                   |diff| <= 1 LSB everywhere, exact on >= 99.99 % of voxels
+  TW_PLAIN / TW_MIXED grid field      max |DF - DF_oracle| <= 1e-2 voxel (typically ~1e-4)
+  fit (FP64 LU / FP32 LU + FP64 IR)   coefficients within 1e-8 relative of the oracle
+"""
+import os
+
+import numpy as np
+import pytest
+
+from terawarp_amd import synth, warp
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DF_TOL = 1e-2  # voxels
+
+
+def case(size, dtype, K, dmax, seed, n_blobs=60):
+    amp = (20, 200) if dtype == np.uint8 else (100, 4000)
+    vol = synth.blob_volume(size[::-1], dtype, n_blobs=n_blobs, amp=amp, seed=seed)
+    sub, tar = synth.landmarks(size, K, dmax, seed=seed)
+    return vol, sub, tar
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("K", [16, 64, 512])
+@pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
+def test_fit_matches_oracle(warper, oracle, K, precision):
+    size = (256, 256, 64)
+    _, sub, tar = case(size, np.uint8, K, 5.0, seed=K, n_blobs=1)
+    c_ref, d_ref = oracle.stps_fit(tar, sub)
+    c, d = warper.tps_fit(tar, sub, precision=precision)
+    t = warper.timings()
+    assert rel_err(c, c_ref) < 1e-8, rel_err(c, c_ref)
+    assert np.abs(d - d_ref).max() < 1e-8
+    assert t["fit_residual"] < 1e-10
+
+
+def test_fit_rejects_degenerate(warper):
+    P = np.zeros((10, 3), np.float32)
+    P[:, 0] = np.arange(10)
+    with pytest.raises(warp.TeraWarpError):
+        warper.tps_fit(P, P)
+    with pytest.raises(warp.TeraWarpError):
+        warper.tps_fit(P[:4], P[:4])
+
+
+@pytest.mark.parametrize("g", [4, 8])
+def test_grid_eval_exact_bitwise(warper, oracle, g):
+    size = (200, 150, 60)
+    _, sub, tar = case(size, np.uint8, 64, 8.0, seed=g, n_blobs=1)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    ref = oracle.grid_eval(tar, c, d, gsh, g, origin=(16, 0, 8))
+    grid = warper.new_grid(gsh, g, origin=(16, 0, 8), device=False)
+    warper.grid_eval(tar, c, d, grid, precision=warp.EXACT)
+    mism = np.count_nonzero(grid.data != ref)
+    assert mism == 0, (mism, np.abs(grid.data - ref).max())
+
+
+@pytest.mark.parametrize("K", [64, 512])
+def test_grid_eval_fp32_accuracy(warper, oracle, K):
+    size = (1024, 1024, 256)
+    g = 8
+    _, sub, tar = case(size, np.uint8, K, 20.0, seed=3, n_blobs=1)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = (warp.grid_dim(1024, g), warp.grid_dim(1024, g), 12)  # a z-slab of nodes
+    ref = oracle.grid_eval(tar, c, d, gsh, g)
+    grid = warper.new_grid(gsh, g, device=False)
+    warper.grid_eval(tar, c, d, grid, precision=warp.PLAIN)
+    err = np.abs(grid.data - ref).max()
+    assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("g", [4, 8])
+def test_upsample_exact_and_plain(warper, oracle, g):
+    size = (70, 45, 33)
+    _, sub, tar = case(size, np.uint8, 32, 6.0, seed=g, n_blobs=1)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    df = oracle.grid_eval(tar, c, d, gsh, g)
+    ref = oracle.dense_df(size, df, g)
+    grid = warper.grid_from_array(df, g)
+    ex = warper.upsample_df(grid, size, precision=warp.EXACT)
+    assert np.array_equal(ex, ref), np.abs(ex - ref).max()
+    pl = warper.upsample_df(grid, size, precision=warp.PLAIN)
+    assert np.abs(pl - ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["tps_u8", "tps_u16"])
+def test_resample_fixture_exact(warper, name):
+    f = dict(np.load(os.path.join(GOLD, name + ".npz")))
+    g = int(f["g"])
+    grid = warper.grid_from_array(f["df"], g)
+    out = warper.resample_tps(f["vol"], grid, precision=warp.EXACT)
+    assert np.array_equal(out, f["out"])
+    out = warper.resample_tps(f["vol"], grid, precision=warp.PLAIN)
+    diff = np.abs(out.astype(int) - f["out"].astype(int))
+    assert diff.max() <= 1 and np.mean(diff == 0) >= 0.9999
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("interp", [warp.LINEAR, warp.NN])
+def test_resample_exact_bitwise(warper, oracle, dtype, interp):
+    size = (131, 77, 45)
+    g = 4
+    vol, sub, tar = case(size, dtype, 40, 5.0, seed=17)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    df = oracle.grid_eval(tar, c, d, gsh, g)
+    ref = oracle.tps_resample(vol, df, g, interp=interp)
+    grid = warper.grid_from_array(df, g)
+    got = warper.resample_tps(vol, grid, interp=interp, precision=warp.EXACT)
+    assert np.array_equal(got, ref), np.count_nonzero(got != ref)
+
+
+@pytest.mark.parametrize("dtype,size,g,K,dmax", [
+    (np.uint8, (256, 256, 64), 4, 64, 5.0),      # CPU-path config
+    (np.uint16, (300, 260, 70), 8, 128, 20.0),
+])
+def test_chain_plain_and_mixed_vs_oracle(warper, oracle, dtype, size, g, K, dmax):
+    vol, sub, tar = case(size, dtype, K, dmax, seed=29, n_blobs=120)
+    ref, df_ref, _ = oracle.warp_tps(vol, tar, sub, g=g)
+    dense_ref = oracle.dense_df(size, df_ref, g)
+    # Plain form: whole chain on the GPU.
+    out = warper.warp_tps(vol, tar, sub, f=g, precision=warp.PLAIN)
+    diff = np.abs(out.astype(np.int64) - ref.astype(np.int64))
+    assert diff.max() <= 1, diff.max()
+    assert np.mean(diff == 0) >= 0.9999, np.mean(diff == 0)
+    # Stage errors: FP32 grid and its upsample against the FP64 oracle.
+    c, d = warper.tps_fit(tar, sub, precision=warp.PLAIN)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    grid = warper.new_grid(gsh, g, device=False)
+    warper.grid_eval(tar, c, d, grid, precision=warp.PLAIN)
+    assert np.abs(grid.data - df_ref).max() < DF_TOL
+    dense = warper.upsample_df(grid, size, precision=warp.PLAIN)
+    assert np.abs(dense - dense_ref).max() < DF_TOL
+    # Mixed form: FP16 grid, FP32 blend.
+    cm, dm = warper.tps_fit(tar, sub, precision=warp.MIXED)
+    g16 = warper.new_grid(gsh, g, storage=warp.F16, device=True)
+    warper.grid_eval(tar, cm, dm, g16, precision=warp.MIXED)
+    dense16 = warper.upsample_df(g16, size, precision=warp.MIXED)
+    assert np.abs(dense16 - dense_ref).max() < DF_TOL
+    outm = warper.warp_tps(vol, tar, sub, f=g, precision=warp.MIXED)
+    dm_ = np.abs(outm.astype(np.int64) - ref.astype(np.int64))
+    # Intensity errors follow from the <= 1e-2 voxel displacement bound.
+    assert np.mean(dm_ <= 1) >= 0.99
+    g16.release()
+
+
+def test_bricks_reassemble_the_full_output(warper, oracle):
+    """Output bricks at offsets, each with its own grid, equal the full warp."""
+    import torch
+    size = (96, 80, 48)
+    g = 4
+    vol, sub, tar = case(size, np.uint16, 32, 6.0, seed=41)
+    c, d = oracle.stps_fit(tar, sub)
+    full_grid = warper.new_grid(tuple(warp.grid_dim(s, g) for s in size), g, device=False)
+    warper.grid_eval(tar, c, d, full_grid, precision=warp.EXACT)
+    full = warper.resample_tps(vol, full_grid, precision=warp.PLAIN)
+    src = torch.from_numpy(vol.view(np.int16)).cuda()
+    for (x0, y0, z0), (nx, ny, nz) in [((0, 0, 0), (40, 33, 20)), ((40, 32, 20), (56, 48, 28)),
+                                       ((52, 8, 4), (30, 21, 13))]:
+        gb = warper.new_grid((warp.grid_dim(nx, g), warp.grid_dim(ny, g), warp.grid_dim(nz, g)), g,
+                             origin=(x0, y0, z0), device=True)
+        warper.grid_eval(tar, c, d, gb, precision=warp.EXACT)
+        dst = torch.zeros((nz, ny, nx), dtype=torch.int16, device="cuda")
+        warper.resample_tps(src, gb, dst=dst, precision=warp.PLAIN, dst_origin=(x0, y0, z0))
+        torch.cuda.synchronize()
+        brick = dst.cpu().numpy().view(np.uint16)
+        # Same nodes at the same global positions only when the brick origin
+        # is grid-aligned; otherwise the B-spline phase differs.
+        if x0 % g == 0 and y0 % g == 0 and z0 % g == 0:
+            assert np.array_equal(brick, full[z0:z0 + nz, y0:y0 + ny, x0:x0 + nx])
+
+
+def test_resample_rejects_uncovered_grid(warper):
+    vol = np.zeros((20, 20, 20), np.uint8)
+    grid = warper.grid_from_array(np.zeros((3, 5, 5, 5), np.float32), 4)
+    with pytest.raises(warp.TeraWarpError):
+        warper.resample_tps(vol, grid)
+
+
+def test_plain_config_size_sampled(warper, oracle):
+    """1024x1024x256 uint16, K=512, g=8: GPU chain vs oracle on sampled block slabs."""
+    import torch
+    size = (1024, 1024, 256)
+    g = 8
+    vol = synth.blob_volume(size[::-1], np.uint16, n_blobs=200, amp=(100, 4000), seed=2)
+    sub, tar = synth.landmarks(size, 512, 20.0, seed=2)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    df = oracle.grid_eval(tar, c, d, gsh, g)
+    grid = warper.grid_from_array(df, g)
+    src = torch.from_numpy(vol.view(np.int16)).cuda()
+    dst = torch.empty_like(src)
+    dgrid = warper.grid_from_array(torch.from_numpy(df).cuda(), g)
+    warper.resample_tps(src, dgrid, dst=dst, precision=warp.PLAIN)
+    torch.cuda.synchronize()
+    out = dst.cpu().numpy().view(np.uint16)
+    for zb in (0, 13, 31):
+        ref = oracle.tps_resample(vol, df, g, zblocks=(zb, zb + 1))
+        sl = slice(zb * g, (zb + 1) * g)
+        diff = np.abs(out[sl].astype(np.int64) - ref[sl].astype(np.int64))
+        assert diff.max() <= 1 and np.mean(diff == 0) >= 0.9999, (zb, diff.max(), np.mean(diff == 0))
+    del grid

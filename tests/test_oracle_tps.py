@@ -1,0 +1,252 @@
+"""Pins of the FP64 oracle's TPS functions against closed forms, library
+routines and invariants (the reference ships no golden vectors).
+
+Each check is chosen so that a plausible slip in the oracle (dropped
+lambda, transposed d, wrong node offset, swapped axes in the B-spline
+ordering, wrong rounding) fails it.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+LAM = 0.2
+
+
+def k_matrix(P):
+    """K_ij = -|p_i - p_j| with float32 differences widened to float64."""
+    d = (P[:, None, :] - P[None, :, :]).astype(np.float32).astype(np.float64)
+    return -np.sqrt((d ** 2).sum(-1))
+
+
+def bordered_solve(P, Q, lam=LAM):
+    """numpy.linalg.solve of [[K+lam I, X], [X', 0]] [c; d] = [Y; 0]."""
+    N = P.shape[0]
+    X = np.hstack([np.ones((N, 1)), P.astype(np.float64)])
+    Y = np.hstack([np.ones((N, 1)), Q.astype(np.float64)])
+    M = np.zeros((N + 4, N + 4))
+    M[:N, :N] = k_matrix(P) + lam * np.eye(N)
+    M[:N, N:] = X
+    M[N:, :N] = X.T
+    rhs = np.vstack([Y, np.zeros((4, 4))])
+    s = np.linalg.solve(M, rhs)
+    return s[:N], s[N:]
+
+
+def make_pts(rng, K, size=(64, 48, 20), dmax=3.0):
+    P = (rng.uniform(0, 1, (K, 3)) * (np.array(size) - 1)).astype(np.float32)
+    Q = (P + rng.uniform(-dmax, dmax, (K, 3))).astype(np.float32)
+    return P, Q
+
+
+@pytest.mark.parametrize("K", [5, 8, 13, 40])
+def test_fit_matches_numpy_bordered_solve(oracle, rng, K):
+    P, Q = make_pts(rng, K)
+    c, d = oracle.stps_fit(P, Q, LAM)
+    c_ref, d_ref = bordered_solve(P, Q)
+    scale = np.abs(c_ref).max() + 1e-30
+    assert np.abs(c - c_ref).max() <= 1e-9 * scale
+    assert np.allclose(d, d_ref, rtol=1e-9, atol=1e-9)
+
+
+def test_fit_satisfies_bordered_system(oracle, rng):
+    P, Q = make_pts(rng, 96)
+    c, d = oracle.stps_fit(P, Q, LAM)
+    N = P.shape[0]
+    X = np.hstack([np.ones((N, 1)), P.astype(np.float64)])
+    Y = np.hstack([np.ones((N, 1)), Q.astype(np.float64)])
+    r1 = (k_matrix(P) + LAM * np.eye(N)) @ c + X @ d - Y
+    r2 = X.T @ c
+    assert np.abs(r1).max() <= 1e-9 * np.abs(Y).max()
+    assert np.abs(r2).max() <= 1e-9 * (np.abs(X).max() * np.abs(c).max() + 1e-12)
+
+
+def test_null_space_and_bordered_paths_agree(oracle, rng):
+    P, Q = make_pts(rng, 200)
+    c1, d1 = oracle.stps_fit(P, Q, LAM)
+    c2, d2 = oracle.stps_fit(P, Q, LAM, bordered=True)
+    assert np.abs(c1 - c2).max() <= 1e-9 * np.abs(c2).max()
+    assert np.abs(d1 - d2).max() <= 1e-9
+
+
+def test_identical_landmarks_give_identity(oracle, rng):
+    P, _ = make_pts(rng, 30)
+    c, d = oracle.stps_fit(P, P, LAM)
+    assert np.abs(c).max() < 1e-10
+    assert np.allclose(d, np.eye(4), atol=1e-10)
+
+
+def test_exact_affine_landmarks_give_affine(oracle, rng):
+    P, _ = make_pts(rng, 30)
+    A = np.eye(3) + rng.uniform(-0.1, 0.1, (3, 3))
+    t = rng.uniform(-5, 5, 3)
+    Q = (P.astype(np.float64) @ A.T + t).astype(np.float32)
+    c, d = oracle.stps_fit(P, Q, LAM)
+    # Y = X d  =>  d = [[1, t'], [0, A']] up to the float32 rounding of Q.
+    d_ref = np.zeros((4, 4))
+    d_ref[0, 0] = 1
+    d_ref[0, 1:] = t
+    d_ref[1:, 1:] = A.T
+    assert np.abs(c).max() < 1e-5
+    assert np.allclose(d, d_ref, atol=1e-4)
+
+
+def test_fit_rejects_coplanar_centres(oracle, rng):
+    P, Q = make_pts(rng, 20)
+    P[:, 2] = 5.0
+    with pytest.raises(RuntimeError):
+        oracle.stps_fit(P, Q, LAM)
+
+
+def test_grid_at_landmarks_reproduces_regularised_fit(oracle, rng):
+    """At a landmark p_i the STPS map equals q_i - lambda c_i (bordered system row i)."""
+    g = 4
+    gsh = (12, 10, 8)
+    # Landmarks exactly on grid nodes: node i sits at (i-1) g.
+    idx = np.stack([rng.integers(0, s, 24) for s in gsh], 1)
+    idx = np.unique(idx, axis=0)
+    P = ((idx - 1) * g).astype(np.float32)
+    Q = (P + rng.uniform(-2, 2, P.shape)).astype(np.float32)
+    c, d = oracle.stps_fit(P, Q, LAM)
+    df = oracle.grid_eval(P, c, d, gsh, g)
+    for n, (i, j, k) in enumerate(idx):
+        expect = Q[n].astype(np.float64) - LAM * c[n, 1:] - P[n]
+        got = df[:, k, j, i]
+        assert np.allclose(got, expect, atol=2e-5), (n, got, expect)
+
+
+def test_grid_of_affine_fit_is_affine(oracle, rng):
+    P, _ = make_pts(rng, 25, size=(40, 40, 40))
+    A = np.eye(3) + rng.uniform(-0.05, 0.05, (3, 3))
+    t = rng.uniform(-3, 3, 3)
+    Q = (P.astype(np.float64) @ A.T + t).astype(np.float32)
+    c, d = oracle.stps_fit(P, Q, LAM)
+    gsh = (13, 11, 9)
+    g = 4
+    df = oracle.grid_eval(P, c, d, gsh, g, origin=(8, 4, 0))
+    k, j, i = np.meshgrid(np.arange(gsh[2]), np.arange(gsh[1]), np.arange(gsh[0]), indexing="ij")
+    node = np.stack([(i - 1) * g + 8, (j - 1) * g + 4, (k - 1) * g + 0], 0).astype(np.float64)
+    expect = np.einsum("ab,bzyx->azyx", A - np.eye(3), node) + t[:, None, None, None]
+    assert np.abs(df - expect).max() < 2e-3
+
+
+def test_bspline_closed_form(oracle):
+    for n in (1, 2, 3, 4, 5, 8):
+        TB, basis = oracle.bspline_basis(n)
+        t = np.arange(n) / n
+        ref = np.stack([(1 - t) ** 3, 3 * t ** 3 - 6 * t ** 2 + 4, -3 * t ** 3 + 3 * t ** 2 + 3 * t + 1, t ** 3], 1) / 6
+        assert np.allclose(TB, ref, atol=1e-15)
+        assert np.allclose(TB.sum(1), 1.0, atol=1e-15)
+        assert np.allclose(basis.sum(1), 1.0, atol=1e-14)
+
+
+def _dense_delta_expect(shape_xyz, n, node):
+    """Dense DF of a grid holding 1 at `node` (i, j, k) in the sx plane."""
+    TB = np.stack([(1 - t) ** 3, 3 * t ** 3 - 6 * t ** 2 + 4, -3 * t ** 3 + 3 * t ** 2 + 3 * t + 1, t ** 3], 1) / 6 \
+        if (t := np.arange(n) / n) is not None else None
+    w = []
+    for ax in range(3):
+        v = np.arange(shape_xyz[ax])
+        b, ph = v // n, v % n
+        k = node[ax] - b
+        ok = (k >= 0) & (k <= 3)
+        w.append(np.where(ok, TB[ph, np.clip(k, 0, 3)], 0.0))
+    return w[2][:, None, None] * w[1][None, :, None] * w[0][None, None, :]
+
+
+def test_bspline_kronecker_ordering_with_a_delta_node(oracle):
+    n = 4
+    size = (17, 13, 11)
+    gsh = tuple(int(np.ceil(s / n)) + 3 for s in size)
+    node = (3, 5, 2)  # distinct per axis: an axis swap moves the lobe
+    df = np.zeros((3,) + gsh[::-1], np.float32)
+    df[0, node[2], node[1], node[0]] = 1.0
+    dense = oracle.dense_df(size, df, n)
+    expect = _dense_delta_expect(size, n, node)
+    assert np.abs(dense[0] - expect).max() < 1e-7
+    assert np.abs(dense[1:]).max() == 0.0
+
+
+def test_bspline_linear_precision(oracle, rng):
+    """An affine DF on the nodes is reproduced exactly at every voxel."""
+    n = 4
+    size = (23, 19, 14)
+    gsh = tuple(int(np.ceil(s / n)) + 3 for s in size)
+    a = rng.uniform(-3, 3, 3)
+    B = rng.uniform(-0.05, 0.05, (3, 3))
+    k, j, i = np.meshgrid(np.arange(gsh[2]), np.arange(gsh[1]), np.arange(gsh[0]), indexing="ij")
+    node = np.stack([(i - 1) * n, (j - 1) * n, (k - 1) * n], 0).astype(np.float64)
+    df = (np.einsum("ab,bzyx->azyx", B, node) + a[:, None, None, None]).astype(np.float32)
+    dense = oracle.dense_df(size, df, n)
+    z, y, x = np.meshgrid(np.arange(size[2]), np.arange(size[1]), np.arange(size[0]), indexing="ij")
+    vox = np.stack([x, y, z], 0).astype(np.float64)
+    expect = np.einsum("ab,bzyx->azyx", B, vox) + a[:, None, None, None]
+    assert np.abs(dense - expect).max() < 2e-6
+
+
+def _const_grid(size, n, vec):
+    gsh = tuple(int(np.ceil(s / n)) + 3 for s in size)
+    df = np.zeros((3,) + gsh[::-1], np.float32)
+    for a in range(3):
+        df[a] = vec[a]
+    return df
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_resample_zero_field_is_identity(oracle, rng, dtype):
+    size = (21, 17, 9)
+    hi = 255 if dtype == np.uint8 else 65535
+    v = rng.integers(0, hi + 1, size[::-1]).astype(dtype)
+    out = oracle.tps_resample(v, _const_grid(size, 4, (0, 0, 0)), 4)
+    assert np.array_equal(out, v)
+
+
+def test_resample_integer_shift_and_strict_bounds(oracle, rng):
+    size = (20, 12, 8)
+    v = rng.integers(1, 256, size[::-1]).astype(np.uint8)
+    out = oracle.tps_resample(v, _const_grid(size, 4, (1, 0, -2)), 4)
+    expect = np.zeros_like(v)
+    expect[2:, :, :-1] = v[:-2, :, 1:]
+    assert np.array_equal(out, expect)
+
+
+def test_resample_boundary_positions(oracle):
+    """pos = sz-1 exactly is kept; pos slightly below 0 is zeroed."""
+    size = (8, 6, 6)
+    v = np.full(size[::-1], 77, np.uint8)
+    out = oracle.tps_resample(v, _const_grid(size, 2, (0.0, 0.0, 0.0)), 2)
+    assert out[-1, -1, -1] == 77
+    out = oracle.tps_resample(v, _const_grid(size, 2, (-1e-3, 0, 0)), 2)
+    assert np.all(out[:, :, 0] == 0) and np.all(out[:, :, 1:] == 77)
+    out = oracle.tps_resample(v, _const_grid(size, 2, (1e-3, 0, 0)), 2)
+    assert np.all(out[:, :, -1] == 0) and np.all(out[:, :, :-1] == 77)
+
+
+@pytest.mark.parametrize("frac", [(0.25, 0.5, 0.75), (0.1, 0.9, 0.3)])
+def test_resample_matches_scipy_trilinear(oracle, rng, frac):
+    """Constant fractional shift vs scipy.ndimage.map_coordinates(order=1)."""
+    size = (16, 12, 10)
+    v = rng.integers(0, 65536, size[::-1]).astype(np.uint16)
+    out = oracle.tps_resample(v, _const_grid(size, 4, frac), 4)
+    z, y, x = np.meshgrid(*[np.arange(s) for s in size[::-1]], indexing="ij")
+    # The sample position is a float32 sum, as in the reference (pos_warp + sx).
+    f32 = np.float32
+    coords = np.stack([(z.astype(f32) + f32(frac[2])), (y.astype(f32) + f32(frac[1])),
+                       (x.astype(f32) + f32(frac[0]))]).astype(np.float64)
+    ref = ndimage.map_coordinates(v.astype(np.float64), coords, order=1, mode="constant", cval=0.0)
+    inside = (coords[0] <= size[2] - 1) & (coords[1] <= size[1] - 1) & (coords[2] <= size[0] - 1)
+    rounded = np.floor(ref + 0.5)
+    tie = np.abs(ref + 0.5 - np.round(ref + 0.5)) < 1e-6
+    assert np.all(out[~inside] == 0)
+    ok = inside & ~tie
+    assert np.array_equal(out[ok], rounded[ok].astype(np.uint16))
+
+
+def test_resample_nearest_neighbour(oracle, rng):
+    size = (12, 10, 8)
+    v = rng.integers(0, 256, size[::-1]).astype(np.uint8)
+    out = oracle.tps_resample(v, _const_grid(size, 4, (0.6, -0.4, 0.2)), 4, interp=1)
+    # x -> round(x + 0.6) = x + 1, y -> round(y - 0.4) = y, z -> round(z + 0.2) = z;
+    # out of bounds (zeroed): x = nx-1, y = 0, z = nz-1.
+    expect = np.zeros_like(v)
+    expect[:-1, 1:, :-1] = v[:-1, 1:, 1:]
+    assert np.array_equal(out, expect)
