@@ -118,51 +118,6 @@ __device__ __forceinline__ void resample_fp64(const TpsArgs& a, i64 X, i64 Y, i6
     }
 }
 
-// FP32 blend: float position (same rounding as the reference), exact
-// fractional part, seven lerps, +0.5, truncation.
-template <typename T, int INTERP>
-__device__ __forceinline__ void resample_fp32(const TpsArgs& a, i64 X, i64 Y, i64 Z, float d0, float d1, float d2,
-                                              i64 o, i64 dvol) {
-    if (INTERP == TW_INTERP_NN) {
-        resample_fp64<T, INTERP>(a, X, Y, Z, d0, d1, d2, o, dvol);
-        return;
-    }
-    const T* __restrict__ src = (const T*)a.src;
-    T* __restrict__ dst = (T*)a.dst;
-    const float p0 = (float)X + d0, p1 = (float)Y + d1, p2 = (float)Z + d2;
-    if (p0 < 0.f || p0 > (float)(a.fnx - 1) || p1 < 0.f || p1 > (float)(a.fny - 1) || p2 < 0.f ||
-        p2 > (float)(a.fnz - 1)) {
-        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;
-        return;
-    }
-    const float fx = floorf(p0), fy = floorf(p1), fz = floorf(p2);
-    const float wx = p0 - fx, wy = p1 - fy, wz = p2 - fz;
-    const i64 lxs = (i64)fx - a.sx0, lys = (i64)fy - a.sy0, lzs = (i64)fz - a.sz0;
-    const i64 lxb = lxs + (wx > 0.f), lyb = lys + (wy > 0.f), lzb = lzs + (wz > 0.f);
-    if (lxs < 0 || lxb >= a.snx || lys < 0 || lyb >= a.sny || lzs < 0 || lzb >= a.snz) {
-        for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;
-        return;
-    }
-    const i64 sxy = a.snx * a.sny, svol = sxy * a.snz;
-    const i64 r00 = lzs * sxy + lys * a.snx, r01 = lzs * sxy + lyb * a.snx;
-    const i64 r10 = lzb * sxy + lys * a.snx, r11 = lzb * sxy + lyb * a.snx;
-    for (i64 c = 0; c < a.nc; c++) {
-        const T* s = src + c * svol;
-        const float v000 = (float)s[r00 + lxs], v100 = (float)s[r00 + lxb];
-        const float v010 = (float)s[r01 + lxs], v110 = (float)s[r01 + lxb];
-        const float v001 = (float)s[r10 + lxs], v101 = (float)s[r10 + lxb];
-        const float v011 = (float)s[r11 + lxs], v111 = (float)s[r11 + lxb];
-        const float a00 = __builtin_fmaf(wx, v100 - v000, v000);
-        const float a10 = __builtin_fmaf(wx, v110 - v010, v010);
-        const float a01 = __builtin_fmaf(wx, v101 - v001, v001);
-        const float a11 = __builtin_fmaf(wx, v111 - v011, v011);
-        const float b0 = __builtin_fmaf(wy, a10 - a00, a00);
-        const float b1 = __builtin_fmaf(wy, a11 - a01, a01);
-        const float v = __builtin_fmaf(wz, b1 - b0, b0) + 0.5f;
-        dst[c * dvol + o] = (T)v;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // TW_EXACT: one voxel per thread, reference-order FP64 upsample.
 
@@ -203,16 +158,118 @@ __global__ __launch_bounds__(256) void k_tps_exact(const TpsArgs a) {
 
 // ---------------------------------------------------------------------------
 // Fused separable kernel (TW_PLAIN / TW_MIXED).
+//
+// Addressing: 32-bit byte offsets through buffer resources (one per
+// channel, num_records = the channel's bytes), so one VGPR offset serves
+// the eight taps: the x+1 taps use the instruction's immediate offset and
+// the y+1 / z+1 taps a row / plane step that is zero on the last row /
+// plane (where the tap's weight is zero anyway).  An x+1 tap past the end
+// of the channel is caught by the buffer range check and reads 0, again
+// under a zero weight.  The host only takes this path when every window
+// fits in 32-bit byte offsets.
 
-struct LdsPlan {
-    int nxn, nyn, nzn;  // maxima of the node tile extents
-    size_t bytes;
-};
+constexpr int BUF_FLAGS = 0x00020000;  // raw buffer, gfx9-family data format
 
+// Upper bound of the node count one tile spans along an axis.
 __host__ __device__ inline int node_span(int tile, int f) { return (tile - 1) / f + 5; }
 
+template <typename T>
+__device__ __forceinline__ uint32_t bld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if (sizeof(T) == 1) return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+    return (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
+    if (sizeof(T) == 1) __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, r, (int)off, 0, 0);
+    else __builtin_amdgcn_raw_buffer_store_b16((uint16_t)v, r, (int)off, 0, 0);
+}
+
+struct FastGeom {
+    int snx, sny, snz, sx0, sy0, sz0;   // source window
+    uint32_t row_b, plane_b, chan_b;    // source strides in bytes
+    float fmx, fmy, fmz;                // full extent - 1 (bounds test)
+    uint32_t dplane_b, dchan_b;         // destination strides in bytes
+    int nc;
+};
+
+// One output voxel: bounds test, taps, blend, store.  MODE selects the
+// FP64 reference blend (TW_PLAIN) or the FP32 lerp blend (TW_MIXED).
+template <typename T, int MODE, int INTERP>
+__device__ __forceinline__ void resample_fast(const TpsArgs& a, const FastGeom& g, int X, int Y, int Z, float d0,
+                                              float d1, float d2, uint32_t obyte) {
+    const float p0 = (float)X + d0, p1 = (float)Y + d1, p2 = (float)Z + d2;
+    const bool inb = p0 >= 0.f && p0 <= g.fmx && p1 >= 0.f && p1 <= g.fmy && p2 >= 0.f && p2 <= g.fmz;
+    const float fx = floorf(p0), fy = floorf(p1), fz = floorf(p2);
+    const float wx = p0 - fx, wy = p1 - fy, wz = p2 - fz;
+    int ix, iy, iz;
+    bool win;
+    if (INTERP == TW_INTERP_NN) {
+        // (long)(pos + 0.5) clamped to the extent, in FP64 like the reference.
+        ix = (int)fmin((double)p0 + 0.5, (double)g.fmx) - g.sx0;
+        iy = (int)fmin((double)p1 + 0.5, (double)g.fmy) - g.sy0;
+        iz = (int)fmin((double)p2 + 0.5, (double)g.fmz) - g.sz0;
+        win = ix >= 0 && ix < g.snx && iy >= 0 && iy < g.sny && iz >= 0 && iz < g.snz;
+    } else {
+        ix = (int)fx - g.sx0;
+        iy = (int)fy - g.sy0;
+        iz = (int)fz - g.sz0;
+        win = ix >= 0 && iy >= 0 && iz >= 0 && ix + (wx > 0.f) < g.snx && iy + (wy > 0.f) < g.sny &&
+              iz + (wz > 0.f) < g.snz;
+    }
+    const bool ok = inb && win;
+    const uint32_t off = ok ? ((uint32_t)iz * (uint32_t)g.sny + (uint32_t)iy) * (uint32_t)g.snx * (uint32_t)sizeof(T) +
+                                  (uint32_t)ix * (uint32_t)sizeof(T)
+                            : 0u;
+    const uint32_t dy = iy + 1 < g.sny ? g.row_b : 0u;
+    const uint32_t dz = iz + 1 < g.snz ? g.plane_b : 0u;
+    for (int c = 0; c < g.nc; c++) {
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)((const uint8_t*)a.src + (size_t)c * g.chan_b), (short)0,
+                                              (int)g.chan_b, BUF_FLAGS);
+        const __amdgpu_buffer_rsrc_t rd =
+            __builtin_amdgcn_make_buffer_rsrc((void*)((uint8_t*)a.dst + (size_t)c * g.dchan_b), (short)0,
+                                              (int)g.dchan_b, BUF_FLAGS);
+        uint32_t v = 0;
+        if (ok) {
+            if (INTERP == TW_INTERP_NN) {
+                v = bld<T>(rs, off);
+            } else {
+                const uint32_t u000 = bld<T>(rs, off), u100 = bld<T>(rs, off + sizeof(T));
+                const uint32_t u010 = bld<T>(rs, off + dy), u110 = bld<T>(rs, off + dy + sizeof(T));
+                const uint32_t u001 = bld<T>(rs, off + dz), u101 = bld<T>(rs, off + dz + sizeof(T));
+                const uint32_t u011 = bld<T>(rs, off + dy + dz), u111 = bld<T>(rs, off + dy + dz + sizeof(T));
+                if (MODE == TW_PLAIN) {
+                    // pos, floor and 1 - frac are exact in FP64; blend in the
+                    // reference's order (q_imgwarp_tps_quicksmallmemory.cpp:141-160).
+                    const double P0 = p0, P1 = p1, P2 = p2;
+                    const double lw = 1.0 - (P0 - (double)fx), rw = 1.0 - lw;
+                    const double tw = 1.0 - (P1 - (double)fy), bw = 1.0 - tw;
+                    const double uw = 1.0 - (P2 - (double)fz), dw = 1.0 - uw;
+                    const double hi = tw * (lw * (double)u000 + rw * (double)u100) +
+                                      bw * (lw * (double)u010 + rw * (double)u110);
+                    const double lo = tw * (lw * (double)u001 + rw * (double)u101) +
+                                      bw * (lw * (double)u011 + rw * (double)u111);
+                    v = (uint32_t)(uw * hi + dw * lo + 0.5);
+                } else {
+                    const float v000 = (float)u000, v100 = (float)u100, v010 = (float)u010, v110 = (float)u110;
+                    const float v001 = (float)u001, v101 = (float)u101, v011 = (float)u011, v111 = (float)u111;
+                    const float a00 = __builtin_fmaf(wx, v100 - v000, v000);
+                    const float a10 = __builtin_fmaf(wx, v110 - v010, v010);
+                    const float a01 = __builtin_fmaf(wx, v101 - v001, v001);
+                    const float a11 = __builtin_fmaf(wx, v111 - v011, v011);
+                    const float b0 = __builtin_fmaf(wy, a10 - a00, a00);
+                    const float b1 = __builtin_fmaf(wy, a11 - a01, a01);
+                    v = (uint32_t)(__builtin_fmaf(wz, b1 - b0, b0) + 0.5f);
+                }
+            }
+        }
+        bst<T>(rd, obyte, v);
+    }
+}
+
 template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF>
-__global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a) {
+__global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGeom g) {
     extern __shared__ float lds[];
     const int f = a.f;
     const int NXM = node_span(TX, f), NYM = node_span(TY, f), NZM = node_span(a.tz, f);
@@ -221,51 +278,53 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a) {
     float* xc = nodes + 3 * NZM * NYM * NXM; // [3][NZM][NYM][TX]
 
     const int lxi = threadIdx.x, lyi = threadIdx.y, tid = lyi * TX + lxi;
-    const i64 tx0 = (i64)blockIdx.x * TX, ty0 = (i64)blockIdx.y * TY, tz0 = (i64)blockIdx.z * a.tz;
-    const i64 tx1 = min(tx0 + TX, a.dnx) - 1, ty1 = min(ty0 + TY, a.dny) - 1, tz1 = min(tz0 + a.tz, a.dnz) - 1;
-    // Node tile: global node indices [n?0, n?0 + extent).
-    const i64 nx0 = (a.dx0 + tx0 - a.ox) / f, ny0 = (a.dy0 + ty0 - a.oy) / f, nz0 = (a.dz0 + tz0 - a.oz) / f;
-    const int nxn = (int)((a.dx0 + tx1 - a.ox) / f + 4 - nx0);
-    const int nyn = (int)((a.dy0 + ty1 - a.oy) / f + 4 - ny0);
-    const int nzn = (int)((a.dz0 + tz1 - a.oz) / f + 4 - nz0);
+    const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
+    const int tx0 = blockIdx.x * TX, ty0 = blockIdx.y * TY, tz0 = blockIdx.z * a.tz;
+    const int tx1 = min(tx0 + TX, dnx) - 1, ty1 = min(ty0 + TY, dny) - 1, tz1 = min(tz0 + a.tz, dnz) - 1;
+    // Window-relative coordinates of the grid (r = global - origin >= 0).
+    const int rx0 = (int)(a.dx0 - a.ox), ry0 = (int)(a.dy0 - a.oy), rz0 = (int)(a.dz0 - a.oz);
+    const int nx0 = (rx0 + tx0) / f, ny0 = (ry0 + ty0) / f, nz0 = (rz0 + tz0) / f;
+    const int nxn = (rx0 + tx1) / f + 4 - nx0;
+    const int nyn = (ry0 + ty1) / f + 4 - ny0;
+    const int nzn = (rz0 + tz1) / f + 4 - nz0;
 
     for (int i = tid; i < 4 * f; i += 256) wsh[i] = a.w32[i];
-    // Stage the node tile (3 planes) in LDS.
     const i64 plane = a.gx * a.gy * a.gz;
+    const int NP = NZM * NYM * NXM;
     const int ntile = nxn * nyn * nzn;
     for (int i = tid; i < ntile; i += 256) {
         const int ix = i % nxn, iy = (i / nxn) % nyn, iz = i / (nxn * nyn);
-        const i64 gi = ((nz0 + iz) * a.gy + (ny0 + iy)) * a.gx + (nx0 + ix);
+        const i64 gi = ((i64)(nz0 + iz) * a.gy + (ny0 + iy)) * a.gx + (nx0 + ix);
         const int li = (iz * NYM + iy) * NXM + ix;
         if (F16) {
-            const __half* g = (const __half*)a.grid;
-            const i64 qb = (((nz0 + iz) >> 3) * a.qby + ((ny0 + iy) >> 3)) * a.qbx + ((nx0 + ix) >> 3);
+            const __half* gh = (const __half*)a.grid;
+            const i64 qb = ((i64)((nz0 + iz) >> 3) * a.qby + ((ny0 + iy) >> 3)) * a.qbx + ((nx0 + ix) >> 3);
             const float* q = a.qtab + qb * 6;
-            nodes[li] = __builtin_fmaf(q[3], __half2float(g[gi]), q[0]);
-            nodes[NZM * NYM * NXM + li] = __builtin_fmaf(q[4], __half2float(g[plane + gi]), q[1]);
-            nodes[2 * NZM * NYM * NXM + li] = __builtin_fmaf(q[5], __half2float(g[2 * plane + gi]), q[2]);
+            nodes[li] = __builtin_fmaf(q[3], __half2float(gh[gi]), q[0]);
+            nodes[NP + li] = __builtin_fmaf(q[4], __half2float(gh[plane + gi]), q[1]);
+            nodes[2 * NP + li] = __builtin_fmaf(q[5], __half2float(gh[2 * plane + gi]), q[2]);
         } else {
-            const float* g = (const float*)a.grid;
-            nodes[li] = g[gi];
-            nodes[NZM * NYM * NXM + li] = g[plane + gi];
-            nodes[2 * NZM * NYM * NXM + li] = g[2 * plane + gi];
+            const float* gf = (const float*)a.grid;
+            nodes[li] = gf[gi];
+            nodes[NP + li] = gf[plane + gi];
+            nodes[2 * NP + li] = gf[2 * plane + gi];
         }
     }
     __syncthreads();
     // Collapse along x for each of the tile's 64 columns.
     {
-        const i64 X = a.dx0 + tx0 + lxi;
-        const i64 rx = X - a.ox;
-        const int bxl = (int)(rx / f - nx0);
-        const int txp = (int)(rx % f);
+        const int rx = rx0 + tx0 + lxi;
+        const int bxl = rx / f - nx0;
+        const int txp = rx - (rx / f) * f;
         const bool live = tx0 + lxi <= tx1;
         const float w0 = wsh[txp * 4 + 0], w1 = wsh[txp * 4 + 1], w2 = wsh[txp * 4 + 2], w3 = wsh[txp * 4 + 3];
         for (int r = lyi; r < nzn * nyn; r += TY) {
             const int iy = r % nyn, iz = r / nyn;
+            #pragma unroll
             for (int c = 0; c < 3; c++) {
                 float v = 0.f;
                 if (live) {
-                    const float* nrow = nodes + c * NZM * NYM * NXM + (iz * NYM + iy) * NXM + bxl;
+                    const float* nrow = nodes + c * NP + (iz * NYM + iy) * NXM + bxl;
                     v = __builtin_fmaf(w3, nrow[3], __builtin_fmaf(w2, nrow[2], __builtin_fmaf(w1, nrow[1], w0 * nrow[0])));
                 }
                 xc[((c * NZM + iz) * NYM + iy) * TX + lxi] = v;
@@ -275,12 +334,13 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a) {
     __syncthreads();
     if (tx0 + lxi > tx1 || ty0 + lyi > ty1) return;
 
-    const i64 X = a.dx0 + tx0 + lxi, Y = a.dy0 + ty0 + lyi;
-    const i64 ry = Y - a.oy;
-    const int byl = (int)(ry / f - ny0);
-    const int typ = (int)(ry % f);
+    const int lx = tx0 + lxi, ly = ty0 + lyi;
+    const int X = (int)a.dx0 + lx, Y = (int)a.dy0 + ly;
+    const int ry = ry0 + ly;
+    const int byl = ry / f - ny0;
+    const int typ = ry - (ry / f) * f;
     const float wy0 = wsh[typ * 4 + 0], wy1 = wsh[typ * 4 + 1], wy2 = wsh[typ * 4 + 2], wy3 = wsh[typ * 4 + 3];
-    const i64 dvol = a.dnx * a.dny * a.dnz;
+    const uint32_t dvol = (uint32_t)dnx * (uint32_t)dny * (uint32_t)dnz;
 
     auto layer = [&](int iz, float* L) {
         #pragma unroll
@@ -290,51 +350,56 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a) {
                                   __builtin_fmaf(wy1, col[TX], wy0 * col[0])));
         }
     };
+    // z walk: the phase advances by one per slice, the node window by one
+    // per f slices (wave-uniform).
+    int rz = rz0 + tz0;
+    int bzl = rz / f - nz0;
+    int tzp = rz - (rz / f) * f;
     float L0[3], L1[3], L2[3], L3[3];
-    int cur = -1;
-    for (i64 lz = tz0; lz <= tz1; lz++) {
-        const i64 Z = a.dz0 + lz;
-        const i64 rz = Z - a.oz;
-        const int bzl = (int)(rz / f - nz0);
-        const int tzp = (int)(rz % f);
-        if (bzl != cur) {
-            if (cur >= 0 && bzl == cur + 1) {
-                #pragma unroll
-                for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
-                layer(bzl + 3, L3);
-            } else {
-                layer(bzl, L0); layer(bzl + 1, L1); layer(bzl + 2, L2); layer(bzl + 3, L3);
-            }
-            cur = bzl;
-        }
+    layer(bzl, L0); layer(bzl + 1, L1); layer(bzl + 2, L2); layer(bzl + 3, L3);
+    const int nzv = tz1 - tz0 + 1;
+    uint32_t o = ((uint32_t)tz0 * (uint32_t)dny + (uint32_t)ly) * (uint32_t)dnx + (uint32_t)lx;
+    const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
+    for (int lz = 0; lz < nzv; lz++) {
+        const int Z = (int)a.dz0 + tz0 + lz;
         const float wz0 = wsh[tzp * 4 + 0], wz1 = wsh[tzp * 4 + 1], wz2 = wsh[tzp * 4 + 2], wz3 = wsh[tzp * 4 + 3];
         float d[3];
         #pragma unroll
         for (int c = 0; c < 3; c++)
             d[c] = __builtin_fmaf(wz3, L3[c], __builtin_fmaf(wz2, L2[c], __builtin_fmaf(wz1, L1[c], wz0 * L0[c])));
-        const i64 o = (lz * a.dny + (ty0 + lyi)) * a.dnx + (tx0 + lxi);
         if (WRITE_DF) {
             a.dfout[o] = d[0];
             a.dfout[dvol + o] = d[1];
             a.dfout[2 * dvol + o] = d[2];
-        } else if (MODE == TW_PLAIN) {
-            resample_fp64<T, INTERP>(a, X, Y, Z, d[0], d[1], d[2], o, dvol);
         } else {
-            resample_fp32<T, INTERP>(a, X, Y, Z, d[0], d[1], d[2], o, dvol);
+            resample_fast<T, MODE, INTERP>(a, g, X, Y, Z, d[0], d[1], d[2], o * (uint32_t)sizeof(T));
+        }
+        o += ostep;
+        if (++tzp == f) {
+            tzp = 0;
+            bzl++;
+            if (lz + 1 < nzv) {
+                #pragma unroll
+                for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
+                layer(bzl + 3, L3);
+            }
         }
     }
-}
-
-int pick_tz(int f) {
-    int tz = 4 * f;
-    if (tz < 16) tz = 16;
-    if (tz > 64) tz = 64;
-    return tz;
 }
 
 size_t fused_lds_bytes(int f, int tz) {
     const int nxm = node_span(TX, f), nym = node_span(TY, f), nzm = node_span(tz, f);
     return sizeof(float) * (size_t)(4 * f + 3 * nzm * nym * nxm + 3 * nzm * nym * TX);
+}
+
+// Voxels per work-group along z: 4 cells, clamped to [16, 64], halved
+// until the LDS tile fits in 64 KiB.
+int pick_tz(int f) {
+    int tz = 4 * f;
+    if (tz < 16) tz = 16;
+    if (tz > 64) tz = 64;
+    while (tz > 1 && fused_lds_bytes(f, tz) > 64 * 1024) tz /= 2;
+    return tz;
 }
 
 template <typename T, bool WRITE_DF>
@@ -359,9 +424,27 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     if (lds > 160 * 1024) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d needs %zu B of LDS", a.f, lds);
     const i64 nbz = (a.dnz + a.tz - 1) / a.tz;
     if (nbz > 65535) return tw_fail(ctx, TW_E_UNSUPPORTED, "volume too deep for one launch");
+    // 32-bit addressing limits of the fused kernel.
+    const uint64_t lim = 0xFFFFFF00ull;
+    const uint64_t schan = (uint64_t)a.snx * a.sny * a.snz * sizeof(T);
+    const uint64_t dchan = (uint64_t)a.dnx * a.dny * a.dnz * (WRITE_DF ? 3 * sizeof(float) : sizeof(T));
+    const i64 cmax = std::max({a.dx0 + a.dnx, a.dy0 + a.dny, a.dz0 + a.dnz, a.fnx, a.fny, a.fnz});
+    if ((!WRITE_DF && schan > lim) || dchan > lim || cmax > (1ll << 24))
+        return tw_fail(ctx, TW_E_UNSUPPORTED,
+                       "window exceeds the fused kernel's 32-bit addressing (split the volume into bricks)");
+    FastGeom g;
+    g.snx = (int)a.snx; g.sny = (int)a.sny; g.snz = (int)a.snz;
+    g.sx0 = (int)a.sx0; g.sy0 = (int)a.sy0; g.sz0 = (int)a.sz0;
+    g.row_b = (uint32_t)(a.snx * sizeof(T));
+    g.plane_b = (uint32_t)(a.snx * a.sny * sizeof(T));
+    g.chan_b = (uint32_t)schan;
+    g.fmx = (float)(a.fnx - 1); g.fmy = (float)(a.fny - 1); g.fmz = (float)(a.fnz - 1);
+    g.dplane_b = (uint32_t)(a.dnx * a.dny * sizeof(T));
+    g.dchan_b = (uint32_t)(a.dnx * a.dny * a.dnz * sizeof(T));
+    g.nc = (int)a.nc;
     const dim3 grid((unsigned)((a.dnx + TX - 1) / TX), (unsigned)((a.dny + TY - 1) / TY), (unsigned)nbz);
 #define TW_FUSED(M, H, I) \
-    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF>), grid, block, lds, ctx->stream, a)
+    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF>), grid, block, lds, ctx->stream, a, g)
     if (precision == TW_PLAIN) {
         if (f16) {
             if (interp == TW_INTERP_NN) TW_FUSED(TW_PLAIN, true, TW_INTERP_NN); else TW_FUSED(TW_PLAIN, true, TW_INTERP_LINEAR);
