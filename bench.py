@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: the two forms' steps and a 1 GiB device copy (counter calibration) only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -157,6 +159,17 @@ def main():
     torch.cuda.synchronize()
 
     w = Warper(0, timing=True)
+    if args.profile:
+        run_form(w, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
+        run_form(w, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
+        # Calibration copy of a known size: 1 GiB read + 1 GiB written.
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "done", "copy_bytes": 1 << 30}))
+        w.close()
+        return
     log("plain form")
     plain, gp, (cp, dp) = run_form(w, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
     out_plain = dst.cpu().numpy().view(np.uint16)

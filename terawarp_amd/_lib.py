@@ -36,6 +36,12 @@ EXPORTS = (
     "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
     "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_grid_dim",
 )
+# Every entry point include/terawarp_host.h declares.
+HOST_EXPORTS = (
+    "tw_v3draw_header", "tw_v3draw_read", "tw_v3draw_write", "tw_markers_read", "tw_markers_write",
+    "tw_io_last_error", "tw_plugin_menu_count", "tw_plugin_menu_item", "tw_plugin_func_count",
+    "tw_plugin_func_item", "tw_plugin_version", "tw_plugin_parse_params", "tw_plugin_dofunc", "tw_bigimagewarp",
+)
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int32
@@ -60,6 +66,12 @@ class TwTimings(ctypes.Structure):
                 ("resample_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double), ("d2h_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("refine_iters", _i32), ("reserved", _i32),
                 ("fit_residual", ctypes.c_double)]
+
+
+class TwWarpParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("device_mode", ctypes.c_int), ("resize", ctypes.c_int),
+                ("downsample", ctypes.c_int), ("para", ctypes.c_double * 3), ("newsize", _i64 * 3),
+                ("legacy", ctypes.c_int)]
 
 
 _lib = None
@@ -97,6 +109,25 @@ def load():
         "tw_warp_tps": (ctypes.c_int, [_P, V, V, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_int]),
         "tw_grid_dim": (_i64, [_i64, ctypes.c_int]),
+        "tw_v3draw_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "tw_v3draw_read": (ctypes.c_int, [ctypes.c_char_p, _P, _i64]),
+        "tw_v3draw_write": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.POINTER(_i64), ctypes.c_int]),
+        "tw_markers_read": (ctypes.c_int, [ctypes.c_char_p, fp, ctypes.c_int]),
+        "tw_markers_write": (ctypes.c_int, [ctypes.c_char_p, fp, ctypes.c_int]),
+        "tw_io_last_error": (ctypes.c_char_p, []),
+        "tw_plugin_menu_count": (ctypes.c_int, []),
+        "tw_plugin_menu_item": (ctypes.c_char_p, [ctypes.c_int]),
+        "tw_plugin_func_count": (ctypes.c_int, []),
+        "tw_plugin_func_item": (ctypes.c_char_p, [ctypes.c_int]),
+        "tw_plugin_version": (ctypes.c_float, []),
+        "tw_plugin_parse_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                                  ctypes.POINTER(TwWarpParams)]),
+        "tw_plugin_dofunc": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_char_p), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]),
+        "tw_bigimagewarp": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.POINTER(TwWarpParams), ctypes.c_char_p, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

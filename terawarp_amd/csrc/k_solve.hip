@@ -1,28 +1,38 @@
-// k_solve.hip -- STPS coefficient solve on the GPU.
+// k_solve.hip -- STPS coefficient solve on the GPU (null-space method).
 //
-// Reference: the GPU branch of compute_df_stps_subsampled_volume_4bspline
-// (q_imgwarp_tps_quicksmallmemory.cpp:449-553: gpu_QR qr.cu:8-131,
+// Reference: compute_df_stps_subsampled_volume_4bspline
+// (q_imgwarp_tps_quicksmallmemory.cpp:449-586).  Its CPU branch computes
+//   QRZ(X) -> q1, r;  extend_orthonormal -> q2;
+//   A = q2' K q2 + 0.2 I;  c = q2 (A^-1 q2' Y);  d = r^-1 q1' (Y - K c),
+// and its GPU branch ports those steps piecewise (gpu_QR qr.cu:8-131,
 // gpu_extendornormal extendornormal.cu:55-150, matrixMultiply aff.cu:37-133,
-// gpu_A_i_new GET_A_i.cu:80-161) and the CPU branch (554-586) it mirrors:
-//   A = q2' K q2 + 0.2 I,  c = q2 A^-1 q2' Y,  d = r^-1 q1' (Y - K c).
-// Because q1' q2 = 0, that (c, d) is the unique solution of the bordered
-// system  [[K + lambda I, X], [X', 0]] [c; d] = [Y; 0]  (X = [1 p],
-// Y = [1 q]).  This file solves the bordered system directly, which needs
-// no orthonormal basis at all:
+// gpu_A_i_new GET_A_i.cu:80-161).  Same algorithm here, laid out for the GPU:
 //
-//  1. assemble M' in scaled coordinates p' = (p - mu)/s (K' = K/s,
-//     lambda' = lambda/s, X' = [1 p']); the solution maps back through
-//     c = c'/s, d = T^-1 d' with X = X' T;
-//  2. blocked right-looking LU with partial pivoting (panel of 64 columns in
-//     one work-group, row swaps, TRSM of U12, GEMM update of A22);
-//  3. blocked forward / backward substitution for the 4 right-hand sides;
-//  4. TW_MIXED: the factorisation runs in FP32 and the FP64 residual
-//     b - M' x (M' regenerated in FP64 from the landmarks) drives iterative
-//     refinement until the correction stalls; if it does not converge the
-//     system is refactored in FP64.
+//  * X' = [1 p'] in scaled coordinates p' = (p - mu)/s (K' = K/s,
+//    lambda' = lambda/s; the solution maps back through c = c'/s and
+//    d = T^-1 d' with X = X' T);
+//  * Householder QR of X' (one work-group): Q = H1..H4 = I - V T V' in
+//    compact WY form, so q2 is never formed: the projected matrix is
+//      B = [Q' (K' + lambda' I) Q]_(4:,4:) = [A' - W V' - V W']_(4:,4:),
+//      W = A' V T - 1/2 V (T' V' A' V T),
+//    with A' = K' + lambda' I generated on the fly from the landmarks (one
+//    matrix-free pass for A' V, one for B);
+//  * B is symmetric positive definite (-|r| is conditionally positive
+//    definite of order 1 and q2 is orthogonal to the constants), so it is
+//    factored by blocked Cholesky with no pivoting: a 64x64 diagonal block
+//    in LDS, a TRSM of the panel below it, a SYRK-style update of the
+//    lower trailing tiles;
+//  * z = B^-1 [Q' Y]_(4:) by forward / backward substitution (one
+//    work-group), c' = Q [0; z], d' = R^-1 [Q' (Y - A' c')]_(0:4);
+//  * TW_MIXED factors B in FP32 and refines z with FP64 residuals
+//    r = [Q'Y]_(4:) - B z, B applied matrix-free in FP64; a stall of the
+//    refinement refactors in FP64.
 //
 // K_ij = -|p_i - p_j| is formed from float coordinate differences widened to
-// FP64, as in the reference (434-444).
+// FP64, as in the reference (434-444).  The (c, d) so computed is the unique
+// solution of the bordered system [[K + lambda I, X], [X', 0]] [c; d] = [Y; 0]
+// (independent of the basis chosen for the null space); its FP64 residual
+// is reported.
 #include "tw_internal.h"
 
 #include <algorithm>
@@ -31,17 +41,18 @@
 
 namespace {
 
-constexpr int NB = 64;  // panel width / block size
+constexpr int NB = 64;  // Cholesky block size
 
 struct SysDesc {
     const float4* pts;   // N centres (x, y, z, 0), original coordinates
-    const double* y;     // N x 4 right-hand side rows [1 qx qy qz]
+    const double* y;     // N x 4 right-hand side rows [1 qx qy qz] (row-major)
     int N, m;            // m = N + 4
     double inv_s;        // 1/s
     double mu[3];        // centroid
     double lam;          // lambda / s
 };
 
+// Entry (r, c) of the scaled bordered matrix [[K' + lambda' I, X'], [X'^T, 0]].
 __device__ __forceinline__ double sys_entry(const SysDesc& s, int r, int c) {
     const int N = s.N;
     if (r < N && c < N) {
@@ -52,155 +63,277 @@ __device__ __forceinline__ double sys_entry(const SysDesc& s, int r, int c) {
         return v;
     }
     if (r >= N && c >= N) return 0.0;
-    const int i = r < N ? r : c;      // landmark index
-    const int a = r < N ? c - N : r - N;  // column of X
+    const int i = r < N ? r : c;
+    const int a = r < N ? c - N : r - N;
     if (a == 0) return 1.0;
     const float4 p = s.pts[i];
     const double v = a == 1 ? (double)p.x : (a == 2 ? (double)p.y : (double)p.z);
     return (v - s.mu[a - 1]) * s.inv_s;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_assemble(T* __restrict__ A, i64 lda, const SysDesc s) {
-    const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (i64)s.m * s.m) return;
-    const int r = (int)(e % s.m), c = (int)(e / s.m);
-    A[(i64)c * lda + r] = (T)sys_entry(s, r, c);
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// b = [Y; 0] in precision T (m x 4, column-major, ld m).
-template <typename T>
-__global__ __launch_bounds__(256) void k_rhs(T* __restrict__ B, const SysDesc s) {
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= s.m) return;
-    for (int j = 0; j < 4; j++) B[(i64)j * s.m + r] = r < s.N ? (T)s.y[r * 4 + j] : (T)0;
+// Sum over a 1024-thread work-group; every thread gets the total.
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double t = 0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++) t += red[i];
+    return t;
 }
 
-// Unblocked GEPP of the panel A[j0:m, j0:j0+jb] in one work-group.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_panel(T* __restrict__ A, i64 lda, int m, int j0, int jb,
-                                                int* __restrict__ ipiv, int* __restrict__ info) {
-    __shared__ double rv[1024];
-    __shared__ int ri[1024];
-    __shared__ int s_prow;
-    const int tid = threadIdx.x;
-    for (int j = 0; j < jb; j++) {
-        const int col = j0 + j;
-        T* Acol = A + (i64)col * lda;
-        double best = -1.0;
-        int bi = m;
-        for (int r = col + tid; r < m; r += 1024) {
-            const double v = fabs((double)Acol[r]);
-            if (v > best) { best = v; bi = r; }
-        }
-        rv[tid] = best;
-        ri[tid] = bi;
+// Small per-fit state shared between kernels (device memory).
+struct FitState {
+    double tau[4];
+    double R[16];   // row-major upper triangle of X' = Q [R; 0]
+    double T[16];   // row-major compact-WY factor, Q = I - V T V'
+    int info;       // 1: rank-deficient X; 2 + k: non-positive pivot at column k
+    int pad;
+    double rnorm, bnorm;
+};
+
+// Householder QR of X' (N x 4) and the compact-WY factor T.  One work-group.
+__global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restrict__ V, FitState* st) {
+    __shared__ double red[16];
+    __shared__ double sh[8];
+    const int N = s.N, tid = threadIdx.x;
+    // X' into V's storage (column-major N x 4).
+    for (int i = tid; i < N; i += 1024) {
+        const float4 p = s.pts[i];
+        V[i] = 1.0;
+        V[N + i] = ((double)p.x - s.mu[0]) * s.inv_s;
+        V[2 * N + i] = ((double)p.y - s.mu[1]) * s.inv_s;
+        V[3 * N + i] = ((double)p.z - s.mu[2]) * s.inv_s;
+    }
+    double R[16] = {0};
+    double tau[4];
+    double scale0 = 0.0;
+    for (int k = 0; k < 4; k++) {
+        double* xk = V + (size_t)k * N;
         __syncthreads();
-        for (int s = 512; s > 0; s >>= 1) {
-            if (tid < s) {
-                const double v2 = rv[tid + s];
-                const int i2 = ri[tid + s];
-                if (v2 > rv[tid] || (v2 == rv[tid] && i2 < ri[tid])) { rv[tid] = v2; ri[tid] = i2; }
-            }
+        double part = 0;
+        for (int i = k + 1 + tid; i < N; i += 1024) part += xk[i] * xk[i];
+        const double xnorm2 = block_sum(part, red);
+        const double alpha = xk[k];
+        double beta, t, scale;
+        if (xnorm2 == 0.0) {
+            beta = alpha; t = 0.0; scale = 0.0;
+        } else {
+            beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        tau[k] = t;
+        R[k * 4 + k] = beta;
+        if (k == 0) scale0 = fabs(beta);
+        __syncthreads();
+        for (int i = tid; i < N; i += 1024) {
+            if (i < k) xk[i] = 0.0;
+            else if (i == k) xk[i] = 1.0;
+            else xk[i] *= scale;
+        }
+        __syncthreads();
+        // Apply H_k to the remaining columns j > k.
+        for (int j = k + 1; j < 4; j++) {
+            double* xj = V + (size_t)j * N;
+            double pw = 0;
+            for (int i = k + tid; i < N; i += 1024) pw += xk[i] * xj[i];
+            const double w = block_sum(pw, red);
             __syncthreads();
+            for (int i = k + tid; i < N; i += 1024) xj[i] -= t * xk[i] * w;
+            __syncthreads();
+            if (tid == 0) sh[j] = xj[k];
+            __syncthreads();
+            R[k * 4 + j] = sh[j];
         }
-        if (tid == 0) {
-            s_prow = ri[0];
-            ipiv[col] = ri[0];
-            if (!(rv[0] > 0.0) && *info == 0) *info = col + 1;
+    }
+    // V'V for T.
+    double G[16];
+    for (int a = 0; a < 4; a++)
+        for (int b = a; b < 4; b++) {
+            double p = 0;
+            for (int i = tid; i < N; i += 1024) p += V[(size_t)a * N + i] * V[(size_t)b * N + i];
+            G[a * 4 + b] = block_sum(p, red);
+            G[b * 4 + a] = G[a * 4 + b];
         }
-        __syncthreads();
-        const int prow = s_prow;
-        if (prow != col && tid < jb) {
-            T* c = A + (i64)(j0 + tid) * lda;
-            const T t = c[col];
-            c[col] = c[prow];
-            c[prow] = t;
-        }
-        __syncthreads();
-        const T piv = Acol[col];
-        if (piv != (T)0)
-            for (int r = col + 1 + tid; r < m; r += 1024) Acol[r] = Acol[r] / piv;
-        __syncthreads();
-        const int nrows = m - col - 1, ncols = j0 + jb - col - 1;
-        if (nrows > 0 && ncols > 0) {
-            const i64 total = (i64)nrows * ncols;
-            for (i64 e = tid; e < total; e += 1024) {
-                const int cc = col + 1 + (int)(e / nrows);
-                const int r = col + 1 + (int)(e % nrows);
-                T* Ac = A + (i64)cc * lda;
-                Ac[r] -= Acol[r] * Ac[col];
+    if (tid == 0) {
+        double T[16] = {0};
+        for (int k = 0; k < 4; k++) {
+            T[k * 4 + k] = tau[k];
+            for (int i = 0; i < k; i++) {
+                double acc = 0;
+                for (int j = i; j < k; j++) acc += T[i * 4 + j] * G[j * 4 + k];
+                T[i * 4 + k] = -tau[k] * acc;
             }
         }
-        __syncthreads();
+        for (int i = 0; i < 16; i++) { st->R[i] = R[i]; st->T[i] = T[i]; }
+        for (int k = 0; k < 4; k++) st->tau[k] = tau[k];
+        int bad = 0;
+        for (int k = 1; k < 4; k++)
+            if (!(fabs(R[k * 4 + k]) > 1e-9 * scale0)) bad = 1;
+        st->info = bad;
     }
 }
 
-// Apply the panel's row swaps to every column outside it.
-template <typename T>
-__global__ __launch_bounds__(256) void k_swap(T* __restrict__ A, i64 lda, int ncols, int j0, int jb,
-                                              const int* __restrict__ ipiv) {
-    int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= ncols - jb) return;
-    if (c >= j0) c += jb;
-    T* Ac = A + (i64)c * lda;
-    for (int j = 0; j < jb; j++) {
-        const int p = ipiv[j0 + j];
-        if (p != j0 + j) {
-            const T t = Ac[j0 + j];
-            Ac[j0 + j] = Ac[p];
-            Ac[p] = t;
-        }
+// P = A' V (N x 4), A' = K' + lambda' I generated on the fly; one row per work-group.
+__global__ __launch_bounds__(256) void k_av(const SysDesc s, const double* __restrict__ V, double* __restrict__ P) {
+    __shared__ double red[4][4];
+    const int i = blockIdx.x, tid = threadIdx.x, N = s.N;
+    double acc[4] = {0, 0, 0, 0};
+    for (int j = tid; j < N; j += 256) {
+        const double a = sys_entry(s, i, j);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += a * V[(size_t)k * N + j];
     }
+    #pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = wave_sum(acc[k]);
+    if ((tid & 63) == 0)
+        for (int k = 0; k < 4; k++) red[tid >> 6][k] = acc[k];
+    __syncthreads();
+    if (tid < 4) P[(size_t)tid * N + i] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
 }
 
-// U12 = L11^-1 A12 for 64 columns per work-group.
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsm(T* __restrict__ A, i64 lda, int ncols, int j0, int jb) {
-    __shared__ T Ls[NB * (NB + 1)];
-    __shared__ T Xs[NB * (NB + 1)];
+// S = V'P, M = T' S T, W = P T - 1/2 V M.  One work-group.
+__global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, const double* __restrict__ P,
+                                                  double* __restrict__ W, const FitState* st, int N) {
+    __shared__ double red[16];
+    __shared__ double M[16], Ts[16];
     const int tid = threadIdx.x;
-    const int c0 = j0 + jb + blockIdx.x * NB;
-    for (int e = tid; e < jb * jb; e += 256) {
-        const int r = e % jb, k = e / jb;
-        Ls[k * (NB + 1) + r] = A[(i64)(j0 + k) * lda + j0 + r];
-    }
-    for (int e = tid; e < jb * NB; e += 256) {
-        const int r = e % jb, cc = e / jb;
-        const int col = c0 + cc;
-        Xs[cc * (NB + 1) + r] = col < ncols ? A[(i64)col * lda + j0 + r] : (T)0;
+    double S[16];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+            double p = 0;
+            for (int i = tid; i < N; i += 1024) p += V[(size_t)a * N + i] * P[(size_t)b * N + i];
+            S[a * 4 + b] = block_sum(p, red);
+        }
+    if (tid == 0) {
+        double TS[16];
+        for (int i = 0; i < 16; i++) Ts[i] = st->T[i];
+        for (int a = 0; a < 4; a++)   // (T' S)
+            for (int b = 0; b < 4; b++) {
+                double acc = 0;
+                for (int k = 0; k < 4; k++) acc += Ts[k * 4 + a] * S[k * 4 + b];
+                TS[a * 4 + b] = acc;
+            }
+        for (int a = 0; a < 4; a++)   // (T' S) T
+            for (int b = 0; b < 4; b++) {
+                double acc = 0;
+                for (int k = 0; k < 4; k++) acc += TS[a * 4 + k] * Ts[k * 4 + b];
+                M[a * 4 + b] = acc;
+            }
     }
     __syncthreads();
-    for (int k = 0; k < jb - 1; k++) {
-        const int rem = jb - k - 1;
-        for (int e = tid; e < rem * NB; e += 256) {
-            const int r = k + 1 + e % rem, cc = e / rem;
-            Xs[cc * (NB + 1) + r] -= Ls[k * (NB + 1) + r] * Xs[cc * (NB + 1) + k];
+    for (int i = tid; i < N; i += 1024) {
+        double p[4], v[4];
+        for (int k = 0; k < 4; k++) { p[k] = P[(size_t)k * N + i]; v[k] = V[(size_t)k * N + i]; }
+        for (int b = 0; b < 4; b++) {
+            double acc = 0;
+            for (int k = 0; k < 4; k++) acc += p[k] * Ts[k * 4 + b] - 0.5 * v[k] * M[k * 4 + b];
+            W[(size_t)b * N + i] = acc;
         }
-        __syncthreads();
-    }
-    for (int e = tid; e < jb * NB; e += 256) {
-        const int r = e % jb, cc = e / jb;
-        const int col = c0 + cc;
-        if (col < ncols) A[(i64)col * lda + j0 + r] = Xs[cc * (NB + 1) + r];
     }
 }
 
-// A22 -= L21 U12 over rows [r0, m) and columns [c0, ncols), inner dim jb.
+// B = [A' - W V' - V W']_(4:,4:), n x n column-major in precision T.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_update(T* __restrict__ A, i64 lda, int m, int ncols, int j0, int jb) {
-    __shared__ T As[NB][NB + 1];
-    __shared__ T Bs[NB][NB + 1];
+__global__ __launch_bounds__(256) void k_build_b(const SysDesc s, const double* __restrict__ V,
+                                                 const double* __restrict__ W, T* __restrict__ B) {
+    const int n = s.N - 4, N = s.N;
+    const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (i64)n * n) return;
+    const int i = (int)(e % n) + 4, j = (int)(e / n) + 4;
+    double v = sys_entry(s, i, j);
+    #pragma unroll
+    for (int k = 0; k < 4; k++)
+        v -= W[(size_t)k * N + i] * V[(size_t)k * N + j] + V[(size_t)k * N + i] * W[(size_t)k * N + j];
+    B[e] = (T)v;
+}
+
+// ---- blocked Cholesky (lower), column-major n x n -----------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, int n, int j0, int jb, FitState* st) {
+    __shared__ T L[NB][NB + 1];
     const int tid = threadIdx.x;
-    const int r0 = j0 + jb, c0 = j0 + jb;
-    const int R = r0 + blockIdx.x * NB, C = c0 + blockIdx.y * NB;
-    for (int e = tid; e < NB * jb; e += 256) {
-        const int r = e % NB, k = e / NB;
-        As[k][r] = (R + r < m) ? A[(i64)(j0 + k) * lda + R + r] : (T)0;
+    for (int e = tid; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        L[r][c] = B[(i64)(j0 + c) * n + j0 + r];
+    }
+    __syncthreads();
+    for (int k = 0; k < jb; k++) {
+        const T d = L[k][k];
+        const T sd = sqrt(d > (T)0 ? d : (T)1);
+        if (tid == 0 && !(d > (T)0) && st->info == 0) st->info = 2 + j0 + k;
+        __syncthreads();
+        for (int r = k + 1 + tid; r < jb; r += 256) L[r][k] = L[r][k] / sd;
+        __syncthreads();
+        if (tid == 0) L[k][k] = sd;
+        const int rem = jb - k - 1;
+        for (int e = tid; e < rem * rem; e += 256) {
+            const int r = k + 1 + e % rem, c = k + 1 + e / rem;
+            if (c <= r) L[r][c] -= L[r][k] * L[c][k];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        if (c <= r) B[(i64)(j0 + c) * n + j0 + r] = L[r][c];
+    }
+}
+
+// L21 = B21 L11^-T for 64 rows per work-group below the diagonal block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, int n, int j0, int jb) {
+    __shared__ T L[NB][NB + 1];
+    __shared__ T X[NB][NB + 1];  // X[row][k]
+    const int tid = threadIdx.x;
+    const int r0 = j0 + jb + blockIdx.x * NB;
+    const int nr = min(NB, n - r0);
+    for (int e = tid; e < jb * jb; e += 256) {
+        const int r = e % jb, c = e / jb;
+        L[r][c] = B[(i64)(j0 + c) * n + j0 + r];
     }
     for (int e = tid; e < NB * jb; e += 256) {
-        const int k = e % jb, c = e / jb;
-        Bs[k][c] = (C + c < ncols) ? A[(i64)(C + c) * lda + j0 + k] : (T)0;
+        const int r = e % NB, c = e / NB;
+        X[r][c] = r < nr ? B[(i64)(j0 + c) * n + r0 + r] : (T)0;
+    }
+    __syncthreads();
+    for (int k = 0; k < jb; k++) {
+        if (tid < NB) X[tid][k] = X[tid][k] / L[k][k];
+        __syncthreads();
+        const int rem = jb - k - 1;
+        for (int e = tid; e < NB * rem; e += 256) {
+            const int r = e % NB, c = k + 1 + e / NB;
+            X[r][c] -= X[r][k] * L[c][k];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < NB * jb; e += 256) {
+        const int r = e % NB, c = e / NB;
+        if (r < nr) B[(i64)(j0 + c) * n + r0 + r] = X[r][c];
+    }
+}
+
+// Lower trailing tiles: B[R, C] -= L21[R] L21[C]' for tile C <= tile R.
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, int jb) {
+    if (blockIdx.y > blockIdx.x) return;
+    __shared__ T As[NB][NB + 1];  // As[k][r]
+    __shared__ T Bs[NB][NB + 1];  // Bs[k][c]
+    const int tid = threadIdx.x;
+    const int r0 = j0 + jb;
+    const int R = r0 + blockIdx.x * NB, C = r0 + blockIdx.y * NB;
+    for (int e = tid; e < NB * jb; e += 256) {
+        const int r = e % NB, k = e / NB;
+        As[k][r] = R + r < n ? B[(i64)(j0 + k) * n + R + r] : (T)0;
+        Bs[k][r] = C + r < n ? B[(i64)(j0 + k) * n + C + r] : (T)0;
     }
     __syncthreads();
     const int tx = tid % 16, ty = tid / 16;
@@ -223,113 +356,224 @@ __global__ __launch_bounds__(256) void k_gemm_update(T* __restrict__ A, i64 lda,
     #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int c = C + ty + 16 * j;
-        if (c >= ncols) continue;
+        if (c >= n) continue;
         #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int r = R + tx + 16 * i;
-            if (r < m) A[(i64)c * lda + r] -= acc[i][j];
+            if (r < n && c <= r) B[(i64)c * n + r] -= acc[i][j];
         }
     }
 }
 
-// ---- solves with the factors, 4 right-hand sides (m x 4, column-major) ----
-
+// Solve L L' x = b for 4 right-hand sides (n x 4 column-major, in place).
+// One work-group; 64-row blocks, each solved by wave 0 with shuffles, the
+// rest of the right-hand side updated by all threads.
 template <typename T>
-__global__ void k_apply_pivots(T* __restrict__ B, int m, const int* __restrict__ ipiv) {
-    const int j = threadIdx.x;
-    if (j >= 4) return;
-    T* b = B + (i64)j * m;
-    for (int i = 0; i < m; i++) {
-        const int p = ipiv[i];
-        if (p != i) {
-            const T t = b[i];
-            b[i] = b[p];
-            b[p] = t;
-        }
-    }
-}
-
-// Diagonal block solve: lower unit (UPPER=false) or upper (UPPER=true).
-template <typename T, bool UPPER>
-__global__ __launch_bounds__(256) void k_diag_solve(const T* __restrict__ A, i64 lda, T* __restrict__ B, int m,
-                                                    int i0, int ib) {
-    __shared__ T Ls[NB * (NB + 1)];
-    __shared__ T Xs[4 * (NB + 1)];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < ib * ib; e += 256) {
-        const int r = e % ib, k = e / ib;
-        Ls[k * (NB + 1) + r] = A[(i64)(i0 + k) * lda + i0 + r];
-    }
-    for (int e = tid; e < ib * 4; e += 256) {
-        const int r = e % ib, j = e / ib;
-        Xs[j * (NB + 1) + r] = B[(i64)j * m + i0 + r];
-    }
-    __syncthreads();
-    if (!UPPER) {
-        for (int k = 0; k < ib - 1; k++) {
-            const int rem = ib - k - 1;
-            for (int e = tid; e < rem * 4; e += 256) {
-                const int r = k + 1 + e % rem, j = e / rem;
-                Xs[j * (NB + 1) + r] -= Ls[k * (NB + 1) + r] * Xs[j * (NB + 1) + k];
+__global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, int n, T* __restrict__ b) {
+    __shared__ T xb[4][NB];
+    const int tid = threadIdx.x, lane = tid & 63;
+    // forward: L y = b
+    for (int i0 = 0; i0 < n; i0 += NB) {
+        const int ib = min(NB, n - i0);
+        if (tid < 64) {
+            T v[4];
+            for (int j = 0; j < 4; j++) v[j] = lane < ib ? b[(i64)j * n + i0 + lane] : (T)0;
+            for (int k = 0; k < ib; k++) {
+                const T lrk = lane < ib ? L[(i64)(i0 + k) * n + i0 + lane] : (T)0;
+                const T lkk = __shfl(lrk, k, 64);
+                for (int j = 0; j < 4; j++) {
+                    T xk = __shfl(v[j], k, 64) / lkk;
+                    if (lane == k) v[j] = xk;
+                    else if (lane > k) v[j] -= lrk * xk;
+                }
             }
-            __syncthreads();
+            if (lane < ib)
+                for (int j = 0; j < 4; j++) xb[j][lane] = v[j];
         }
-    } else {
-        for (int k = ib - 1; k >= 0; k--) {
-            if (tid < 4) Xs[tid * (NB + 1) + k] = Xs[tid * (NB + 1) + k] / Ls[k * (NB + 1) + k];
-            __syncthreads();
-            for (int e = tid; e < k * 4; e += 256) {
-                const int r = e % k, j = e / k;
-                Xs[j * (NB + 1) + r] -= Ls[k * (NB + 1) + r] * Xs[j * (NB + 1) + k];
+        __syncthreads();
+        for (int e = tid; e < ib * 4; e += 1024) b[(i64)(e / ib) * n + i0 + e % ib] = xb[e / ib][e % ib];
+        for (int r = i0 + ib + tid; r < n; r += 1024) {
+            T acc[4] = {0, 0, 0, 0};
+            for (int k = 0; k < ib; k++) {
+                const T l = L[(i64)(i0 + k) * n + r];
+                for (int j = 0; j < 4; j++) acc[j] = fma(l, xb[j][k], acc[j]);
             }
-            __syncthreads();
+            for (int j = 0; j < 4; j++) b[(i64)j * n + r] -= acc[j];
         }
-    }
-    for (int e = tid; e < ib * 4; e += 256) {
-        const int r = e % ib, j = e / ib;
-        B[(i64)j * m + i0 + r] = Xs[j * (NB + 1) + r];
-    }
-}
-
-// B[rows] -= A[rows, i0:i0+ib] * B[i0:i0+ib] for rows in [ra, rb).
-template <typename T>
-__global__ __launch_bounds__(256) void k_block_update(const T* __restrict__ A, i64 lda, T* __restrict__ B, int m,
-                                                      int i0, int ib, int ra, int rb) {
-    const int r = ra + blockIdx.x * 256 + threadIdx.x;
-    if (r >= rb) return;
-    T acc[4] = {0, 0, 0, 0};
-    for (int k = 0; k < ib; k++) {
-        const T a = A[(i64)(i0 + k) * lda + r];
-        #pragma unroll
-        for (int j = 0; j < 4; j++) acc[j] = fma(a, B[(i64)j * m + i0 + k], acc[j]);
-    }
-    #pragma unroll
-    for (int j = 0; j < 4; j++) B[(i64)j * m + r] -= acc[j];
-}
-
-// FP64 residual  r = b - M' x  with M' regenerated from the landmarks.
-// One work-group per row; also returns max |r| and max |b| per row.
-__global__ __launch_bounds__(256) void k_residual(const SysDesc s, const double* __restrict__ x,
-                                                  double* __restrict__ res) {
-    __shared__ double red[4][256];
-    const int r = blockIdx.x, tid = threadIdx.x;
-    double acc[4] = {0, 0, 0, 0};
-    for (int k = tid; k < s.m; k += 256) {
-        const double a = sys_entry(s, r, k);
-        if (a != 0.0)
-            for (int j = 0; j < 4; j++) acc[j] += a * x[(i64)j * s.m + k];
-    }
-    for (int j = 0; j < 4; j++) red[j][tid] = acc[j];
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st)
-            for (int j = 0; j < 4; j++) red[j][tid] += red[j][tid + st];
         __syncthreads();
     }
-    if (tid < 4) {
-        const double b = r < s.N ? s.y[r * 4 + tid] : 0.0;
-        res[(i64)tid * s.m + r] = b - red[tid][0];
+    // backward: L' x = y
+    const int last = ((n - 1) / NB) * NB;
+    for (int i0 = last; i0 >= 0; i0 -= NB) {
+        const int ib = min(NB, n - i0);
+        if (tid < 64) {
+            T v[4];
+            for (int j = 0; j < 4; j++) v[j] = lane < ib ? b[(i64)j * n + i0 + lane] : (T)0;
+            // Row `lane` of L' within the block is column `lane` of L: L[(i0+lane)..][i0+k] for k > lane.
+            for (int k = ib - 1; k >= 0; k--) {
+                const T lkr = lane < ib ? L[(i64)(i0 + lane) * n + i0 + k] : (T)0;  // L[k][lane]
+                const T lkk = __shfl(lkr, k, 64);
+                for (int j = 0; j < 4; j++) {
+                    T xk = __shfl(v[j], k, 64) / lkk;
+                    if (lane == k) v[j] = xk;
+                    else if (lane < k) v[j] -= lkr * xk;
+                }
+            }
+            if (lane < ib)
+                for (int j = 0; j < 4; j++) xb[j][lane] = v[j];
+        }
+        __syncthreads();
+        for (int e = tid; e < ib * 4; e += 1024) b[(i64)(e / ib) * n + i0 + e % ib] = xb[e / ib][e % ib];
+        for (int r = tid; r < i0; r += 1024) {
+            T acc[4] = {0, 0, 0, 0};
+            for (int k = 0; k < ib; k++) {
+                const T l = L[(i64)r * n + i0 + k];  // L[i0+k][r]
+                for (int j = 0; j < 4; j++) acc[j] = fma(l, xb[j][k], acc[j]);
+            }
+            for (int j = 0; j < 4; j++) b[(i64)j * n + r] -= acc[j];
+        }
+        __syncthreads();
     }
+}
+
+// U (N x 4, column-major) <- Q' U (trans = 1) or Q U (trans = 0), Q = I - V T V'.
+__global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, const FitState* st,
+                                                  double* __restrict__ U, int N, int trans) {
+    __shared__ double red[16];
+    __shared__ double H[16];
+    const int tid = threadIdx.x;
+    double G[16];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+            double p = 0;
+            for (int i = tid; i < N; i += 1024) p += V[(size_t)a * N + i] * U[(size_t)b * N + i];
+            G[a * 4 + b] = block_sum(p, red);
+        }
+    if (tid == 0)
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                double acc = 0;
+                for (int k = 0; k < 4; k++) acc += (trans ? st->T[k * 4 + a] : st->T[a * 4 + k]) * G[k * 4 + b];
+                H[a * 4 + b] = acc;
+            }
+    __syncthreads();
+    for (int i = tid; i < N; i += 1024)
+        for (int b = 0; b < 4; b++) {
+            double acc = 0;
+            for (int k = 0; k < 4; k++) acc += V[(size_t)k * N + i] * H[k * 4 + b];
+            U[(size_t)b * N + i] -= acc;
+        }
+}
+
+// out = (with_y ? Y : 0) - A' x, x and out N x 4 column-major.
+__global__ __launch_bounds__(256) void k_matvec(const SysDesc s, const double* __restrict__ x,
+                                                double* __restrict__ out, int with_y) {
+    __shared__ double red[4][4];
+    const int i = blockIdx.x, tid = threadIdx.x, N = s.N;
+    double acc[4] = {0, 0, 0, 0};
+    for (int j = tid; j < N; j += 256) {
+        const double a = sys_entry(s, i, j);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += a * x[(size_t)k * N + j];
+    }
+    #pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = wave_sum(acc[k]);
+    if ((tid & 63) == 0)
+        for (int k = 0; k < 4; k++) red[tid >> 6][k] = acc[k];
+    __syncthreads();
+    if (tid < 4) {
+        const double ax = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        out[(size_t)tid * N + i] = (with_y ? s.y[i * 4 + tid] : 0.0) - ax;
+    }
+}
+
+// U = Y (N x 4 column-major from the row-major Y rows).
+__global__ void k_load_y(const SysDesc s, double* __restrict__ U) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < s.N)
+        for (int k = 0; k < 4; k++) U[(size_t)k * s.N + i] = s.y[i * 4 + k];
+}
+
+// U = [0; z] (z is n x 4 column-major, FP64).
+__global__ void k_embed(const double* __restrict__ z, double* __restrict__ U, int N) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const int n = N - 4;
+    for (int k = 0; k < 4; k++) U[(size_t)k * N + i] = i < 4 ? 0.0 : z[(size_t)k * n + i - 4];
+}
+
+// b2 (n x 4) = rows 4.. of U; also converts to the factor precision.
+template <typename T>
+__global__ void k_tail(const double* __restrict__ U, double* __restrict__ b64, T* __restrict__ bt, int N) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int n = N - 4;
+    if (i >= n) return;
+    for (int k = 0; k < 4; k++) {
+        const double v = U[(size_t)k * N + 4 + i];
+        if (b64) b64[(size_t)k * n + i] = v;
+        bt[(size_t)k * n + i] = (T)v;
+    }
+}
+
+// Refinement residual r = b2 - B z with B = [A' - W V' - V W']_(4:,4:):
+// given Au = -A' [0; z] (from k_matvec), r_i = b2_i + Au_(i+4) + W (V'u) + V (W'u).
+// Writes r in FP32 for the correction solve and the max norms to st.
+__global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__ V, const double* __restrict__ W,
+                                                      const double* __restrict__ U, const double* __restrict__ negAu,
+                                                      const double* __restrict__ b2, float* __restrict__ r32,
+                                                      FitState* st, int N) {
+    __shared__ double red[16];
+    __shared__ double G1[16], G2[16];
+    const int tid = threadIdx.x, n = N - 4;
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) {
+            double p1 = 0, p2 = 0;
+            for (int i = tid; i < N; i += 1024) {
+                p1 += V[(size_t)a * N + i] * U[(size_t)b * N + i];
+                p2 += W[(size_t)a * N + i] * U[(size_t)b * N + i];
+            }
+            p1 = block_sum(p1, red);
+            p2 = block_sum(p2, red);
+            if (tid == 0) { G1[a * 4 + b] = p1; G2[a * 4 + b] = p2; }
+        }
+    __syncthreads();
+    double rmax = 0, bmax = 0;
+    for (int i = tid; i < n; i += 1024)
+        for (int b = 0; b < 4; b++) {
+            double corr = 0;
+            for (int k = 0; k < 4; k++)
+                corr += W[(size_t)k * N + 4 + i] * G1[k * 4 + b] + V[(size_t)k * N + 4 + i] * G2[k * 4 + b];
+            const double bb = b2[(size_t)b * n + i];
+            const double r = bb + negAu[(size_t)b * N + 4 + i] + corr;
+            r32[(size_t)b * n + i] = (float)r;
+            rmax = fmax(rmax, fabs(r));
+            bmax = fmax(bmax, fabs(bb));
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        rmax = fmax(rmax, __shfl_xor(rmax, o, 64));
+        bmax = fmax(bmax, __shfl_xor(bmax, o, 64));
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) { red[tid >> 6] = rmax; }
+    __syncthreads();
+    if (tid == 0) {
+        double m = 0;
+        for (int w = 0; w < 16; w++) m = fmax(m, red[w]);
+        st->rnorm = m;
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) { red[tid >> 6] = bmax; }
+    __syncthreads();
+    if (tid == 0) {
+        double m = 0;
+        for (int w = 0; w < 16; w++) m = fmax(m, red[w]);
+        st->bnorm = m;
+    }
+}
+
+__global__ void k_axpy_f32(double* __restrict__ x, const float* __restrict__ d, i64 n) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) x[i] += (double)d[i];
 }
 
 template <typename Src, typename Dst>
@@ -338,60 +582,42 @@ __global__ void k_convert(const Src* __restrict__ a, Dst* __restrict__ b, i64 n)
     if (i < n) b[i] = (Dst)a[i];
 }
 
-__global__ void k_axpy_f32(double* __restrict__ x, const float* __restrict__ d, i64 n) {
-    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) x[i] += (double)d[i];
+// FP64 residual of the bordered system  r = [Y; 0] - M' x  (x is m x 4).
+__global__ __launch_bounds__(256) void k_residual(const SysDesc s, const double* __restrict__ x,
+                                                  double* __restrict__ res) {
+    __shared__ double red[4][4];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    for (int k = tid; k < s.m; k += 256) {
+        const double a = sys_entry(s, r, k);
+        #pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] += a * x[(i64)j * s.m + k];
+    }
+    #pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = wave_sum(acc[j]);
+    if ((tid & 63) == 0)
+        for (int j = 0; j < 4; j++) red[tid >> 6][j] = acc[j];
+    __syncthreads();
+    if (tid < 4) {
+        const double b = r < s.N ? s.y[r * 4 + tid] : 0.0;
+        res[(i64)tid * s.m + r] = b - (red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
+    }
 }
 
 inline unsigned nblk(i64 n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 template <typename T>
-int lu_factor(tw_ctx* ctx, T* A, int m, int* ipiv, int* info) {
-    const i64 lda = m;
-    for (int j0 = 0; j0 < m; j0 += NB) {
-        const int jb = std::min(NB, m - j0);
-        hipLaunchKernelGGL(k_panel<T>, dim3(1), dim3(1024), 0, ctx->stream, A, lda, m, j0, jb, ipiv, info);
-        if (m - jb > 0)
-            hipLaunchKernelGGL(k_swap<T>, dim3(nblk(m - jb)), dim3(256), 0, ctx->stream, A, lda, m, j0, jb, ipiv);
-        const int rest = m - j0 - jb;
+void cholesky(tw_ctx* ctx, T* B, int n, FitState* st) {
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int jb = std::min(NB, n - j0);
+        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, n, j0, jb, st);
+        const int rest = n - j0 - jb;
         if (rest > 0) {
-            hipLaunchKernelGGL(k_trsm<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, A, lda, m, j0, jb);
-            hipLaunchKernelGGL(k_gemm_update<T>, dim3(nblk(rest, NB), nblk(rest, NB)), dim3(256), 0, ctx->stream, A,
-                               lda, m, m, j0, jb);
+            hipLaunchKernelGGL(k_trsm_chol<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, B, n, j0, jb);
+            hipLaunchKernelGGL(k_syrk<T>, dim3(nblk(rest, NB), nblk(rest, NB)), dim3(256), 0, ctx->stream, B, n, j0,
+                               jb);
         }
-        TW_LAUNCH_CHECK(ctx);
     }
-    return TW_OK;
-}
-
-template <typename T>
-int lu_solve(tw_ctx* ctx, const T* A, int m, const int* ipiv, T* B) {
-    const i64 lda = m;
-    hipLaunchKernelGGL(k_apply_pivots<T>, dim3(1), dim3(64), 0, ctx->stream, B, m, ipiv);
-    for (int i0 = 0; i0 < m; i0 += NB) {
-        const int ib = std::min(NB, m - i0);
-        hipLaunchKernelGGL((k_diag_solve<T, false>), dim3(1), dim3(256), 0, ctx->stream, A, lda, B, m, i0, ib);
-        if (i0 + ib < m)
-            hipLaunchKernelGGL(k_block_update<T>, dim3(nblk(m - i0 - ib)), dim3(256), 0, ctx->stream, A, lda, B, m,
-                               i0, ib, i0 + ib, m);
-    }
-    const int last = ((m - 1) / NB) * NB;
-    for (int i0 = last; i0 >= 0; i0 -= NB) {
-        const int ib = std::min(NB, m - i0);
-        hipLaunchKernelGGL((k_diag_solve<T, true>), dim3(1), dim3(256), 0, ctx->stream, A, lda, B, m, i0, ib);
-        if (i0 > 0)
-            hipLaunchKernelGGL(k_block_update<T>, dim3(nblk(i0)), dim3(256), 0, ctx->stream, A, lda, B, m, i0, ib, 0,
-                               i0);
-    }
-    TW_LAUNCH_CHECK(ctx);
-    return TW_OK;
-}
-
-// max |v| over an m x 4 host array
-double amax(const std::vector<double>& v) {
-    double r = 0;
-    for (double x : v) r = std::max(r, std::fabs(x));
-    return r;
 }
 
 }  // namespace
@@ -413,8 +639,7 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     tw_stage_begin(ctx, ST_TOTAL);
     tw_stage_begin(ctx, ST_FIT);
 
-    const int N = K, m = N + 4;
-    // Scaling: centroid and max deviation of the centres.
+    const int N = K, n = N - 4, m = N + 4;
     double mu[3] = {0, 0, 0};
     for (int i = 0; i < N; i++)
         for (int a = 0; a < 3; a++) mu[a] += centres[3 * i + a];
@@ -424,7 +649,7 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
         for (int a = 0; a < 3; a++) s = std::max(s, std::fabs(centres[3 * i + a] - mu[a]));
     if (!(s > 0)) return tw_fail(ctx, TW_E_SINGULAR, "all centres coincide");
 
-    // Device packs: centres (float4), Y rows (double x4).
+    // Landmark packs.
     const size_t pb = (size_t)N * sizeof(float4), yb = (size_t)N * 4 * sizeof(double);
     void* pack = nullptr;
     if ((rc = tw_ws(ctx, WS_PTS, pb + yb, &pack))) return rc;
@@ -446,109 +671,148 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     for (int a = 0; a < 3; a++) sd.mu[a] = mu[a];
     sd.lam = lambda / s;
 
-    // Workspaces.
+    // Workspaces: B (n x n, FP64 or FP32); vectors in one FP64 arena.
     const bool mixed = precision == TW_MIXED;
-    const size_t esz = mixed ? sizeof(float) : sizeof(double);
-    void *Aw = nullptr, *Bw = nullptr, *aux = nullptr, *aux2 = nullptr;
-    if ((rc = tw_ws(ctx, WS_SOLVE_A, (size_t)m * m * esz, &Aw))) return rc;
-    if ((rc = tw_ws(ctx, WS_SOLVE_B, (size_t)m * 4 * sizeof(double) * 2, &Bw))) return rc;
-    if ((rc = tw_ws(ctx, WS_SOLVE_AUX, (size_t)(m + 16) * sizeof(int), &aux))) return rc;
-    if ((rc = tw_ws(ctx, WS_SOLVE_AUX2, (size_t)m * 4 * sizeof(double), &aux2))) return rc;
-    int* ipiv = (int*)aux;
-    int* info = ipiv + m;
-    double* x64 = (double*)Bw;              // solution, FP64
-    double* r64 = x64 + (size_t)m * 4;      // residual, FP64
-    TW_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), ctx->stream));
+    void *Bw = nullptr, *arena = nullptr, *aux = nullptr;
+    if ((rc = tw_ws(ctx, WS_SOLVE_A, (size_t)n * n * (mixed ? sizeof(float) : sizeof(double)), &Bw))) return rc;
+    const size_t NV = (size_t)N * 4;
+    // V, P, W, U, U2, x(m x 4), res(m x 4), b2(n x 4), z(n x 4)
+    const size_t arena_d = 5 * NV + 2 * (size_t)m * 4 + 2 * (size_t)n * 4;
+    if ((rc = tw_ws(ctx, WS_SOLVE_B, arena_d * sizeof(double), &arena))) return rc;
+    if ((rc = tw_ws(ctx, WS_SOLVE_AUX, sizeof(FitState) + (size_t)n * 4 * sizeof(float) * 2, &aux))) return rc;
+    double* V = (double*)arena;
+    double* P = V + NV;
+    double* W = P + NV;
+    double* U = W + NV;
+    double* U2 = U + NV;
+    double* x = U2 + NV;
+    double* res = x + (size_t)m * 4;
+    double* b2 = res + (size_t)m * 4;
+    double* z = b2 + (size_t)n * 4;
+    FitState* st = (FitState*)aux;
+    float* f32a = (float*)((uint8_t*)aux + sizeof(FitState));
+    float* f32b = f32a + (size_t)n * 4;
 
-    std::vector<double> hx((size_t)m * 4), hr((size_t)m * 4);
-    auto fetch = [&](const double* dev, std::vector<double>& h) -> int {
-        TW_HIP(ctx, hipMemcpyAsync(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FitState hst;
+    auto fetch_state = [&]() -> int {
+        TW_HIP(ctx, hipMemcpyAsync(&hst, st, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
         TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return TW_OK;
     };
-    int hinfo = 0;
-    double bmax = 0;
-    for (int i = 0; i < N; i++)
-        for (int a = 0; a < 4; a++) bmax = std::max(bmax, std::fabs(hy[4 * i + a]));
 
-    auto solve_fp64 = [&]() -> int {
-        double* A = (double*)Aw;
-        if (!mixed) {
-            hipLaunchKernelGGL(k_assemble<double>, dim3(nblk((i64)m * m)), dim3(256), 0, ctx->stream, A, (i64)m, sd);
-        } else {
-            // Refactor in FP64: the FP32 workspace is too small, grow it.
-            int r2 = tw_ws(ctx, WS_SOLVE_A, (size_t)m * m * sizeof(double), &Aw);
+    // 1. QR of X', W, B.
+    hipLaunchKernelGGL(k_hh_x, dim3(1), dim3(1024), 0, ctx->stream, sd, V, st);
+    hipLaunchKernelGGL(k_av, dim3(N), dim3(256), 0, ctx->stream, sd, V, P);
+    hipLaunchKernelGGL(k_build_w, dim3(1), dim3(1024), 0, ctx->stream, V, P, W, st, N);
+    // 2. Q'Y -> b2
+    hipLaunchKernelGGL(k_load_y, dim3(nblk(N)), dim3(256), 0, ctx->stream, sd, U);
+    hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 1);
+    TW_LAUNCH_CHECK(ctx);
+
+    auto factor_solve_fp64 = [&]() -> int {
+        double* B = (double*)Bw;
+        if (mixed) {
+            int r2 = tw_ws(ctx, WS_SOLVE_A, (size_t)n * n * sizeof(double), &Bw);
             if (r2) return r2;
-            A = (double*)Aw;
-            hipLaunchKernelGGL(k_assemble<double>, dim3(nblk((i64)m * m)), dim3(256), 0, ctx->stream, A, (i64)m, sd);
+            B = (double*)Bw;
         }
-        TW_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), ctx->stream));
-        int r2 = lu_factor<double>(ctx, A, m, ipiv, info);
-        if (r2) return r2;
-        TW_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (hinfo) return tw_fail(ctx, TW_E_SINGULAR, "singular TPS system (zero pivot at column %d)", hinfo - 1);
-        hipLaunchKernelGGL(k_rhs<double>, dim3(nblk(m)), dim3(256), 0, ctx->stream, x64, sd);
-        return lu_solve<double>(ctx, A, m, ipiv, x64);
+        hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
+        cholesky<double>(ctx, B, n, st);
+        hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
+        hipLaunchKernelGGL(k_chol_solve<double>, dim3(1), dim3(1024), 0, ctx->stream, B, n, z);
+        TW_LAUNCH_CHECK(ctx);
+        return TW_OK;
     };
 
+    int iters = 0;
     if (!mixed) {
-        if ((rc = solve_fp64())) return rc;
-        hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x64, r64);
-        TW_LAUNCH_CHECK(ctx);
-        if ((rc = fetch(r64, hr))) return rc;
-        ctx->fit_residual = amax(hr) / bmax;
-        ctx->refine_iters = 0;
+        if ((rc = factor_solve_fp64())) return rc;
     } else {
-        float* A = (float*)Aw;
-        float* b32 = (float*)aux2;
-        hipLaunchKernelGGL(k_assemble<float>, dim3(nblk((i64)m * m)), dim3(256), 0, ctx->stream, A, (i64)m, sd);
-        if ((rc = lu_factor<float>(ctx, A, m, ipiv, info))) return rc;
-        TW_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        bool ok = hinfo == 0;
-        int iters = 0;
+        float* B = (float*)Bw;
+        hipLaunchKernelGGL(k_build_b<float>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
+        cholesky<float>(ctx, B, n, st);
+        hipLaunchKernelGGL(k_tail<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, b2, f32a, N);
+        hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(1024), 0, ctx->stream, B, n, f32a);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, f32a, z,
+                           (i64)n * 4);
+        TW_LAUNCH_CHECK(ctx);
+        if ((rc = fetch_state())) return rc;
+        if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
+        bool ok = hst.info == 0;
+        double prev = INFINITY;
         if (ok) {
-            hipLaunchKernelGGL(k_rhs<float>, dim3(nblk(m)), dim3(256), 0, ctx->stream, b32, sd);
-            if ((rc = lu_solve<float>(ctx, A, m, ipiv, b32))) return rc;
-            hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk((i64)m * 4)), dim3(256), 0, ctx->stream, b32,
-                               x64, (i64)m * 4);
-            double prev = INFINITY;
             ok = false;
-            for (iters = 1; iters <= 30; iters++) {
-                hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x64, r64);
+            for (iters = 1; iters <= 20; iters++) {
+                hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U2, N);
+                hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U2, P, 0);  // P = -A'[0;z]
+                hipLaunchKernelGGL(k_ir_residual, dim3(1), dim3(1024), 0, ctx->stream, V, W, U2, P, b2, f32b, st, N);
                 TW_LAUNCH_CHECK(ctx);
-                if ((rc = fetch(r64, hr))) return rc;
-                const double rel = amax(hr) / bmax;
-                if (rel < 1e-14 || !(rel < 0.5 * prev)) {
-                    ok = rel < 1e-10;
-                    ctx->fit_residual = rel;
+                if ((rc = fetch_state())) return rc;
+                const double rel = hst.rnorm / (hst.bnorm > 0 ? hst.bnorm : 1.0);
+                if (rel < 1e-15 || !(rel < 0.5 * prev)) {
+                    ok = rel < 1e-11;
                     break;
                 }
                 prev = rel;
-                hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk((i64)m * 4)), dim3(256), 0, ctx->stream, r64,
-                                   b32, (i64)m * 4);
-                if ((rc = lu_solve<float>(ctx, A, m, ipiv, b32))) return rc;
-                hipLaunchKernelGGL(k_axpy_f32, dim3(nblk((i64)m * 4)), dim3(256), 0, ctx->stream, x64, b32, (i64)m * 4);
+                hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(1024), 0, ctx->stream, B, n, f32b);
+                hipLaunchKernelGGL(k_axpy_f32, dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, z, f32b,
+                                   (i64)n * 4);
             }
         }
-        ctx->refine_iters = iters;
         if (!ok) {
-            // FP32 factors too inaccurate for this system: FP64 refactor.
-            if ((rc = solve_fp64())) return rc;
-            hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x64, r64);
-            if ((rc = fetch(r64, hr))) return rc;
-            ctx->fit_residual = amax(hr) / bmax;
-            ctx->refine_iters = -iters;  // negative: fell back to FP64 factorisation
+            // FP32 factors too weak for this system: factor in FP64 (U still
+            // holds Q'Y).
+            const int zero = 0;
+            TW_HIP(ctx, hipMemcpyAsync(&st->info, &zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream));
+            if ((rc = factor_solve_fp64())) return rc;
+            iters = -iters;
         }
     }
-    if ((rc = fetch(x64, hx))) return rc;
+    // 3. c' = Q [0; z];  t = Y - A' c';  Q' t (first 4 rows give q1' t).
+    hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U, N);
+    hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 0);
+    hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U, U2, 1);
+    hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U2, N, 1);
+    TW_LAUNCH_CHECK(ctx);
+    std::vector<double> hc(NV), ht(NV);
+    TW_HIP(ctx, hipMemcpyAsync(hc.data(), U, NV * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TW_HIP(ctx, hipMemcpyAsync(ht.data(), U2, NV * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = fetch_state())) return rc;
+    if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
+    if (hst.info >= 2)
+        return tw_fail(ctx, TW_E_SINGULAR, "projected TPS system not positive definite (column %d)", hst.info - 2);
+    // d' = R^-1 (q1' t)
+    double dp[16];
+    for (int j = 0; j < 4; j++)
+        for (int a = 3; a >= 0; a--) {
+            double v = ht[(size_t)j * N + a];
+            for (int q = a + 1; q < 4; q++) v -= hst.R[a * 4 + q] * dp[q * 4 + j];
+            dp[a * 4 + j] = v / hst.R[a * 4 + a];
+        }
+    // Residual of the bordered system (FP64), for the record.
+    {
+        std::vector<double> hx((size_t)m * 4);
+        for (int j = 0; j < 4; j++) {
+            for (int i = 0; i < N; i++) hx[(size_t)j * m + i] = hc[(size_t)j * N + i];
+            for (int a = 0; a < 4; a++) hx[(size_t)j * m + N + a] = dp[a * 4 + j];
+        }
+        TW_HIP(ctx, hipMemcpyAsync(x, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x, res);
+        TW_LAUNCH_CHECK(ctx);
+        std::vector<double> hr((size_t)m * 4);
+        TW_HIP(ctx, hipMemcpyAsync(hr.data(), res, hr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double rmax = 0, bmax = 0;
+        for (double v : hr) rmax = std::max(rmax, std::fabs(v));
+        for (int i = 0; i < 4 * N; i++) bmax = std::max(bmax, std::fabs(hy[i]));
+        ctx->fit_residual = rmax / bmax;
+    }
+    ctx->refine_iters = iters;
     // Undo the scaling: c = c'/s, d = T^-1 d'.
     for (int i = 0; i < N; i++)
-        for (int j = 0; j < 4; j++) c[4 * i + j] = hx[(size_t)j * m + i] / s;
+        for (int j = 0; j < 4; j++) c[4 * i + j] = hc[(size_t)j * N + i] / s;
     for (int j = 0; j < 4; j++) {
-        const double d0 = hx[(size_t)j * m + N], d1 = hx[(size_t)j * m + N + 1], d2 = hx[(size_t)j * m + N + 2],
-                     d3 = hx[(size_t)j * m + N + 3];
+        const double d0 = dp[0 * 4 + j], d1 = dp[1 * 4 + j], d2 = dp[2 * 4 + j], d3 = dp[3 * 4 + j];
         d[0 * 4 + j] = d0 - (mu[0] * d1 + mu[1] * d2 + mu[2] * d3) / s;
         d[1 * 4 + j] = d1 / s;
         d[2 * 4 + j] = d2 / s;

@@ -189,92 +189,107 @@ struct FastGeom {
     int snx, sny, snz, sx0, sy0, sz0;   // source window
     uint32_t row_b, plane_b, chan_b;    // source strides in bytes
     float fmx, fmy, fmz;                // full extent - 1 (bounds test)
-    uint32_t dplane_b, dchan_b;         // destination strides in bytes
-    int nc;
+    uint32_t dchan_b;                   // destination channel stride in bytes
+    float wt[4 * 64];                   // FP32 B-spline weights TB[t][k], f <= 64
 };
 
-// One output voxel: bounds test, taps, blend, store.  MODE selects the
-// FP64 reference blend (TW_PLAIN) or the FP32 lerp blend (TW_MIXED).
-template <typename T, int MODE, int INTERP>
-__device__ __forceinline__ void resample_fast(const TpsArgs& a, const FastGeom& g, int X, int Y, int Z, float d0,
-                                              float d1, float d2, uint32_t obyte) {
-    const float p0 = (float)X + d0, p1 = (float)Y + d1, p2 = (float)Z + d2;
-    const bool inb = p0 >= 0.f && p0 <= g.fmx && p1 >= 0.f && p1 <= g.fmy && p2 >= 0.f && p2 <= g.fmz;
-    const float fx = floorf(p0), fy = floorf(p1), fz = floorf(p2);
-    const float wx = p0 - fx, wy = p1 - fy, wz = p2 - fz;
+// Tap geometry of one output voxel: byte offset of (x_s, y_s, z_s) in the
+// channel (0 when the sample is out of bounds), exact fractional parts and
+// the in-bounds flag.  Branch-free.
+struct Tap {
+    uint32_t off;
+    float wx, wy, wz;
+    bool ok;
+};
+
+template <typename T, int INTERP, bool WIN>
+__device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, float p2) {
+    Tap t;
+    const bool inb = (p0 >= 0.f) & (p0 <= g.fmx) & (p1 >= 0.f) & (p1 <= g.fmy) & (p2 >= 0.f) & (p2 <= g.fmz);
     int ix, iy, iz;
-    bool win;
     if (INTERP == TW_INTERP_NN) {
-        // (long)(pos + 0.5) clamped to the extent, in FP64 like the reference.
-        ix = (int)fmin((double)p0 + 0.5, (double)g.fmx) - g.sx0;
-        iy = (int)fmin((double)p1 + 0.5, (double)g.fmy) - g.sy0;
-        iz = (int)fmin((double)p2 + 0.5, (double)g.fmz) - g.sz0;
-        win = ix >= 0 && ix < g.snx && iy >= 0 && iy < g.sny && iz >= 0 && iz < g.snz;
+        // (long)(pos + 0.5), clamped to the extent; pos + 0.5 is exact in FP32
+        // for positions below 2^23.
+        ix = (int)fminf(floorf(p0 + 0.5f), g.fmx);
+        iy = (int)fminf(floorf(p1 + 0.5f), g.fmy);
+        iz = (int)fminf(floorf(p2 + 0.5f), g.fmz);
+        t.wx = t.wy = t.wz = 0.f;
     } else {
-        ix = (int)fx - g.sx0;
-        iy = (int)fy - g.sy0;
-        iz = (int)fz - g.sz0;
-        win = ix >= 0 && iy >= 0 && iz >= 0 && ix + (wx > 0.f) < g.snx && iy + (wy > 0.f) < g.sny &&
-              iz + (wz > 0.f) < g.snz;
+        // pos - floor(pos) is exact for the non-negative in-bounds positions.
+        ix = __float2int_rd(p0);
+        iy = __float2int_rd(p1);
+        iz = __float2int_rd(p2);
+        t.wx = __builtin_amdgcn_fractf(p0);
+        t.wy = __builtin_amdgcn_fractf(p1);
+        t.wz = __builtin_amdgcn_fractf(p2);
     }
-    const bool ok = inb && win;
-    const uint32_t off = ok ? ((uint32_t)iz * (uint32_t)g.sny + (uint32_t)iy) * (uint32_t)g.snx * (uint32_t)sizeof(T) +
-                                  (uint32_t)ix * (uint32_t)sizeof(T)
-                            : 0u;
-    const uint32_t dy = iy + 1 < g.sny ? g.row_b : 0u;
-    const uint32_t dz = iz + 1 < g.snz ? g.plane_b : 0u;
-    for (int c = 0; c < g.nc; c++) {
-        const __amdgpu_buffer_rsrc_t rs =
-            __builtin_amdgcn_make_buffer_rsrc((void*)((const uint8_t*)a.src + (size_t)c * g.chan_b), (short)0,
-                                              (int)g.chan_b, BUF_FLAGS);
-        const __amdgpu_buffer_rsrc_t rd =
-            __builtin_amdgcn_make_buffer_rsrc((void*)((uint8_t*)a.dst + (size_t)c * g.dchan_b), (short)0,
-                                              (int)g.dchan_b, BUF_FLAGS);
-        uint32_t v = 0;
-        if (ok) {
-            if (INTERP == TW_INTERP_NN) {
-                v = bld<T>(rs, off);
-            } else {
-                const uint32_t u000 = bld<T>(rs, off), u100 = bld<T>(rs, off + sizeof(T));
-                const uint32_t u010 = bld<T>(rs, off + dy), u110 = bld<T>(rs, off + dy + sizeof(T));
-                const uint32_t u001 = bld<T>(rs, off + dz), u101 = bld<T>(rs, off + dz + sizeof(T));
-                const uint32_t u011 = bld<T>(rs, off + dy + dz), u111 = bld<T>(rs, off + dy + dz + sizeof(T));
-                if (MODE == TW_PLAIN) {
-                    // pos, floor and 1 - frac are exact in FP64; blend in the
-                    // reference's order (q_imgwarp_tps_quicksmallmemory.cpp:141-160).
-                    const double P0 = p0, P1 = p1, P2 = p2;
-                    const double lw = 1.0 - (P0 - (double)fx), rw = 1.0 - lw;
-                    const double tw = 1.0 - (P1 - (double)fy), bw = 1.0 - tw;
-                    const double uw = 1.0 - (P2 - (double)fz), dw = 1.0 - uw;
-                    const double hi = tw * (lw * (double)u000 + rw * (double)u100) +
-                                      bw * (lw * (double)u010 + rw * (double)u110);
-                    const double lo = tw * (lw * (double)u001 + rw * (double)u101) +
-                                      bw * (lw * (double)u011 + rw * (double)u111);
-                    v = (uint32_t)(uw * hi + dw * lo + 0.5);
-                } else {
-                    const float v000 = (float)u000, v100 = (float)u100, v010 = (float)u010, v110 = (float)u110;
-                    const float v001 = (float)u001, v101 = (float)u101, v011 = (float)u011, v111 = (float)u111;
-                    const float a00 = __builtin_fmaf(wx, v100 - v000, v000);
-                    const float a10 = __builtin_fmaf(wx, v110 - v010, v010);
-                    const float a01 = __builtin_fmaf(wx, v101 - v001, v001);
-                    const float a11 = __builtin_fmaf(wx, v111 - v011, v011);
-                    const float b0 = __builtin_fmaf(wy, a10 - a00, a00);
-                    const float b1 = __builtin_fmaf(wy, a11 - a01, a01);
-                    v = (uint32_t)(__builtin_fmaf(wz, b1 - b0, b0) + 0.5f);
-                }
-            }
-        }
-        bst<T>(rd, obyte, v);
+    bool win = true;
+    if (WIN) {
+        ix -= g.sx0;
+        iy -= g.sy0;
+        iz -= g.sz0;
+        if (INTERP == TW_INTERP_NN)
+            win = (ix >= 0) & (iy >= 0) & (iz >= 0) & (ix < g.snx) & (iy < g.sny) & (iz < g.snz);
+        else
+            win = (ix >= 0) & (iy >= 0) & (iz >= 0) & (ix + (t.wx > 0.f) < g.snx) & (iy + (t.wy > 0.f) < g.sny) &
+                  (iz + (t.wz > 0.f) < g.snz);
     }
+    t.ok = inb & win;
+    const uint32_t e = __umul24(__umul24((uint32_t)iz, (uint32_t)g.sny) + (uint32_t)iy, (uint32_t)g.snx) + (uint32_t)ix;
+    t.off = t.ok ? e * (uint32_t)sizeof(T) : 0u;
+    return t;
 }
 
-template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF>
+// The eight taps of a voxel through one buffer resource.  The x+1 taps use
+// the instruction's immediate offset; y+1 / z+1 add a row / plane.  Taps of
+// zero weight may land in the next row or plane, or past the channel, where
+// the buffer range check returns 0.
+template <typename T, int MODE, int INTERP>
+__device__ __forceinline__ uint32_t blend(__amdgpu_buffer_rsrc_t rs, const Tap& t, uint32_t row, uint32_t plane) {
+    uint32_t off = t.off;
+    __builtin_assume(off <= 0x7FFF0000u);
+    if (INTERP == TW_INTERP_NN) {
+        const uint32_t v = bld<T>(rs, off);
+        return t.ok ? v : 0u;
+    }
+    const uint32_t o1 = off + row, o2 = off + plane, o3 = o1 + plane;
+    __builtin_assume(o1 <= 0x7FFF0000u);
+    __builtin_assume(o2 <= 0x7FFF0000u);
+    __builtin_assume(o3 <= 0x7FFF0000u);
+    const uint32_t u000 = bld<T>(rs, off), u100 = bld<T>(rs, off + (uint32_t)sizeof(T));
+    const uint32_t u010 = bld<T>(rs, o1), u110 = bld<T>(rs, o1 + (uint32_t)sizeof(T));
+    const uint32_t u001 = bld<T>(rs, o2), u101 = bld<T>(rs, o2 + (uint32_t)sizeof(T));
+    const uint32_t u011 = bld<T>(rs, o3), u111 = bld<T>(rs, o3 + (uint32_t)sizeof(T));
+    uint32_t v;
+    if (MODE == TW_PLAIN) {
+        // The reference's FP64 blend (q_imgwarp_tps_quicksmallmemory.cpp:141-160);
+        // 1 - (pos - floor(pos)) is exact, so the weights equal the reference's.
+        const double lw = 1.0 - (double)t.wx, rw = 1.0 - lw;
+        const double tw = 1.0 - (double)t.wy, bw = 1.0 - tw;
+        const double uw = 1.0 - (double)t.wz, dw = 1.0 - uw;
+        const double hi = tw * (lw * (double)u000 + rw * (double)u100) + bw * (lw * (double)u010 + rw * (double)u110);
+        const double lo = tw * (lw * (double)u001 + rw * (double)u101) + bw * (lw * (double)u011 + rw * (double)u111);
+        v = (uint32_t)(uw * hi + dw * lo + 0.5);
+    } else {
+        const float v000 = (float)u000, v100 = (float)u100, v010 = (float)u010, v110 = (float)u110;
+        const float v001 = (float)u001, v101 = (float)u101, v011 = (float)u011, v111 = (float)u111;
+        const float a00 = __builtin_fmaf(t.wx, v100 - v000, v000);
+        const float a10 = __builtin_fmaf(t.wx, v110 - v010, v010);
+        const float a01 = __builtin_fmaf(t.wx, v101 - v001, v001);
+        const float a11 = __builtin_fmaf(t.wx, v111 - v011, v011);
+        const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
+        const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
+        v = (uint32_t)(__builtin_fmaf(t.wz, b1 - b0, b0) + 0.5f);
+    }
+    return t.ok ? v : 0u;
+}
+
+template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF, bool WIN>
 __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGeom g) {
     extern __shared__ float lds[];
     const int f = a.f;
     const int NXM = node_span(TX, f), NYM = node_span(TY, f), NZM = node_span(a.tz, f);
-    float* wsh = lds;                        // f x 4 weights
-    float* nodes = wsh + 4 * f;              // [3][NZM][NYM][NXM]
+    float* nodes = lds;                      // [3][NZM][NYM][NXM]
     float* xc = nodes + 3 * NZM * NYM * NXM; // [3][NZM][NYM][TX]
 
     const int lxi = threadIdx.x, lyi = threadIdx.y, tid = lyi * TX + lxi;
@@ -288,7 +303,6 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
     const int nyn = (ry0 + ty1) / f + 4 - ny0;
     const int nzn = (rz0 + tz1) / f + 4 - nz0;
 
-    for (int i = tid; i < 4 * f; i += 256) wsh[i] = a.w32[i];
     const i64 plane = a.gx * a.gy * a.gz;
     const int NP = NZM * NYM * NXM;
     const int ntile = nxn * nyn * nzn;
@@ -317,7 +331,7 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
         const int bxl = rx / f - nx0;
         const int txp = rx - (rx / f) * f;
         const bool live = tx0 + lxi <= tx1;
-        const float w0 = wsh[txp * 4 + 0], w1 = wsh[txp * 4 + 1], w2 = wsh[txp * 4 + 2], w3 = wsh[txp * 4 + 3];
+        const float w0 = g.wt[txp * 4 + 0], w1 = g.wt[txp * 4 + 1], w2 = g.wt[txp * 4 + 2], w3 = g.wt[txp * 4 + 3];
         for (int r = lyi; r < nzn * nyn; r += TY) {
             const int iy = r % nyn, iz = r / nyn;
             #pragma unroll
@@ -335,11 +349,11 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
     if (tx0 + lxi > tx1 || ty0 + lyi > ty1) return;
 
     const int lx = tx0 + lxi, ly = ty0 + lyi;
-    const int X = (int)a.dx0 + lx, Y = (int)a.dy0 + ly;
+    const float Xf = (float)((int)a.dx0 + lx), Yf = (float)((int)a.dy0 + ly);
     const int ry = ry0 + ly;
     const int byl = ry / f - ny0;
     const int typ = ry - (ry / f) * f;
-    const float wy0 = wsh[typ * 4 + 0], wy1 = wsh[typ * 4 + 1], wy2 = wsh[typ * 4 + 2], wy3 = wsh[typ * 4 + 3];
+    const float wy0 = g.wt[typ * 4 + 0], wy1 = g.wt[typ * 4 + 1], wy2 = g.wt[typ * 4 + 2], wy3 = g.wt[typ * 4 + 3];
     const uint32_t dvol = (uint32_t)dnx * (uint32_t)dny * (uint32_t)dnz;
 
     auto layer = [&](int iz, float* L) {
@@ -358,38 +372,84 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
     float L0[3], L1[3], L2[3], L3[3];
     layer(bzl, L0); layer(bzl + 1, L1); layer(bzl + 2, L2); layer(bzl + 3, L3);
     const int nzv = tz1 - tz0 + 1;
-    uint32_t o = ((uint32_t)tz0 * (uint32_t)dny + (uint32_t)ly) * (uint32_t)dnx + (uint32_t)lx;
     const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
-    for (int lz = 0; lz < nzv; lz++) {
-        const int Z = (int)a.dz0 + tz0 + lz;
-        const float wz0 = wsh[tzp * 4 + 0], wz1 = wsh[tzp * 4 + 1], wz2 = wsh[tzp * 4 + 2], wz3 = wsh[tzp * 4 + 3];
-        float d[3];
+    uint32_t o = (uint32_t)tz0 * ostep + (uint32_t)ly * (uint32_t)dnx + (uint32_t)lx;
+    float Zf = (float)((int)a.dz0 + tz0);
+
+    auto df_ph = [&](float* d, int ph) {
+        const float wz0 = g.wt[ph * 4 + 0], wz1 = g.wt[ph * 4 + 1], wz2 = g.wt[ph * 4 + 2], wz3 = g.wt[ph * 4 + 3];
         #pragma unroll
         for (int c = 0; c < 3; c++)
             d[c] = __builtin_fmaf(wz3, L3[c], __builtin_fmaf(wz2, L2[c], __builtin_fmaf(wz1, L1[c], wz0 * L0[c])));
-        if (WRITE_DF) {
-            a.dfout[o] = d[0];
-            a.dfout[dvol + o] = d[1];
-            a.dfout[2 * dvol + o] = d[2];
-        } else {
-            resample_fast<T, MODE, INTERP>(a, g, X, Y, Z, d[0], d[1], d[2], o * (uint32_t)sizeof(T));
-        }
-        o += ostep;
+    };
+    auto df_at = [&](float* d) { df_ph(d, tzp); };
+    // Advance one slice; `more` says whether another slice follows (the new
+    // node layer is only read when it exists in the tile).
+    auto advance = [&](bool more) {
         if (++tzp == f) {
             tzp = 0;
             bzl++;
-            if (lz + 1 < nzv) {
+            if (more) {
                 #pragma unroll
                 for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
                 layer(bzl + 3, L3);
             }
         }
+    };
+
+    if (WRITE_DF) {
+        for (int lz = 0; lz < nzv; lz++) {
+            float d[3];
+            df_at(d);
+            a.dfout[o] = d[0];
+            a.dfout[dvol + o] = d[1];
+            a.dfout[2 * dvol + o] = d[2];
+            o += ostep;
+            advance(lz + 1 < nzv);
+        }
+        return;
+    }
+    // One channel per launch (the host offsets src / dst per channel).
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)a.src, (short)0, (int)g.chan_b, BUF_FLAGS);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(a.dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
+    // Cell by cell: the node-layer window moves once per cell; inside a
+    // cell two slices per iteration keep both voxels' 16 taps in flight.
+    int lz = 0;
+    while (true) {
+        const int cnt = min(f - tzp, nzv - lz);
+        for (int k = 0; k < cnt; k += 2) {
+            const bool hasB = k + 1 < cnt;
+            float dA[3], dB[3];
+            df_ph(dA, tzp);
+            df_ph(dB, hasB ? tzp + 1 : tzp);
+            tzp += 2;
+            const Tap tA = tap_setup<T, INTERP, WIN>(g, Xf + dA[0], Yf + dA[1], Zf + dA[2]);
+            const Tap tB = tap_setup<T, INTERP, WIN>(g, Xf + dB[0], Yf + dB[1], (Zf + 1.0f) + dB[2]);
+            const uint32_t vA = blend<T, MODE, INTERP>(rs, tA, g.row_b, g.plane_b);
+            const uint32_t vB = blend<T, MODE, INTERP>(rs, tB, g.row_b, g.plane_b);
+            bst<T>(rd, o * (uint32_t)sizeof(T), vA);
+            if (hasB) bst<T>(rd, (o + ostep) * (uint32_t)sizeof(T), vB);
+            o += 2 * ostep;
+            Zf += 2.0f;
+        }
+        // An odd count advanced one slice too far: step back.
+        if (cnt & 1) {
+            o -= ostep;
+            Zf -= 1.0f;
+        }
+        lz += cnt;
+        if (lz >= nzv) break;
+        tzp = 0;
+        bzl++;
+        #pragma unroll
+        for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
+        layer(bzl + 3, L3);
     }
 }
 
 size_t fused_lds_bytes(int f, int tz) {
     const int nxm = node_span(TX, f), nym = node_span(TY, f), nzm = node_span(tz, f);
-    return sizeof(float) * (size_t)(4 * f + 3 * nzm * nym * nxm + 3 * nzm * nym * TX);
+    return sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
 }
 
 // Voxels per work-group along z: 4 cells, clamped to [16, 64], halved
@@ -419,47 +479,69 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
         }
         return TW_OK;
     }
+    if (a.f > 64) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d above 64 is not supported", a.f);
     a.tz = pick_tz(a.f);
     const size_t lds = fused_lds_bytes(a.f, a.tz);
     if (lds > 160 * 1024) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d needs %zu B of LDS", a.f, lds);
     const i64 nbz = (a.dnz + a.tz - 1) / a.tz;
     if (nbz > 65535) return tw_fail(ctx, TW_E_UNSUPPORTED, "volume too deep for one launch");
-    // 32-bit addressing limits of the fused kernel.
-    const uint64_t lim = 0xFFFFFF00ull;
+    // 32-bit addressing limits of the fused kernel: byte offsets of every tap
+    // below 2 GiB (so the x+1 taps fold into the immediate offset), 24-bit
+    // row indices and coordinates.
     const uint64_t schan = (uint64_t)a.snx * a.sny * a.snz * sizeof(T);
     const uint64_t dchan = (uint64_t)a.dnx * a.dny * a.dnz * (WRITE_DF ? 3 * sizeof(float) : sizeof(T));
+    const uint64_t taps = schan + (uint64_t)a.snx * (a.sny + 1) * sizeof(T) + 16;  // offsets of every tap
     const i64 cmax = std::max({a.dx0 + a.dnx, a.dy0 + a.dny, a.dz0 + a.dnz, a.fnx, a.fny, a.fnz});
-    if ((!WRITE_DF && schan > lim) || dchan > lim || cmax > (1ll << 24))
+    if ((!WRITE_DF && (taps > 0x7FFF0000ull || a.sny * a.snz >= (1ll << 24))) || dchan > 0xF0000000ull ||
+        cmax > (1ll << 24))
         return tw_fail(ctx, TW_E_UNSUPPORTED,
                        "window exceeds the fused kernel's 32-bit addressing (split the volume into bricks)");
     FastGeom g;
+    memset(&g, 0, sizeof g);
     g.snx = (int)a.snx; g.sny = (int)a.sny; g.snz = (int)a.snz;
     g.sx0 = (int)a.sx0; g.sy0 = (int)a.sy0; g.sz0 = (int)a.sz0;
     g.row_b = (uint32_t)(a.snx * sizeof(T));
     g.plane_b = (uint32_t)(a.snx * a.sny * sizeof(T));
     g.chan_b = (uint32_t)schan;
     g.fmx = (float)(a.fnx - 1); g.fmy = (float)(a.fny - 1); g.fmz = (float)(a.fnz - 1);
-    g.dplane_b = (uint32_t)(a.dnx * a.dny * sizeof(T));
     g.dchan_b = (uint32_t)(a.dnx * a.dny * a.dnz * sizeof(T));
-    g.nc = (int)a.nc;
+    {
+        std::vector<double> tb((size_t)a.f * 4);
+        tw_bspline_tb(a.f, tb.data());
+        for (size_t i = 0; i < tb.size(); i++) g.wt[i] = (float)tb[i];
+    }
+    // The source window is the whole volume: no window test needed.
+    const bool win = !(a.sx0 == 0 && a.sy0 == 0 && a.sz0 == 0 && a.snx == a.fnx && a.sny == a.fny && a.snz == a.fnz);
     const dim3 grid((unsigned)((a.dnx + TX - 1) / TX), (unsigned)((a.dny + TY - 1) / TY), (unsigned)nbz);
-#define TW_FUSED(M, H, I) \
-    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF>), grid, block, lds, ctx->stream, a, g)
-    if (precision == TW_PLAIN) {
+    const void* src0 = a.src;
+    void* dst0 = a.dst;
+    for (int ch = 0; ch < (WRITE_DF ? 1 : (int)a.nc); ch++) {
+    a.src = src0 ? (const void*)((const uint8_t*)src0 + (size_t)ch * g.chan_b) : nullptr;
+    a.dst = dst0 ? (void*)((uint8_t*)dst0 + (size_t)ch * g.dchan_b) : nullptr;
+#define TW_FUSED(M, H, I, W) \
+    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF, W>), grid, block, lds, ctx->stream, a, g)
+    if (WRITE_DF) {
+        if (f16) TW_FUSED(TW_MIXED, true, TW_INTERP_LINEAR, false);
+        else TW_FUSED(TW_MIXED, false, TW_INTERP_LINEAR, false);
+    } else if (interp == TW_INTERP_NN) {
+        if (f16) TW_FUSED(TW_PLAIN, true, TW_INTERP_NN, true);
+        else TW_FUSED(TW_PLAIN, false, TW_INTERP_NN, true);
+    } else if (precision == TW_PLAIN) {
         if (f16) {
-            if (interp == TW_INTERP_NN) TW_FUSED(TW_PLAIN, true, TW_INTERP_NN); else TW_FUSED(TW_PLAIN, true, TW_INTERP_LINEAR);
+            if (win) TW_FUSED(TW_PLAIN, true, TW_INTERP_LINEAR, true); else TW_FUSED(TW_PLAIN, true, TW_INTERP_LINEAR, false);
         } else {
-            if (interp == TW_INTERP_NN) TW_FUSED(TW_PLAIN, false, TW_INTERP_NN); else TW_FUSED(TW_PLAIN, false, TW_INTERP_LINEAR);
+            if (win) TW_FUSED(TW_PLAIN, false, TW_INTERP_LINEAR, true); else TW_FUSED(TW_PLAIN, false, TW_INTERP_LINEAR, false);
         }
     } else {
         if (f16) {
-            if (interp == TW_INTERP_NN) TW_FUSED(TW_MIXED, true, TW_INTERP_NN); else TW_FUSED(TW_MIXED, true, TW_INTERP_LINEAR);
+            if (win) TW_FUSED(TW_MIXED, true, TW_INTERP_LINEAR, true); else TW_FUSED(TW_MIXED, true, TW_INTERP_LINEAR, false);
         } else {
-            if (interp == TW_INTERP_NN) TW_FUSED(TW_MIXED, false, TW_INTERP_NN); else TW_FUSED(TW_MIXED, false, TW_INTERP_LINEAR);
+            if (win) TW_FUSED(TW_MIXED, false, TW_INTERP_LINEAR, true); else TW_FUSED(TW_MIXED, false, TW_INTERP_LINEAR, false);
         }
     }
 #undef TW_FUSED
     TW_LAUNCH_CHECK(ctx);
+    }
     return TW_OK;
 }
 

@@ -11,17 +11,21 @@ from terawarp_amd import _lib as L
 from terawarp_amd import warp
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "terawarp.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("terawarp.h", "terawarp_host.h")]
 
 
-def declared_symbols():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(tw_[a-z0-9_]+)\s*\(", text)))
+def declared_symbols(path=None):
+    names = set()
+    for h in ([path] if path else HEADERS):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(tw_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_and_binding_agree():
-    assert declared_symbols() == sorted(L.EXPORTS)
+    assert declared_symbols(HEADERS[0]) == sorted(L.EXPORTS)
+    assert declared_symbols(HEADERS[1]) == sorted(L.HOST_EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():
