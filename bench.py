@@ -9,8 +9,10 @@ resident in HBM before the timed region.
 One step = one whole warp: TPS fit (GPU), grid displacement evaluation
 (GPU), fused upsample + resample (GPU).  Two precision forms run side by
 side:
-  plain : FP64 LU solve, FP32 grid field, FP64 blend (reference order)
-  mixed : FP32 LU + FP64 refinement, FP16 offset-scaled field, FP32 blend
+  plain : FP64 null-space Cholesky solve, FP32 grid field, FP64 blend
+          (reference order)
+  mixed : FP32 Cholesky + FP64 iterative refinement, FP16 offset-scaled
+          field, FP32 blend
 `value` is the mixed form's end-to-end Gvox/s; both forms, the per-stage
 kernel times (HIP events on the library's stream) and the displacement
 error against the FP64 oracle are in the same JSON line.

@@ -53,9 +53,9 @@ extern "C" {
  * TW_EXACT : FP64 throughout, in the reference's operation order (parity).
  * TW_PLAIN : FP64 TPS solve; FP32 grid field; FP32 B-spline upsample;
  *            FP64 trilinear in the reference's order.
- * TW_MIXED : FP32 LU (MFMA trailing updates) + FP64 iterative refinement;
- *            FP32 compensated grid field stored as offset-scaled FP16;
- *            FP32 upsample and trilinear. */
+ * TW_MIXED : FP32 Cholesky of the projected system + FP64 iterative
+ *            refinement; FP32 compensated grid field stored as
+ *            offset-scaled FP16; FP32 upsample and trilinear. */
 #define TW_EXACT  2
 #define TW_PLAIN  0
 #define TW_MIXED  1
@@ -154,8 +154,10 @@ int tw_resample_affine(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const 
  * kernel centres (target space; the file-target markers at the plugin call
  * site, plugin_Bigimagewarp.cpp:197), targets the subject-space points.
  * c: K x 4 row-major, d: 4 x 4 row-major, host memory (the reference's
- * xnx4_c, x4x4_d).  precision: TW_PLAIN/TW_EXACT (FP64 LU) or TW_MIXED
- * (FP32 LU + FP64 refinement). */
+ * xnx4_c, x4x4_d).  Computed by the reference's null-space method
+ * (Householder QR of X, Cholesky of q2'(K + lambda I)q2); precision:
+ * TW_PLAIN/TW_EXACT (FP64 factor) or TW_MIXED (FP32 factor + FP64
+ * iterative refinement, FP64 refactor if the refinement stalls). */
 int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda,
                int precision, double* c, double* d);
 

@@ -12,13 +12,17 @@
 //
 // TW_EXACT: FP64, the reference's summation order (affine part, then the
 //   landmark sum in ascending n, then their sum), one thread per node.
-// Otherwise FP32: the affine part is evaluated in FP64 as
-//   d0 + P (d_lin - I) so that the large identity term never enters FP32,
-//   and the landmark sum is FP32 with TwoSum error compensation.
+// Otherwise the distances are FP32 (float differences, FP32 squares and
+//   v_sqrt_f32) and the landmark sum is compensated by accumulating in FP64:
+//   on MI355X an FP64 FMA issues at the FP32 rate, so the FP64 accumulator
+//   (one cvt + three FMAs per landmark) is cheaper than TwoSum compensation
+//   (six FP32 ops per component) and rounds the products only once.  The
+//   affine part is evaluated in FP64 as d0 + P (d_lin - I) so that the large
+//   identity term never enters the field.
 // Landmarks are read with wave-uniform addresses (scalar loads).
 //
-// Roofline: compute-bound; ~28 VALU ops per (node, landmark) pair in FP32,
-// so nodes*K*28 / 3.9e13 lane-ops/s on MI355X.
+// Roofline: compute-bound; ~11 VALU issue slots per (node, landmark) pair
+// (the quarter-rate sqrt counts four), so nodes*K*11 / 3.9e13 per second.
 #include "tw_internal.h"
 
 #include <cmath>
@@ -29,21 +33,13 @@ namespace {
 struct GridArgs {
     const float4* pts;   // K centres (x, y, z, 0)
     const double* c64;   // K x 3 coefficients (cols 1..3), TW_EXACT
-    const float4* c32;   // K coefficients (c1, c2, c3, 0), FP32 path
+    const double* cf;    // K x 3 coefficients (cols 1..3), FP32-distance path
     int K;
     int f;
     i64 gx, gy, gz, ox, oy, oz;
     double d[16];        // affine block, row-major 4x4
     float* out;          // 3 planes gx*gy*gz
 };
-
-__device__ __forceinline__ void two_sum(float& s, float& comp, float y) {
-    const float t = s + y;
-    const float bp = t - s;
-    const float err = (s - (t - bp)) + (y - bp);
-    s = t;
-    comp += err;
-}
 
 template <bool EXACT>
 __global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
@@ -76,25 +72,24 @@ __global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
         a.out[plane + idx] = (float)((a2 + b2) - (double)Y);
         a.out[2 * plane + idx] = (float)((a3 + b3) - (double)Z);
     } else {
-        float s1 = 0.f, s2 = 0.f, s3 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+        double s1 = 0.0, s2 = 0.0, s3 = 0.0;
         for (int n = 0; n < a.K; n++) {
             const float4 p = a.pts[n];
-            const float4 c = a.c32[n];
             const float dx = Xf - p.x, dy = Yf - p.y, dz = Zf - p.z;
             const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            const float r = __builtin_amdgcn_sqrtf(r2);
-            two_sum(s1, e1, -r * c.x);
-            two_sum(s2, e2, -r * c.y);
-            two_sum(s3, e3, -r * c.z);
+            const double r = (double)__builtin_amdgcn_sqrtf(r2);
+            s1 = fma(-r, a.cf[3 * n + 0], s1);
+            s2 = fma(-r, a.cf[3 * n + 1], s2);
+            s3 = fma(-r, a.cf[3 * n + 2], s3);
         }
         // Affine part minus identity, in FP64.
         const double Xd = (double)X, Yd = (double)Y, Zd = (double)Z;
         const double l1 = a.d[1] + Xd * (a.d[5] - 1.0) + Yd * a.d[9] + Zd * a.d[13];
         const double l2 = a.d[2] + Xd * a.d[6] + Yd * (a.d[10] - 1.0) + Zd * a.d[14];
         const double l3 = a.d[3] + Xd * a.d[7] + Yd * a.d[11] + Zd * (a.d[15] - 1.0);
-        a.out[idx] = (float)(l1 + ((double)s1 + (double)e1));
-        a.out[plane + idx] = (float)(l2 + ((double)s2 + (double)e2));
-        a.out[2 * plane + idx] = (float)(l3 + ((double)s3 + (double)e3));
+        a.out[idx] = (float)(l1 + s1);
+        a.out[plane + idx] = (float)(l2 + s2);
+        a.out[2 * plane + idx] = (float)(l3 + s3);
     }
 }
 
@@ -208,28 +203,22 @@ extern "C" int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const 
 
     // Landmark packs: centres (float4), coefficients (FP64 x3 or float4).
     const size_t pts_bytes = (size_t)K * sizeof(float4);
-    const size_t coef_bytes = exact ? (size_t)K * 3 * sizeof(double) : (size_t)K * sizeof(float4);
+    const size_t coef_bytes = (size_t)K * 3 * sizeof(double);
     void* pack = nullptr;
     if ((rc = tw_ws(ctx, WS_PTS, pts_bytes + coef_bytes, &pack))) return rc;
     std::vector<uint8_t> host(pts_bytes + coef_bytes);
     float4* hp = (float4*)host.data();
     for (int n = 0; n < K; n++) hp[n] = make_float4(centres[3 * n], centres[3 * n + 1], centres[3 * n + 2], 0.f);
-    if (exact) {
-        double* hc = (double*)(host.data() + pts_bytes);
-        for (int n = 0; n < K; n++)
-            for (int q = 0; q < 3; q++) hc[3 * n + q] = c[4 * n + 1 + q];
-    } else {
-        float4* hc = (float4*)(host.data() + pts_bytes);
-        for (int n = 0; n < K; n++)
-            hc[n] = make_float4((float)c[4 * n + 1], (float)c[4 * n + 2], (float)c[4 * n + 3], 0.f);
-    }
+    double* hc = (double*)(host.data() + pts_bytes);
+    for (int n = 0; n < K; n++)
+        for (int q = 0; q < 3; q++) hc[3 * n + q] = c[4 * n + 1 + q];
     TW_HIP(ctx, hipMemcpyAsync(pack, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
 
     GridArgs a;
     memset(&a, 0, sizeof a);
     a.pts = (const float4*)pack;
-    a.c64 = exact ? (const double*)((uint8_t*)pack + pts_bytes) : nullptr;
-    a.c32 = exact ? nullptr : (const float4*)((uint8_t*)pack + pts_bytes);
+    a.c64 = (const double*)((uint8_t*)pack + pts_bytes);
+    a.cf = a.c64;
     a.K = K;
     a.f = grid->f;
     a.gx = grid->gx; a.gy = grid->gy; a.gz = grid->gz;

@@ -89,6 +89,27 @@ __device__ double block_sum(double v, double* red) {
     return t;
 }
 
+// Sums of NV values over a 1024-thread work-group with one barrier pair;
+// red must hold 16 * NV doubles.  Every thread gets the totals in v.
+template <int NV>
+__device__ void block_sum_n(double* v, double* red) {
+    #pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0)
+        #pragma unroll
+        for (int i = 0; i < NV; i++) red[w * NV + i] = v[i];
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    #pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double t = 0;
+        for (int k = 0; k < nw; k++) t += red[k * NV + i];
+        v[i] = t;
+    }
+}
+
 // Small per-fit state shared between kernels (device memory).
 struct FitState {
     double tau[4];
@@ -203,16 +224,18 @@ __global__ __launch_bounds__(256) void k_av(const SysDesc s, const double* __res
 // S = V'P, M = T' S T, W = P T - 1/2 V M.  One work-group.
 __global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, const double* __restrict__ P,
                                                   double* __restrict__ W, const FitState* st, int N) {
-    __shared__ double red[16];
+    __shared__ double red[16 * 16];
     __shared__ double M[16], Ts[16];
     const int tid = threadIdx.x;
     double S[16];
-    for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 4; b++) {
-            double p = 0;
-            for (int i = tid; i < N; i += 1024) p += V[(size_t)a * N + i] * P[(size_t)b * N + i];
-            S[a * 4 + b] = block_sum(p, red);
-        }
+    for (int i = 0; i < 16; i++) S[i] = 0;
+    for (int i = tid; i < N; i += 1024) {
+        double v[4], p[4];
+        for (int k = 0; k < 4; k++) { v[k] = V[(size_t)k * N + i]; p[k] = P[(size_t)k * N + i]; }
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) S[a * 4 + b] += v[a] * p[b];
+    }
+    block_sum_n<16>(S, red);
     if (tid == 0) {
         double TS[16];
         for (int i = 0; i < 16; i++) Ts[i] = st->T[i];
@@ -260,39 +283,47 @@ __global__ __launch_bounds__(256) void k_build_b(const SysDesc s, const double* 
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, int n, int j0, int jb, FitState* st) {
+    // Right-looking on the unscaled factor (L D L' form): step k subtracts
+    // L[r][k] L[c][k] / d_k from the trailing block; the columns are scaled
+    // by 1/sqrt(d_k) at the end.  Four threads per row, one barrier per step.
     __shared__ T L[NB][NB + 1];
-    const int tid = threadIdx.x;
+    __shared__ T dk[NB];
+    const int tid = threadIdx.x, r = tid >> 2, q = tid & 3;
     for (int e = tid; e < jb * jb; e += 256) {
-        const int r = e % jb, c = e / jb;
-        L[r][c] = B[(i64)(j0 + c) * n + j0 + r];
+        const int rr = e % jb, c = e / jb;
+        L[rr][c] = B[(i64)(j0 + c) * n + j0 + rr];
     }
     __syncthreads();
     for (int k = 0; k < jb; k++) {
         const T d = L[k][k];
-        const T sd = sqrt(d > (T)0 ? d : (T)1);
-        if (tid == 0 && !(d > (T)0) && st->info == 0) st->info = 2 + j0 + k;
-        __syncthreads();
-        for (int r = k + 1 + tid; r < jb; r += 256) L[r][k] = L[r][k] / sd;
-        __syncthreads();
-        if (tid == 0) L[k][k] = sd;
-        const int rem = jb - k - 1;
-        for (int e = tid; e < rem * rem; e += 256) {
-            const int r = k + 1 + e % rem, c = k + 1 + e / rem;
-            if (c <= r) L[r][c] -= L[r][k] * L[c][k];
+        if (tid == 0) {
+            dk[k] = d;
+            if (!(d > (T)0) && st->info == 0) st->info = 2 + j0 + k;
+        }
+        const T inv = (T)1 / (d > (T)0 ? d : (T)1);
+        if (r > k && r < jb) {
+            const T lrk = L[r][k] * inv;
+            for (int c = k + 1 + q; c <= r; c += 4) L[r][c] -= lrk * L[c][k];
         }
         __syncthreads();
     }
     for (int e = tid; e < jb * jb; e += 256) {
-        const int r = e % jb, c = e / jb;
-        if (c <= r) B[(i64)(j0 + c) * n + j0 + r] = L[r][c];
+        const int rr = e % jb, c = e / jb;
+        if (c <= rr) {
+            const T dc = dk[c] > (T)0 ? dk[c] : (T)1;
+            B[(i64)(j0 + c) * n + j0 + rr] = L[rr][c] / sqrt(dc);
+        }
     }
 }
 
 // L21 = B21 L11^-T for 64 rows per work-group below the diagonal block.
+// Each row is solved by the four lanes of one wave that own it, so the
+// substitution needs no barrier.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, int n, int j0, int jb) {
     __shared__ T L[NB][NB + 1];
     __shared__ T X[NB][NB + 1];  // X[row][k]
+    __shared__ T rdiag[NB];
     const int tid = threadIdx.x;
     const int r0 = j0 + jb + blockIdx.x * NB;
     const int nr = min(NB, n - r0);
@@ -305,19 +336,18 @@ __global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, int n, int
         X[r][c] = r < nr ? B[(i64)(j0 + c) * n + r0 + r] : (T)0;
     }
     __syncthreads();
+    if (tid < jb) rdiag[tid] = (T)1 / L[tid][tid];
+    __syncthreads();
+    const int r = tid >> 2, q = tid & 3;
     for (int k = 0; k < jb; k++) {
-        if (tid < NB) X[tid][k] = X[tid][k] / L[k][k];
-        __syncthreads();
-        const int rem = jb - k - 1;
-        for (int e = tid; e < NB * rem; e += 256) {
-            const int r = e % NB, c = k + 1 + e / NB;
-            X[r][c] -= X[r][k] * L[c][k];
-        }
-        __syncthreads();
+        const T xk = X[r][k] * rdiag[k];
+        for (int c = k + 1 + q; c < jb; c += 4) X[r][c] -= xk * L[c][k];
+        if (q == (k & 3)) X[r][k] = xk;
     }
+    __syncthreads();
     for (int e = tid; e < NB * jb; e += 256) {
-        const int r = e % NB, c = e / NB;
-        if (r < nr) B[(i64)(j0 + c) * n + r0 + r] = X[r][c];
+        const int rr = e % NB, c = e / NB;
+        if (rr < nr) B[(i64)(j0 + c) * n + r0 + rr] = X[rr][c];
     }
 }
 
@@ -366,23 +396,33 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
 }
 
 // Solve L L' x = b for 4 right-hand sides (n x 4 column-major, in place).
-// One work-group; 64-row blocks, each solved by wave 0 with shuffles, the
-// rest of the right-hand side updated by all threads.
+// One work-group; per 64-row block the diagonal block and its reciprocal
+// diagonal are staged in LDS, wave 0 substitutes with shuffles, and all
+// threads update the remaining right-hand side rows.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, int n, T* __restrict__ b) {
+    __shared__ T Ld[NB][NB + 1];
+    __shared__ T rd[NB];
     __shared__ T xb[4][NB];
     const int tid = threadIdx.x, lane = tid & 63;
     // forward: L y = b
     for (int i0 = 0; i0 < n; i0 += NB) {
         const int ib = min(NB, n - i0);
+        for (int e = tid; e < ib * ib; e += 1024) {
+            const int r = e % ib, c = e / ib;
+            Ld[r][c] = L[(i64)(i0 + c) * n + i0 + r];
+        }
+        __syncthreads();
+        if (tid < ib) rd[tid] = (T)1 / Ld[tid][tid];
+        __syncthreads();
         if (tid < 64) {
             T v[4];
             for (int j = 0; j < 4; j++) v[j] = lane < ib ? b[(i64)j * n + i0 + lane] : (T)0;
             for (int k = 0; k < ib; k++) {
-                const T lrk = lane < ib ? L[(i64)(i0 + k) * n + i0 + lane] : (T)0;
-                const T lkk = __shfl(lrk, k, 64);
+                const T lrk = lane < ib ? Ld[lane][k] : (T)0;
+                const T rk = rd[k];
                 for (int j = 0; j < 4; j++) {
-                    T xk = __shfl(v[j], k, 64) / lkk;
+                    const T xk = __shfl(v[j], k, 64) * rk;
                     if (lane == k) v[j] = xk;
                     else if (lane > k) v[j] -= lrk * xk;
                 }
@@ -406,15 +446,22 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, in
     const int last = ((n - 1) / NB) * NB;
     for (int i0 = last; i0 >= 0; i0 -= NB) {
         const int ib = min(NB, n - i0);
+        for (int e = tid; e < ib * ib; e += 1024) {
+            const int r = e % ib, c = e / ib;
+            Ld[r][c] = L[(i64)(i0 + c) * n + i0 + r];
+        }
+        __syncthreads();
+        if (tid < ib) rd[tid] = (T)1 / Ld[tid][tid];
+        __syncthreads();
         if (tid < 64) {
             T v[4];
             for (int j = 0; j < 4; j++) v[j] = lane < ib ? b[(i64)j * n + i0 + lane] : (T)0;
-            // Row `lane` of L' within the block is column `lane` of L: L[(i0+lane)..][i0+k] for k > lane.
+            // Row `lane` of L' is column `lane` of L: L[k][lane] for k > lane.
             for (int k = ib - 1; k >= 0; k--) {
-                const T lkr = lane < ib ? L[(i64)(i0 + lane) * n + i0 + k] : (T)0;  // L[k][lane]
-                const T lkk = __shfl(lkr, k, 64);
+                const T lkr = lane < ib ? Ld[k][lane] : (T)0;
+                const T rk = rd[k];
                 for (int j = 0; j < 4; j++) {
-                    T xk = __shfl(v[j], k, 64) / lkk;
+                    const T xk = __shfl(v[j], k, 64) * rk;
                     if (lane == k) v[j] = xk;
                     else if (lane < k) v[j] -= lkr * xk;
                 }
@@ -439,16 +486,18 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, in
 // U (N x 4, column-major) <- Q' U (trans = 1) or Q U (trans = 0), Q = I - V T V'.
 __global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, const FitState* st,
                                                   double* __restrict__ U, int N, int trans) {
-    __shared__ double red[16];
+    __shared__ double red[16 * 16];
     __shared__ double H[16];
     const int tid = threadIdx.x;
     double G[16];
-    for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 4; b++) {
-            double p = 0;
-            for (int i = tid; i < N; i += 1024) p += V[(size_t)a * N + i] * U[(size_t)b * N + i];
-            G[a * 4 + b] = block_sum(p, red);
-        }
+    for (int i = 0; i < 16; i++) G[i] = 0;
+    for (int i = tid; i < N; i += 1024) {
+        double v[4], u[4];
+        for (int k = 0; k < 4; k++) { v[k] = V[(size_t)k * N + i]; u[k] = U[(size_t)k * N + i]; }
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) G[a * 4 + b] += v[a] * u[b];
+    }
+    block_sum_n<16>(G, red);
     if (tid == 0)
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++) {
@@ -522,20 +571,26 @@ __global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__
                                                       const double* __restrict__ U, const double* __restrict__ negAu,
                                                       const double* __restrict__ b2, float* __restrict__ r32,
                                                       FitState* st, int N) {
-    __shared__ double red[16];
+    __shared__ double red[16 * 32];
     __shared__ double G1[16], G2[16];
     const int tid = threadIdx.x, n = N - 4;
-    for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 4; b++) {
-            double p1 = 0, p2 = 0;
-            for (int i = tid; i < N; i += 1024) {
-                p1 += V[(size_t)a * N + i] * U[(size_t)b * N + i];
-                p2 += W[(size_t)a * N + i] * U[(size_t)b * N + i];
-            }
-            p1 = block_sum(p1, red);
-            p2 = block_sum(p2, red);
-            if (tid == 0) { G1[a * 4 + b] = p1; G2[a * 4 + b] = p2; }
+    double G[32];
+    for (int i = 0; i < 32; i++) G[i] = 0;
+    for (int i = tid; i < N; i += 1024) {
+        double v[4], w[4], u[4];
+        for (int k = 0; k < 4; k++) {
+            v[k] = V[(size_t)k * N + i];
+            w[k] = W[(size_t)k * N + i];
+            u[k] = U[(size_t)k * N + i];
         }
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                G[a * 4 + b] += v[a] * u[b];
+                G[16 + a * 4 + b] += w[a] * u[b];
+            }
+    }
+    block_sum_n<32>(G, red);
+    if (tid < 16) { G1[tid] = G[tid]; G2[tid] = G[16 + tid]; }
     __syncthreads();
     double rmax = 0, bmax = 0;
     for (int i = tid; i < n; i += 1024)

@@ -190,6 +190,7 @@ struct FastGeom {
     uint32_t row_b, plane_b, chan_b;    // source strides in bytes
     float fmx, fmy, fmz;                // full extent - 1 (bounds test)
     uint32_t dchan_b;                   // destination channel stride in bytes
+    int tiles_x, tiles_y, ntiles, per_xcd;  // tile grid and XCD-major order
     float wt[4 * 64];                   // FP32 B-spline weights TB[t][k], f <= 64
 };
 
@@ -294,7 +295,14 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
 
     const int lxi = threadIdx.x, lyi = threadIdx.y, tid = lyi * TX + lxi;
     const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
-    const int tx0 = blockIdx.x * TX, ty0 = blockIdx.y * TY, tz0 = blockIdx.z * a.tz;
+    // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD
+    // a contiguous run of tiles (x fastest, then y, then z) so that tiles
+    // sharing source rows share that XCD's L2.
+    const int wg = blockIdx.x;
+    const int tile = (wg & 7) * g.per_xcd + (wg >> 3);
+    if (tile >= g.ntiles) return;
+    const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
+    const int tx0 = tix * TX, ty0 = tiy * TY, tz0 = tiz * a.tz;
     const int tx1 = min(tx0 + TX, dnx) - 1, ty1 = min(ty0 + TY, dny) - 1, tz1 = min(tz0 + a.tz, dnz) - 1;
     // Window-relative coordinates of the grid (r = global - origin >= 0).
     const int rx0 = (int)(a.dx0 - a.ox), ry0 = (int)(a.dy0 - a.oy), rz0 = (int)(a.dz0 - a.oz);
@@ -484,7 +492,6 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     const size_t lds = fused_lds_bytes(a.f, a.tz);
     if (lds > 160 * 1024) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d needs %zu B of LDS", a.f, lds);
     const i64 nbz = (a.dnz + a.tz - 1) / a.tz;
-    if (nbz > 65535) return tw_fail(ctx, TW_E_UNSUPPORTED, "volume too deep for one launch");
     // 32-bit addressing limits of the fused kernel: byte offsets of every tap
     // below 2 GiB (so the x+1 taps fold into the immediate offset), 24-bit
     // row indices and coordinates.
@@ -512,7 +519,13 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     }
     // The source window is the whole volume: no window test needed.
     const bool win = !(a.sx0 == 0 && a.sy0 == 0 && a.sz0 == 0 && a.snx == a.fnx && a.sny == a.fny && a.snz == a.fnz);
-    const dim3 grid((unsigned)((a.dnx + TX - 1) / TX), (unsigned)((a.dny + TY - 1) / TY), (unsigned)nbz);
+    g.tiles_x = (int)((a.dnx + TX - 1) / TX);
+    g.tiles_y = (int)((a.dny + TY - 1) / TY);
+    const i64 ntiles = (i64)g.tiles_x * g.tiles_y * nbz;
+    if (ntiles > (1ll << 30)) return tw_fail(ctx, TW_E_UNSUPPORTED, "too many tiles for one launch");
+    g.ntiles = (int)ntiles;
+    g.per_xcd = (int)((ntiles + 7) / 8);
+    const dim3 grid((unsigned)(8 * g.per_xcd), 1, 1);
     const void* src0 = a.src;
     void* dst0 = a.dst;
     for (int ch = 0; ch < (WRITE_DF ? 1 : (int)a.nc); ch++) {
