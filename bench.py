@@ -134,6 +134,50 @@ def oracle_slab_resample(oracle, vol, df_slab, zb0, zb1, threads):
     return oracle.tps_resample(vol, full, G, zblocks=(zb0, zb1), nthreads=threads)
 
 
+MIXED_KERNEL = "k_tps_fused<unsigned short, 1, true, 0, false, false>"
+
+
+def measure_traffic(vol):
+    """HBM bytes per launch of the mixed resample kernel from two rocprofv3
+    counter passes of this script (--profile), FETCH_SIZE then WRITE_SIZE, each
+    scaled by what the same counter reads for the 1 GiB device copy that the
+    profile run performs after the warps."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    rocprof = shutil.which("rocprofv3")
+    if not rocprof:
+        return None
+    tmp = tempfile.mkdtemp(prefix="tw_pmc_")
+    vpath = os.path.join(tmp, "vol.npy")
+    np.save(vpath, vol)
+    out = {}
+    try:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, ctr)
+            cmd = [rocprof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--profile", "--steps", "1", "--warmup", "0",
+                   "--volume", vpath]
+            log("counter pass", ctr)
+            subprocess.run(cmd, check=True, timeout=240, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           cwd=tmp, env=dict(os.environ, TMPDIR=tmp))
+            path = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")][0]
+            rows = list(csv.DictReader(open(path)))
+            fused = [r for r in rows if MIXED_KERNEL in r["Kernel_Name"]]
+            last = max(int(r["Dispatch_Id"]) for r in fused)
+            copy = [r for r in rows if "copyBuffer" in r["Kernel_Name"] and int(r["Dispatch_Id"]) > last]
+            kib = float(np.mean([float(r["Counter_Value"]) for r in fused]))
+            copy_kib = sum(float(r["Counter_Value"]) for r in copy)
+            scale = (1 << 30) / (copy_kib * 1024.0)
+            out[ctr] = {"kernel_kib": kib, "copy_kib": copy_kib, "scale": scale, "bytes": kib * 1024.0 * scale}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    total = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return {"bytes_per_launch": total, "counters": out, "kernel": MIXED_KERNEL,
+            "calibration": "1 GiB device copy in the same pass"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +187,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: the two forms' steps and a 1 GiB device copy (counter calibration) only")
+    ap.add_argument("--volume", default=None, help="load the workload volume from this .npy instead of generating it")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes for roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -153,8 +199,11 @@ def main():
     from terawarp_amd import synth, warp
     from terawarp_amd.warp import Warper
 
-    log("generating workload", SIZE, "uint16, K =", K)
-    vol = synth.blob_volume(SIZE[::-1], np.uint16, n_blobs=200, amp=(100, 4000), sigma=(2, 8), seed=SEED)
+    if args.volume:
+        vol = np.load(args.volume)
+    else:
+        log("generating workload", SIZE, "uint16, K =", K)
+        vol = synth.blob_volume(SIZE[::-1], np.uint16, n_blobs=200, amp=(100, 4000), sigma=(2, 8), seed=SEED)
     sub, tar = synth.landmarks(SIZE, K, DMAX, seed=SEED)
     src = torch.from_numpy(vol.view(np.int16)).cuda()
     dst = torch.empty_like(src)
@@ -219,6 +268,13 @@ def main():
         log("oracle leg failed:", repr(e))
         acc["error"] = repr(e)
 
+    traffic = None
+    if not args.no_traffic:
+        try:
+            traffic = measure_traffic(vol)
+        except Exception as e:  # counters are optional evidence, never fatal
+            log("traffic passes failed:", repr(e))
+
     line = {
         "metric": "TPS warp Gvox/s + resample % HBM peak; mixed vs plain at <=1e-2 vox err",
         "value": mixed["gvox_s"],
@@ -236,13 +292,15 @@ def main():
                                "B-spline upsample + trilinear",
                    "size_xyz": list(SIZE), "landmarks": K, "grid_spacing": G, "forms": "mixed (value) and plain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": "k_tps_fused<uint16, MIXED, F16, LINEAR>", "b_alg_per_voxel": b_mixed},
         "cpu_baseline": cpu,
         "plain": plain,
         "mixed": mixed,
         "mixed_over_plain_end_to_end": mixed["gvox_s"] / plain["gvox_s"],
         "accuracy": acc,
+        "traffic_detail": traffic,
     }
     print(json.dumps(line))
     w.close()

@@ -89,6 +89,17 @@ __device__ double block_sum(double v, double* red) {
     return t;
 }
 
+// Value of lane `lane` (wave-uniform) in every lane, through an SGPR.
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double bcast(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Sums of NV values over a 1024-thread work-group with one barrier pair;
 // red must hold 16 * NV doubles.  Every thread gets the totals in v.
 template <int NV>
@@ -282,32 +293,34 @@ __global__ __launch_bounds__(256) void k_build_b(const SysDesc s, const double* 
 // ---- blocked Cholesky (lower), column-major n x n -----------------------
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, int n, int j0, int jb, FitState* st) {
-    // Right-looking on the unscaled factor (L D L' form): step k subtracts
-    // L[r][k] L[c][k] / d_k from the trailing block; the columns are scaled
-    // by 1/sqrt(d_k) at the end.  Four threads per row, one barrier per step.
+__global__ __launch_bounds__(64) void k_potrf_diag(T* __restrict__ B, int n, int j0, int jb, FitState* st) {
+    // One wave; lane r owns row r.  Right-looking on the unscaled factor
+    // (L D L' form): step k subtracts L[r][k] L[c][k] / d_k from row r; the
+    // columns are scaled by 1/sqrt(d_k) at the end.  The wave's LDS accesses
+    // execute in order, so the steps need no barrier.
     __shared__ T L[NB][NB + 1];
     __shared__ T dk[NB];
-    const int tid = threadIdx.x, r = tid >> 2, q = tid & 3;
-    for (int e = tid; e < jb * jb; e += 256) {
+    const int r = threadIdx.x;
+    for (int e = r; e < jb * jb; e += 64) {
         const int rr = e % jb, c = e / jb;
         L[rr][c] = B[(i64)(j0 + c) * n + j0 + rr];
     }
     __syncthreads();
     for (int k = 0; k < jb; k++) {
         const T d = L[k][k];
-        if (tid == 0) {
+        if (r == 0) {
             dk[k] = d;
             if (!(d > (T)0) && st->info == 0) st->info = 2 + j0 + k;
         }
         const T inv = (T)1 / (d > (T)0 ? d : (T)1);
         if (r > k && r < jb) {
             const T lrk = L[r][k] * inv;
-            for (int c = k + 1 + q; c <= r; c += 4) L[r][c] -= lrk * L[c][k];
+            for (int c = k + 1; c <= r; c++) L[r][c] -= lrk * L[c][k];
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
     }
-    for (int e = tid; e < jb * jb; e += 256) {
+    __syncthreads();
+    for (int e = r; e < jb * jb; e += 64) {
         const int rr = e % jb, c = e / jb;
         if (c <= rr) {
             const T dc = dk[c] > (T)0 ? dk[c] : (T)1;
@@ -422,7 +435,7 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, in
                 const T lrk = lane < ib ? Ld[lane][k] : (T)0;
                 const T rk = rd[k];
                 for (int j = 0; j < 4; j++) {
-                    const T xk = __shfl(v[j], k, 64) * rk;
+                    const T xk = bcast(v[j], k) * rk;
                     if (lane == k) v[j] = xk;
                     else if (lane > k) v[j] -= lrk * xk;
                 }
@@ -461,7 +474,7 @@ __global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, in
                 const T lkr = lane < ib ? Ld[k][lane] : (T)0;
                 const T rk = rd[k];
                 for (int j = 0; j < 4; j++) {
-                    const T xk = __shfl(v[j], k, 64) * rk;
+                    const T xk = bcast(v[j], k) * rk;
                     if (lane == k) v[j] = xk;
                     else if (lane < k) v[j] -= lkr * xk;
                 }
@@ -665,7 +678,7 @@ template <typename T>
 void cholesky(tw_ctx* ctx, T* B, int n, FitState* st) {
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int jb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, n, j0, jb, st);
+        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(64), 0, ctx->stream, B, n, j0, jb, st);
         const int rest = n - j0 - jb;
         if (rest > 0) {
             hipLaunchKernelGGL(k_trsm_chol<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, B, n, j0, jb);

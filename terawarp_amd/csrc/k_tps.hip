@@ -191,6 +191,7 @@ struct FastGeom {
     float fmx, fmy, fmz;                // full extent - 1 (bounds test)
     uint32_t dchan_b;                   // destination channel stride in bytes
     int tiles_x, tiles_y, ntiles, per_xcd;  // tile grid and XCD-major order
+    int nxm, nym, nzm;                  // node-tile extents (maxima over the launch's tiles)
     float wt[4 * 64];                   // FP32 B-spline weights TB[t][k], f <= 64
 };
 
@@ -202,6 +203,13 @@ struct Tap {
     float wx, wy, wz;
     bool ok;
 };
+
+// floor(x) as int in one instruction (v_cvt_flr_i32_f32).
+__device__ __forceinline__ int cvt_flr(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 
 template <typename T, int INTERP, bool WIN>
 __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, float p2) {
@@ -217,9 +225,9 @@ __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, 
         t.wx = t.wy = t.wz = 0.f;
     } else {
         // pos - floor(pos) is exact for the non-negative in-bounds positions.
-        ix = __float2int_rd(p0);
-        iy = __float2int_rd(p1);
-        iz = __float2int_rd(p2);
+        ix = cvt_flr(p0);
+        iy = cvt_flr(p1);
+        iz = cvt_flr(p2);
         t.wx = __builtin_amdgcn_fractf(p0);
         t.wy = __builtin_amdgcn_fractf(p1);
         t.wz = __builtin_amdgcn_fractf(p2);
@@ -289,7 +297,7 @@ template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF, bool WIN>
 __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGeom g) {
     extern __shared__ float lds[];
     const int f = a.f;
-    const int NXM = node_span(TX, f), NYM = node_span(TY, f), NZM = node_span(a.tz, f);
+    const int NXM = g.nxm, NYM = g.nym, NZM = g.nzm;
     float* nodes = lds;                      // [3][NZM][NYM][NXM]
     float* xc = nodes + 3 * NZM * NYM * NXM; // [3][NZM][NYM][TX]
 
@@ -460,6 +468,25 @@ size_t fused_lds_bytes(int f, int tz) {
     return sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
 }
 
+// Largest node count a tile of `len` voxels spans along one axis over the
+// tiles [t*len, min((t+1)*len, n)) of a window starting at grid offset r0.
+int max_node_span(i64 r0, int len, i64 n, int f) {
+    int best = 0;
+    const i64 ntile = (n + len - 1) / len;
+    // The span depends on the tile start modulo f and on the clipped last tile.
+    const i64 period = f / std::__gcd(f, len);
+    for (i64 t = 0; t < ntile && t < period; t++) {
+        const i64 a = r0 + t * len, b = r0 + std::min<i64>((t + 1) * len, n) - 1;
+        best = std::max(best, (int)(b / f - a / f + 4));
+    }
+    {
+        const i64 t = ntile - 1;
+        const i64 a = r0 + t * len, b = r0 + n - 1;
+        best = std::max(best, (int)(b / f - a / f + 4));
+    }
+    return best;
+}
+
 // Voxels per work-group along z: 4 cells, clamped to [16, 64], halved
 // until the LDS tile fits in 64 KiB.
 int pick_tz(int f) {
@@ -489,7 +516,12 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     }
     if (a.f > 64) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d above 64 is not supported", a.f);
     a.tz = pick_tz(a.f);
-    const size_t lds = fused_lds_bytes(a.f, a.tz);
+    // Exact node-tile extents of this launch (tiles aligned to the grid need
+    // fewer nodes than the worst case), which sets the LDS per work-group.
+    const int nxm = max_node_span(a.dx0 - a.ox, TX, a.dnx, a.f);
+    const int nym = max_node_span(a.dy0 - a.oy, TY, a.dny, a.f);
+    const int nzm = max_node_span(a.dz0 - a.oz, a.tz, a.dnz, a.f);
+    const size_t lds = sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
     if (lds > 160 * 1024) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d needs %zu B of LDS", a.f, lds);
     const i64 nbz = (a.dnz + a.tz - 1) / a.tz;
     // 32-bit addressing limits of the fused kernel: byte offsets of every tap
@@ -505,6 +537,7 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
                        "window exceeds the fused kernel's 32-bit addressing (split the volume into bricks)");
     FastGeom g;
     memset(&g, 0, sizeof g);
+    g.nxm = nxm; g.nym = nym; g.nzm = nzm;
     g.snx = (int)a.snx; g.sny = (int)a.sny; g.snz = (int)a.snz;
     g.sx0 = (int)a.sx0; g.sy0 = (int)a.sy0; g.sz0 = (int)a.sz0;
     g.row_b = (uint32_t)(a.snx * sizeof(T));
