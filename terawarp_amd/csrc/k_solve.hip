@@ -100,25 +100,27 @@ __device__ __forceinline__ double bcast(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sums of NV values over a 1024-thread work-group with one barrier pair;
-// red must hold 16 * NV doubles.  Every thread gets the totals in v.
+// Sums of NV values over a 1024-thread work-group; red must hold 17 * NV
+// doubles.  Every thread gets the totals in v.
 template <int NV>
 __device__ void block_sum_n(double* v, double* red) {
     #pragma unroll
     for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int nw = (blockDim.x + 63) >> 6;
     __syncthreads();
     if (l == 0)
         #pragma unroll
         for (int i = 0; i < NV; i++) red[w * NV + i] = v[i];
     __syncthreads();
-    const int nw = (blockDim.x + 63) >> 6;
-    #pragma unroll
-    for (int i = 0; i < NV; i++) {
+    if (threadIdx.x < NV) {
         double t = 0;
-        for (int k = 0; k < nw; k++) t += red[k * NV + i];
-        v[i] = t;
+        for (int k = 0; k < nw; k++) t += red[k * NV + threadIdx.x];
+        red[16 * NV + threadIdx.x] = t;
     }
+    __syncthreads();
+    #pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = red[16 * NV + i];
 }
 
 // Small per-fit state shared between kernels (device memory).
@@ -127,8 +129,12 @@ struct FitState {
     double R[16];   // row-major upper triangle of X' = Q [R; 0]
     double T[16];   // row-major compact-WY factor, Q = I - V T V'
     int info;       // 1: rank-deficient X; 2 + k: non-positive pivot at column k
-    int pad;
+    int ir_stop;    // refinement: 0 running, 1 converged, 2 stalled
     double rnorm, bnorm;
+    double ir_prev; // last relative residual
+    int ir_iters;   // corrections applied
+    int pad;
+    double fit_rmax;  // max |residual| of the bordered system
 };
 
 // Householder QR of X' (N x 4) and the compact-WY factor T.  One work-group.
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(256) void k_av(const SysDesc s, const double* __res
 // S = V'P, M = T' S T, W = P T - 1/2 V M.  One work-group.
 __global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, const double* __restrict__ P,
                                                   double* __restrict__ W, const FitState* st, int N) {
-    __shared__ double red[16 * 16];
+    __shared__ double red[17 * 16];
     __shared__ double M[16], Ts[16];
     const int tid = threadIdx.x;
     double S[16];
@@ -291,77 +297,129 @@ __global__ __launch_bounds__(256) void k_build_b(const SysDesc s, const double* 
 }
 
 // ---- blocked Cholesky (lower), column-major n x n -----------------------
+// Each 64 x 64 diagonal block is factored and inverted in registers; Dinv
+// holds, per block b, X = L_bb^-1 as a row-major 64 x 64 tile (rows and
+// columns past the matrix are the identity).  With X the panel solve is a
+// product and the triangular solves are block products.
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_potrf_diag(T* __restrict__ B, int n, int j0, int jb, FitState* st) {
-    // One wave; lane r owns row r.  Right-looking on the unscaled factor
-    // (L D L' form): step k subtracts L[r][k] L[c][k] / d_k from row r; the
-    // columns are scaled by 1/sqrt(d_k) at the end.  The wave's LDS accesses
-    // execute in order, so the steps need no barrier.
-    __shared__ T L[NB][NB + 1];
-    __shared__ T dk[NB];
-    const int r = threadIdx.x;
-    for (int e = r; e < jb * jb; e += 64) {
-        const int rr = e % jb, c = e / jb;
-        L[rr][c] = B[(i64)(j0 + c) * n + j0 + rr];
-    }
-    __syncthreads();
-    for (int k = 0; k < jb; k++) {
-        const T d = L[k][k];
-        if (r == 0) {
-            dk[k] = d;
-            if (!(d > (T)0) && st->info == 0) st->info = 2 + j0 + k;
-        }
-        const T inv = (T)1 / (d > (T)0 ? d : (T)1);
-        if (r > k && r < jb) {
-            const T lrk = L[r][k] * inv;
-            for (int c = k + 1; c <= r; c++) L[r][c] -= lrk * L[c][k];
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
-    for (int e = r; e < jb * jb; e += 64) {
-        const int rr = e % jb, c = e / jb;
-        if (c <= rr) {
-            const T dc = dk[c] > (T)0 ? dk[c] : (T)1;
-            B[(i64)(j0 + c) * n + j0 + rr] = L[rr][c] / sqrt(dc);
-        }
-    }
+// Orders one wave's LDS writes before its later reads of other lanes'
+// entries (a wave's LDS operations execute in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// L21 = B21 L11^-T for 64 rows per work-group below the diagonal block.
-// Each row is solved by the four lanes of one wave that own it, so the
-// substitution needs no barrier.
+// Factor and invert the diagonal block at j0.  Four waves: every wave
+// factors the block (lane r owns row r; each new column of L reaches the
+// other lanes through the wave's LDS column buffer), then wave w inverts
+// columns c = 4t + w.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, int n, int j0, int jb) {
-    __shared__ T L[NB][NB + 1];
-    __shared__ T X[NB][NB + 1];  // X[row][k]
-    __shared__ T rdiag[NB];
-    const int tid = threadIdx.x;
-    const int r0 = j0 + jb + blockIdx.x * NB;
-    const int nr = min(NB, n - r0);
-    for (int e = tid; e < jb * jb; e += 256) {
-        const int r = e % jb, c = e / jb;
-        L[r][c] = B[(i64)(j0 + c) * n + j0 + r];
+__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
+                                                    FitState* st) {
+    __shared__ T colb[4][NB];      // per wave: column k of L
+    __shared__ T rinv[4][NB];      // per wave: 1 / L[k][k]
+    __shared__ T rowb[4][NB / 4];  // per wave: row i of X (the wave's columns)
+    const int r = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool rin = r < jb;
+    T a[NB];
+    {
+        const T* col = B + (i64)j0 * n + j0 + r;
+        #pragma unroll
+        for (int c = 0; c < NB; c++) a[c] = (rin && c < jb) ? col[(i64)c * n] : (T)(r == c ? 1 : 0);
     }
-    for (int e = tid; e < NB * jb; e += 256) {
-        const int r = e % NB, c = e / NB;
-        X[r][c] = r < nr ? B[(i64)(j0 + c) * n + r0 + r] : (T)0;
+    __syncthreads();  // every wave has read the block before wave 0 writes L
+    T* cb = colb[w];
+    T* ri = rinv[w];
+    int badk = -1;
+    #pragma unroll
+    for (int k = 0; k < NB; k++) {
+        T d = bcast(a[k], k);
+        if (!(d > (T)0)) {
+            if (badk < 0) badk = k;
+            d = (T)1;
+        }
+        const T sd = sqrt(d), inv = (T)1 / sd;
+        a[k] = r > k ? a[k] * inv : (r == k ? sd : (T)0);
+        cb[r] = a[k];
+        if (r == k) ri[k] = inv;
+        wave_lds_sync();
+        #pragma unroll
+        for (int c = k + 1; c < NB; c++) a[c] = fma(-a[k], cb[c], a[c]);
+        wave_lds_sync();
     }
-    __syncthreads();
-    if (tid < jb) rdiag[tid] = (T)1 / L[tid][tid];
-    __syncthreads();
-    const int r = tid >> 2, q = tid & 3;
-    for (int k = 0; k < jb; k++) {
-        const T xk = X[r][k] * rdiag[k];
-        for (int c = k + 1 + q; c < jb; c += 4) X[r][c] -= xk * L[c][k];
-        if (q == (k & 3)) X[r][k] = xk;
+    if (w == 0 && rin) {
+        T* col = B + (i64)j0 * n + j0 + r;
+        #pragma unroll
+        for (int c = 0; c < NB; c++)
+            if (c < jb && c <= r) col[(i64)c * n] = a[c];
     }
-    __syncthreads();
-    for (int e = tid; e < NB * jb; e += 256) {
-        const int rr = e % NB, c = e / NB;
-        if (rr < nr) B[(i64)(j0 + c) * n + r0 + rr] = X[rr][c];
+    if (threadIdx.x == 0 && badk >= 0 && st->info == 0) st->info = 2 + j0 + badk;
+    // X = L^-1 by rows: row i of X is final at step i (lane i scales its
+    // accumulated row by 1 / L[i][i]); rows below subtract L[r][i] X[i][:]
+    // (L[r][i] = 0 above the diagonal leaves finished rows alone).
+    T* rb = rowb[w];
+    T x[NB / 4];
+    #pragma unroll
+    for (int t = 0; t < NB / 4; t++) x[t] = r == 4 * t + w ? (T)1 : (T)0;
+    #pragma unroll
+    for (int i = 0; i < NB; i++) {
+        if (r == i) {
+            const T dinv = ri[i];
+            #pragma unroll
+            for (int t = 0; t <= i / 4; t++) rb[t] = x[t] * dinv;
+        }
+        wave_lds_sync();
+        const T l = a[i];
+        #pragma unroll
+        for (int t = 0; t <= i / 4; t++) {
+            const T xi = rb[t];
+            x[t] = r == i ? xi : fma(-l, xi, x[t]);
+        }
+        wave_lds_sync();
     }
+    T* D = Dinv + (i64)(j0 / NB) * NB * NB + r * NB + w;
+    #pragma unroll
+    for (int t = 0; t < NB / 4; t++) D[4 * t] = x[t];
+}
+
+// Panel below the diagonal block: L21 = B21 X', 64 rows per work-group; wave
+// q computes columns c = 4t + q of every row (row i in lane i).
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, const T* __restrict__ Dinv, int n, int j0,
+                                                   int jb) {
+    __shared__ T Xs[NB][NB];
+    const int lane = threadIdx.x & 63;
+    const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = j0 + jb + blockIdx.x * NB + lane;
+    const bool ok = i < n;
+    const T* D = Dinv + (i64)(j0 / NB) * NB * NB;
+    T xv[NB * NB / 256];
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) xv[u] = D[threadIdx.x + 256 * u];
+    T a[NB];
+    #pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = (ok && j < jb) ? B[(i64)(j0 + j) * n + i] : (T)0;
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) (&Xs[0][0])[threadIdx.x + 256 * u] = xv[u];
+    __syncthreads();  // also: all waves have read the panel before any writes it
+    T acc[NB / 4];
+    #pragma unroll
+    for (int t = 0; t < NB / 4; t++) {
+        const T* xr = &Xs[4 * t + q][0];
+        T s = 0;
+        #pragma unroll
+        for (int j = 0; j < 4 * t + 4; j++) {
+            s = fma(a[j], xr[j], s);
+            if ((j & 7) == 7) asm volatile("" : "+v"(s)::"memory");  // bound the LDS reads in flight
+        }
+        acc[t] = s;
+    }
+    if (ok)
+        #pragma unroll
+        for (int t = 0; t < NB / 4; t++)
+            if (4 * t + q < jb) B[(i64)(j0 + 4 * t + q) * n + i] = acc[t];
 }
 
 // Lower trailing tiles: B[R, C] -= L21[R] L21[C]' for tile C <= tile R.
@@ -373,19 +431,35 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
     const int tid = threadIdx.x;
     const int r0 = j0 + jb;
     const int R = r0 + blockIdx.x * NB, C = r0 + blockIdx.y * NB;
-    for (int e = tid; e < NB * jb; e += 256) {
-        const int r = e % NB, k = e / NB;
-        As[k][r] = R + r < n ? B[(i64)(j0 + k) * n + R + r] : (T)0;
-        Bs[k][r] = C + r < n ? B[(i64)(j0 + k) * n + C + r] : (T)0;
+    const int tx = tid % 16, ty = tid / 16;
+    T va[NB * NB / 256], vb[NB * NB / 256], old[4][4];
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) {
+        const int e = tid + 256 * u, r = e & (NB - 1), k = e / NB;
+        va[u] = (k < jb && R + r < n) ? B[(i64)(j0 + k) * n + R + r] : (T)0;
+        vb[u] = (k < jb && C + r < n) ? B[(i64)(j0 + k) * n + C + r] : (T)0;
+    }
+    #pragma unroll
+    for (int j = 0; j < 4; j++)
+        #pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = R + tx + 16 * i, c = C + ty + 16 * j;
+            old[i][j] = (r < n && c < n && c <= r) ? B[(i64)c * n + r] : (T)0;
+        }
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) {
+        const int e = tid + 256 * u;
+        As[e / NB][e & (NB - 1)] = va[u];
+        Bs[e / NB][e & (NB - 1)] = vb[u];
     }
     __syncthreads();
-    const int tx = tid % 16, ty = tid / 16;
     T acc[4][4];
     #pragma unroll
     for (int i = 0; i < 4; i++)
         #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    for (int k = 0; k < jb; k++) {
+    #pragma unroll 8
+    for (int k = 0; k < NB; k++) {
         T av[4], bv[4];
         #pragma unroll
         for (int i = 0; i < 4; i++) av[i] = As[k][tx + 16 * i];
@@ -397,109 +471,106 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
             for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
     }
     #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int c = C + ty + 16 * j;
-        if (c >= n) continue;
+    for (int j = 0; j < 4; j++)
         #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int r = R + tx + 16 * i;
-            if (r < n && c <= r) B[(i64)c * n + r] -= acc[i][j];
+            const int r = R + tx + 16 * i, c = C + ty + 16 * j;
+            if (r < n && c < n && c <= r) B[(i64)c * n + r] = old[i][j] - acc[i][j];
         }
-    }
 }
 
+constexpr int SOLVE_THREADS = 512;
+
 // Solve L L' x = b for 4 right-hand sides (n x 4 column-major, in place).
-// One work-group; per 64-row block the diagonal block and its reciprocal
-// diagonal are staged in LDS, wave 0 substitutes with shuffles, and all
-// threads update the remaining right-hand side rows.
+// One work-group.  Per 64-row block: wave 0 multiplies the block's
+// right-hand side by X (forward) or X' (backward), then every thread
+// updates one remaining row with the block's 64 entries of L, loaded
+// together.  skip (optional): a nonzero *skip makes the call a no-op.
 template <typename T>
-__global__ __launch_bounds__(1024) void k_chol_solve(const T* __restrict__ L, int n, T* __restrict__ b) {
-    __shared__ T Ld[NB][NB + 1];
-    __shared__ T rd[NB];
-    __shared__ T xb[4][NB];
+__global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                              int n, T* __restrict__ b, const int* skip) {
+    if (skip && *skip) return;
+    __shared__ T Xs[NB][NB + 1];
+    __shared__ T vk[NB][4];
+    __shared__ T yk[NB][4];
     const int tid = threadIdx.x, lane = tid & 63;
-    // forward: L y = b
-    for (int i0 = 0; i0 < n; i0 += NB) {
-        const int ib = min(NB, n - i0);
-        for (int e = tid; e < ib * ib; e += 1024) {
-            const int r = e % ib, c = e / ib;
-            Ld[r][c] = L[(i64)(i0 + c) * n + i0 + r];
-        }
-        __syncthreads();
-        if (tid < ib) rd[tid] = (T)1 / Ld[tid][tid];
-        __syncthreads();
-        if (tid < 64) {
-            T v[4];
-            for (int j = 0; j < 4; j++) v[j] = lane < ib ? b[(i64)j * n + i0 + lane] : (T)0;
-            for (int k = 0; k < ib; k++) {
-                const T lrk = lane < ib ? Ld[lane][k] : (T)0;
-                const T rk = rd[k];
-                for (int j = 0; j < 4; j++) {
-                    const T xk = bcast(v[j], k) * rk;
-                    if (lane == k) v[j] = xk;
-                    else if (lane > k) v[j] -= lrk * xk;
+    const int nb = (n + NB - 1) / NB;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int s = 0; s < nb; s++) {
+            const int kb = pass == 0 ? s : nb - 1 - s;
+            const int k0 = kb * NB, ib = min(NB, n - k0);
+            const T* D = Dinv + (i64)kb * NB * NB;
+            T xv[NB * NB / SOLVE_THREADS];
+            #pragma unroll
+            for (int u = 0; u < NB * NB / SOLVE_THREADS; u++) xv[u] = D[tid + SOLVE_THREADS * u];
+            if (tid < 4 * NB) {
+                const int r = tid & (NB - 1), j = tid / NB;
+                vk[r][j] = r < ib ? b[(i64)j * n + k0 + r] : (T)0;
+            }
+            #pragma unroll
+            for (int u = 0; u < NB * NB / SOLVE_THREADS; u++) {
+                const int e = tid + SOLVE_THREADS * u;
+                Xs[e / NB][e & (NB - 1)] = xv[u];
+            }
+            __syncthreads();
+            if (tid < 64) {
+                T y[4] = {0, 0, 0, 0};
+                #pragma unroll 8
+                for (int c = 0; c < NB; c++) {
+                    const T xe = pass == 0 ? Xs[lane][c] : Xs[c][lane];
+                    #pragma unroll
+                    for (int j = 0; j < 4; j++) y[j] = fma(xe, vk[c][j], y[j]);
+                }
+                #pragma unroll
+                for (int j = 0; j < 4; j++) yk[lane][j] = y[j];
+                if (lane < ib)
+                    #pragma unroll
+                    for (int j = 0; j < 4; j++) b[(i64)j * n + k0 + lane] = y[j];
+            }
+            __syncthreads();
+            if (pass == 0) {
+                // rows below: b_i -= L[i, k0:k0+64] y_k (ib == NB whenever rows remain)
+                for (int i = k0 + NB + tid; i < n; i += SOLVE_THREADS) {
+                    T l[NB];
+                    #pragma unroll
+                    for (int c = 0; c < NB; c++) l[c] = L[(i64)(k0 + c) * n + i];
+                    T acc[4] = {0, 0, 0, 0};
+                    #pragma unroll
+                    for (int c = 0; c < NB; c++) {
+                        #pragma unroll
+                        for (int j = 0; j < 4; j++) acc[j] = fma(l[c], yk[c][j], acc[j]);
+                        if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+                    }
+                    #pragma unroll
+                    for (int j = 0; j < 4; j++) b[(i64)j * n + i] -= acc[j];
+                }
+            } else {
+                // rows above: b_i -= L[k0:k0+ib, i]' x_k (column i of L, contiguous)
+                for (int i = tid; i < k0; i += SOLVE_THREADS) {
+                    T l[NB];
+                    const T* col = L + (i64)i * n + k0;
+                    #pragma unroll
+                    for (int c = 0; c < NB; c++) l[c] = c < ib ? col[c] : (T)0;
+                    T acc[4] = {0, 0, 0, 0};
+                    #pragma unroll
+                    for (int c = 0; c < NB; c++) {
+                        #pragma unroll
+                        for (int j = 0; j < 4; j++) acc[j] = fma(l[c], yk[c][j], acc[j]);
+                        if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+                    }
+                    #pragma unroll
+                    for (int j = 0; j < 4; j++) b[(i64)j * n + i] -= acc[j];
                 }
             }
-            if (lane < ib)
-                for (int j = 0; j < 4; j++) xb[j][lane] = v[j];
+            __syncthreads();
         }
-        __syncthreads();
-        for (int e = tid; e < ib * 4; e += 1024) b[(i64)(e / ib) * n + i0 + e % ib] = xb[e / ib][e % ib];
-        for (int r = i0 + ib + tid; r < n; r += 1024) {
-            T acc[4] = {0, 0, 0, 0};
-            for (int k = 0; k < ib; k++) {
-                const T l = L[(i64)(i0 + k) * n + r];
-                for (int j = 0; j < 4; j++) acc[j] = fma(l, xb[j][k], acc[j]);
-            }
-            for (int j = 0; j < 4; j++) b[(i64)j * n + r] -= acc[j];
-        }
-        __syncthreads();
-    }
-    // backward: L' x = y
-    const int last = ((n - 1) / NB) * NB;
-    for (int i0 = last; i0 >= 0; i0 -= NB) {
-        const int ib = min(NB, n - i0);
-        for (int e = tid; e < ib * ib; e += 1024) {
-            const int r = e % ib, c = e / ib;
-            Ld[r][c] = L[(i64)(i0 + c) * n + i0 + r];
-        }
-        __syncthreads();
-        if (tid < ib) rd[tid] = (T)1 / Ld[tid][tid];
-        __syncthreads();
-        if (tid < 64) {
-            T v[4];
-            for (int j = 0; j < 4; j++) v[j] = lane < ib ? b[(i64)j * n + i0 + lane] : (T)0;
-            // Row `lane` of L' is column `lane` of L: L[k][lane] for k > lane.
-            for (int k = ib - 1; k >= 0; k--) {
-                const T lkr = lane < ib ? Ld[k][lane] : (T)0;
-                const T rk = rd[k];
-                for (int j = 0; j < 4; j++) {
-                    const T xk = bcast(v[j], k) * rk;
-                    if (lane == k) v[j] = xk;
-                    else if (lane < k) v[j] -= lkr * xk;
-                }
-            }
-            if (lane < ib)
-                for (int j = 0; j < 4; j++) xb[j][lane] = v[j];
-        }
-        __syncthreads();
-        for (int e = tid; e < ib * 4; e += 1024) b[(i64)(e / ib) * n + i0 + e % ib] = xb[e / ib][e % ib];
-        for (int r = tid; r < i0; r += 1024) {
-            T acc[4] = {0, 0, 0, 0};
-            for (int k = 0; k < ib; k++) {
-                const T l = L[(i64)r * n + i0 + k];  // L[i0+k][r]
-                for (int j = 0; j < 4; j++) acc[j] = fma(l, xb[j][k], acc[j]);
-            }
-            for (int j = 0; j < 4; j++) b[(i64)j * n + r] -= acc[j];
-        }
-        __syncthreads();
     }
 }
 
 // U (N x 4, column-major) <- Q' U (trans = 1) or Q U (trans = 0), Q = I - V T V'.
 __global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, const FitState* st,
                                                   double* __restrict__ U, int N, int trans) {
-    __shared__ double red[16 * 16];
+    __shared__ double red[17 * 16];
     __shared__ double H[16];
     const int tid = threadIdx.x;
     double G[16];
@@ -529,7 +600,8 @@ __global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, 
 
 // out = (with_y ? Y : 0) - A' x, x and out N x 4 column-major.
 __global__ __launch_bounds__(256) void k_matvec(const SysDesc s, const double* __restrict__ x,
-                                                double* __restrict__ out, int with_y) {
+                                                double* __restrict__ out, int with_y, const int* skip) {
+    if (skip && *skip) return;
     __shared__ double red[4][4];
     const int i = blockIdx.x, tid = threadIdx.x, N = s.N;
     double acc[4] = {0, 0, 0, 0};
@@ -557,7 +629,8 @@ __global__ void k_load_y(const SysDesc s, double* __restrict__ U) {
 }
 
 // U = [0; z] (z is n x 4 column-major, FP64).
-__global__ void k_embed(const double* __restrict__ z, double* __restrict__ U, int N) {
+__global__ void k_embed(const double* __restrict__ z, double* __restrict__ U, int N, const int* skip) {
+    if (skip && *skip) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
     const int n = N - 4;
@@ -584,19 +657,24 @@ __global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__
                                                       const double* __restrict__ U, const double* __restrict__ negAu,
                                                       const double* __restrict__ b2, float* __restrict__ r32,
                                                       FitState* st, int N) {
-    __shared__ double red[16 * 32];
+    if (st->ir_stop) return;
+    __shared__ double red[17 * 32];
     __shared__ double G1[16], G2[16];
     const int tid = threadIdx.x, n = N - 4;
     double G[32];
+    #pragma unroll
     for (int i = 0; i < 32; i++) G[i] = 0;
     for (int i = tid; i < N; i += 1024) {
         double v[4], w[4], u[4];
+        #pragma unroll
         for (int k = 0; k < 4; k++) {
             v[k] = V[(size_t)k * N + i];
             w[k] = W[(size_t)k * N + i];
             u[k] = U[(size_t)k * N + i];
         }
+        #pragma unroll
         for (int a = 0; a < 4; a++)
+            #pragma unroll
             for (int b = 0; b < 4; b++) {
                 G[a * 4 + b] += v[a] * u[b];
                 G[16 + a * 4 + b] += w[a] * u[b];
@@ -607,8 +685,10 @@ __global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__
     __syncthreads();
     double rmax = 0, bmax = 0;
     for (int i = tid; i < n; i += 1024)
+        #pragma unroll
         for (int b = 0; b < 4; b++) {
             double corr = 0;
+            #pragma unroll
             for (int k = 0; k < 4; k++)
                 corr += W[(size_t)k * N + 4 + i] * G1[k * 4 + b] + V[(size_t)k * N + 4 + i] * G2[k * 4 + b];
             const double bb = b2[(size_t)b * n + i];
@@ -636,10 +716,27 @@ __global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__
         double m = 0;
         for (int w = 0; w < 16; w++) m = fmax(m, red[w]);
         st->bnorm = m;
+        // Stop on convergence to FP64 level or when a sweep no longer
+        // halves the residual; 1 marks a residual usable as FP64.
+        const double rel = st->rnorm / (m > 0 ? m : 1.0);
+        if (rel < 1e-15 || !(rel < 0.5 * st->ir_prev)) {
+            st->ir_stop = rel < 1e-11 ? 1 : 2;
+        } else {
+            st->ir_prev = rel;
+            st->ir_iters += 1;
+        }
     }
 }
 
-__global__ void k_axpy_f32(double* __restrict__ x, const float* __restrict__ d, i64 n) {
+__global__ void k_ir_init(FitState* st) {
+    st->ir_stop = 0;
+    st->ir_prev = INFINITY;
+    st->ir_iters = 0;
+    st->info = 0;
+}
+
+__global__ void k_axpy_f32(double* __restrict__ x, const float* __restrict__ d, i64 n, const int* skip) {
+    if (skip && *skip) return;
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     if (i < n) x[i] += (double)d[i];
 }
@@ -672,16 +769,51 @@ __global__ __launch_bounds__(256) void k_residual(const SysDesc s, const double*
     }
 }
 
+// x (m x 4) = [c'; d'] with d' = R^-1 [Q'(Y - A'c')]_(0:4); U = c', U2 = Q'(Y - A'c').
+__global__ void k_finish(const double* __restrict__ U, const double* __restrict__ U2, const FitState* st,
+                         double* __restrict__ x, int N) {
+    const int m = N + 4;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N)
+        #pragma unroll
+        for (int j = 0; j < 4; j++) x[(i64)j * m + i] = U[(i64)j * N + i];
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        const int j = threadIdx.x;
+        double dp[4];
+        for (int a = 3; a >= 0; a--) {
+            double v = U2[(i64)j * N + a];
+            for (int q = a + 1; q < 4; q++) v -= st->R[a * 4 + q] * dp[q];
+            dp[a] = v / st->R[a * 4 + a];
+        }
+        for (int a = 0; a < 4; a++) x[(i64)j * m + N + a] = dp[a];
+    }
+}
+
+// st->fit_rmax = max |res| over cnt entries.  One work-group of 1024.
+__global__ __launch_bounds__(1024) void k_res_max(const double* __restrict__ res, i64 cnt, FitState* st) {
+    __shared__ double red[16];
+    double m = 0;
+    for (i64 i = threadIdx.x; i < cnt; i += 1024) m = fmax(m, fabs(res[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < 16; w++) t = fmax(t, red[w]);
+        st->fit_rmax = t;
+    }
+}
+
 inline unsigned nblk(i64 n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 template <typename T>
-void cholesky(tw_ctx* ctx, T* B, int n, FitState* st) {
+void cholesky(tw_ctx* ctx, T* B, T* Dinv, int n, FitState* st) {
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int jb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(64), 0, ctx->stream, B, n, j0, jb, st);
+        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, Dinv, n, j0, jb, st);
         const int rest = n - j0 - jb;
         if (rest > 0) {
-            hipLaunchKernelGGL(k_trsm_chol<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, B, n, j0, jb);
+            hipLaunchKernelGGL(k_trsm_chol<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, B, Dinv, n, j0, jb);
             hipLaunchKernelGGL(k_syrk<T>, dim3(nblk(rest, NB), nblk(rest, NB)), dim3(256), 0, ctx->stream, B, n, j0,
                                jb);
         }
@@ -739,7 +871,8 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     for (int a = 0; a < 3; a++) sd.mu[a] = mu[a];
     sd.lam = lambda / s;
 
-    // Workspaces: B (n x n, FP64 or FP32); vectors in one FP64 arena.
+    // Workspaces: B (n x n, FP64 or FP32); vectors in one FP64 arena;
+    // state, FP32 vectors and the inverted diagonal blocks in aux.
     const bool mixed = precision == TW_MIXED;
     void *Bw = nullptr, *arena = nullptr, *aux = nullptr;
     if ((rc = tw_ws(ctx, WS_SOLVE_A, (size_t)n * n * (mixed ? sizeof(float) : sizeof(double)), &Bw))) return rc;
@@ -747,7 +880,11 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     // V, P, W, U, U2, x(m x 4), res(m x 4), b2(n x 4), z(n x 4)
     const size_t arena_d = 5 * NV + 2 * (size_t)m * 4 + 2 * (size_t)n * 4;
     if ((rc = tw_ws(ctx, WS_SOLVE_B, arena_d * sizeof(double), &arena))) return rc;
-    if ((rc = tw_ws(ctx, WS_SOLVE_AUX, sizeof(FitState) + (size_t)n * 4 * sizeof(float) * 2, &aux))) return rc;
+    const int nblocks = (n + NB - 1) / NB;
+    const size_t st_b = (sizeof(FitState) + 255) & ~(size_t)255;
+    const size_t f32_b = ((size_t)n * 4 * sizeof(float) + 255) & ~(size_t)255;
+    const size_t dinv_b = (size_t)nblocks * NB * NB * sizeof(double);
+    if ((rc = tw_ws(ctx, WS_SOLVE_AUX, st_b + 2 * f32_b + dinv_b, &aux))) return rc;
     double* V = (double*)arena;
     double* P = V + NV;
     double* W = P + NV;
@@ -758,8 +895,10 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     double* b2 = res + (size_t)m * 4;
     double* z = b2 + (size_t)n * 4;
     FitState* st = (FitState*)aux;
-    float* f32a = (float*)((uint8_t*)aux + sizeof(FitState));
-    float* f32b = f32a + (size_t)n * 4;
+    float* f32a = (float*)((uint8_t*)aux + st_b);
+    float* f32b = (float*)((uint8_t*)aux + st_b + f32_b);
+    void* dinv = (uint8_t*)aux + st_b + 2 * f32_b;
+    const int* stop = &st->ir_stop;
 
     FitState hst;
     auto fetch_state = [&]() -> int {
@@ -769,6 +908,7 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     };
 
     // 1. QR of X', W, B.
+    hipLaunchKernelGGL(k_ir_init, dim3(1), dim3(1), 0, ctx->stream, st);
     hipLaunchKernelGGL(k_hh_x, dim3(1), dim3(1024), 0, ctx->stream, sd, V, st);
     hipLaunchKernelGGL(k_av, dim3(N), dim3(256), 0, ctx->stream, sd, V, P);
     hipLaunchKernelGGL(k_build_w, dim3(1), dim3(1024), 0, ctx->stream, V, P, W, st, N);
@@ -785,9 +925,10 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
             B = (double*)Bw;
         }
         hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
-        cholesky<double>(ctx, B, n, st);
+        cholesky<double>(ctx, B, (double*)dinv, n, st);
         hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
-        hipLaunchKernelGGL(k_chol_solve<double>, dim3(1), dim3(1024), 0, ctx->stream, B, n, z);
+        hipLaunchKernelGGL(k_chol_solve<double>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, B,
+                           (const double*)dinv, n, z, (const int*)nullptr);
         TW_LAUNCH_CHECK(ctx);
         return TW_OK;
     };
@@ -797,88 +938,77 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
         if ((rc = factor_solve_fp64())) return rc;
     } else {
         float* B = (float*)Bw;
+        float* Df = (float*)dinv;
         hipLaunchKernelGGL(k_build_b<float>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
-        cholesky<float>(ctx, B, n, st);
+        cholesky<float>(ctx, B, Df, n, st);
         hipLaunchKernelGGL(k_tail<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, b2, f32a, N);
-        hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(1024), 0, ctx->stream, B, n, f32a);
+        hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, B, (const float*)Df, n,
+                           f32a, (const int*)nullptr);
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, f32a, z,
                            (i64)n * 4);
         TW_LAUNCH_CHECK(ctx);
-        if ((rc = fetch_state())) return rc;
-        if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
-        bool ok = hst.info == 0;
-        double prev = INFINITY;
-        if (ok) {
-            ok = false;
-            for (iters = 1; iters <= 20; iters++) {
-                hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U2, N);
-                hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U2, P, 0);  // P = -A'[0;z]
-                hipLaunchKernelGGL(k_ir_residual, dim3(1), dim3(1024), 0, ctx->stream, V, W, U2, P, b2, f32b, st, N);
-                TW_LAUNCH_CHECK(ctx);
-                if ((rc = fetch_state())) return rc;
-                const double rel = hst.rnorm / (hst.bnorm > 0 ? hst.bnorm : 1.0);
-                if (rel < 1e-15 || !(rel < 0.5 * prev)) {
-                    ok = rel < 1e-11;
-                    break;
-                }
-                prev = rel;
-                hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(1024), 0, ctx->stream, B, n, f32b);
+        // Refinement sweeps run in batches; the device stops a sweep (and
+        // turns the rest of its batch into no-ops) once the residual reaches
+        // FP64 level or stalls, so the host checks once per batch.
+        constexpr int kBatch = 3, kMaxSweeps = 21;
+        bool ok = false;
+        for (int done = 0; done < kMaxSweeps; done += kBatch) {
+            for (int it = 0; it < kBatch; it++) {
+                hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U2, N, stop);
+                hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U2, P, 0, stop);  // -A'[0;z]
+                hipLaunchKernelGGL(k_ir_residual, dim3(1), dim3(1024), 0, ctx->stream, V, W, U2, P, b2, f32b, st,
+                                   N);
+                hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, B,
+                                   (const float*)Df, n, f32b, stop);
                 hipLaunchKernelGGL(k_axpy_f32, dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, z, f32b,
-                                   (i64)n * 4);
+                                   (i64)n * 4, stop);
+            }
+            TW_LAUNCH_CHECK(ctx);
+            if ((rc = fetch_state())) return rc;
+            if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
+            if (hst.info != 0) break;  // FP32 factor not positive definite
+            if (hst.ir_stop) {
+                ok = hst.ir_stop == 1;
+                break;
             }
         }
+        iters = hst.ir_iters;
         if (!ok) {
             // FP32 factors too weak for this system: factor in FP64 (U still
             // holds Q'Y).
-            const int zero = 0;
-            TW_HIP(ctx, hipMemcpyAsync(&st->info, &zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(k_ir_init, dim3(1), dim3(1), 0, ctx->stream, st);
             if ((rc = factor_solve_fp64())) return rc;
-            iters = -iters;
+            iters = -std::max(iters, 1);
         }
     }
-    // 3. c' = Q [0; z];  t = Y - A' c';  Q' t (first 4 rows give q1' t).
-    hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U, N);
+    // 3. c' = Q [0; z];  t = Y - A' c';  Q' t (first 4 rows give q1' t);
+    //    d' = R^-1 (q1' t); FP64 residual of the bordered system.
+    hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U, N, (const int*)nullptr);
     hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 0);
-    hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U, U2, 1);
+    hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U, U2, 1, (const int*)nullptr);
     hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U2, N, 1);
+    hipLaunchKernelGGL(k_finish, dim3(nblk(N)), dim3(256), 0, ctx->stream, U, U2, st, x, N);
+    hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x, res);
+    hipLaunchKernelGGL(k_res_max, dim3(1), dim3(1024), 0, ctx->stream, res, (i64)m * 4, st);
     TW_LAUNCH_CHECK(ctx);
-    std::vector<double> hc(NV), ht(NV);
-    TW_HIP(ctx, hipMemcpyAsync(hc.data(), U, NV * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    TW_HIP(ctx, hipMemcpyAsync(ht.data(), U2, NV * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> hx((size_t)m * 4);
+    TW_HIP(ctx, hipMemcpyAsync(hx.data(), x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if ((rc = fetch_state())) return rc;
     if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
     if (hst.info >= 2)
         return tw_fail(ctx, TW_E_SINGULAR, "projected TPS system not positive definite (column %d)", hst.info - 2);
-    // d' = R^-1 (q1' t)
-    double dp[16];
-    for (int j = 0; j < 4; j++)
-        for (int a = 3; a >= 0; a--) {
-            double v = ht[(size_t)j * N + a];
-            for (int q = a + 1; q < 4; q++) v -= hst.R[a * 4 + q] * dp[q * 4 + j];
-            dp[a * 4 + j] = v / hst.R[a * 4 + a];
-        }
-    // Residual of the bordered system (FP64), for the record.
     {
-        std::vector<double> hx((size_t)m * 4);
-        for (int j = 0; j < 4; j++) {
-            for (int i = 0; i < N; i++) hx[(size_t)j * m + i] = hc[(size_t)j * N + i];
-            for (int a = 0; a < 4; a++) hx[(size_t)j * m + N + a] = dp[a * 4 + j];
-        }
-        TW_HIP(ctx, hipMemcpyAsync(x, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x, res);
-        TW_LAUNCH_CHECK(ctx);
-        std::vector<double> hr((size_t)m * 4);
-        TW_HIP(ctx, hipMemcpyAsync(hr.data(), res, hr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        double rmax = 0, bmax = 0;
-        for (double v : hr) rmax = std::max(rmax, std::fabs(v));
+        double bmax = 0;
         for (int i = 0; i < 4 * N; i++) bmax = std::max(bmax, std::fabs(hy[i]));
-        ctx->fit_residual = rmax / bmax;
+        ctx->fit_residual = hst.fit_rmax / bmax;
     }
     ctx->refine_iters = iters;
+    double dp[16];
+    for (int j = 0; j < 4; j++)
+        for (int a = 0; a < 4; a++) dp[a * 4 + j] = hx[(size_t)j * m + N + a];
     // Undo the scaling: c = c'/s, d = T^-1 d'.
     for (int i = 0; i < N; i++)
-        for (int j = 0; j < 4; j++) c[4 * i + j] = hc[(size_t)j * N + i] / s;
+        for (int j = 0; j < 4; j++) c[4 * i + j] = hx[(size_t)j * m + i] / s;
     for (int j = 0; j < 4; j++) {
         const double d0 = dp[0 * 4 + j], d1 = dp[1 * 4 + j], d2 = dp[2 * 4 + j], d3 = dp[3 * 4 + j];
         d[0 * 4 + j] = d0 - (mu[0] * d1 + mu[1] * d2 + mu[2] * d3) / s;

@@ -38,6 +38,7 @@
 #include "tw_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace {
@@ -49,6 +50,7 @@ struct TpsArgs {
     i64 fnx, fny, fnz;                 // full subject extent
     i64 dnx, dny, dnz, dx0, dy0, dz0;  // destination window (global origin)
     i64 nc;
+    i64 src_cstride, dst_cstride;      // bytes between channels (whole buffers)
     const void* grid;                  // 3 planes, float or half
     i64 gx, gy, gz, ox, oy, oz;
     int f;
@@ -562,8 +564,8 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     const void* src0 = a.src;
     void* dst0 = a.dst;
     for (int ch = 0; ch < (WRITE_DF ? 1 : (int)a.nc); ch++) {
-    a.src = src0 ? (const void*)((const uint8_t*)src0 + (size_t)ch * g.chan_b) : nullptr;
-    a.dst = dst0 ? (void*)((uint8_t*)dst0 + (size_t)ch * g.dchan_b) : nullptr;
+    a.src = src0 ? (const void*)((const uint8_t*)src0 + (size_t)ch * a.src_cstride) : nullptr;
+    a.dst = dst0 ? (void*)((uint8_t*)dst0 + (size_t)ch * a.dst_cstride) : nullptr;
 #define TW_FUSED(M, H, I, W) \
     hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF, W>), grid, block, lds, ctx->stream, a, g)
     if (WRITE_DF) {
@@ -651,6 +653,101 @@ void fill_grid_args(TpsArgs& a, const tw_grid* g, const void* gdev) {
 
 }  // namespace
 
+// Per node layer k: min and max of each displacement component over the
+// layer (out[k*6 + 2c], out[k*6 + 2c + 1]).  The upsampled field is a convex
+// combination of 4x4x4 nodes, so these bound the field of every voxel whose
+// nodes lie in the layers.
+__global__ __launch_bounds__(256) void k_layer_bounds(const void* __restrict__ grid, int f16,
+                                                      const float* __restrict__ qtab, i64 qbx, i64 qby, i64 gx,
+                                                      i64 gy, i64 gz, float* __restrict__ out) {
+    __shared__ float red[6][256];
+    const i64 k = blockIdx.x, plane = gx * gy * gz;
+    float v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (i64 e = threadIdx.x; e < gx * gy; e += 256) {
+        const i64 i = e % gx, j = e / gx, gi = (k * gy + j) * gx + i;
+        for (int c = 0; c < 3; c++) {
+            float x;
+            if (f16) {
+                const float* q = qtab + (((k >> 3) * qby + (j >> 3)) * qbx + (i >> 3)) * 6;
+                x = q[c] + q[3 + c] * __half2float(((const __half*)grid)[c * plane + gi]);
+            } else {
+                x = ((const float*)grid)[c * plane + gi];
+            }
+            v[2 * c] = fminf(v[2 * c], x);
+            v[2 * c + 1] = fmaxf(v[2 * c + 1], x);
+        }
+    }
+    for (int c = 0; c < 6; c++) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int c = 0; c < 6; c++)
+                red[c][threadIdx.x] = (c & 1) ? fmaxf(red[c][threadIdx.x], red[c][threadIdx.x + st])
+                                              : fminf(red[c][threadIdx.x], red[c][threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[k * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Does a source window of nx*ny*nz voxels of `bpv` bytes fit the fused
+// kernel's 32-bit tap offsets?
+bool fits_fast(i64 nx, i64 ny, i64 nz, int bpv) {
+    const uint64_t taps = (uint64_t)nx * ny * nz * bpv + (uint64_t)nx * (ny + 1) * bpv + 16;
+    return taps <= 0x7FFF0000ull && ny * nz < (1ll << 24);
+}
+
+// Resample dst in z-slabs whose source windows (from the grid's layer bounds)
+// fit the fused kernel; used when the whole source does not.
+template <typename T>
+int resample_slabs(tw_ctx* ctx, TpsArgs a, const tw_grid* grid, int interp, int precision, bool f16) {
+    const i64 gz = grid->gz;
+    void* bp = nullptr;
+    int rc = tw_ws(ctx, WS_AUX_BOUNDS, (size_t)gz * 6 * sizeof(float), &bp);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_layer_bounds, dim3((unsigned)gz), dim3(256), 0, ctx->stream, a.grid, f16 ? 1 : 0, a.qtab,
+                       a.qbx, a.qby, a.gx, a.gy, gz, (float*)bp);
+    TW_LAUNCH_CHECK(ctx);
+    std::vector<float> b((size_t)gz * 6);
+    TW_HIP(ctx, hipMemcpyAsync(b.data(), bp, b.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const i64 plane_s = a.snx * a.sny * (i64)sizeof(T), plane_d = a.dnx * a.dny * (i64)sizeof(T);
+    const TpsArgs full = a;
+    i64 slab = 256;
+    for (i64 z0 = 0; z0 < full.dnz;) {
+        i64 n = std::min(slab, full.dnz - z0);
+        i64 lo = 0, hi = 0;
+        for (;;) {
+            const i64 Zs = full.dz0 + z0, Ze = full.dz0 + z0 + n - 1;
+            const i64 kmin = (Zs - full.oz) / full.f, kmax = std::min<i64>((Ze - full.oz) / full.f + 3, gz - 1);
+            float dmin = INFINITY, dmax = -INFINITY;
+            for (i64 k = kmin; k <= kmax; k++) {
+                dmin = std::min(dmin, b[k * 6 + 4]);
+                dmax = std::max(dmax, b[k * 6 + 5]);
+            }
+            if (!std::isfinite(dmin) || !std::isfinite(dmax)) { dmin = -1e30f; dmax = 1e30f; }
+            // Two voxels of margin cover the float rounding of the position.
+            lo = std::max<i64>(full.sz0, (i64)std::floor((double)Zs + dmin) - 2);
+            hi = std::min<i64>(full.sz0 + full.snz, (i64)std::ceil((double)Ze + dmax) + 3);
+            if (hi <= lo) { lo = full.sz0; hi = full.sz0 + 1; }
+            if (fits_fast(full.snx, full.sny, hi - lo, sizeof(T)) || n == 1) break;
+            n = std::max<i64>(1, n / 2);
+            slab = n;
+        }
+        if (!fits_fast(full.snx, full.sny, hi - lo, sizeof(T)))
+            return tw_fail(ctx, TW_E_UNSUPPORTED, "displacement range too large for a 2 GiB source window per slab");
+        TpsArgs s = full;
+        s.src = (const uint8_t*)full.src + (lo - full.sz0) * plane_s;
+        s.sz0 = lo;
+        s.snz = hi - lo;
+        s.dst = (uint8_t*)full.dst + z0 * plane_d;
+        s.dz0 = full.dz0 + z0;
+        s.dnz = n;
+        if ((rc = launch_tps<T, false>(ctx, s, interp, precision, f16))) return rc;
+        z0 += n;
+    }
+    return TW_OK;
+}
+
 extern "C" int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
                                int precision) {
     if (!ctx) return TW_E_ARG;
@@ -688,12 +785,19 @@ extern "C" int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst
     a.dnx = dst->nx; a.dny = dst->ny; a.dnz = dst->nz;
     a.dx0 = dst->x0; a.dy0 = dst->y0; a.dz0 = dst->z0;
     a.nc = src->nc;
+    a.src_cstride = src->nx * src->ny * src->nz * (i64)dtype_size(src->dtype);
+    a.dst_cstride = dst->nx * dst->ny * dst->nz * (i64)dtype_size(dst->dtype);
     fill_grid_args(a, grid, gdev);
 
     const bool f16 = grid->storage == TW_DF_F16;
+    const bool slabs = precision != TW_EXACT && !fits_fast(a.snx, a.sny, a.snz, dtype_size(src->dtype));
     tw_stage_begin(ctx, ST_RESAMPLE);
-    rc = src->dtype == TW_U8 ? launch_tps<uint8_t, false>(ctx, a, interp, precision, f16)
-                             : launch_tps<uint16_t, false>(ctx, a, interp, precision, f16);
+    if (slabs)
+        rc = src->dtype == TW_U8 ? resample_slabs<uint8_t>(ctx, a, grid, interp, precision, f16)
+                                 : resample_slabs<uint16_t>(ctx, a, grid, interp, precision, f16);
+    else
+        rc = src->dtype == TW_U8 ? launch_tps<uint8_t, false>(ctx, a, interp, precision, f16)
+                                 : launch_tps<uint16_t, false>(ctx, a, interp, precision, f16);
     if (rc) return rc;
     tw_stage_end(ctx, ST_RESAMPLE);
     if ((rc = tw_stage_out_finish(ctx, dst, ddev))) return rc;

@@ -27,6 +27,7 @@ enum TwSlot {
     WS_PTS,         // landmark / coefficient packs for the grid kernel
     WS_DF,          // dense DF staging
     WS_WEIGHTS,     // B-spline weight tables
+    WS_AUX_BOUNDS,  // grid layer bounds
     WS_COUNT
 };
 

@@ -212,3 +212,37 @@ def test_plain_config_size_sampled(warper, oracle):
         diff = np.abs(out[sl].astype(np.int64) - ref[sl].astype(np.int64))
         assert diff.max() <= 1 and np.mean(diff == 0) >= 0.999, (zb, diff.max(), np.mean(diff == 0))
     del grid
+
+
+def test_plain_source_over_2gib_in_slabs(warper, oracle):
+    """A uint16 source past 2 GiB (1280x1280x700) runs in z-slabs whose
+    source windows come from the grid's per-layer displacement bounds; the
+    slices on both sides of each slab boundary match the oracle."""
+    import torch
+    size = (1280, 1280, 700)
+    g = 8
+    tile = synth.blob_volume((100, size[1], size[0]), np.uint16, n_blobs=60, amp=(100, 4000), seed=5)
+    vol = np.ascontiguousarray(np.tile(tile, (7, 1, 1)))
+    assert vol.nbytes > 2 ** 31
+    sub, tar = synth.landmarks(size, 256, 20.0, seed=5)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    df = oracle.grid_eval(tar, c, d, gsh, g)
+    src = torch.from_numpy(vol.view(np.int16)).cuda()
+    dst = torch.empty_like(src)
+    dgrid = warper.grid_from_array(torch.from_numpy(df).cuda(), g)
+    warper.resample_tps(src, dgrid, dst=dst, precision=warp.PLAIN)
+    torch.cuda.synchronize()
+    out = dst.cpu().numpy().view(np.uint16)
+    # The whole mixed chain (FP16 grid, whose layer bounds come from the
+    # block tables) on the same volume.
+    warper.warp_tps(src, tar, sub, f=g, precision=warp.MIXED, dst=dst)
+    torch.cuda.synchronize()
+    out_m = dst.cpu().numpy().view(np.uint16)
+    del src, dst
+    for zb in (0, 31, 32, 63, 64, 87):
+        ref = oracle.tps_resample(vol, df, g, zblocks=(zb, zb + 1))
+        sl = slice(zb * g, min(size[2], (zb + 1) * g))
+        for o in (out, out_m):
+            diff = np.abs(o[sl].astype(np.int64) - ref[sl].astype(np.int64))
+            assert diff.max() <= 1 and np.mean(diff == 0) >= 0.999, (zb, diff.max(), np.mean(diff == 0))
