@@ -53,9 +53,11 @@ extern "C" {
  * TW_EXACT : FP64 throughout, in the reference's operation order (parity).
  * TW_PLAIN : FP64 TPS solve; FP32 grid field; FP32 B-spline upsample;
  *            FP64 trilinear in the reference's order.
- * TW_MIXED : FP32 Cholesky of the projected system + FP64 iterative
- *            refinement; FP32 compensated grid field stored as
- *            offset-scaled FP16; FP32 upsample and trilinear. */
+ * TW_MIXED : the FP64 solve up to 4096 projected unknowns (MI355X runs
+ *            FP64 vector FMAs at the FP32 rate), FP32 Cholesky + FP64
+ *            iterative refinement above; FP32 grid field with FP64
+ *            accumulation stored as offset-scaled FP16; FP32 upsample and
+ *            trilinear. */
 #define TW_EXACT  2
 #define TW_PLAIN  0
 #define TW_MIXED  1
@@ -156,8 +158,9 @@ int tw_resample_affine(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const 
  * c: K x 4 row-major, d: 4 x 4 row-major, host memory (the reference's
  * xnx4_c, x4x4_d).  Computed by the reference's null-space method
  * (Householder QR of X, Cholesky of q2'(K + lambda I)q2); precision:
- * TW_PLAIN/TW_EXACT (FP64 factor) or TW_MIXED (FP32 factor + FP64
- * iterative refinement, FP64 refactor if the refinement stalls). */
+ * TW_PLAIN/TW_EXACT (FP64 factor) or TW_MIXED (the FP64 factor up to 4096
+ * projected unknowns; above, FP32 factor + FP64 iterative refinement, FP64
+ * refactor if the refinement stalls).  K <= 16384. */
 int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda,
                int precision, double* c, double* d);
 
@@ -187,7 +190,10 @@ int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int64_t ny, int
 
 /* Whole in-memory chain of imgwarp_smallmemory_image's GPU branch
  * (q_imgwarp_tps_quicksmallmemory.cpp:1997-2162): fit, grid, resample.
- * Grid spacing f (the reference uses 4), lambda 0.2. */
+ * Grid spacing f (the reference uses 4), lambda 0.2.  Every stage stays on
+ * the device (the coefficients and the grid never cross to the host) and
+ * the stream is synchronised once, at the end; a failed fit is reported
+ * then and leaves dst unspecified.  tw_get_timings gives each stage. */
 int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
                 const float* targets, int K, int f, double lambda, int interp, int precision);
 

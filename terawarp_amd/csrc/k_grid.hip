@@ -37,7 +37,7 @@ struct GridArgs {
     int K;
     int f;
     i64 gx, gy, gz, ox, oy, oz;
-    double d[16];        // affine block, row-major 4x4
+    const double* d;     // affine block, row-major 4x4 (device)
     float* out;          // 3 planes gx*gy*gz
 };
 
@@ -183,47 +183,21 @@ extern "C" int tw_grid_release(tw_ctx* ctx, tw_grid* grid) {
     return TW_OK;
 }
 
-extern "C" int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const double* c, const double* d,
-                                int precision, tw_grid* grid) {
-    if (!ctx) return TW_E_ARG;
-    if (!centres || !c || !d || !grid || !grid->data) return tw_fail(ctx, TW_E_ARG, "null argument");
-    if (K < 1) return tw_fail(ctx, TW_E_ARG, "K must be >= 1");
-    if (grid->gx < 1 || grid->gy < 1 || grid->gz < 1 || grid->f < 1)
-        return tw_fail(ctx, TW_E_ARG, "grid extent and spacing must be positive");
-    if (grid->storage != TW_DF_F32 && grid->storage != TW_DF_F16)
-        return tw_fail(ctx, TW_E_ARG, "grid storage must be TW_DF_F32 or TW_DF_F16");
-    if (grid->storage == TW_DF_F16 && precision == TW_EXACT)
-        return tw_fail(ctx, TW_E_ARG, "TW_EXACT needs an FP32 grid");
+int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const double* d, int K, int precision,
+                     tw_grid* grid) {
     int rc;
-    if ((rc = tw_bind(ctx))) return rc;
-    tw_stages_reset(ctx);
-    tw_stage_begin(ctx, ST_TOTAL);
     const bool exact = precision == TW_EXACT;
     const i64 plane = grid->gx * grid->gy * grid->gz;
-
-    // Landmark packs: centres (float4), coefficients (FP64 x3 or float4).
-    const size_t pts_bytes = (size_t)K * sizeof(float4);
-    const size_t coef_bytes = (size_t)K * 3 * sizeof(double);
-    void* pack = nullptr;
-    if ((rc = tw_ws(ctx, WS_PTS, pts_bytes + coef_bytes, &pack))) return rc;
-    std::vector<uint8_t> host(pts_bytes + coef_bytes);
-    float4* hp = (float4*)host.data();
-    for (int n = 0; n < K; n++) hp[n] = make_float4(centres[3 * n], centres[3 * n + 1], centres[3 * n + 2], 0.f);
-    double* hc = (double*)(host.data() + pts_bytes);
-    for (int n = 0; n < K; n++)
-        for (int q = 0; q < 3; q++) hc[3 * n + q] = c[4 * n + 1 + q];
-    TW_HIP(ctx, hipMemcpyAsync(pack, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
-
     GridArgs a;
     memset(&a, 0, sizeof a);
-    a.pts = (const float4*)pack;
-    a.c64 = (const double*)((uint8_t*)pack + pts_bytes);
-    a.cf = a.c64;
+    a.pts = pts;
+    a.c64 = cf;
+    a.cf = cf;
     a.K = K;
     a.f = grid->f;
     a.gx = grid->gx; a.gy = grid->gy; a.gz = grid->gz;
     a.ox = grid->ox; a.oy = grid->oy; a.oz = grid->oz;
-    for (int i = 0; i < 16; i++) a.d[i] = d[i];
+    a.d = d;
 
     // FP32 output: the grid itself (device F32), else a workspace.
     float* out32 = nullptr;
@@ -252,6 +226,42 @@ extern "C" int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const 
         TW_HIP(ctx, hipMemcpyAsync(grid->data, out32, (size_t)plane * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                    ctx->stream));
     }
+    return TW_OK;
+}
+
+extern "C" int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const double* c, const double* d,
+                                int precision, tw_grid* grid) {
+    if (!ctx) return TW_E_ARG;
+    if (!centres || !c || !d || !grid || !grid->data) return tw_fail(ctx, TW_E_ARG, "null argument");
+    if (K < 1) return tw_fail(ctx, TW_E_ARG, "K must be >= 1");
+    if (grid->gx < 1 || grid->gy < 1 || grid->gz < 1 || grid->f < 1)
+        return tw_fail(ctx, TW_E_ARG, "grid extent and spacing must be positive");
+    if (grid->storage != TW_DF_F32 && grid->storage != TW_DF_F16)
+        return tw_fail(ctx, TW_E_ARG, "grid storage must be TW_DF_F32 or TW_DF_F16");
+    if (grid->storage == TW_DF_F16 && precision == TW_EXACT)
+        return tw_fail(ctx, TW_E_ARG, "TW_EXACT needs an FP32 grid");
+    int rc;
+    if ((rc = tw_bind(ctx))) return rc;
+    tw_stages_reset(ctx);
+    tw_stage_begin(ctx, ST_TOTAL);
+
+    // Landmark pack: centres (float4), coefficients (FP64 x3), affine block.
+    const size_t pts_bytes = (size_t)K * sizeof(float4);
+    const size_t coef_bytes = (size_t)K * 3 * sizeof(double);
+    void* pack = nullptr;
+    if ((rc = tw_ws(ctx, WS_PTS, pts_bytes + coef_bytes + 16 * sizeof(double), &pack))) return rc;
+    std::vector<uint8_t> host(pts_bytes + coef_bytes + 16 * sizeof(double));
+    float4* hp = (float4*)host.data();
+    for (int n = 0; n < K; n++) hp[n] = make_float4(centres[3 * n], centres[3 * n + 1], centres[3 * n + 2], 0.f);
+    double* hc = (double*)(host.data() + pts_bytes);
+    for (int n = 0; n < K; n++)
+        for (int q = 0; q < 3; q++) hc[3 * n + q] = c[4 * n + 1 + q];
+    memcpy(host.data() + pts_bytes + coef_bytes, d, 16 * sizeof(double));
+    TW_HIP(ctx, hipMemcpyAsync(pack, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+    const uint8_t* pk = (const uint8_t*)pack;
+    rc = tw_grid_eval_dev(ctx, (const float4*)pk, (const double*)(pk + pts_bytes),
+                          (const double*)(pk + pts_bytes + coef_bytes), K, precision, grid);
+    if (rc) return rc;
     tw_stage_end(ctx, ST_TOTAL);
     // The host pack must outlive the copy; host grids need the result now.
     TW_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -42,6 +42,7 @@
 namespace {
 
 constexpr int NB = 64;  // Cholesky block size
+constexpr int MIXED_FP32_MIN = 4097;  // TW_MIXED: smallest projected order factored in FP32
 
 struct SysDesc {
     const float4* pts;   // N centres (x, y, z, 0), original coordinates
@@ -304,84 +305,117 @@ __global__ __launch_bounds__(256) void k_build_b(const SysDesc s, const double* 
 
 // Orders one wave's LDS writes before its later reads of other lanes'
 // entries (a wave's LDS operations execute in order).
+// The fences emit no instruction at wavefront scope, so an empty asm with
+// a memory clobber also keeps the scheduler from moving LDS accesses
+// across steps.
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    asm volatile("" ::: "memory");
 }
 
-// Factor and invert the diagonal block at j0.  Four waves: every wave
-// factors the block (lane r owns row r; each new column of L reaches the
-// other lanes through the wave's LDS column buffer), then wave w inverts
-// columns c = 4t + w.
+// Keeps a step's arithmetic on the 16 values inside the step: the
+// scheduler may not sink it past the step's barrier (which would keep the
+// step's loaded operands alive across many steps).
+template <typename T>
+__device__ __forceinline__ void pin16(T* v) {
+    asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]), "+v"(v[7]));
+    asm volatile("" : "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]), "+v"(v[13]), "+v"(v[14]),
+                 "+v"(v[15]));
+}
+
+// Factor and invert the diagonal block at j0.  Four waves; lane r owns
+// row r and wave w columns c = 4t + w.  The factor is built as L~ D^-1 L~'
+// (L~[r][k] the Schur-complement column, d_k = L~[k][k]) so a step's
+// critical path holds one reciprocal; L = L~ D^-1/2 at the end.  Per step
+// the owner of column k publishes it through LDS and every wave updates
+// its 16 columns.  Then X = L^-1 by columns: lane c of wave 0 substitutes
+// column c alone, reading the rows of L from LDS (no cross-lane steps).
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
                                                     FitState* st) {
-    __shared__ T colb[4][NB];      // per wave: column k of L
-    __shared__ T rinv[4][NB];      // per wave: 1 / L[k][k]
-    __shared__ T rowb[4][NB / 4];  // per wave: row i of X (the wave's columns)
+    constexpr int LS = NB + 2;           // row stride of Lr (16-byte aligned rows)
+    __shared__ T colb[2][NB];            // L~[:, k], double-buffered by step parity
+    __shared__ T colp[2][4][NB / 4];     // L~[4t + w][k] / d_k for 4t + w > k, else 0
+    __shared__ T dsh[NB];                // pivots d_k (non-positive ones replaced by 1)
+    __shared__ __align__(16) T Lr[NB][LS];  // L, row-major
+    __shared__ T rsqsh[NB];              // 1 / L[c][c] = d_c^-1/2
+    __shared__ int bad;
     const int r = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool rin = r < jb;
-    T a[NB];
+    T a[NB / 4];
     {
         const T* col = B + (i64)j0 * n + j0 + r;
         #pragma unroll
-        for (int c = 0; c < NB; c++) a[c] = (rin && c < jb) ? col[(i64)c * n] : (T)(r == c ? 1 : 0);
+        for (int t = 0; t < NB / 4; t++) {
+            const int c = 4 * t + w;
+            a[t] = (rin && c < jb) ? col[(i64)c * n] : (T)(r == c ? 1 : 0);
+        }
     }
-    __syncthreads();  // every wave has read the block before wave 0 writes L
-    T* cb = colb[w];
-    T* ri = rinv[w];
-    int badk = -1;
+    if (threadIdx.x == 0) bad = -1;
     #pragma unroll
     for (int k = 0; k < NB; k++) {
-        T d = bcast(a[k], k);
-        if (!(d > (T)0)) {
-            if (badk < 0) badk = k;
-            d = (T)1;
+        const int buf = k & 1;
+        if (w == (k & 3)) {
+            T d = bcast(a[k >> 2], k);
+            if (!(d > (T)0)) {
+                if (r == 0 && bad < 0) bad = k;
+                d = (T)1;
+            }
+            const T rd = (T)1 / d;
+            const T lrk = a[k >> 2];
+            colb[buf][r] = r >= k ? lrk : (T)0;
+            colp[buf][r & 3][r >> 2] = r > k ? lrk * rd : (T)0;
+            if (r == 0) dsh[k] = d;
         }
-        const T sd = sqrt(d), inv = (T)1 / sd;
-        a[k] = r > k ? a[k] * inv : (r == k ? sd : (T)0);
-        cb[r] = a[k];
-        if (r == k) ri[k] = inv;
-        wave_lds_sync();
+        __syncthreads();
+        const T l = colb[buf][r];
         #pragma unroll
-        for (int c = k + 1; c < NB; c++) a[c] = fma(-a[k], cb[c], a[c]);
-        wave_lds_sync();
+        for (int t = 0; t < NB / 4; t++) a[t] = fma(-l, colp[buf][w][t], a[t]);
+        pin16(a);
     }
-    if (w == 0 && rin) {
+    // L = L~ D^-1/2 (lower part; the rest is zero).
+    #pragma unroll
+    for (int t = 0; t < NB / 4; t++) {
+        const int c = 4 * t + w;
+        const T rs = (T)1 / sqrt(dsh[c]);
+        a[t] = r >= c ? a[t] * rs : (T)0;
+        Lr[r][c] = a[t];
+        if (r == 0) rsqsh[c] = rs;
+    }
+    {
         T* col = B + (i64)j0 * n + j0 + r;
         #pragma unroll
-        for (int c = 0; c < NB; c++)
-            if (c < jb && c <= r) col[(i64)c * n] = a[c];
-    }
-    if (threadIdx.x == 0 && badk >= 0 && st->info == 0) st->info = 2 + j0 + badk;
-    // X = L^-1 by rows: row i of X is final at step i (lane i scales its
-    // accumulated row by 1 / L[i][i]); rows below subtract L[r][i] X[i][:]
-    // (L[r][i] = 0 above the diagonal leaves finished rows alone).
-    T* rb = rowb[w];
-    T x[NB / 4];
-    #pragma unroll
-    for (int t = 0; t < NB / 4; t++) x[t] = r == 4 * t + w ? (T)1 : (T)0;
-    #pragma unroll
-    for (int i = 0; i < NB; i++) {
-        if (r == i) {
-            const T dinv = ri[i];
-            #pragma unroll
-            for (int t = 0; t <= i / 4; t++) rb[t] = x[t] * dinv;
+        for (int t = 0; t < NB / 4; t++) {
+            const int c = 4 * t + w;
+            if (rin && c < jb && c <= r) col[(i64)c * n] = a[t];
         }
-        wave_lds_sync();
-        const T l = a[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && bad >= 0 && st->info == 0) st->info = 2 + j0 + bad;
+    if (w != 0) return;
+    // Column c = r of X: x[q] = (delta_qc - sum_{j<q} L[q][j] x[j]) / L[q][q]
+    // (zero above the diagonal by construction); two partial sums halve the
+    // dependent chain.
+    T x[NB];
+    #pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const T* lq = Lr[q];
+        T s0 = r == q ? (T)1 : (T)0, s1 = 0;
         #pragma unroll
-        for (int t = 0; t <= i / 4; t++) {
-            const T xi = rb[t];
-            x[t] = r == i ? xi : fma(-l, xi, x[t]);
+        for (int j = 0; j + 1 < q; j += 2) {
+            s0 = fma(-lq[j], x[j], s0);
+            s1 = fma(-lq[j + 1], x[j + 1], s1);
         }
-        wave_lds_sync();
+        if (q & 1) s0 = fma(-lq[q - 1], x[q - 1], s0);
+        x[q] = (s0 + s1) * rsqsh[q];
+        asm volatile("" : "+v"(x[q])::"memory");  // one row's LDS reads at a time
     }
-    T* D = Dinv + (i64)(j0 / NB) * NB * NB + r * NB + w;
+    T* D = Dinv + (i64)(j0 / NB) * NB * NB + r;
     #pragma unroll
-    for (int t = 0; t < NB / 4; t++) D[4 * t] = x[t];
+    for (int q = 0; q < NB; q++) D[q * NB] = x[q];
 }
 
 // Panel below the diagonal block: L21 = B21 X', 64 rows per work-group; wave
@@ -480,12 +514,26 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
 }
 
 constexpr int SOLVE_THREADS = 512;
+constexpr int BUF_FLAGS = 0x00020000;  // raw buffer, gfx9-family data format
+
+// One float / double through a buffer resource: byte offsets voff (per
+// lane) + soff (wave-uniform).
+template <typename T>
+__device__ __forceinline__ T ld_buf(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    } else {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    }
+}
 
 // Solve L L' x = b for 4 right-hand sides (n x 4 column-major, in place).
 // One work-group.  Per 64-row block: wave 0 multiplies the block's
 // right-hand side by X (forward) or X' (backward), then every thread
-// updates one remaining row with the block's 64 entries of L, loaded
-// together.  skip (optional): a nonzero *skip makes the call a no-op.
+// updates one remaining row with the block's 64 entries of L.  Those
+// entries do not depend on the solution, so they are loaded (together with
+// the block's X) before the block's product and their latency overlaps it.
+// skip (optional): a nonzero *skip makes the call a no-op.
 template <typename T>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restrict__ L, const T* __restrict__ Dinv,
                                                               int n, T* __restrict__ b, const int* skip) {
@@ -495,18 +543,46 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
     __shared__ T yk[NB][4];
     const int tid = threadIdx.x, lane = tid & 63;
     const int nb = (n + NB - 1) / NB;
+    // L through a buffer resource (the host keeps n * n * sizeof(T) < 2^31).
+    const uint32_t colb = (uint32_t)n * (uint32_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rl =
+        __builtin_amdgcn_make_buffer_rsrc((void*)L, (short)0, (int)(colb * (uint32_t)n), BUF_FLAGS);
     for (int pass = 0; pass < 2; pass++) {
         for (int s = 0; s < nb; s++) {
             const int kb = pass == 0 ? s : nb - 1 - s;
             const int k0 = kb * NB, ib = min(NB, n - k0);
+            // Rows to update: below the block (forward; ib == NB whenever
+            // rows remain) or above it (backward).
+            const int r0 = pass == 0 ? k0 + NB : 0, r1 = pass == 0 ? n : k0;
+            int i = r0 + tid;
+            T l[NB];
+            auto load_row = [&](int row) {
+                if (pass == 0) {
+                    // L[row][k0 + c]: one lane offset, the column in the scalar offset.
+                    const uint32_t v = (uint32_t)row * (uint32_t)sizeof(T);
+                    #pragma unroll
+                    for (int c = 0; c < NB; c++) l[c] = ld_buf<T>(rl, v, (uint32_t)(k0 + c) * colb);
+                } else {
+                    // Column `row` of L, rows k0..k0+ib-1 (contiguous).
+                    const uint32_t v = ((uint32_t)row * (uint32_t)n + (uint32_t)k0) * (uint32_t)sizeof(T);
+                    #pragma unroll
+                    for (int c = 0; c < NB; c++) l[c] = c < ib ? ld_buf<T>(rl, v + (uint32_t)(c * sizeof(T)), 0u) : (T)0;
+                }
+            };
+            // The block's X and right-hand side first: loads retire in order,
+            // so the row loads issued after them stay in flight past the
+            // barrier.
             const T* D = Dinv + (i64)kb * NB * NB;
             T xv[NB * NB / SOLVE_THREADS];
             #pragma unroll
             for (int u = 0; u < NB * NB / SOLVE_THREADS; u++) xv[u] = D[tid + SOLVE_THREADS * u];
+            T bv = 0;
             if (tid < 4 * NB) {
                 const int r = tid & (NB - 1), j = tid / NB;
-                vk[r][j] = r < ib ? b[(i64)j * n + k0 + r] : (T)0;
+                bv = r < ib ? b[(i64)j * n + k0 + r] : (T)0;
             }
+            if (i < r1) load_row(i);
+            if (tid < 4 * NB) vk[tid & (NB - 1)][tid / NB] = bv;
             #pragma unroll
             for (int u = 0; u < NB * NB / SOLVE_THREADS; u++) {
                 const int e = tid + SOLVE_THREADS * u;
@@ -515,11 +591,13 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
             __syncthreads();
             if (tid < 64) {
                 T y[4] = {0, 0, 0, 0};
-                #pragma unroll 8
+                #pragma unroll 4
                 for (int c = 0; c < NB; c++) {
                     const T xe = pass == 0 ? Xs[lane][c] : Xs[c][lane];
                     #pragma unroll
                     for (int j = 0; j < 4; j++) y[j] = fma(xe, vk[c][j], y[j]);
+                    if ((c & 3) == 3)  // the row loads of l stay live here: few LDS reads in flight
+                        asm volatile("" : "+v"(y[0]), "+v"(y[1]), "+v"(y[2]), "+v"(y[3])::"memory");
                 }
                 #pragma unroll
                 for (int j = 0; j < 4; j++) yk[lane][j] = y[j];
@@ -528,39 +606,18 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
                     for (int j = 0; j < 4; j++) b[(i64)j * n + k0 + lane] = y[j];
             }
             __syncthreads();
-            if (pass == 0) {
-                // rows below: b_i -= L[i, k0:k0+64] y_k (ib == NB whenever rows remain)
-                for (int i = k0 + NB + tid; i < n; i += SOLVE_THREADS) {
-                    T l[NB];
+            for (bool first = true; i < r1; i += SOLVE_THREADS, first = false) {
+                if (!first) load_row(i);
+                T acc[4] = {0, 0, 0, 0};
+                #pragma unroll
+                for (int c = 0; c < NB; c++) {
                     #pragma unroll
-                    for (int c = 0; c < NB; c++) l[c] = L[(i64)(k0 + c) * n + i];
-                    T acc[4] = {0, 0, 0, 0};
-                    #pragma unroll
-                    for (int c = 0; c < NB; c++) {
-                        #pragma unroll
-                        for (int j = 0; j < 4; j++) acc[j] = fma(l[c], yk[c][j], acc[j]);
-                        if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-                    }
-                    #pragma unroll
-                    for (int j = 0; j < 4; j++) b[(i64)j * n + i] -= acc[j];
+                    for (int j = 0; j < 4; j++) acc[j] = fma(l[c], yk[c][j], acc[j]);
+                    if ((c & 7) == 7)  // bound the LDS reads in flight
+                        asm volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])::"memory");
                 }
-            } else {
-                // rows above: b_i -= L[k0:k0+ib, i]' x_k (column i of L, contiguous)
-                for (int i = tid; i < k0; i += SOLVE_THREADS) {
-                    T l[NB];
-                    const T* col = L + (i64)i * n + k0;
-                    #pragma unroll
-                    for (int c = 0; c < NB; c++) l[c] = c < ib ? col[c] : (T)0;
-                    T acc[4] = {0, 0, 0, 0};
-                    #pragma unroll
-                    for (int c = 0; c < NB; c++) {
-                        #pragma unroll
-                        for (int j = 0; j < 4; j++) acc[j] = fma(l[c], yk[c][j], acc[j]);
-                        if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-                    }
-                    #pragma unroll
-                    for (int j = 0; j < 4; j++) b[(i64)j * n + i] -= acc[j];
-                }
+                #pragma unroll
+                for (int j = 0; j < 4; j++) b[(i64)j * n + i] -= acc[j];
             }
             __syncthreads();
         }
@@ -820,25 +877,37 @@ void cholesky(tw_ctx* ctx, T* B, T* Dinv, int n, FitState* st) {
     }
 }
 
-}  // namespace
 
-extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda,
-                          int precision, double* c, double* d) {
-    if (!ctx) return TW_E_ARG;
-    if (!centres || !targets || !c || !d) return tw_fail(ctx, TW_E_ARG, "null argument");
+// Device results of one fit run (scaled coordinates: c = c'/s, d = T^-1 d').
+struct FitRun {
+    double* x;        // m x 4 column-major: [c'; d'], device
+    FitState* st;     // device
+    const float4* pts;  // centres, device
+    double s, mu[3];
+    double bmax;      // largest |entry| of [1 targets]
+    int N, m, iters;
+};
+
+int check_fit_args(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda, int precision) {
+    if (!centres || !targets) return tw_fail(ctx, TW_E_ARG, "null argument");
     if (K < 5) return tw_fail(ctx, TW_E_ARG, "the STPS fit needs at least 5 landmarks (got %d)", K);
+    if (K > 16384)  // the factor's byte offsets stay below 2^31
+        return tw_fail(ctx, TW_E_UNSUPPORTED, "the STPS fit takes at most 16384 landmarks (got %d)", K);
     if (!(lambda >= 0.0) || !std::isfinite(lambda)) return tw_fail(ctx, TW_E_ARG, "lambda must be finite and >= 0");
     if (precision != TW_EXACT && precision != TW_PLAIN && precision != TW_MIXED)
         return tw_fail(ctx, TW_E_ARG, "unknown precision %d", precision);
     for (int i = 0; i < 3 * K; i++)
         if (!std::isfinite(centres[i]) || !std::isfinite(targets[i]))
             return tw_fail(ctx, TW_E_ARG, "landmark coordinate %d is not finite", i);
-    int rc;
-    if ((rc = tw_bind(ctx))) return rc;
-    tw_stages_reset(ctx);
-    tw_stage_begin(ctx, ST_TOTAL);
-    tw_stage_begin(ctx, ST_FIT);
+    return TW_OK;
+}
 
+// Every kernel of the fit on ctx's stream, through the bordered-system
+// residual; synchronises only for the refinement control of a large
+// TW_MIXED system.
+int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda, int precision,
+            FitRun& out) {
+    int rc;
     const int N = K, n = N - 4, m = N + 4;
     double mu[3] = {0, 0, 0};
     for (int i = 0; i < N; i++)
@@ -873,7 +942,10 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
 
     // Workspaces: B (n x n, FP64 or FP32); vectors in one FP64 arena;
     // state, FP32 vectors and the inverted diagonal blocks in aux.
-    const bool mixed = precision == TW_MIXED;
+    // MI355X runs FP64 vector FMAs at the FP32 rate, so below MIXED_FP32_MIN
+    // the FP64 factor (one solve) beats the FP32 factor plus refinement
+    // sweeps (several solves and residuals); TW_MIXED then takes it too.
+    const bool mixed = precision == TW_MIXED && n >= MIXED_FP32_MIN;
     void *Bw = nullptr, *arena = nullptr, *aux = nullptr;
     if ((rc = tw_ws(ctx, WS_SOLVE_A, (size_t)n * n * (mixed ? sizeof(float) : sizeof(double)), &Bw))) return rc;
     const size_t NV = (size_t)N * 4;
@@ -991,34 +1063,117 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x, res);
     hipLaunchKernelGGL(k_res_max, dim3(1), dim3(1024), 0, ctx->stream, res, (i64)m * 4, st);
     TW_LAUNCH_CHECK(ctx);
-    std::vector<double> hx((size_t)m * 4);
-    TW_HIP(ctx, hipMemcpyAsync(hx.data(), x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if ((rc = fetch_state())) return rc;
+    out.x = x;
+    out.st = st;
+    out.pts = sd.pts;
+    out.s = s;
+    for (int a = 0; a < 3; a++) out.mu[a] = mu[a];
+    double bmax = 1.0;
+    for (int i = 0; i < 3 * N; i++) bmax = std::max(bmax, std::fabs((double)targets[i]));
+    out.bmax = bmax;
+    out.N = N;
+    out.m = m;
+    out.iters = iters;
+    return TW_OK;
+}
+
+// Fit state after a run: errors, residual, refinement count (synchronises).
+int fit_finish(tw_ctx* ctx, const FitState* st, double bmax, int iters) {
+    FitState hst;
+    TW_HIP(ctx, hipMemcpyAsync(&hst, st, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
+    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
     if (hst.info >= 2)
         return tw_fail(ctx, TW_E_SINGULAR, "projected TPS system not positive definite (column %d)", hst.info - 2);
-    {
-        double bmax = 0;
-        for (int i = 0; i < 4 * N; i++) bmax = std::max(bmax, std::fabs(hy[i]));
-        ctx->fit_residual = hst.fit_rmax / bmax;
-    }
+    ctx->fit_residual = hst.fit_rmax / bmax;
     ctx->refine_iters = iters;
-    double dp[16];
-    for (int j = 0; j < 4; j++)
-        for (int a = 0; a < 4; a++) dp[a * 4 + j] = hx[(size_t)j * m + N + a];
-    // Undo the scaling: c = c'/s, d = T^-1 d'.
-    for (int i = 0; i < N; i++)
-        for (int j = 0; j < 4; j++) c[4 * i + j] = hx[(size_t)j * m + i] / s;
-    for (int j = 0; j < 4; j++) {
-        const double d0 = dp[0 * 4 + j], d1 = dp[1 * 4 + j], d2 = dp[2 * 4 + j], d3 = dp[3 * 4 + j];
-        d[0 * 4 + j] = d0 - (mu[0] * d1 + mu[1] * d2 + mu[2] * d3) / s;
+    if (!std::isfinite(ctx->fit_residual)) return tw_fail(ctx, TW_E_SINGULAR, "non-finite TPS coefficients");
+    return TW_OK;
+}
+
+// Grid-kernel coefficients from the scaled solution x (m x 4, column-major):
+// cf[3i + q] = c'[i][q+1] / s and d = T^-1 d' (d[0][j] = d'0j - (mu . d'[1..3][j]) / s,
+// d[a][j] = d'[a][j] / s), as the host unscaling of tw_tps_fit.
+__global__ void k_unscale(const double* __restrict__ x, int N, double s, double mu0, double mu1, double mu2,
+                          double* __restrict__ cf, double* __restrict__ d) {
+    const int m = N + 4;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N)
+        #pragma unroll
+        for (int q = 0; q < 3; q++) cf[3 * i + q] = x[(i64)(q + 1) * m + i] / s;
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        const int j = threadIdx.x;
+        const double d0 = x[(i64)j * m + N], d1 = x[(i64)j * m + N + 1], d2 = x[(i64)j * m + N + 2],
+                     d3 = x[(i64)j * m + N + 3];
+        d[0 * 4 + j] = d0 - (mu0 * d1 + mu1 * d2 + mu2 * d3) / s;
         d[1 * 4 + j] = d1 / s;
         d[2 * 4 + j] = d2 / s;
         d[3 * 4 + j] = d3 / s;
     }
+}
+
+}  // namespace
+
+extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda,
+                          int precision, double* c, double* d) {
+    if (!ctx) return TW_E_ARG;
+    if (!c || !d) return tw_fail(ctx, TW_E_ARG, "null argument");
+    int rc;
+    if ((rc = check_fit_args(ctx, centres, targets, K, lambda, precision))) return rc;
+    if ((rc = tw_bind(ctx))) return rc;
+    tw_stages_reset(ctx);
+    tw_stage_begin(ctx, ST_TOTAL);
+    tw_stage_begin(ctx, ST_FIT);
+    FitRun r;
+    if ((rc = fit_run(ctx, centres, targets, K, lambda, precision, r))) return rc;
+    const int N = r.N, m = r.m;
+    std::vector<double> hx((size_t)m * 4);
+    TW_HIP(ctx, hipMemcpyAsync(hx.data(), r.x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     tw_stage_end(ctx, ST_FIT);
     tw_stage_end(ctx, ST_TOTAL);
+    if ((rc = fit_finish(ctx, r.st, r.bmax, r.iters))) return rc;
+    // Undo the scaling: c = c'/s, d = T^-1 d'.
+    const double s = r.s;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < 4; j++) c[4 * i + j] = hx[(size_t)j * m + i] / s;
+    for (int j = 0; j < 4; j++) {
+        const double d0 = hx[(size_t)j * m + N], d1 = hx[(size_t)j * m + N + 1], d2 = hx[(size_t)j * m + N + 2],
+                     d3 = hx[(size_t)j * m + N + 3];
+        d[0 * 4 + j] = d0 - (r.mu[0] * d1 + r.mu[1] * d2 + r.mu[2] * d3) / s;
+        d[1 * 4 + j] = d1 / s;
+        d[2 * 4 + j] = d2 / s;
+        d[3 * 4 + j] = d3 / s;
+    }
     for (int i = 0; i < 4 * N; i++)
         if (!std::isfinite(c[i])) return tw_fail(ctx, TW_E_SINGULAR, "non-finite TPS coefficients");
     return TW_OK;
+}
+
+int tw_tps_fit_dev(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda, int precision,
+                   TwFitDev* out) {
+    int rc;
+    if ((rc = check_fit_args(ctx, centres, targets, K, lambda, precision))) return rc;
+    tw_stage_begin(ctx, ST_FIT);
+    FitRun r;
+    if ((rc = fit_run(ctx, centres, targets, K, lambda, precision, r))) return rc;
+    void* p = nullptr;
+    if ((rc = tw_ws(ctx, WS_COEF, ((size_t)K * 3 + 16) * sizeof(double), &p))) return rc;
+    double* cf = (double*)p;
+    double* d = cf + (size_t)K * 3;
+    hipLaunchKernelGGL(k_unscale, dim3(nblk(K)), dim3(256), 0, ctx->stream, r.x, K, r.s, r.mu[0], r.mu[1], r.mu[2],
+                       cf, d);
+    TW_LAUNCH_CHECK(ctx);
+    tw_stage_end(ctx, ST_FIT);
+    out->pts = r.pts;
+    out->cf = cf;
+    out->d = d;
+    out->state = r.st;
+    out->K = K;
+    out->bmax = r.bmax;
+    ctx->refine_iters = r.iters;
+    return TW_OK;
+}
+
+int tw_tps_fit_check(tw_ctx* ctx, const TwFitDev* fit) {
+    return fit_finish(ctx, (const FitState*)fit->state, fit->bmax, ctx->refine_iters);
 }

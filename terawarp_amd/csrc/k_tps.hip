@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace {
 
@@ -63,7 +64,16 @@ struct TpsArgs {
     i64 zbase;                         // first z block of this launch (exact kernel)
 };
 
-constexpr int TX = 64, TY = 4;
+constexpr int TX = 64, TY = 4;  // reference-order kernel block
+// Fused kernel tile: TX x FTY threads, each walking one (x, y) column over
+// TZ_CELLS grid cells of z.
+#ifndef TW_FUSED_TY
+#define TW_FUSED_TY 4
+#endif
+#ifndef TW_TZ_CELLS
+#define TW_TZ_CELLS 4
+#endif
+constexpr int FTY = TW_FUSED_TY, FT = TX * FTY;
 
 // ---------------------------------------------------------------------------
 // Shared per-voxel resample tail.
@@ -181,6 +191,18 @@ __device__ __forceinline__ uint32_t bld(__amdgpu_buffer_rsrc_t r, uint32_t off) 
     return (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
 }
 
+// The pair (x, x+1) in one access: bits 0-15 / 16-31 (uint16) or 0-7 /
+// 8-15 (uint8).  The access may be unaligned; callers keep x+1 in the row.
+template <typename T>
+__device__ __forceinline__ uint32_t bld2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if (sizeof(T) == 1) return (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ uint32_t pair_lo(uint32_t p) { return sizeof(T) == 1 ? (p & 0xffu) : (p & 0xffffu); }
+template <typename T>
+__device__ __forceinline__ uint32_t pair_hi(uint32_t p) { return sizeof(T) == 1 ? ((p >> 8) & 0xffu) : (p >> 16); }
+
 template <typename T>
 __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t v) {
     if (sizeof(T) == 1) __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v, r, (int)off, 0, 0);
@@ -190,7 +212,10 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, uint
 struct FastGeom {
     int snx, sny, snz, sx0, sy0, sz0;   // source window
     uint32_t row_b, plane_b, chan_b;    // source strides in bytes
+    uint32_t nrec01, nrec10, nrec11;    // channel bytes past +row, +plane, +row+plane
     float fmx, fmy, fmz;                // full extent - 1 (bounds test)
+    int imx, imy, imz;                  // the same as integers
+    float invf;                         // 1 / f
     uint32_t dchan_b;                   // destination channel stride in bytes
     int tiles_x, tiles_y, ntiles, per_xcd;  // tile grid and XCD-major order
     int nxm, nym, nzm;                  // node-tile extents (maxima over the launch's tiles)
@@ -198,13 +223,30 @@ struct FastGeom {
 };
 
 // Tap geometry of one output voxel: byte offset of (x_s, y_s, z_s) in the
-// channel (0 when the sample is out of bounds), exact fractional parts and
-// the in-bounds flag.  Branch-free.
+// channel and the exact fractional parts.  A sample that fails the bounds
+// test gets OOB_OFF, past every buffer's range, so its taps read 0.
 struct Tap {
     uint32_t off;
     float wx, wy, wz;
-    bool ok;
 };
+constexpr uint32_t OOB_OFF = 0x7FFF0000u;
+
+// The source channel through four buffer resources based at +0, +row,
+// +plane and +row+plane bytes, so the eight taps of a voxel share one
+// offset register; each resource's range ends at the channel's end.
+struct SrcRs {
+    __amdgpu_buffer_rsrc_t r00, r01, r10, r11;
+};
+
+__device__ __forceinline__ SrcRs make_src_rs(const void* src, const FastGeom& g) {
+    const char* p = (const char*)src;
+    SrcRs r;
+    r.r00 = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)g.chan_b, BUF_FLAGS);
+    r.r01 = __builtin_amdgcn_make_buffer_rsrc((void*)(p + g.row_b), (short)0, (int)g.nrec01, BUF_FLAGS);
+    r.r10 = __builtin_amdgcn_make_buffer_rsrc((void*)(p + g.plane_b), (short)0, (int)g.nrec10, BUF_FLAGS);
+    r.r11 = __builtin_amdgcn_make_buffer_rsrc((void*)(p + g.row_b + g.plane_b), (short)0, (int)g.nrec11, BUF_FLAGS);
+    return r;
+}
 
 // floor(x) as int in one instruction (v_cvt_flr_i32_f32).
 __device__ __forceinline__ int cvt_flr(float x) {
@@ -213,10 +255,27 @@ __device__ __forceinline__ int cvt_flr(float x) {
     return r;
 }
 
-template <typename T, int INTERP, bool WIN>
+// floor(x + 1/2) as int in one instruction (v_cvt_rpi_i32_f32; x >= 0 here).
+__device__ __forceinline__ uint32_t cvt_rpi(float x) {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return (uint32_t)r;
+}
+
+// a * b + c on 24-bit operands (v_mad_u32_u24; b wave-uniform).
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
+// CHECK = false: the caller has shown that every sample of the cell lies
+// inside the volume and the source window with a voxel to spare.
+template <typename T, int INTERP, bool WIN, bool CHECK>
 __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, float p2) {
     Tap t;
-    const bool inb = (p0 >= 0.f) & (p0 <= g.fmx) & (p1 >= 0.f) & (p1 <= g.fmy) & (p2 >= 0.f) & (p2 <= g.fmz);
+    bool ok = true;
+    if (CHECK) ok = (p0 >= 0.f) & (p0 <= g.fmx) & (p1 >= 0.f) & (p1 <= g.fmy) & (p2 >= 0.f) & (p2 <= g.fmz);
     int ix, iy, iz;
     if (INTERP == TW_INTERP_NN) {
         // (long)(pos + 0.5), clamped to the extent; pos + 0.5 is exact in FP32
@@ -234,44 +293,48 @@ __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, 
         t.wy = __builtin_amdgcn_fractf(p1);
         t.wz = __builtin_amdgcn_fractf(p2);
     }
-    bool win = true;
     if (WIN) {
         ix -= g.sx0;
         iy -= g.sy0;
         iz -= g.sz0;
-        if (INTERP == TW_INTERP_NN)
-            win = (ix >= 0) & (iy >= 0) & (iz >= 0) & (ix < g.snx) & (iy < g.sny) & (iz < g.snz);
-        else
-            win = (ix >= 0) & (iy >= 0) & (iz >= 0) & (ix + (t.wx > 0.f) < g.snx) & (iy + (t.wy > 0.f) < g.sny) &
-                  (iz + (t.wz > 0.f) < g.snz);
     }
-    t.ok = inb & win;
-    const uint32_t e = __umul24(__umul24((uint32_t)iz, (uint32_t)g.sny) + (uint32_t)iy, (uint32_t)g.snx) + (uint32_t)ix;
-    t.off = t.ok ? e * (uint32_t)sizeof(T) : 0u;
+    if (CHECK && INTERP != TW_INTERP_NN) {
+        // A sample on the window's last column has weight 0 on x+1: take the
+        // pair (x-1, x) with weight 1 instead, so no pair leaves its row.
+        // The blend is unchanged: lerp(v(x-1), v(x), 1) = v(x) exactly.
+        const bool last = ix == g.snx - 1;
+        ix -= last ? 1 : 0;
+        t.wx = last ? 1.0f : t.wx;
+    }
+    if (WIN) {
+        if (CHECK) {
+            if (INTERP == TW_INTERP_NN)
+                ok &= (ix >= 0) & (iy >= 0) & (iz >= 0) & (ix < g.snx) & (iy < g.sny) & (iz < g.snz);
+            else
+                ok &= (ix >= 0) & (iy >= 0) & (iz >= 0) & (ix + (t.wx > 0.f) < g.snx) & (iy + (t.wy > 0.f) < g.sny) &
+                      (iz + (t.wz > 0.f) < g.snz);
+        }
+    }
+    const uint32_t e = __umul24(mad24((uint32_t)iz, (uint32_t)g.sny, (uint32_t)iy), (uint32_t)g.snx) + (uint32_t)ix;
+    t.off = e * (uint32_t)sizeof(T);
+    if (CHECK) t.off = ok ? t.off : OOB_OFF;
     return t;
 }
 
-// The eight taps of a voxel through one buffer resource.  The x+1 taps use
-// the instruction's immediate offset; y+1 / z+1 add a row / plane.  Taps of
-// zero weight may land in the next row or plane, or past the channel, where
-// the buffer range check returns 0.
+// The eight taps of a voxel as four (x, x+1) pairs: y+1 / z+1 through the
+// shifted resources.  Pairs of zero weight may land in the next plane or
+// past the channel, where the range check returns 0.
 template <typename T, int MODE, int INTERP>
-__device__ __forceinline__ uint32_t blend(__amdgpu_buffer_rsrc_t rs, const Tap& t, uint32_t row, uint32_t plane) {
-    uint32_t off = t.off;
-    __builtin_assume(off <= 0x7FFF0000u);
-    if (INTERP == TW_INTERP_NN) {
-        const uint32_t v = bld<T>(rs, off);
-        return t.ok ? v : 0u;
-    }
-    const uint32_t o1 = off + row, o2 = off + plane, o3 = o1 + plane;
-    __builtin_assume(o1 <= 0x7FFF0000u);
-    __builtin_assume(o2 <= 0x7FFF0000u);
-    __builtin_assume(o3 <= 0x7FFF0000u);
-    const uint32_t u000 = bld<T>(rs, off), u100 = bld<T>(rs, off + (uint32_t)sizeof(T));
-    const uint32_t u010 = bld<T>(rs, o1), u110 = bld<T>(rs, o1 + (uint32_t)sizeof(T));
-    const uint32_t u001 = bld<T>(rs, o2), u101 = bld<T>(rs, o2 + (uint32_t)sizeof(T));
-    const uint32_t u011 = bld<T>(rs, o3), u111 = bld<T>(rs, o3 + (uint32_t)sizeof(T));
-    uint32_t v;
+__device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
+    const uint32_t off = t.off;
+    __builtin_assume(off <= OOB_OFF);
+    if (INTERP == TW_INTERP_NN) return bld<T>(rs.r00, off);
+    const uint32_t p00 = bld2<T>(rs.r00, off), p10 = bld2<T>(rs.r01, off);
+    const uint32_t p01 = bld2<T>(rs.r10, off), p11 = bld2<T>(rs.r11, off);
+    const uint32_t u000 = pair_lo<T>(p00), u100 = pair_hi<T>(p00);
+    const uint32_t u010 = pair_lo<T>(p10), u110 = pair_hi<T>(p10);
+    const uint32_t u001 = pair_lo<T>(p01), u101 = pair_hi<T>(p01);
+    const uint32_t u011 = pair_lo<T>(p11), u111 = pair_hi<T>(p11);
     if (MODE == TW_PLAIN) {
         // The reference's FP64 blend (q_imgwarp_tps_quicksmallmemory.cpp:141-160);
         // 1 - (pos - floor(pos)) is exact, so the weights equal the reference's.
@@ -280,24 +343,36 @@ __device__ __forceinline__ uint32_t blend(__amdgpu_buffer_rsrc_t rs, const Tap& 
         const double uw = 1.0 - (double)t.wz, dw = 1.0 - uw;
         const double hi = tw * (lw * (double)u000 + rw * (double)u100) + bw * (lw * (double)u010 + rw * (double)u110);
         const double lo = tw * (lw * (double)u001 + rw * (double)u101) + bw * (lw * (double)u011 + rw * (double)u111);
-        v = (uint32_t)(uw * hi + dw * lo + 0.5);
-    } else {
-        const float v000 = (float)u000, v100 = (float)u100, v010 = (float)u010, v110 = (float)u110;
-        const float v001 = (float)u001, v101 = (float)u101, v011 = (float)u011, v111 = (float)u111;
-        const float a00 = __builtin_fmaf(t.wx, v100 - v000, v000);
-        const float a10 = __builtin_fmaf(t.wx, v110 - v010, v010);
-        const float a01 = __builtin_fmaf(t.wx, v101 - v001, v001);
-        const float a11 = __builtin_fmaf(t.wx, v111 - v011, v011);
-        const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
-        const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
-        v = (uint32_t)(__builtin_fmaf(t.wz, b1 - b0, b0) + 0.5f);
+        return (uint32_t)(uw * hi + dw * lo + 0.5);
     }
-    return t.ok ? v : 0u;
+    const float v000 = (float)u000, v100 = (float)u100, v010 = (float)u010, v110 = (float)u110;
+    const float v001 = (float)u001, v101 = (float)u101, v011 = (float)u011, v111 = (float)u111;
+    const float a00 = __builtin_fmaf(t.wx, v100 - v000, v000);
+    const float a10 = __builtin_fmaf(t.wx, v110 - v010, v010);
+    const float a01 = __builtin_fmaf(t.wx, v101 - v001, v001);
+    const float a11 = __builtin_fmaf(t.wx, v111 - v011, v011);
+    const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
+    const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
+    return cvt_rpi(__builtin_fmaf(t.wz, b1 - b0, b0));
+}
+
+// floor(i / d) for 0 <= i < 2^21 as trunc((i + 1/2) / d) in FP32, with
+// inv = fl(1/d), hinv = inv / 2: (i + 1/2) / d sits at least 1/(2d) from an
+// integer and the FP32 evaluation errs by under 3 * 2^-24 * i / d < 3/(8d).
+__device__ __forceinline__ int div_f(int i, float inv, float hinv) {
+    return (int)__builtin_fmaf((float)i, inv, hinv);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
 template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF, bool WIN>
-__global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGeom g) {
+__global__ __launch_bounds__(FT) void k_tps_fused(const TpsArgs a, const FastGeom g) {
     extern __shared__ float lds[];
+    __shared__ float rmax_w[FTY];
     const int f = a.f;
     const int NXM = g.nxm, NYM = g.nym, NZM = g.nzm;
     float* nodes = lds;                      // [3][NZM][NYM][NXM]
@@ -312,54 +387,80 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
     const int tile = (wg & 7) * g.per_xcd + (wg >> 3);
     if (tile >= g.ntiles) return;
     const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
-    const int tx0 = tix * TX, ty0 = tiy * TY, tz0 = tiz * a.tz;
-    const int tx1 = min(tx0 + TX, dnx) - 1, ty1 = min(ty0 + TY, dny) - 1, tz1 = min(tz0 + a.tz, dnz) - 1;
+    const int tx0 = tix * TX, ty0 = tiy * FTY, tz0 = tiz * a.tz;
+    const int tx1 = min(tx0 + TX, dnx) - 1, ty1 = min(ty0 + FTY, dny) - 1, tz1 = min(tz0 + a.tz, dnz) - 1;
     // Window-relative coordinates of the grid (r = global - origin >= 0).
     const int rx0 = (int)(a.dx0 - a.ox), ry0 = (int)(a.dy0 - a.oy), rz0 = (int)(a.dz0 - a.oz);
     const int nx0 = (rx0 + tx0) / f, ny0 = (ry0 + ty0) / f, nz0 = (rz0 + tz0) / f;
     const int nxn = (rx0 + tx1) / f + 4 - nx0;
     const int nyn = (ry0 + ty1) / f + 4 - ny0;
     const int nzn = (rz0 + tz1) / f + 4 - nz0;
+    const float ivx = 1.0f / (float)nxn, ivy = 1.0f / (float)nyn;
 
-    const i64 plane = a.gx * a.gy * a.gz;
+    // Stage the tile's nodes (24-bit index arithmetic: the host keeps the
+    // grid's planes below 2^31 values).
+    const uint32_t plane = (uint32_t)(a.gx * a.gy * a.gz);
     const int NP = NZM * NYM * NXM;
     const int ntile = nxn * nyn * nzn;
-    for (int i = tid; i < ntile; i += 256) {
-        const int ix = i % nxn, iy = (i / nxn) % nyn, iz = i / (nxn * nyn);
-        const i64 gi = ((i64)(nz0 + iz) * a.gy + (ny0 + iy)) * a.gx + (nx0 + ix);
+    float rm = 0.f;  // largest |displacement| among this thread's nodes
+    for (int i = tid; i < ntile; i += FT) {
+        const int q = div_f(i, ivx, 0.5f * ivx), iz = div_f(q, ivy, 0.5f * ivy);
+        const int ix = i - q * nxn, iy = q - iz * nyn;
+        const uint32_t gi = mad24(mad24((uint32_t)(nz0 + iz), (uint32_t)a.gy, (uint32_t)(ny0 + iy)), (uint32_t)a.gx,
+                                  (uint32_t)(nx0 + ix));
         const int li = (iz * NYM + iy) * NXM + ix;
+        float v0, v1, v2;
         if (F16) {
             const __half* gh = (const __half*)a.grid;
-            const i64 qb = ((i64)((nz0 + iz) >> 3) * a.qby + ((ny0 + iy) >> 3)) * a.qbx + ((nx0 + ix) >> 3);
-            const float* q = a.qtab + qb * 6;
-            nodes[li] = __builtin_fmaf(q[3], __half2float(gh[gi]), q[0]);
-            nodes[NP + li] = __builtin_fmaf(q[4], __half2float(gh[plane + gi]), q[1]);
-            nodes[2 * NP + li] = __builtin_fmaf(q[5], __half2float(gh[2 * plane + gi]), q[2]);
+            const uint32_t qb = mad24(mad24((uint32_t)((nz0 + iz) >> 3), (uint32_t)a.qby, (uint32_t)((ny0 + iy) >> 3)),
+                                      (uint32_t)a.qbx, (uint32_t)((nx0 + ix) >> 3));
+            const float* qt = a.qtab + qb * 6;
+            v0 = __builtin_fmaf(qt[3], __half2float(gh[gi]), qt[0]);
+            v1 = __builtin_fmaf(qt[4], __half2float(gh[plane + gi]), qt[1]);
+            v2 = __builtin_fmaf(qt[5], __half2float(gh[2 * plane + gi]), qt[2]);
         } else {
             const float* gf = (const float*)a.grid;
-            nodes[li] = gf[gi];
-            nodes[NP + li] = gf[plane + gi];
-            nodes[2 * NP + li] = gf[2 * plane + gi];
+            v0 = gf[gi];
+            v1 = gf[plane + gi];
+            v2 = gf[2 * plane + gi];
         }
+        nodes[li] = v0;
+        nodes[NP + li] = v1;
+        nodes[2 * NP + li] = v2;
+        rm = fmaxf(rm, fmaxf(fabsf(v0), fmaxf(fabsf(v1), fabsf(v2))));
     }
+    rm = wave_max(rm);
+    if (lxi == 0) rmax_w[lyi] = rm;
+    // Node index and phase of a column: base / f for the tile (scalar) plus
+    // (base % f + lane) / f < 2 + 64 / f for the lane.
+    auto split = [&](int base, int lane, int n0, int& b, int& ph) {
+        const int q0 = base / f, r0 = base - q0 * f;
+        const int rr = r0 + lane;
+        const int qq = div_f(rr, g.invf, 0.5f * g.invf);
+        b = q0 + qq - n0;
+        ph = rr - qq * f;
+    };
     __syncthreads();
     // Collapse along x for each of the tile's 64 columns.
     {
-        const int rx = rx0 + tx0 + lxi;
-        const int bxl = rx / f - nx0;
-        const int txp = rx - (rx / f) * f;
+        int bxl, txp;
+        split(rx0 + tx0, lxi, nx0, bxl, txp);
         const bool live = tx0 + lxi <= tx1;
         const float w0 = g.wt[txp * 4 + 0], w1 = g.wt[txp * 4 + 1], w2 = g.wt[txp * 4 + 2], w3 = g.wt[txp * 4 + 3];
-        for (int r = lyi; r < nzn * nyn; r += TY) {
-            const int iy = r % nyn, iz = r / nyn;
-            #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float v = 0.f;
-                if (live) {
-                    const float* nrow = nodes + c * NP + (iz * NYM + iy) * NXM + bxl;
-                    v = __builtin_fmaf(w3, nrow[3], __builtin_fmaf(w2, nrow[2], __builtin_fmaf(w1, nrow[1], w0 * nrow[0])));
+        const int nstep = NYM * NXM, xstep = NYM * TX, xplane = NZM * NYM * TX;
+        for (int iy = lyi; iy < nyn; iy += FTY) {
+            const float* nrow = nodes + iy * NXM + bxl;
+            float* xrow = xc + iy * TX + lxi;
+            for (int iz = 0; iz < nzn; iz++) {
+                #pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float* nr = nrow + c * NP;
+                    xrow[c * xplane] =
+                        live ? __builtin_fmaf(w3, nr[3], __builtin_fmaf(w2, nr[2], __builtin_fmaf(w1, nr[1], w0 * nr[0])))
+                             : 0.f;
                 }
-                xc[((c * NZM + iz) * NYM + iy) * TX + lxi] = v;
+                nrow += nstep;
+                xrow += xstep;
             }
         }
     }
@@ -368,18 +469,20 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
 
     const int lx = tx0 + lxi, ly = ty0 + lyi;
     const float Xf = (float)((int)a.dx0 + lx), Yf = (float)((int)a.dy0 + ly);
-    const int ry = ry0 + ly;
-    const int byl = ry / f - ny0;
-    const int typ = ry - (ry / f) * f;
+    int byl, typ;
+    split(ry0 + ty0, lyi, ny0, byl, typ);
     const float wy0 = g.wt[typ * 4 + 0], wy1 = g.wt[typ * 4 + 1], wy2 = g.wt[typ * 4 + 2], wy3 = g.wt[typ * 4 + 3];
     const uint32_t dvol = (uint32_t)dnx * (uint32_t)dny * (uint32_t)dnz;
 
+    // Layer k of the thread's column: xcol + k * lstep (+ c * cstep).
+    const float* xcol = xc + byl * TX + lxi;
+    const int lstep = NYM * TX, cstep = NZM * NYM * TX;
     auto layer = [&](int iz, float* L) {
+        const float* col = xcol + iz * lstep;
         #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float* col = xc + ((c * NZM + iz) * NYM + byl) * TX + lxi;
-            L[c] = __builtin_fmaf(wy3, col[3 * TX], __builtin_fmaf(wy2, col[2 * TX],
-                                  __builtin_fmaf(wy1, col[TX], wy0 * col[0])));
+            const float* cc = col + c * cstep;
+            L[c] = __builtin_fmaf(wy3, cc[3 * TX], __builtin_fmaf(wy2, cc[2 * TX], __builtin_fmaf(wy1, cc[TX], wy0 * cc[0])));
         }
     };
     // z walk: the phase advances by one per slice, the node window by one
@@ -400,73 +503,89 @@ __global__ __launch_bounds__(256) void k_tps_fused(const TpsArgs a, const FastGe
         for (int c = 0; c < 3; c++)
             d[c] = __builtin_fmaf(wz3, L3[c], __builtin_fmaf(wz2, L2[c], __builtin_fmaf(wz1, L1[c], wz0 * L0[c])));
     };
-    auto df_at = [&](float* d) { df_ph(d, tzp); };
-    // Advance one slice; `more` says whether another slice follows (the new
-    // node layer is only read when it exists in the tile).
-    auto advance = [&](bool more) {
-        if (++tzp == f) {
-            tzp = 0;
-            bzl++;
-            if (more) {
-                #pragma unroll
-                for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
-                layer(bzl + 3, L3);
-            }
-        }
-    };
-
-    if (WRITE_DF) {
-        for (int lz = 0; lz < nzv; lz++) {
-            float d[3];
-            df_at(d);
-            a.dfout[o] = d[0];
-            a.dfout[dvol + o] = d[1];
-            a.dfout[2 * dvol + o] = d[2];
-            o += ostep;
-            advance(lz + 1 < nzv);
-        }
-        return;
-    }
-    // One channel per launch (the host offsets src / dst per channel).
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)a.src, (short)0, (int)g.chan_b, BUF_FLAGS);
-    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(a.dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
-    // Cell by cell: the node-layer window moves once per cell; inside a
-    // cell two slices per iteration keep both voxels' 16 taps in flight.
-    int lz = 0;
-    while (true) {
-        const int cnt = min(f - tzp, nzv - lz);
-        for (int k = 0; k < cnt; k += 2) {
-            const bool hasB = k + 1 < cnt;
-            float dA[3], dB[3];
-            df_ph(dA, tzp);
-            df_ph(dB, hasB ? tzp + 1 : tzp);
-            tzp += 2;
-            const Tap tA = tap_setup<T, INTERP, WIN>(g, Xf + dA[0], Yf + dA[1], Zf + dA[2]);
-            const Tap tB = tap_setup<T, INTERP, WIN>(g, Xf + dB[0], Yf + dB[1], (Zf + 1.0f) + dB[2]);
-            const uint32_t vA = blend<T, MODE, INTERP>(rs, tA, g.row_b, g.plane_b);
-            const uint32_t vB = blend<T, MODE, INTERP>(rs, tB, g.row_b, g.plane_b);
-            bst<T>(rd, o * (uint32_t)sizeof(T), vA);
-            if (hasB) bst<T>(rd, (o + ostep) * (uint32_t)sizeof(T), vB);
-            o += 2 * ostep;
-            Zf += 2.0f;
-        }
-        // An odd count advanced one slice too far: step back.
-        if (cnt & 1) {
-            o -= ostep;
-            Zf -= 1.0f;
-        }
-        lz += cnt;
-        if (lz >= nzv) break;
+    auto next_cell = [&]() {
         tzp = 0;
         bzl++;
         #pragma unroll
         for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
         layer(bzl + 3, L3);
+    };
+
+    if (WRITE_DF) {
+        for (int lz = 0; lz < nzv; lz++) {
+            float d[3];
+            df_ph(d, tzp);
+            a.dfout[o] = d[0];
+            a.dfout[dvol + o] = d[1];
+            a.dfout[2 * dvol + o] = d[2];
+            o += ostep;
+            if (++tzp == f && lz + 1 < nzv) next_cell();
+        }
+        return;
+    }
+    // Samples of this tile lie within rmax (+ one voxel for rounding) of
+    // their voxel, rmax bounding the field (a convex combination of the
+    // staged nodes).  Cells whose samples all stay a voxel inside the
+    // volume and the source window skip the per-voxel bounds test.
+    // (Integer margin R = ceil(rmax) + 1, clamped past any coordinate, so
+    // the tests below are scalar.)
+    float rmx = rmax_w[0];
+    #pragma unroll
+    for (int w = 1; w < FTY; w++) rmx = fmaxf(rmx, rmax_w[w]);
+    const int R = __builtin_amdgcn_readfirstlane((int)ceilf(fminf(rmx, 16777216.f))) + 1;
+    auto lo_bound = [](int s0) { return max(0, s0); };
+    auto hi_bound = [](int im, int s0, int sn) { return min(im, s0 + sn - 1) - 1; };
+    const int loz = lo_bound(g.sz0), hiz = hi_bound(g.imz, g.sz0, g.snz);
+    const bool fast_xy = (int)a.dx0 + tx0 - R >= lo_bound(g.sx0) && (int)a.dx0 + tx1 + R <= hi_bound(g.imx, g.sx0, g.snx) &&
+                         (int)a.dy0 + ty0 - R >= lo_bound(g.sy0) && (int)a.dy0 + ty1 + R <= hi_bound(g.imy, g.sy0, g.sny);
+    // One channel per launch (the host offsets src / dst per channel).
+    const SrcRs rs = make_src_rs(a.src, g);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(a.dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
+    // Cell by cell: the node-layer window moves once per cell; inside a
+    // cell four slices per iteration keep 32 taps in flight (a short last
+    // group repeats its last slice and stores only the real ones).
+    auto group = [&](auto check, auto full, int nh) {
+        constexpr bool CHECK = decltype(check)::value, FULL = decltype(full)::value;
+        Tap tp[4];
+        #pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int jj = FULL ? j : min(j, nh - 1);
+            float d[3];
+            df_ph(d, tzp + jj);
+            const float zj = FULL ? Zf + (float)j : Zf + (float)jj;
+            tp[j] = tap_setup<T, INTERP, WIN, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2]);
+        }
+        uint32_t v[4];
+        #pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
+        #pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (FULL || j < nh) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
+        tzp += nh;
+        o += (uint32_t)nh * ostep;
+        Zf += (float)nh;
+    };
+    auto cell = [&](auto check, int cnt) {
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) group(check, std::true_type{}, 4);
+        if (k < cnt) group(check, std::false_type{}, cnt - k);
+    };
+    int lz = 0;
+    while (true) {
+        const int cnt = min(f - tzp, nzv - lz);
+        const int zc = (int)a.dz0 + tz0 + lz;  // first slice of the cell
+        if (fast_xy && zc - R >= loz && zc + cnt - 1 + R <= hiz)
+            cell(std::false_type{}, cnt);
+        else
+            cell(std::true_type{}, cnt);
+        lz += cnt;
+        if (lz >= nzv) break;
+        next_cell();
     }
 }
 
 size_t fused_lds_bytes(int f, int tz) {
-    const int nxm = node_span(TX, f), nym = node_span(TY, f), nzm = node_span(tz, f);
+    const int nxm = node_span(TX, f), nym = node_span(FTY, f), nzm = node_span(tz, f);
     return sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
 }
 
@@ -492,7 +611,7 @@ int max_node_span(i64 r0, int len, i64 n, int f) {
 // Voxels per work-group along z: 4 cells, clamped to [16, 64], halved
 // until the LDS tile fits in 64 KiB.
 int pick_tz(int f) {
-    int tz = 4 * f;
+    int tz = TW_TZ_CELLS * f;
     if (tz < 16) tz = 16;
     if (tz > 64) tz = 64;
     while (tz > 1 && fused_lds_bytes(f, tz) > 64 * 1024) tz /= 2;
@@ -502,7 +621,9 @@ int pick_tz(int f) {
 template <typename T, bool WRITE_DF>
 int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     const dim3 block(TX, TY, 1);
-    if (precision == TW_EXACT) {
+    // A source window one voxel wide has no (x, x+1) pairs: the reference-
+    // order kernel serves it (its results are the FP64 reference's).
+    if (precision == TW_EXACT || (!WRITE_DF && a.snx < 2)) {
         const i64 zmax = 65535;
         for (i64 z0 = 0; z0 < a.dnz; z0 += zmax) {
             a.zbase = z0;
@@ -521,7 +642,7 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     // Exact node-tile extents of this launch (tiles aligned to the grid need
     // fewer nodes than the worst case), which sets the LDS per work-group.
     const int nxm = max_node_span(a.dx0 - a.ox, TX, a.dnx, a.f);
-    const int nym = max_node_span(a.dy0 - a.oy, TY, a.dny, a.f);
+    const int nym = max_node_span(a.dy0 - a.oy, FTY, a.dny, a.f);
     const int nzm = max_node_span(a.dz0 - a.oz, a.tz, a.dnz, a.f);
     const size_t lds = sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
     if (lds > 160 * 1024) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d needs %zu B of LDS", a.f, lds);
@@ -534,7 +655,7 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     const uint64_t taps = schan + (uint64_t)a.snx * (a.sny + 1) * sizeof(T) + 16;  // offsets of every tap
     const i64 cmax = std::max({a.dx0 + a.dnx, a.dy0 + a.dny, a.dz0 + a.dnz, a.fnx, a.fny, a.fnz});
     if ((!WRITE_DF && (taps > 0x7FFF0000ull || a.sny * a.snz >= (1ll << 24))) || dchan > 0xF0000000ull ||
-        cmax > (1ll << 24))
+        cmax > (1ll << 24) || a.gy * a.gz >= (1ll << 24) || 3 * a.gx * a.gy * a.gz >= (1ll << 31))
         return tw_fail(ctx, TW_E_UNSUPPORTED,
                        "window exceeds the fused kernel's 32-bit addressing (split the volume into bricks)");
     FastGeom g;
@@ -545,7 +666,13 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     g.row_b = (uint32_t)(a.snx * sizeof(T));
     g.plane_b = (uint32_t)(a.snx * a.sny * sizeof(T));
     g.chan_b = (uint32_t)schan;
+    auto past = [&](uint64_t skip) { return (uint32_t)(schan > skip ? schan - skip : 0); };
+    g.nrec01 = past(g.row_b);
+    g.nrec10 = past(g.plane_b);
+    g.nrec11 = past((uint64_t)g.row_b + g.plane_b);
     g.fmx = (float)(a.fnx - 1); g.fmy = (float)(a.fny - 1); g.fmz = (float)(a.fnz - 1);
+    g.imx = (int)(a.fnx - 1); g.imy = (int)(a.fny - 1); g.imz = (int)(a.fnz - 1);
+    g.invf = 1.0f / (float)a.f;
     g.dchan_b = (uint32_t)(a.dnx * a.dny * a.dnz * sizeof(T));
     {
         std::vector<double> tb((size_t)a.f * 4);
@@ -555,19 +682,19 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     // The source window is the whole volume: no window test needed.
     const bool win = !(a.sx0 == 0 && a.sy0 == 0 && a.sz0 == 0 && a.snx == a.fnx && a.sny == a.fny && a.snz == a.fnz);
     g.tiles_x = (int)((a.dnx + TX - 1) / TX);
-    g.tiles_y = (int)((a.dny + TY - 1) / TY);
+    g.tiles_y = (int)((a.dny + FTY - 1) / FTY);
     const i64 ntiles = (i64)g.tiles_x * g.tiles_y * nbz;
     if (ntiles > (1ll << 30)) return tw_fail(ctx, TW_E_UNSUPPORTED, "too many tiles for one launch");
     g.ntiles = (int)ntiles;
     g.per_xcd = (int)((ntiles + 7) / 8);
-    const dim3 grid((unsigned)(8 * g.per_xcd), 1, 1);
+    const dim3 grid((unsigned)(8 * g.per_xcd), 1, 1), fblock(TX, FTY, 1);
     const void* src0 = a.src;
     void* dst0 = a.dst;
     for (int ch = 0; ch < (WRITE_DF ? 1 : (int)a.nc); ch++) {
     a.src = src0 ? (const void*)((const uint8_t*)src0 + (size_t)ch * a.src_cstride) : nullptr;
     a.dst = dst0 ? (void*)((uint8_t*)dst0 + (size_t)ch * a.dst_cstride) : nullptr;
 #define TW_FUSED(M, H, I, W) \
-    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF, W>), grid, block, lds, ctx->stream, a, g)
+    hipLaunchKernelGGL((k_tps_fused<T, M, H, I, WRITE_DF, W>), grid, fblock, lds, ctx->stream, a, g)
     if (WRITE_DF) {
         if (f16) TW_FUSED(TW_MIXED, true, TW_INTERP_LINEAR, false);
         else TW_FUSED(TW_MIXED, false, TW_INTERP_LINEAR, false);
@@ -593,19 +720,24 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     return TW_OK;
 }
 
+// FP64 and FP32 B-spline weight tables of spacing f on the device, cached
+// per context: uploaded only when f changes (ordered on the stream after
+// every earlier use of the old tables).
 int upload_weights(tw_ctx* ctx, int f, const float** w32, const double** w64) {
-    std::vector<double> tb((size_t)f * 4);
-    tw_bspline_tb(f, tb.data());
-    const size_t b64 = tb.size() * sizeof(double), b32 = tb.size() * sizeof(float);
+    const size_t n = (size_t)f * 4, b64 = n * sizeof(double), b32 = n * sizeof(float);
     void* p = nullptr;
     int rc = tw_ws(ctx, WS_WEIGHTS, b64 + b32, &p);
     if (rc) return rc;
-    std::vector<uint8_t> host(b64 + b32);
-    memcpy(host.data(), tb.data(), b64);
-    float* h32 = (float*)(host.data() + b64);
-    for (size_t i = 0; i < tb.size(); i++) h32[i] = (float)tb[i];
-    TW_HIP(ctx, hipMemcpyAsync(p, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
-    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->weights_f != f || ctx->weights_host.size() != b64 + b32) {
+        std::vector<double> tb(n);
+        tw_bspline_tb(f, tb.data());
+        ctx->weights_host.assign(b64 + b32, 0);
+        memcpy(ctx->weights_host.data(), tb.data(), b64);
+        float* h32 = (float*)(ctx->weights_host.data() + b64);
+        for (size_t i = 0; i < n; i++) h32[i] = (float)tb[i];
+        TW_HIP(ctx, hipMemcpyAsync(p, ctx->weights_host.data(), b64 + b32, hipMemcpyHostToDevice, ctx->stream));
+        ctx->weights_f = f;
+    }
     *w64 = (const double*)p;
     *w32 = (const float*)((uint8_t*)p + b64);
     return TW_OK;
@@ -765,7 +897,17 @@ extern "C" int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst
     if ((rc = tw_bind(ctx))) return rc;
     tw_stages_reset(ctx);
     tw_stage_begin(ctx, ST_TOTAL);
+    if ((rc = tw_resample_tps_run(ctx, src, dst, grid, interp, precision))) return rc;
+    tw_stage_end(ctx, ST_TOTAL);
+    if (src->mem == TW_MEM_HOST || grid->mem == TW_MEM_HOST) TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TW_OK;
+}
 
+// The resample of tw_resample_tps on validated arguments, without a stage
+// reset (the chained warp records its stages in one set).
+int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
+                        int precision) {
+    int rc;
     TpsArgs a;
     memset(&a, 0, sizeof a);
     const void* sdev = nullptr;
@@ -800,10 +942,7 @@ extern "C" int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst
                                  : launch_tps<uint16_t, false>(ctx, a, interp, precision, f16);
     if (rc) return rc;
     tw_stage_end(ctx, ST_RESAMPLE);
-    if ((rc = tw_stage_out_finish(ctx, dst, ddev))) return rc;
-    tw_stage_end(ctx, ST_TOTAL);
-    if (src->mem == TW_MEM_HOST || grid->mem == TW_MEM_HOST) TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return TW_OK;
+    return tw_stage_out_finish(ctx, dst, ddev);
 }
 
 extern "C" int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int64_t ny, int64_t nz, int64_t x0,

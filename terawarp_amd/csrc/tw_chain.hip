@@ -3,9 +3,11 @@
 // Reference: GPU branch of imgwarp_smallmemory_image
 // (q_imgwarp_tps_quicksmallmemory.cpp:1997-2162): STPS fit and grid DF
 // (compute_df_stps_subsampled_volume_4bspline, 2066), then the fused
-// upsample + resample (gpu_interpolation_stps, 2143).  Here the volume is
-// staged once, the grid never leaves the device, and no stage resets the
-// device.
+// upsample + resample (gpu_interpolation_stps, 2143).  Here every stage
+// stays on the device and on the context's stream: the fit leaves its
+// coefficients where the grid kernel reads them, the grid never leaves the
+// device, nothing resets the device, and the stream is synchronised once,
+// at the end, where the fit's status is checked.
 #include "tw_internal.h"
 
 #include <cmath>
@@ -16,17 +18,23 @@ extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, co
     if (!ctx) return TW_E_ARG;
     int rc;
     if ((rc = tw_check_volume(ctx, src, "src")) || (rc = tw_check_volume(ctx, dst, "dst"))) return rc;
+    if (src->dtype != dst->dtype || src->nc != dst->nc)
+        return tw_fail(ctx, TW_E_ARG, "src and dst must share dtype and channel count");
+    if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
+        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN");
+    if (precision != TW_EXACT && precision != TW_PLAIN && precision != TW_MIXED)
+        return tw_fail(ctx, TW_E_ARG, "unknown precision %d", precision);
     if (f < 1) return tw_fail(ctx, TW_E_ARG, "grid spacing must be >= 1");
     // Block-size checks of imgwarp_smallmemory_image (2011-2020).
     if (f >= src->nx || f >= src->ny || f >= src->nz)
         return tw_fail(ctx, TW_E_ARG, "block size should be smaller than the image size");
     if ((rc = tw_bind(ctx))) return rc;
+    tw_stages_reset(ctx);
+    tw_stage_begin(ctx, ST_TOTAL);
 
-    std::vector<double> c((size_t)K * 4), d(16);
+    TwFitDev fit;
     const int fit_prec = precision == TW_MIXED ? TW_MIXED : TW_PLAIN;
-    if ((rc = tw_tps_fit(ctx, centres, targets, K, lambda, fit_prec, c.data(), d.data()))) return rc;
-    tw_timings tf;
-    tw_get_timings(ctx, &tf);
+    if ((rc = tw_tps_fit_dev(ctx, centres, targets, K, lambda, fit_prec, &fit))) return rc;
 
     // Grid over the output window (global coordinates of dst).
     tw_grid g = {};
@@ -41,21 +49,19 @@ extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, co
     g.storage = precision == TW_MIXED ? TW_DF_F16 : TW_DF_F32;
     const size_t plane = (size_t)g.gx * g.gy * g.gz;
     void* gbuf = nullptr;
-    // The grid buffer lives in the WS_DF slot: the grid kernel stages its own
-    // FP32 copy in WS_GRID when quantising.
+    // The grid buffer lives in the WS_DF slot (the grid kernel stages its
+    // FP32 field in WS_GRID when quantising); the FP16 block table in WS_QTAB.
     if ((rc = tw_ws(ctx, WS_DF, plane * 3 * (g.storage == TW_DF_F16 ? 2 : 4), &gbuf))) return rc;
     g.data = gbuf;
-    if ((rc = tw_tps_grid_eval(ctx, centres, K, c.data(), d.data(), precision, &g))) return rc;
-    tw_timings tg;
-    tw_get_timings(ctx, &tg);
-    rc = tw_resample_tps(ctx, src, dst, &g, interp, precision);
-    tw_timings tr;
-    tw_get_timings(ctx, &tr);
-    tw_grid_release(ctx, &g);
-    // Report the stages of the whole chain.
-    ctx->refine_iters = tf.refine_iters;
-    ctx->fit_residual = tf.fit_residual;
-    (void)tg;
-    (void)tr;
-    return rc;
+    if (g.storage == TW_DF_F16) {
+        const size_t nb = (size_t)((g.gx + 7) / 8) * ((g.gy + 7) / 8) * ((g.gz + 7) / 8);
+        void* q = nullptr;
+        if ((rc = tw_ws(ctx, WS_QTAB, nb * 6 * sizeof(float), &q))) return rc;
+        g.qtab = (float*)q;
+    }
+    if ((rc = tw_grid_eval_dev(ctx, fit.pts, fit.cf, fit.d, K, precision, &g))) return rc;
+    if ((rc = tw_resample_tps_run(ctx, src, dst, &g, interp, precision))) return rc;
+    tw_stage_end(ctx, ST_TOTAL);
+    // One synchronisation: the fit's status (a failed fit leaves dst unspecified).
+    return tw_tps_fit_check(ctx, &fit);
 }

@@ -28,6 +28,7 @@ enum TwSlot {
     WS_DF,          // dense DF staging
     WS_WEIGHTS,     // B-spline weight tables
     WS_AUX_BOUNDS,  // grid layer bounds
+    WS_COEF,        // device-side grid coefficients of the chained warp
     WS_COUNT
 };
 
@@ -48,6 +49,10 @@ struct tw_ctx {
     double fit_residual = 0.0;
     // Device properties used for launch shaping.
     int num_cus = 256;
+    // B-spline weight tables on the device (WS_WEIGHTS) are for spacing
+    // weights_f (0: none yet); their host image must outlive the copy.
+    int weights_f = 0;
+    std::vector<uint8_t> weights_host;
 };
 
 // Set the context error string and return code.
@@ -94,3 +99,24 @@ void tw_bspline_tb(int n, double* TB);
 
 // Internal launchers shared between translation units.
 int tw_grid_quantize_f16(tw_ctx* ctx, const float* df32, tw_grid* grid);
+
+// Device-resident pieces of the chained warp (no host round trip, no
+// stage reset).  A fit left on the device: the centres (float4), the
+// grid-kernel coefficients (K x 3 FP64, columns 1..3 of c), the affine
+// block d (4 x 4 FP64) and the fit state, checked later by
+// tw_tps_fit_check (which synchronises the stream).
+struct TwFitDev {
+    const float4* pts;
+    const double* cf;
+    const double* d;
+    void* state;
+    int K;
+    double bmax;  // largest |target| (scale of the fit residual)
+};
+int tw_tps_fit_dev(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda, int precision,
+                   TwFitDev* out);
+int tw_tps_fit_check(tw_ctx* ctx, const TwFitDev* fit);
+int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const double* d, int K, int precision,
+                     tw_grid* grid);
+int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
+                        int precision);

@@ -29,6 +29,22 @@ def rel_err(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def test_fit_fp32_refinement_large(warper):
+    """Above 4096 projected unknowns TW_MIXED factors in FP32 and refines the
+    solution with FP64 residuals (an FP64 refactor if refinement stalls);
+    either way the coefficients match the FP64 factor's."""
+    sub, tar = synth.landmarks((2048, 2048, 512), 4200, 20.0, seed=7)
+    c64, d64 = warper.tps_fit(tar, sub, precision=warp.PLAIN)
+    r64 = warper.timings()["fit_residual"]
+    c32, d32 = warper.tps_fit(tar, sub, precision=warp.MIXED)
+    t = warper.timings()
+    print("refine_iters", t["refine_iters"], "residual", t["fit_residual"], "fp64 residual", r64)
+    assert t["refine_iters"] != 0
+    assert rel_err(c32, c64) < 1e-8, rel_err(c32, c64)
+    assert rel_err(d32, d64) < 1e-8, rel_err(d32, d64)
+    assert t["fit_residual"] < 1e-10
+
+
 @pytest.mark.parametrize("K", [16, 64, 512])
 @pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
 def test_fit_matches_oracle(warper, oracle, K, precision):
