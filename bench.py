@@ -144,6 +144,94 @@ def oracle_slab_resample(oracle, vol, df_slab, zb0, zb1, threads):
 
 MIXED_KERNEL = "k_tps_fused<unsigned short, 1, true, 0, false, false>"
 
+# The precision configuration: 2048 x 2048 x 512 uint16 (4 GiB), K = 4096
+# markers on a jittered 16^3 lattice with 10 % uniform outliers, |d| <= 40,
+# grid spacing 8.
+SIZE_P = (2048, 2048, 512)
+K_P, DMAX_P, G_P, SEED_P = 4096, 40.0, 8, 3
+
+
+def precision_run(args):
+    """Per-stage error of both forms against the FP64 oracle at the precision
+    configuration, plus the chain's timing there.  Oracle work is bounded:
+    the full fit, the grid nodes of two block rows and their resample."""
+    import torch
+    from terawarp_amd import synth, warp
+    from terawarp_amd.warp import Warper
+    from oracle import oracle
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    log("generating precision workload", SIZE_P, "uint16, K =", K_P)
+    tile = synth.blob_volume((64, 512, 512), np.uint16, n_blobs=40, amp=(100, 4000), sigma=(2, 8), seed=SEED_P)
+    vol = np.ascontiguousarray(np.tile(tile, (SIZE_P[2] // 64, SIZE_P[1] // 512, SIZE_P[0] // 512)))
+    sub, tar = synth.landmarks(SIZE_P, K_P, DMAX_P, seed=SEED_P, layout="lattice", outliers=0.1)
+    src = torch.from_numpy(vol.view(np.int16)).cuda()
+    dst = torch.empty_like(src)
+    w = Warper(0, timing=True)
+    vox = SIZE_P[0] * SIZE_P[1] * SIZE_P[2]
+
+    log("oracle fit (bordered FP64 LU,", threads, "threads)")
+    t0 = time.perf_counter()
+    c_ref, d_ref = oracle.stps_fit(tar, sub, nthreads=threads, bordered=True)
+    t_fit = time.perf_counter() - t0
+    zb0, zb1 = 31, 33                       # two block rows in the middle of z
+    gsh = oracle.grid_shape(SIZE_P, G_P)
+    log("oracle grid slab")
+    df_sl = oracle.grid_eval(tar, c_ref, d_ref, (gsh[0], gsh[1], zb1 - zb0 + 3), G_P, origin=(0, 0, zb0 * G_P),
+                             nthreads=threads)
+    box = (SIZE_P[0], SIZE_P[1], (zb1 - zb0) * G_P)
+    dense_ref = oracle.dense_df(box, df_sl, G_P, nthreads=threads)
+    log("oracle resample slab")
+    full = np.zeros((3, gsh[2], gsh[1], gsh[0]), np.float32)
+    full[:, zb0:zb0 + df_sl.shape[1]] = df_sl
+    ref_out = oracle.tps_resample(vol, full, G_P, zblocks=(zb0, zb1), nthreads=threads)
+    del full
+    sl = slice(zb0 * G_P, zb1 * G_P)
+    out = {"config": {"workload": "precision: %dx%dx%d uint16, K=%d jittered 16^3 lattice + 10%% outliers, "
+                                  "|d|<=%g, grid %d" % (SIZE_P + (K_P, DMAX_P, G_P)),
+                      "oracle_sample": "full fit; grid nodes and resample of z-slices [%d,%d)" % (zb0 * G_P, zb1 * G_P)},
+           "oracle_fit_s": t_fit}
+    for name, prec in (("plain", warp.PLAIN), ("mixed", warp.MIXED)):
+        log(name, "form")
+        keys = ("fit_ms", "grid_ms", "quant_ms", "resample_ms", "total_ms", "refine_iters")
+        rec = {k: [] for k in keys}
+        w.warp_tps(src, tar, sub, f=G_P, lam=0.2, precision=prec, dst=dst)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            w.warp_tps(src, tar, sub, f=G_P, lam=0.2, precision=prec, dst=dst)
+            t = w.timings()
+            for k in keys:
+                rec[k].append(t[k])
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / 3
+        o = dst.cpu().numpy().view(np.uint16)
+        diff = np.abs(o[sl].astype(np.int64) - ref_out[sl].astype(np.int64))
+        c, d = w.tps_fit(tar, sub, precision=prec)
+        tf = w.timings()
+        storage = warp.F16 if prec == warp.MIXED else warp.F32
+        grid = w.new_grid(gsh, G_P, storage=storage, device=True)
+        w.grid_eval(tar, c, d, grid, precision=prec)
+        dense = w.upsample_df(grid, box, origin=(0, 0, zb0 * G_P), precision=prec)
+        node_err = None
+        if storage == warp.F32:  # the FP32 node field (both forms evaluate it the same way)
+            nodes = grid.data[:, zb0:zb0 + df_sl.shape[1]].cpu().numpy()
+            node_err = float(np.abs(nodes - df_sl).max())
+        out[name] = {
+            "gvox_s": vox / wall / 1e9, "ms_per_warp": wall * 1e3,
+            **{k: float(np.mean(rec[k])) for k in keys if k != "refine_iters"},
+            "refine_iters": int(np.max(rec["refine_iters"])),
+            "coef_rel_err": float(np.abs(c - c_ref).max() / np.abs(c_ref).max()),
+            "affine_abs_err": float(np.abs(d - d_ref).max()),
+            "fit_residual": tf["fit_residual"],
+            "node_df_err_max": node_err,
+            "voxel_df_err_max": float(np.abs(dense - dense_ref).max()),
+            "lsb_exact_frac": float(np.mean(diff == 0)), "lsb_le1_frac": float(np.mean(diff <= 1)),
+            "lsb_max": int(diff.max()),
+        }
+        del grid, dense
+    w.close()
+    print(json.dumps(out))
+
 
 def measure_traffic(vol):
     """HBM bytes per launch of the mixed resample kernel from two rocprofv3
@@ -197,11 +285,17 @@ def main():
                     help="profiler run: the two forms' steps and a 1 GiB device copy (counter calibration) only")
     ap.add_argument("--volume", default=None, help="load the workload volume from this .npy instead of generating it")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes for roofline.traffic")
+    ap.add_argument("--precision", action="store_true",
+                    help="the precision configuration (2048x2048x512, K=4096): per-stage errors vs the FP64 oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.precision:
+        precision_run(args)
+        return
 
     import torch
     from terawarp_amd import synth, warp
