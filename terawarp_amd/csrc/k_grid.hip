@@ -93,6 +93,62 @@ __global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
     }
 }
 
+// FP32-distance grid kernel, tiled: a work-group owns 16 x 16 nodes of one
+// z-plane (wave w: rows 4w..4w+3, 16 nodes each).  For every chunk of 256
+// landmarks it stages (Y - py)^2 + (Z - pz)^2 per row in LDS, so a node-
+// landmark pair costs dx, one FMA, the square root and the three FP64
+// accumulations.
+constexpr int GT = 16, GCH = 256, GRS = GCH + 4;  // tile side, landmark chunk, LDS row stride
+
+__global__ __launch_bounds__(256) void k_grid_tiled(const GridArgs a) {
+    __shared__ __align__(16) float syz[GT][GRS];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const i64 i = (i64)blockIdx.x * GT + (lane & 15);
+    const i64 j = (i64)blockIdx.y * GT + 4 * w + (lane >> 4);
+    const i64 k = blockIdx.z;
+    const bool live = i < a.gx && j < a.gy;
+    const i64 X = (i - 1) * a.f + a.ox, Z = (k - 1) * a.f + a.oz;
+    const float Xf = (float)X, Zf = (float)Z;
+    const int row = 4 * w + (lane >> 4);
+    const float Y0 = (float)(((i64)blockIdx.y * GT - 1) * a.f + a.oy);  // row 0 of the tile
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int n0 = 0; n0 < a.K; n0 += GCH) {
+        const int cnt = min(GCH, a.K - n0);
+        __syncthreads();
+        if (t < cnt) {
+            const float4 p = a.pts[n0 + t];
+            const float dz = Zf - p.z;
+            const float dz2 = dz * dz;
+            #pragma unroll
+            for (int r = 0; r < GT; r++) {
+                const float dy = (Y0 + (float)(r * a.f)) - p.y;
+                syz[r][t] = __builtin_fmaf(dy, dy, dz2);
+            }
+        }
+        __syncthreads();
+        if (live) {
+            const float* sr = syz[row];
+            for (int n = 0; n < cnt; n++) {
+                const int m = n0 + n;
+                const float dx = Xf - a.pts[m].x;
+                const double r = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, sr[n]));
+                s1 = fma(-r, a.cf[3 * m + 0], s1);
+                s2 = fma(-r, a.cf[3 * m + 1], s2);
+                s3 = fma(-r, a.cf[3 * m + 2], s3);
+            }
+        }
+    }
+    if (!live) return;
+    const i64 plane = a.gx * a.gy * a.gz, idx = (k * a.gy + j) * a.gx + i;
+    const double Xd = (double)X, Yd = (double)((j - 1) * a.f + a.oy), Zd = (double)Z;
+    const double l1 = a.d[1] + Xd * (a.d[5] - 1.0) + Yd * a.d[9] + Zd * a.d[13];
+    const double l2 = a.d[2] + Xd * a.d[6] + Yd * (a.d[10] - 1.0) + Zd * a.d[14];
+    const double l3 = a.d[3] + Xd * a.d[7] + Yd * a.d[11] + Zd * (a.d[15] - 1.0);
+    a.out[idx] = (float)(l1 + s1);
+    a.out[plane + idx] = (float)(l2 + s2);
+    a.out[2 * plane + idx] = (float)(l3 + s3);
+}
+
 // FP16 offset-scaled storage: per block of 8x8x8 nodes and component,
 // value = offset + scale * h with h in [-1, 1].
 constexpr int QB = 8;
@@ -212,10 +268,14 @@ int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const dou
     a.out = out32;
     tw_stage_begin(ctx, ST_GRID);
     const unsigned nblk = (unsigned)((plane + 255) / 256);
-    if (exact)
+    if (exact) {
         hipLaunchKernelGGL(k_grid_eval<true>, dim3(nblk), dim3(256), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(k_grid_eval<false>, dim3(nblk), dim3(256), 0, ctx->stream, a);
+    } else {
+        if (grid->gz > 65535 || (grid->gy + GT - 1) / GT > 65535)
+            return tw_fail(ctx, TW_E_UNSUPPORTED, "grid too large for one launch");
+        const dim3 g((unsigned)((grid->gx + GT - 1) / GT), (unsigned)((grid->gy + GT - 1) / GT), (unsigned)grid->gz);
+        hipLaunchKernelGGL(k_grid_tiled, g, dim3(256), 0, ctx->stream, a);
+    }
     TW_LAUNCH_CHECK(ctx);
     tw_stage_end(ctx, ST_GRID);
 

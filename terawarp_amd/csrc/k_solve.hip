@@ -303,17 +303,6 @@ __global__ __launch_bounds__(256) void k_build_b(const SysDesc s, const double* 
 // columns past the matrix are the identity).  With X the panel solve is a
 // product and the triangular solves are block products.
 
-// Orders one wave's LDS writes before its later reads of other lanes'
-// entries (a wave's LDS operations execute in order).
-// The fences emit no instruction at wavefront scope, so an empty asm with
-// a memory clobber also keeps the scheduler from moving LDS accesses
-// across steps.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    asm volatile("" ::: "memory");
-}
 
 // Keeps a step's arithmetic on the 16 values inside the step: the
 // scheduler may not sink it past the step's barrier (which would keep the
@@ -325,19 +314,52 @@ __device__ __forceinline__ void pin16(T* v) {
                  "+v"(v[15]));
 }
 
-// Factor and invert the diagonal block at j0.  Four waves; lane r owns
-// row r and wave w columns c = 4t + w.  The factor is built as L~ D^-1 L~'
-// (L~[r][k] the Schur-complement column, d_k = L~[k][k]) so a step's
-// critical path holds one reciprocal; L = L~ D^-1/2 at the end.  Per step
-// the owner of column k publishes it through LDS and every wave updates
-// its 16 columns.  Then X = L^-1 by columns: lane c of wave 0 substitutes
-// column c alone, reading the rows of L from LDS (no cross-lane steps).
+// Value of the quad neighbour lane ^ 1 / lane ^ 2 (DPP quad permutes).
+__device__ __forceinline__ float quad_swap1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float quad_swap2(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double quad_dpp(double v, int ctrl) {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffff), hi = (int)(b >> 32);
+    lo = ctrl == 1 ? __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, true) : __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, true);
+    hi = ctrl == 1 ? __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, true) : __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double quad_swap1(double v) { return quad_dpp(v, 1); }
+__device__ __forceinline__ double quad_swap2(double v) { return quad_dpp(v, 2); }
+
+// 1/d in a few instructions: the hardware reciprocal and one Newton step
+// (d > 0 and normal here; the result is within an ulp of 1/d).
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    const double e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, r, 1.0f);
+    return fmaf(r, e, r);
+}
+
+// Factor and invert the diagonal block at j0.  Four waves; lane r owns row
+// r; the 16 blocks of 4 columns are dealt round-robin, wave w holding
+// blocks b = 4t + w (columns 16t + 4w + j).  The factor is built as
+// L~ D^-1 L~' (L~[r][k] the Schur-complement column, d_k = L~[k][k]); per
+// block step the owner factors its 4 columns locally (pivots through
+// readlane), publishes them through LDS, and every wave applies the 4
+// rank-1 updates to its columns right of the block: 16 barriers in all.
+// L = L~ D^-1/2 at the end.  Then X = L^-1 by columns: lane c of wave 0
+// substitutes column c alone, reading the rows of L from LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
                                                     FitState* st) {
     constexpr int LS = NB + 2;           // row stride of Lr (16-byte aligned rows)
-    __shared__ T colb[2][NB];            // L~[:, k], double-buffered by step parity
-    __shared__ T colp[2][4][NB / 4];     // L~[4t + w][k] / d_k for 4t + w > k, else 0
+    // Lr[q][16p + i] = L[q][4i + p]: lane p of a quad reads its terms contiguously.
+    __shared__ T colb[2][4][NB];         // L~[r][k] (0 for r < k), by step parity
+    __shared__ __align__(16) T colp[2][4][NB];  // L~[c][k] / d_k for columns c right of the block, else 0
     __shared__ T dsh[NB];                // pivots d_k (non-positive ones replaced by 1)
     __shared__ __align__(16) T Lr[NB][LS];  // L, row-major
     __shared__ T rsqsh[NB];              // 1 / L[c][c] = d_c^-1/2
@@ -345,77 +367,108 @@ __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __rest
     const int r = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool rin = r < jb;
-    T a[NB / 4];
+    T a[NB / 4];  // a[4t + j]: column 16t + 4w + j
     {
         const T* col = B + (i64)j0 * n + j0 + r;
         #pragma unroll
-        for (int t = 0; t < NB / 4; t++) {
-            const int c = 4 * t + w;
-            a[t] = (rin && c < jb) ? col[(i64)c * n] : (T)(r == c ? 1 : 0);
-        }
+        for (int t = 0; t < 4; t++)
+            #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int c = 16 * t + 4 * w + j;
+                a[4 * t + j] = (rin && c < jb) ? col[(i64)c * n] : (T)(r == c ? 1 : 0);
+            }
     }
     if (threadIdx.x == 0) bad = -1;
     #pragma unroll
-    for (int k = 0; k < NB; k++) {
-        const int buf = k & 1;
-        if (w == (k & 3)) {
-            T d = bcast(a[k >> 2], k);
-            if (!(d > (T)0)) {
-                if (r == 0 && bad < 0) bad = k;
-                d = (T)1;
+    for (int b = 0; b < NB / 4; b++) {
+        const int buf = b & 1, tb = b >> 2, k0 = 4 * b;
+        if (w == (b & 3)) {
+            #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int k = k0 + j;
+                T d = bcast(a[4 * tb + j], k);
+                if (!(d > (T)0)) {
+                    if (r == 0 && bad < 0) bad = k;
+                    d = (T)1;
+                }
+                const T rd = recip(d);
+                const T lrk = r >= k ? a[4 * tb + j] : (T)0;
+                colb[buf][j][r] = lrk;
+                colp[buf][j][r] = r > k0 + 3 ? lrk * rd : (T)0;
+                if (r == 0) dsh[k] = d;
+                // the block's later columns
+                #pragma unroll
+                for (int jj = j + 1; jj < 4; jj++)
+                    a[4 * tb + jj] = fma(-lrk * rd, bcast(a[4 * tb + j], k0 + jj), a[4 * tb + jj]);
             }
-            const T rd = (T)1 / d;
-            const T lrk = a[k >> 2];
-            colb[buf][r] = r >= k ? lrk : (T)0;
-            colp[buf][r & 3][r >> 2] = r > k ? lrk * rd : (T)0;
-            if (r == 0) dsh[k] = d;
         }
         __syncthreads();
-        const T l = colb[buf][r];
+        // Only columns right of the block change: the wave's groups t > tb,
+        // and group tb when the wave comes after the owner.
         #pragma unroll
-        for (int t = 0; t < NB / 4; t++) a[t] = fma(-l, colp[buf][w][t], a[t]);
+        for (int j = 0; j < 4; j++) {
+            const T l = colb[buf][j][r];
+            #pragma unroll
+            for (int t = tb; t < 4; t++) {
+                if (t == tb && w <= (b & 3)) continue;
+                #pragma unroll
+                for (int jj = 0; jj < 4; jj++)
+                    a[4 * t + jj] = fma(-l, colp[buf][j][16 * t + 4 * w + jj], a[4 * t + jj]);
+            }
+        }
         pin16(a);
     }
     // L = L~ D^-1/2 (lower part; the rest is zero).
     #pragma unroll
-    for (int t = 0; t < NB / 4; t++) {
-        const int c = 4 * t + w;
-        const T rs = (T)1 / sqrt(dsh[c]);
-        a[t] = r >= c ? a[t] * rs : (T)0;
-        Lr[r][c] = a[t];
-        if (r == 0) rsqsh[c] = rs;
-    }
+    for (int t = 0; t < 4; t++)
+        #pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = 16 * t + 4 * w + j;
+            const T rs = (T)1 / sqrt(dsh[c]);
+            a[4 * t + j] = r >= c ? a[4 * t + j] * rs : (T)0;
+            Lr[r][16 * (c & 3) + (c >> 2)] = a[4 * t + j];
+            if (r == 0) rsqsh[c] = rs;
+        }
     {
         T* col = B + (i64)j0 * n + j0 + r;
         #pragma unroll
-        for (int t = 0; t < NB / 4; t++) {
-            const int c = 4 * t + w;
-            if (rin && c < jb && c <= r) col[(i64)c * n] = a[t];
-        }
+        for (int t = 0; t < 4; t++)
+            #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int c = 16 * t + 4 * w + j;
+                if (rin && c < jb && c <= r) col[(i64)c * n] = a[4 * t + j];
+            }
     }
     __syncthreads();
     if (threadIdx.x == 0 && bad >= 0 && st->info == 0) st->info = 2 + j0 + bad;
-    if (w != 0) return;
-    // Column c = r of X: x[q] = (delta_qc - sum_{j<q} L[q][j] x[j]) / L[q][q]
-    // (zero above the diagonal by construction); two partial sums halve the
-    // dependent chain.
-    T x[NB];
+    // X = L^-1 by columns: column c = 16w + r/4 of X belongs to the quad
+    // r/4 of wave w; lane p of the quad keeps x[4i + p] and sums those terms
+    // of x[q] = (delta_qc - sum_{j<q} L[q][j] x[j]) / L[q][q], the quad
+    // combining its four partial sums with two DPP exchanges per row.
+    const int p = r & 3, c = 16 * w + (r >> 2);
+    T xr[NB / 4];
+    #pragma unroll
+    for (int i = 0; i < NB / 4; i++) xr[i] = 0;  // x[j] not yet formed reads as 0
     #pragma unroll
     for (int q = 0; q < NB; q++) {
-        const T* lq = Lr[q];
-        T s0 = r == q ? (T)1 : (T)0, s1 = 0;
+        const T* lq = &Lr[q][16 * p];
+        T s0 = 0, s1 = 0;
         #pragma unroll
-        for (int j = 0; j + 1 < q; j += 2) {
-            s0 = fma(-lq[j], x[j], s0);
-            s1 = fma(-lq[j + 1], x[j + 1], s1);
+        for (int i = 0; 4 * i < q; i++) {  // terms j = 4i + p; those with j >= q are 0
+            if (i & 1) s1 = fma(lq[i], xr[i], s1);
+            else s0 = fma(lq[i], xr[i], s0);
         }
-        if (q & 1) s0 = fma(-lq[q - 1], x[q - 1], s0);
-        x[q] = (s0 + s1) * rsqsh[q];
-        asm volatile("" : "+v"(x[q])::"memory");  // one row's LDS reads at a time
+        T sum = s0 + s1;
+        sum += quad_swap1(sum);
+        sum += quad_swap2(sum);
+        const T xq = ((q == c ? (T)1 : (T)0) - sum) * rsqsh[q];
+        if ((q & 3) == p) xr[q >> 2] = xq;
+        asm volatile("" : "+v"(xr[q >> 2])::"memory");  // one row's LDS reads at a time
     }
-    T* D = Dinv + (i64)(j0 / NB) * NB * NB + r;
+    // xr[i] = X[4i + p][c]; Dinv row-major: D[row][c].
+    T* D = Dinv + (i64)(j0 / NB) * NB * NB + c;
     #pragma unroll
-    for (int q = 0; q < NB; q++) D[q * NB] = x[q];
+    for (int i = 0; i < NB / 4; i++) D[(4 * i + p) * NB] = xr[i];
 }
 
 // Panel below the diagonal block: L21 = B21 X', 64 rows per work-group; wave
@@ -622,6 +675,107 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
             __syncthreads();
         }
     }
+}
+
+// The same solve for large n, one launch per 64-row block so that many
+// work-groups share the rows to update (the single work-group above streams
+// the whole factor through one CU).  Forward step kb: every work-group forms
+// y_kb = X_kb b_kb from LDS (work-group 0 stores it in y) and updates its 256
+// rows below the block in b.  Backward step kb: x_kb = X_kb' y_kb (stored in b
+// by work-group 0), rows above the block updated in y.  skip as above.
+constexpr int STEP_THREADS = 256;
+
+template <typename T, bool FWD>
+__global__ __launch_bounds__(STEP_THREADS) void k_solve_step(const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                             int n, int kb, T* __restrict__ b, T* __restrict__ y,
+                                                             const int* skip) {
+    if (skip && *skip) return;
+    __shared__ T Xs[NB][NB + 1];
+    __shared__ T vk[NB][4];
+    __shared__ T yk[NB][4];
+    const int tid = threadIdx.x;
+    const int k0 = kb * NB, ib = min(NB, n - k0);
+    const uint32_t colb = (uint32_t)n * (uint32_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rl =
+        __builtin_amdgcn_make_buffer_rsrc((void*)L, (short)0, (int)(colb * (uint32_t)n), BUF_FLAGS);
+    // The block's X and right-hand side first (loads retire in order), then
+    // this thread's row of L.
+    const T* D = Dinv + (i64)kb * NB * NB;
+    T xv[NB * NB / STEP_THREADS];
+    #pragma unroll
+    for (int u = 0; u < NB * NB / STEP_THREADS; u++) xv[u] = D[tid + STEP_THREADS * u];
+    const T* rhs = FWD ? b : y;  // the block's current right-hand side
+    const int r = tid & (NB - 1), j = tid / NB;
+    const T bv = r < ib ? rhs[(i64)j * n + k0 + r] : (T)0;
+    const int i = FWD ? k0 + NB + blockIdx.x * STEP_THREADS + tid : blockIdx.x * STEP_THREADS + tid;
+    const bool row = FWD ? i < n : i < k0;
+    T l[NB];
+    if (row) {
+        if (FWD) {
+            const uint32_t v = (uint32_t)i * (uint32_t)sizeof(T);
+            #pragma unroll
+            for (int c = 0; c < NB; c++) l[c] = ld_buf<T>(rl, v, (uint32_t)(k0 + c) * colb);
+        } else {
+            const uint32_t v = ((uint32_t)i * (uint32_t)n + (uint32_t)k0) * (uint32_t)sizeof(T);
+            #pragma unroll
+            for (int c = 0; c < NB; c++) l[c] = c < ib ? ld_buf<T>(rl, v + (uint32_t)(c * sizeof(T)), 0u) : (T)0;
+        }
+    }
+    #pragma unroll
+    for (int u = 0; u < NB * NB / STEP_THREADS; u++) {
+        const int e = tid + STEP_THREADS * u;
+        Xs[e / NB][e & (NB - 1)] = xv[u];
+    }
+    vk[r][j] = bv;
+    __syncthreads();
+    {
+        T acc = 0;
+        #pragma unroll 8
+        for (int c = 0; c < NB; c++) acc = fma(FWD ? Xs[r][c] : Xs[c][r], vk[c][j], acc);
+        yk[r][j] = acc;
+        if (blockIdx.x == 0 && r < ib) (FWD ? y : b)[(i64)j * n + k0 + r] = acc;
+    }
+    __syncthreads();
+    if (row) {
+        T acc[4] = {0, 0, 0, 0};
+        #pragma unroll
+        for (int c = 0; c < NB; c++) {
+            #pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] = fma(l[c], yk[c][q], acc[q]);
+            if ((c & 7) == 7)  // bound the LDS reads in flight
+                asm volatile("" : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])::"memory");
+        }
+        T* out = FWD ? b : y;
+        #pragma unroll
+        for (int q = 0; q < 4; q++) out[(i64)q * n + i] -= acc[q];
+    }
+}
+
+// Systems above this order use the per-block launches.
+constexpr int SOLVE_MULTI_MIN = 1025;
+
+// L L' x = b (n x 4 column-major, in place); y: n x 4 scratch for large n.
+template <typename T>
+int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip) {
+    if (n < SOLVE_MULTI_MIN) {
+        hipLaunchKernelGGL(k_chol_solve<T>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, L, Dinv, n, b, skip);
+    } else {
+        const int nb = (n + NB - 1) / NB;
+        for (int kb = 0; kb < nb; kb++) {
+            const int below = n - (kb + 1) * NB;
+            const unsigned g = (unsigned)std::max(1, (below + STEP_THREADS - 1) / STEP_THREADS);
+            hipLaunchKernelGGL((k_solve_step<T, true>), dim3(g), dim3(STEP_THREADS), 0, ctx->stream, L, Dinv, n, kb,
+                               b, y, skip);
+        }
+        for (int kb = nb - 1; kb >= 0; kb--) {
+            const int above = kb * NB;
+            const unsigned g = (unsigned)std::max(1, (above + STEP_THREADS - 1) / STEP_THREADS);
+            hipLaunchKernelGGL((k_solve_step<T, false>), dim3(g), dim3(STEP_THREADS), 0, ctx->stream, L, Dinv, n, kb,
+                               b, y, skip);
+        }
+    }
+    TW_LAUNCH_CHECK(ctx);
+    return TW_OK;
 }
 
 // U (N x 4, column-major) <- Q' U (trans = 1) or Q U (trans = 0), Q = I - V T V'.
@@ -956,7 +1110,8 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
     const size_t st_b = (sizeof(FitState) + 255) & ~(size_t)255;
     const size_t f32_b = ((size_t)n * 4 * sizeof(float) + 255) & ~(size_t)255;
     const size_t dinv_b = (size_t)nblocks * NB * NB * sizeof(double);
-    if ((rc = tw_ws(ctx, WS_SOLVE_AUX, st_b + 2 * f32_b + dinv_b, &aux))) return rc;
+    const size_t ys_b = ((size_t)n * 4 * sizeof(double) + 255) & ~(size_t)255;  // solve scratch (large n)
+    if ((rc = tw_ws(ctx, WS_SOLVE_AUX, st_b + 2 * f32_b + ys_b + dinv_b, &aux))) return rc;
     double* V = (double*)arena;
     double* P = V + NV;
     double* W = P + NV;
@@ -969,7 +1124,8 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
     FitState* st = (FitState*)aux;
     float* f32a = (float*)((uint8_t*)aux + st_b);
     float* f32b = (float*)((uint8_t*)aux + st_b + f32_b);
-    void* dinv = (uint8_t*)aux + st_b + 2 * f32_b;
+    void* ys = (uint8_t*)aux + st_b + 2 * f32_b;
+    void* dinv = (uint8_t*)aux + st_b + 2 * f32_b + ys_b;
     const int* stop = &st->ir_stop;
 
     FitState hst;
@@ -999,10 +1155,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
         cholesky<double>(ctx, B, (double*)dinv, n, st);
         hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
-        hipLaunchKernelGGL(k_chol_solve<double>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, B,
-                           (const double*)dinv, n, z, (const int*)nullptr);
-        TW_LAUNCH_CHECK(ctx);
-        return TW_OK;
+        return chol_solve<double>(ctx, B, (const double*)dinv, n, z, (double*)ys, nullptr);
     };
 
     int iters = 0;
@@ -1014,8 +1167,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         hipLaunchKernelGGL(k_build_b<float>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
         cholesky<float>(ctx, B, Df, n, st);
         hipLaunchKernelGGL(k_tail<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, b2, f32a, N);
-        hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, B, (const float*)Df, n,
-                           f32a, (const int*)nullptr);
+        if ((rc = chol_solve<float>(ctx, B, (const float*)Df, n, f32a, (float*)ys, nullptr))) return rc;
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, f32a, z,
                            (i64)n * 4);
         TW_LAUNCH_CHECK(ctx);
@@ -1030,8 +1182,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
                 hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U2, P, 0, stop);  // -A'[0;z]
                 hipLaunchKernelGGL(k_ir_residual, dim3(1), dim3(1024), 0, ctx->stream, V, W, U2, P, b2, f32b, st,
                                    N);
-                hipLaunchKernelGGL(k_chol_solve<float>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, B,
-                                   (const float*)Df, n, f32b, stop);
+                if ((rc = chol_solve<float>(ctx, B, (const float*)Df, n, f32b, (float*)ys, stop))) return rc;
                 hipLaunchKernelGGL(k_axpy_f32, dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, z, f32b,
                                    (i64)n * 4, stop);
             }

@@ -29,6 +29,19 @@ def rel_err(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def test_fit_large_system_matches_oracle(warper, oracle):
+    """K = 1200: the projected system (1196 unknowns) is solved by per-block
+    launches over many work-groups; coefficients match the oracle's bordered
+    FP64 solve."""
+    sub, tar = synth.landmarks((1024, 1024, 256), 1200, 20.0, seed=11)
+    c_ref, d_ref = oracle.stps_fit(tar, sub, bordered=True)
+    for precision in (warp.PLAIN, warp.MIXED):
+        c, d = warper.tps_fit(tar, sub, precision=precision)
+        assert rel_err(c, c_ref) < 1e-8, (precision, rel_err(c, c_ref))
+        assert np.abs(d - d_ref).max() < 1e-8
+        assert warper.timings()["fit_residual"] < 1e-10
+
+
 def test_fit_fp32_refinement_large(warper):
     """Above 4096 projected unknowns TW_MIXED factors in FP32 and refines the
     solution with FP64 residuals (an FP64 refactor if refinement stalls);
