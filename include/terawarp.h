@@ -197,6 +197,43 @@ int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int64_t ny, int
 int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
                 const float* targets, int K, int f, double lambda, int interp, int precision);
 
+/* ---- batches ------------------------------------------------------------- */
+#define TW_WARP_TPS     0
+#define TW_WARP_AFFINE  1
+
+/* One warp of a batch.  src is a whole volume (origin 0) and dst a volume or
+ * window (origin x0, y0, z0) of the same dtype and channel count, both in
+ * device memory.  TW_WARP_TPS: K landmark pairs (host memory, as for
+ * tw_tps_fit), fitted with the batch's lambda, grid of the batch's spacing f
+ * over dst.  TW_WARP_AFFINE: M, the 3 x 4 row-major target -> subject matrix
+ * (rigid transforms and the README modes through tw_mode_matrix). */
+typedef struct {
+    tw_volume    src;
+    tw_volume    dst;
+    int32_t      kind;      /* TW_WARP_TPS / TW_WARP_AFFINE */
+    int32_t      K;
+    const float* centres;   /* TPS: K x 3 */
+    const float* targets;   /* TPS: K x 3 */
+    double       M[12];     /* AFFINE */
+} tw_warp_item;
+
+/* Many independent warps of different sizes, dtypes and modes, one launch per
+ * stage: every TPS fit in one launch (a work-group per system up to 256
+ * landmarks), every grid in one, and every output voxel of every item in one
+ * persistent resample launch that finishes the items in order.  The
+ * reference warps one volume or tera block per call (its block driver,
+ * q_imgwarp_tps_quicksmallmemory.cpp:1161-1994, calls gpu_interpolation,
+ * chazhi.cu:557-663, or gpu_interpolation_new, aff.cu:449-536, per block);
+ * each item's output equals tw_warp_tps (TPS, same f, lambda, interp and
+ * precision) or tw_resample_affine (AFFINE) on the same arguments.
+ * precision: TW_PLAIN or TW_MIXED.  Synchronises once, at the end.
+ * status (optional, n entries): per-item TW_OK or error code (a failed fit
+ * leaves its item's dst unspecified; the call returns the first error).
+ * done_ms (optional, n entries): device-clock time from the batch's first
+ * kernel to the end of the item's last output tile. */
+int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int f, double lambda, int interp,
+                  int precision, int32_t* status, double* done_ms);
+
 /* Grid node count per axis for a volume extent n and spacing f:
  * ceil(n / f) + 3 (q_imgwarp_tps_quicksmallmemory.cpp:592). */
 int64_t tw_grid_dim(int64_t n, int f);

@@ -29,12 +29,14 @@ TW_INTERP_LINEAR = 0
 TW_INTERP_NN = 1
 TW_DF_F32 = 0
 TW_DF_F16 = 1
+TW_WARP_TPS = 0
+TW_WARP_AFFINE = 1
 
 # Every entry point include/terawarp.h declares.
 EXPORTS = (
     "tw_create", "tw_destroy", "tw_set_stream", "tw_sync", "tw_set_timing", "tw_get_timings", "tw_last_error",
     "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
-    "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_grid_dim",
+    "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_warp_batch", "tw_grid_dim",
 )
 # Every entry point include/terawarp_host.h declares.
 HOST_EXPORTS = (
@@ -66,6 +68,12 @@ class TwTimings(ctypes.Structure):
                 ("resample_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double), ("d2h_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("refine_iters", _i32), ("reserved", _i32),
                 ("fit_residual", ctypes.c_double)]
+
+
+class TwWarpItem(ctypes.Structure):
+    _fields_ = [("src", TwVolume), ("dst", TwVolume), ("kind", _i32), ("K", _i32),
+                ("centres", ctypes.POINTER(ctypes.c_float)), ("targets", ctypes.POINTER(ctypes.c_float)),
+                ("M", ctypes.c_double * 12)]
 
 
 class TwWarpParams(ctypes.Structure):
@@ -108,6 +116,8 @@ def load():
                                           ctypes.c_int]),
         "tw_warp_tps": (ctypes.c_int, [_P, V, V, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_int]),
+        "tw_warp_batch": (ctypes.c_int, [_P, ctypes.POINTER(TwWarpItem), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i32), dp]),
         "tw_grid_dim": (_i64, [_i64, ctypes.c_int]),
         "tw_v3draw_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),

@@ -100,17 +100,16 @@ __global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
 // accumulations.
 constexpr int GT = 16, GCH = 256, GRS = GCH + 4;  // tile side, landmark chunk, LDS row stride
 
-__global__ __launch_bounds__(256) void k_grid_tiled(const GridArgs a) {
-    __shared__ __align__(16) float syz[GT][GRS];
+__device__ __forceinline__ void grid_tiled_body(const GridArgs& a, int bx, int by, int bz, float (*syz)[GRS]) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const i64 i = (i64)blockIdx.x * GT + (lane & 15);
-    const i64 j = (i64)blockIdx.y * GT + 4 * w + (lane >> 4);
-    const i64 k = blockIdx.z;
+    const i64 i = (i64)bx * GT + (lane & 15);
+    const i64 j = (i64)by * GT + 4 * w + (lane >> 4);
+    const i64 k = bz;
     const bool live = i < a.gx && j < a.gy;
     const i64 X = (i - 1) * a.f + a.ox, Z = (k - 1) * a.f + a.oz;
     const float Xf = (float)X, Zf = (float)Z;
     const int row = 4 * w + (lane >> 4);
-    const float Y0 = (float)(((i64)blockIdx.y * GT - 1) * a.f + a.oy);  // row 0 of the tile
+    const float Y0 = (float)(((i64)by * GT - 1) * a.f + a.oy);  // row 0 of the tile
     double s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (int n0 = 0; n0 < a.K; n0 += GCH) {
         const int cnt = min(GCH, a.K - n0);
@@ -149,17 +148,26 @@ __global__ __launch_bounds__(256) void k_grid_tiled(const GridArgs a) {
     a.out[2 * plane + idx] = (float)(l3 + s3);
 }
 
+__global__ __launch_bounds__(256) void k_grid_tiled(const GridArgs a) {
+    __shared__ __align__(16) float syz[GT][GRS];
+    grid_tiled_body(a, blockIdx.x, blockIdx.y, blockIdx.z, syz);
+}
+
 // FP16 offset-scaled storage: per block of 8x8x8 nodes and component,
 // value = offset + scale * h with h in [-1, 1].
 constexpr int QB = 8;
 
-__global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ df, __half* __restrict__ out,
-                                                  float* __restrict__ qtab, i64 gx, i64 gy, i64 gz, i64 qbx,
-                                                  i64 qby) {
-    __shared__ float red[2][3][256];
-    __shared__ float prm[6];
+struct QuantLds {
+    float red[2][3][256];
+    float prm[6];
+};
+
+__device__ __forceinline__ void quantize_body(const float* __restrict__ df, __half* __restrict__ out,
+                                              float* __restrict__ qtab, i64 gx, i64 gy, i64 gz, i64 qbx, i64 qby,
+                                              i64 b, QuantLds& S) {
+    auto& red = S.red;
+    auto& prm = S.prm;
     const i64 plane = gx * gy * gz;
-    const i64 b = blockIdx.x;
     const i64 bx = b % qbx, by = (b / qbx) % qby, bz = b / (qbx * qby);
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int t = threadIdx.x; t < QB * QB * QB; t += 256) {
@@ -206,6 +214,54 @@ __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ df, 
             out[c * plane + idx] = __float2half_rn(fminf(1.f, fmaxf(-1.f, h)));
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ df, __half* __restrict__ out,
+                                                  float* __restrict__ qtab, i64 gx, i64 gy, i64 gz, i64 qbx,
+                                                  i64 qby) {
+    __shared__ QuantLds S;
+    quantize_body(df, out, qtab, gx, gy, gz, qbx, qby, blockIdx.x, S);
+}
+
+// ---- batches: every system's grid in one launch ---------------------------
+// Work-group w serves tile w - pref[j] of job j, pref[j] <= w < pref[j + 1]
+// (pref: exclusive prefix sums of the jobs' tile counts).
+struct GridJob {
+    GridArgs a;         // a.out: the FP32 planes
+    int tiles_x, tiles_y;
+    i64 qbx, qby;       // FP16 block grid
+    __half* h;          // FP16 planes, or null (FP32 grid)
+    float* qtab;
+};
+
+__device__ __forceinline__ int job_of(const int* __restrict__ pref, int njobs, int w) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pref[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_grid_batch(const GridJob* __restrict__ jobs, const int* __restrict__ pref,
+                                                    int njobs) {
+    __shared__ __align__(16) float syz[GT][GRS];
+    const int w = blockIdx.x;
+    const int j = job_of(pref, njobs, w);
+    const GridJob& J = jobs[j];
+    const int t = w - pref[j];
+    const int bx = t % J.tiles_x, by = (t / J.tiles_x) % J.tiles_y, bz = t / (J.tiles_x * J.tiles_y);
+    grid_tiled_body(J.a, bx, by, bz, syz);
+}
+
+__global__ __launch_bounds__(256) void k_quantize_batch(const GridJob* __restrict__ jobs, const int* __restrict__ pref,
+                                                        int njobs) {
+    __shared__ QuantLds S;
+    const int w = blockIdx.x;
+    const int j = job_of(pref, njobs, w);
+    const GridJob& J = jobs[j];
+    quantize_body(J.a.out, J.h, J.qtab, J.a.gx, J.a.gy, J.a.gz, J.qbx, J.qby, w - pref[j], S);
 }
 
 }  // namespace
@@ -325,5 +381,70 @@ extern "C" int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const 
     tw_stage_end(ctx, ST_TOTAL);
     // The host pack must outlive the copy; host grids need the result now.
     TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return TW_OK;
+}
+
+// Grids of a batch of systems: every FP32 field in one launch of the tiled
+// kernel, then (FP16 items) every quantisation in one launch.  Per node the
+// arithmetic is that of tw_grid_eval_dev.  The job tables are staged in
+// host memory that the context keeps until the next batch.
+int tw_grid_eval_batch(tw_ctx* ctx, const TwGridItem* items, int n) {
+    int rc;
+    std::vector<GridJob> jobs((size_t)n);
+    std::vector<int> tpref((size_t)n + 1, 0), qpref((size_t)n + 1, 0);
+    bool any16 = false;
+    for (int i = 0; i < n; i++) {
+        const TwGridItem& it = items[i];
+        const tw_grid& g = it.grid;
+        GridJob& J = jobs[i];
+        memset(&J, 0, sizeof J);
+        J.a.pts = it.pts;
+        J.a.c64 = it.cf;
+        J.a.cf = it.cf;
+        J.a.K = it.K;
+        J.a.f = g.f;
+        J.a.gx = g.gx; J.a.gy = g.gy; J.a.gz = g.gz;
+        J.a.ox = g.ox; J.a.oy = g.oy; J.a.oz = g.oz;
+        J.a.d = it.d;
+        J.a.out = it.f32;
+        J.tiles_x = (int)((g.gx + GT - 1) / GT);
+        J.tiles_y = (int)((g.gy + GT - 1) / GT);
+        J.qbx = (g.gx + QB - 1) / QB;
+        J.qby = (g.gy + QB - 1) / QB;
+        const i64 qbz = (g.gz + QB - 1) / QB;
+        if (g.storage == TW_DF_F16) {
+            J.h = (__half*)g.data;
+            J.qtab = g.qtab;
+            any16 = true;
+        }
+        const i64 nt = (i64)J.tiles_x * J.tiles_y * g.gz;
+        const i64 nq = g.storage == TW_DF_F16 ? J.qbx * J.qby * qbz : 0;
+        if (tpref[i] + nt >= (1ll << 31) || qpref[i] + nq >= (1ll << 31))
+            return tw_fail(ctx, TW_E_UNSUPPORTED, "batch grids too large for one launch");
+        tpref[i + 1] = (int)(tpref[i] + nt);
+        qpref[i + 1] = (int)(qpref[i] + nq);
+    }
+    const size_t jb = ((size_t)n * sizeof(GridJob) + 255) & ~(size_t)255;
+    const size_t pb = ((size_t)(n + 1) * sizeof(int) + 255) & ~(size_t)255;
+    uint8_t* host = tw_host_stage(ctx, jb + 2 * pb);
+    memcpy(host, jobs.data(), (size_t)n * sizeof(GridJob));
+    memcpy(host + jb, tpref.data(), (size_t)(n + 1) * sizeof(int));
+    memcpy(host + jb + pb, qpref.data(), (size_t)(n + 1) * sizeof(int));
+    void* dev = nullptr;
+    if ((rc = tw_ws(ctx, WS_BATCH_GTAB, jb + 2 * pb, &dev))) return rc;
+    TW_HIP(ctx, hipMemcpyAsync(dev, host, jb + 2 * pb, hipMemcpyHostToDevice, ctx->stream));
+    const GridJob* djobs = (const GridJob*)dev;
+    const int* dt = (const int*)((uint8_t*)dev + jb);
+    const int* dq = (const int*)((uint8_t*)dev + jb + pb);
+    tw_stage_begin(ctx, ST_GRID);
+    if (tpref[n] > 0) hipLaunchKernelGGL(k_grid_batch, dim3((unsigned)tpref[n]), dim3(256), 0, ctx->stream, djobs, dt, n);
+    TW_LAUNCH_CHECK(ctx);
+    tw_stage_end(ctx, ST_GRID);
+    if (any16 && qpref[n] > 0) {
+        tw_stage_begin(ctx, ST_QUANT);
+        hipLaunchKernelGGL(k_quantize_batch, dim3((unsigned)qpref[n]), dim3(256), 0, ctx->stream, djobs, dq, n);
+        TW_LAUNCH_CHECK(ctx);
+        tw_stage_end(ctx, ST_QUANT);
+    }
     return TW_OK;
 }

@@ -77,7 +77,7 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum over a 1024-thread work-group; every thread gets the total.
+// Sum over a work-group of up to 1024 threads; every thread gets the total.
 __device__ double block_sum(double v, double* red) {
     v = wave_sum(v);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -101,7 +101,7 @@ __device__ __forceinline__ double bcast(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sums of NV values over a 1024-thread work-group; red must hold 17 * NV
+// Sums of NV values over a work-group of up to 1024 threads; red must hold 17 * NV
 // doubles.  Every thread gets the totals in v.
 template <int NV>
 __device__ void block_sum_n(double* v, double* red) {
@@ -139,12 +139,11 @@ struct FitState {
 };
 
 // Householder QR of X' (N x 4) and the compact-WY factor T.  One work-group.
-__global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restrict__ V, FitState* st) {
-    __shared__ double red[16];
-    __shared__ double sh[8];
+template <int NT>
+__device__ void hh_x_body(const SysDesc& s, double* __restrict__ V, FitState* st, double* red, double* sh) {
     const int N = s.N, tid = threadIdx.x;
     // X' into V's storage (column-major N x 4).
-    for (int i = tid; i < N; i += 1024) {
+    for (int i = tid; i < N; i += NT) {
         const float4 p = s.pts[i];
         V[i] = 1.0;
         V[N + i] = ((double)p.x - s.mu[0]) * s.inv_s;
@@ -158,7 +157,7 @@ __global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restri
         double* xk = V + (size_t)k * N;
         __syncthreads();
         double part = 0;
-        for (int i = k + 1 + tid; i < N; i += 1024) part += xk[i] * xk[i];
+        for (int i = k + 1 + tid; i < N; i += NT) part += xk[i] * xk[i];
         const double xnorm2 = block_sum(part, red);
         const double alpha = xk[k];
         double beta, t, scale;
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restri
         R[k * 4 + k] = beta;
         if (k == 0) scale0 = fabs(beta);
         __syncthreads();
-        for (int i = tid; i < N; i += 1024) {
+        for (int i = tid; i < N; i += NT) {
             if (i < k) xk[i] = 0.0;
             else if (i == k) xk[i] = 1.0;
             else xk[i] *= scale;
@@ -183,10 +182,10 @@ __global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restri
         for (int j = k + 1; j < 4; j++) {
             double* xj = V + (size_t)j * N;
             double pw = 0;
-            for (int i = k + tid; i < N; i += 1024) pw += xk[i] * xj[i];
+            for (int i = k + tid; i < N; i += NT) pw += xk[i] * xj[i];
             const double w = block_sum(pw, red);
             __syncthreads();
-            for (int i = k + tid; i < N; i += 1024) xj[i] -= t * xk[i] * w;
+            for (int i = k + tid; i < N; i += NT) xj[i] -= t * xk[i] * w;
             __syncthreads();
             if (tid == 0) sh[j] = xj[k];
             __syncthreads();
@@ -198,7 +197,7 @@ __global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restri
     for (int a = 0; a < 4; a++)
         for (int b = a; b < 4; b++) {
             double p = 0;
-            for (int i = tid; i < N; i += 1024) p += V[(size_t)a * N + i] * V[(size_t)b * N + i];
+            for (int i = tid; i < N; i += NT) p += V[(size_t)a * N + i] * V[(size_t)b * N + i];
             G[a * 4 + b] = block_sum(p, red);
             G[b * 4 + a] = G[a * 4 + b];
         }
@@ -221,6 +220,13 @@ __global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restri
     }
 }
 
+__global__ __launch_bounds__(1024) void k_hh_x(const SysDesc s, double* __restrict__ V, FitState* st) {
+    __shared__ double red[16];
+    __shared__ double sh[8];
+    hh_x_body<1024>(s, V, st, red, sh);
+}
+
+
 // P = A' V (N x 4), A' = K' + lambda' I generated on the fly; one row per work-group.
 __global__ __launch_bounds__(256) void k_av(const SysDesc s, const double* __restrict__ V, double* __restrict__ P) {
     __shared__ double red[4][4];
@@ -240,14 +246,13 @@ __global__ __launch_bounds__(256) void k_av(const SysDesc s, const double* __res
 }
 
 // S = V'P, M = T' S T, W = P T - 1/2 V M.  One work-group.
-__global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, const double* __restrict__ P,
-                                                  double* __restrict__ W, const FitState* st, int N) {
-    __shared__ double red[17 * 16];
-    __shared__ double M[16], Ts[16];
+template <int NT>
+__device__ void build_w_body(const double* __restrict__ V, const double* __restrict__ P, double* __restrict__ W,
+                             const FitState* st, int N, double* red, double* M, double* Ts) {
     const int tid = threadIdx.x;
     double S[16];
     for (int i = 0; i < 16; i++) S[i] = 0;
-    for (int i = tid; i < N; i += 1024) {
+    for (int i = tid; i < N; i += NT) {
         double v[4], p[4];
         for (int k = 0; k < 4; k++) { v[k] = V[(size_t)k * N + i]; p[k] = P[(size_t)k * N + i]; }
         for (int a = 0; a < 4; a++)
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, 
             }
     }
     __syncthreads();
-    for (int i = tid; i < N; i += 1024) {
+    for (int i = tid; i < N; i += NT) {
         double p[4], v[4];
         for (int k = 0; k < 4; k++) { p[k] = P[(size_t)k * N + i]; v[k] = V[(size_t)k * N + i]; }
         for (int b = 0; b < 4; b++) {
@@ -281,6 +286,14 @@ __global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, 
         }
     }
 }
+
+__global__ __launch_bounds__(1024) void k_build_w(const double* __restrict__ V, const double* __restrict__ P,
+                                                  double* __restrict__ W, const FitState* st, int N) {
+    __shared__ double red[17 * 16];
+    __shared__ double M[16], Ts[16];
+    build_w_body<1024>(V, P, W, st, N, red, M, Ts);
+}
+
 
 // B = [A' - W V' - V W']_(4:,4:), n x n column-major in precision T.
 template <typename T>
@@ -353,17 +366,28 @@ __device__ __forceinline__ float recip(float d) {
 // rank-1 updates to its columns right of the block: 16 barriers in all.
 // L = L~ D^-1/2 at the end.  Then X = L^-1 by columns: lane c of wave 0
 // substitutes column c alone, reading the rows of L from LDS.
+constexpr int LS = NB + 2;  // row stride of the diagonal block's L (16-byte aligned rows)
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
-                                                    FitState* st) {
-    constexpr int LS = NB + 2;           // row stride of Lr (16-byte aligned rows)
+struct PotrfLds {
+    T colb[2][4][NB];                  // L~[r][k] (0 for r < k), by step parity
+    __align__(16) T colp[2][4][NB];    // L~[c][k] / d_k for columns c right of the block, else 0
+    T dsh[NB];                         // pivots d_k (non-positive ones replaced by 1)
     // Lr[q][16p + i] = L[q][4i + p]: lane p of a quad reads its terms contiguously.
-    __shared__ T colb[2][4][NB];         // L~[r][k] (0 for r < k), by step parity
-    __shared__ __align__(16) T colp[2][4][NB];  // L~[c][k] / d_k for columns c right of the block, else 0
-    __shared__ T dsh[NB];                // pivots d_k (non-positive ones replaced by 1)
-    __shared__ __align__(16) T Lr[NB][LS];  // L, row-major
-    __shared__ T rsqsh[NB];              // 1 / L[c][c] = d_c^-1/2
-    __shared__ int bad;
+    __align__(16) T Lr[NB][LS];        // L, row-major
+    T rsqsh[NB];                       // 1 / L[c][c] = d_c^-1/2
+    int bad;
+};
+
+template <typename T>
+__device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
+                                                FitState* st, PotrfLds<T>& S) {
+    auto& colb = S.colb;
+    auto& colp = S.colp;
+    auto& dsh = S.dsh;
+    auto& Lr = S.Lr;
+    auto& rsqsh = S.rsqsh;
+    int& bad = S.bad;
     const int r = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool rin = r < jb;
@@ -471,15 +495,27 @@ __global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __rest
     for (int i = 0; i < NB / 4; i++) D[(4 * i + p) * NB] = xr[i];
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
+                                                    FitState* st) {
+    __shared__ PotrfLds<T> S;
+    potrf_diag_body<T>(B, Dinv, n, j0, jb, st, S);
+}
+
 // Panel below the diagonal block: L21 = B21 X', 64 rows per work-group; wave
 // q computes columns c = 4t + q of every row (row i in lane i).
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, const T* __restrict__ Dinv, int n, int j0,
-                                                   int jb) {
-    __shared__ T Xs[NB][NB];
+struct TrsmLds {
+    T Xs[NB][NB];
+};
+
+template <typename T>
+__device__ __forceinline__ void trsm_body(T* __restrict__ B, const T* __restrict__ Dinv, int n, int j0, int jb,
+                                          int bx, TrsmLds<T>& S) {
+    auto& Xs = S.Xs;
     const int lane = threadIdx.x & 63;
     const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = j0 + jb + blockIdx.x * NB + lane;
+    const int i = j0 + jb + bx * NB + lane;
     const bool ok = i < n;
     const T* D = Dinv + (i64)(j0 / NB) * NB * NB;
     T xv[NB * NB / 256];
@@ -509,15 +545,27 @@ __global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, const T* _
             if (4 * t + q < jb) B[(i64)(j0 + 4 * t + q) * n + i] = acc[t];
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, const T* __restrict__ Dinv, int n, int j0,
+                                                   int jb) {
+    __shared__ TrsmLds<T> S;
+    trsm_body<T>(B, Dinv, n, j0, jb, blockIdx.x, S);
+}
+
 // Lower trailing tiles: B[R, C] -= L21[R] L21[C]' for tile C <= tile R.
 template <typename T>
-__global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, int jb) {
-    if (blockIdx.y > blockIdx.x) return;
-    __shared__ T As[NB][NB + 1];  // As[k][r]
-    __shared__ T Bs[NB][NB + 1];  // Bs[k][c]
+struct SyrkLds {
+    T As[NB][NB + 1];  // As[k][r]
+    T Bs[NB][NB + 1];  // Bs[k][c]
+};
+
+template <typename T>
+__device__ __forceinline__ void syrk_body(T* __restrict__ B, int n, int j0, int jb, int bx, int by, SyrkLds<T>& S) {
+    auto& As = S.As;
+    auto& Bs = S.Bs;
     const int tid = threadIdx.x;
     const int r0 = j0 + jb;
-    const int R = r0 + blockIdx.x * NB, C = r0 + blockIdx.y * NB;
+    const int R = r0 + bx * NB, C = r0 + by * NB;
     const int tx = tid % 16, ty = tid / 16;
     T va[NB * NB / 256], vb[NB * NB / 256], old[4][4];
     #pragma unroll
@@ -566,6 +614,13 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
         }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, int jb) {
+    if (blockIdx.y > blockIdx.x) return;
+    __shared__ SyrkLds<T> S;
+    syrk_body<T>(B, n, j0, jb, blockIdx.x, blockIdx.y, S);
+}
+
 constexpr int SOLVE_THREADS = 512;
 constexpr int BUF_FLAGS = 0x00020000;  // raw buffer, gfx9-family data format
 
@@ -588,12 +643,19 @@ __device__ __forceinline__ T ld_buf(__amdgpu_buffer_rsrc_t r, uint32_t voff, uin
 // the block's X) before the block's product and their latency overlaps it.
 // skip (optional): a nonzero *skip makes the call a no-op.
 template <typename T>
-__global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restrict__ L, const T* __restrict__ Dinv,
-                                                              int n, T* __restrict__ b, const int* skip) {
-    if (skip && *skip) return;
-    __shared__ T Xs[NB][NB + 1];
-    __shared__ T vk[NB][4];
-    __shared__ T yk[NB][4];
+struct SolveLds {
+    T Xs[NB][NB + 1];
+    T vk[NB][4];
+    T yk[NB][4];
+};
+
+template <typename T, int NT>
+__device__ __forceinline__ void chol_solve_body(const T* __restrict__ L, const T* __restrict__ Dinv, int n,
+                                                T* __restrict__ b, SolveLds<T>& S) {
+    static_assert(NT >= 4 * NB && NT % NB == 0, "one thread per right-hand-side entry of a block");
+    auto& Xs = S.Xs;
+    auto& vk = S.vk;
+    auto& yk = S.yk;
     const int tid = threadIdx.x, lane = tid & 63;
     const int nb = (n + NB - 1) / NB;
     // L through a buffer resource (the host keeps n * n * sizeof(T) < 2^31).
@@ -626,9 +688,9 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
             // so the row loads issued after them stay in flight past the
             // barrier.
             const T* D = Dinv + (i64)kb * NB * NB;
-            T xv[NB * NB / SOLVE_THREADS];
+            T xv[NB * NB / NT];
             #pragma unroll
-            for (int u = 0; u < NB * NB / SOLVE_THREADS; u++) xv[u] = D[tid + SOLVE_THREADS * u];
+            for (int u = 0; u < NB * NB / NT; u++) xv[u] = D[tid + NT * u];
             T bv = 0;
             if (tid < 4 * NB) {
                 const int r = tid & (NB - 1), j = tid / NB;
@@ -637,8 +699,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
             if (i < r1) load_row(i);
             if (tid < 4 * NB) vk[tid & (NB - 1)][tid / NB] = bv;
             #pragma unroll
-            for (int u = 0; u < NB * NB / SOLVE_THREADS; u++) {
-                const int e = tid + SOLVE_THREADS * u;
+            for (int u = 0; u < NB * NB / NT; u++) {
+                const int e = tid + NT * u;
                 Xs[e / NB][e & (NB - 1)] = xv[u];
             }
             __syncthreads();
@@ -659,7 +721,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
                     for (int j = 0; j < 4; j++) b[(i64)j * n + k0 + lane] = y[j];
             }
             __syncthreads();
-            for (bool first = true; i < r1; i += SOLVE_THREADS, first = false) {
+            for (bool first = true; i < r1; i += NT, first = false) {
                 if (!first) load_row(i);
                 T acc[4] = {0, 0, 0, 0};
                 #pragma unroll
@@ -676,6 +738,15 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restric
         }
     }
 }
+
+template <typename T>
+__global__ __launch_bounds__(SOLVE_THREADS) void k_chol_solve(const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                              int n, T* __restrict__ b, const int* skip) {
+    if (skip && *skip) return;
+    __shared__ SolveLds<T> S;
+    chol_solve_body<T, SOLVE_THREADS>(L, Dinv, n, b, S);
+}
+
 
 // The same solve for large n, one launch per 64-row block so that many
 // work-groups share the rows to update (the single work-group above streams
@@ -779,14 +850,13 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
 }
 
 // U (N x 4, column-major) <- Q' U (trans = 1) or Q U (trans = 0), Q = I - V T V'.
-__global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, const FitState* st,
-                                                  double* __restrict__ U, int N, int trans) {
-    __shared__ double red[17 * 16];
-    __shared__ double H[16];
+template <int NT>
+__device__ void q_apply_body(const double* __restrict__ V, const FitState* st, double* __restrict__ U, int N,
+                             int trans, double* red, double* H) {
     const int tid = threadIdx.x;
     double G[16];
     for (int i = 0; i < 16; i++) G[i] = 0;
-    for (int i = tid; i < N; i += 1024) {
+    for (int i = tid; i < N; i += NT) {
         double v[4], u[4];
         for (int k = 0; k < 4; k++) { v[k] = V[(size_t)k * N + i]; u[k] = U[(size_t)k * N + i]; }
         for (int a = 0; a < 4; a++)
@@ -801,13 +871,21 @@ __global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, 
                 H[a * 4 + b] = acc;
             }
     __syncthreads();
-    for (int i = tid; i < N; i += 1024)
+    for (int i = tid; i < N; i += NT)
         for (int b = 0; b < 4; b++) {
             double acc = 0;
             for (int k = 0; k < 4; k++) acc += V[(size_t)k * N + i] * H[k * 4 + b];
             U[(size_t)b * N + i] -= acc;
         }
 }
+
+__global__ __launch_bounds__(1024) void k_q_apply(const double* __restrict__ V, const FitState* st,
+                                                  double* __restrict__ U, int N, int trans) {
+    __shared__ double red[17 * 16];
+    __shared__ double H[16];
+    q_apply_body<1024>(V, st, U, N, trans, red, H);
+}
+
 
 // out = (with_y ? Y : 0) - A' x, x and out N x 4 column-major.
 __global__ __launch_bounds__(256) void k_matvec(const SysDesc s, const double* __restrict__ x,
@@ -1032,6 +1110,262 @@ void cholesky(tw_ctx* ctx, T* B, T* Dinv, int n, FitState* st) {
 }
 
 
+// Grid-kernel coefficients from the scaled solution x (m x 4, column-major):
+// cf[3i + q] = c'[i][q+1] / s and d = T^-1 d' (d[0][j] = d'0j - (mu . d'[1..3][j]) / s,
+// d[a][j] = d'[a][j] / s), as the host unscaling of tw_tps_fit.
+__device__ __forceinline__ void unscale_coef(const double* __restrict__ x, int N, double s, double* __restrict__ cf,
+                                             int i) {
+    const int m = N + 4;
+    #pragma unroll
+    for (int q = 0; q < 3; q++) cf[3 * i + q] = x[(i64)(q + 1) * m + i] / s;
+}
+__device__ __forceinline__ void unscale_affine(const double* __restrict__ x, int N, double s, double mu0, double mu1,
+                                               double mu2, double* __restrict__ d, int j) {
+    const int m = N + 4;
+    const double d0 = x[(i64)j * m + N], d1 = x[(i64)j * m + N + 1], d2 = x[(i64)j * m + N + 2],
+                 d3 = x[(i64)j * m + N + 3];
+    d[0 * 4 + j] = d0 - (mu0 * d1 + mu1 * d2 + mu2 * d3) / s;
+    d[1 * 4 + j] = d1 / s;
+    d[2 * 4 + j] = d2 / s;
+    d[3 * 4 + j] = d3 / s;
+}
+
+__global__ void k_unscale(const double* __restrict__ x, int N, double s, double mu0, double mu1, double mu2,
+                          double* __restrict__ cf, double* __restrict__ d) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N) unscale_coef(x, N, s, cf, i);
+    if (blockIdx.x == 0 && threadIdx.x < 4) unscale_affine(x, N, s, mu0, mu1, mu2, d, threadIdx.x);
+}
+
+// ---- small systems: the whole fit in one work-group ------------------------
+// The FP64 steps of fit_run (QR of X', W, B, blocked Cholesky with inverted
+// diagonal blocks, substitution, c' and d', bordered residual) run in one
+// work-group of 256 threads with work-group barriers between them, so a
+// single small system costs one launch instead of a kernel sequence and a
+// batch of systems takes one launch with a work-group per system.  The
+// tile kernels' bodies are shared with the large-system path; the
+// matrix-free products go one row per thread.
+constexpr int FIT_SMALL_THREADS = 256;
+
+struct SmallFitJob {
+    SysDesc s;
+    double* ws;       // small_ws_doubles(N) doubles
+    FitState* st;
+    double* cf;       // grid-kernel coefficients (K x 3) and affine block (4 x 4), or null
+    double* d;
+    double scale;     // s (the solution is in scaled coordinates)
+};
+
+// Workspace of one small system: V, P, W, U, U2 (N x 4), x (m x 4), z (n x 4),
+// B (n x n), the inverted diagonal blocks.
+inline size_t small_ws_doubles(int N) {
+    const size_t n = (size_t)N - 4, m = (size_t)N + 4, nb = (n + NB - 1) / NB;
+    return 20 * (size_t)N + 4 * m + 4 * n + n * n + nb * NB * NB;
+}
+// Offset of x in that workspace.
+inline size_t small_ws_x(int N) { return 20 * (size_t)N; }
+
+union SmallFitLds {
+    PotrfLds<double> potrf;
+    TrsmLds<double> trsm;
+    SyrkLds<double> syrk;
+    SolveLds<double> solve;
+    struct {
+        double red[17 * 16];
+        double M[16], Ts[16], H[16], sh[8];
+    } v;
+};
+
+// acc = row i of M' x, x being len x 4 column-major: len = N takes the A'
+// block of the bordered matrix M', len = m the whole of it.
+__device__ __forceinline__ void row_product(const SysDesc& s, const double* __restrict__ x, int len, int i,
+                                            double* acc) {
+    acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
+    for (int k = 0; k < len; k++) {
+        const double a = sys_entry(s, i, k);
+        #pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] += a * x[(i64)j * len + k];
+    }
+}
+
+__device__ void fit_small_body(const SmallFitJob& J, SmallFitLds& L) {
+    constexpr int NT = FIT_SMALL_THREADS;
+    const SysDesc& s = J.s;
+    const int N = s.N, n = N - 4, m = N + 4, tid = threadIdx.x;
+    const size_t NV = (size_t)N * 4;
+    double* V = J.ws;
+    double* P = V + NV;
+    double* W = P + NV;
+    double* U = W + NV;
+    double* U2 = U + NV;
+    double* x = U2 + NV;
+    double* z = x + (size_t)m * 4;
+    double* B = z + (size_t)n * 4;
+    double* Dinv = B + (size_t)n * n;
+    FitState* st = J.st;
+    if (tid == 0) {
+        st->ir_stop = 0;
+        st->ir_prev = INFINITY;
+        st->ir_iters = 0;
+        st->info = 0;
+    }
+    __syncthreads();
+    // 1. QR of X', P = A' V, W.
+    hh_x_body<NT>(s, V, st, L.v.red, L.v.sh);
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {
+        double acc[4];
+        row_product(s, V, N, i, acc);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) P[(size_t)k * N + i] = acc[k];
+    }
+    __syncthreads();
+    build_w_body<NT>(V, P, W, st, N, L.v.red, L.v.M, L.v.Ts);
+    // 2. U = Q'Y; B; z = U[4:].
+    for (int i = tid; i < N; i += NT)
+        #pragma unroll
+        for (int k = 0; k < 4; k++) U[(size_t)k * N + i] = s.y[i * 4 + k];
+    __syncthreads();
+    q_apply_body<NT>(V, st, U, N, 1, L.v.red, L.v.H);
+    __syncthreads();
+    for (i64 e = tid; e < (i64)n * n; e += NT) {
+        const int i = (int)(e % n) + 4, j = (int)(e / n) + 4;
+        double v = sys_entry(s, i, j);
+        #pragma unroll
+        for (int k = 0; k < 4; k++)
+            v -= W[(size_t)k * N + i] * V[(size_t)k * N + j] + V[(size_t)k * N + i] * W[(size_t)k * N + j];
+        B[e] = v;
+    }
+    for (int i = tid; i < n; i += NT)
+        #pragma unroll
+        for (int k = 0; k < 4; k++) z[(size_t)k * n + i] = U[(size_t)k * N + 4 + i];
+    __syncthreads();
+    // 3. Blocked Cholesky of B, tile by tile.
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int jb = min(NB, n - j0);
+        potrf_diag_body<double>(B, Dinv, n, j0, jb, st, L.potrf);
+        __syncthreads();
+        const int nt = (n - j0 - jb + NB - 1) / NB;
+        for (int bx = 0; bx < nt; bx++) {
+            trsm_body<double>(B, Dinv, n, j0, jb, bx, L.trsm);
+            __syncthreads();
+        }
+        for (int bx = 0; bx < nt; bx++)
+            for (int by = 0; by <= bx; by++) {
+                syrk_body<double>(B, n, j0, jb, bx, by, L.syrk);
+                __syncthreads();
+            }
+    }
+    chol_solve_body<double, NT>(B, Dinv, n, z, L.solve);
+    __syncthreads();
+    // 4. c' = Q [0; z]; U2 = Q'(Y - A'c'); d' = R^-1 U2[0:4].
+    for (int i = tid; i < N; i += NT)
+        #pragma unroll
+        for (int k = 0; k < 4; k++) U[(size_t)k * N + i] = i < 4 ? 0.0 : z[(size_t)k * n + i - 4];
+    __syncthreads();
+    q_apply_body<NT>(V, st, U, N, 0, L.v.red, L.v.H);
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {
+        double acc[4];
+        row_product(s, U, N, i, acc);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) U2[(size_t)k * N + i] = s.y[i * 4 + k] - acc[k];
+    }
+    __syncthreads();
+    q_apply_body<NT>(V, st, U2, N, 1, L.v.red, L.v.H);
+    __syncthreads();
+    for (int i = tid; i < N; i += NT)
+        #pragma unroll
+        for (int j = 0; j < 4; j++) x[(i64)j * m + i] = U[(i64)j * N + i];
+    if (tid < 4) {
+        const int j = tid;
+        double dp[4];
+        for (int a = 3; a >= 0; a--) {
+            double v = U2[(i64)j * N + a];
+            for (int q = a + 1; q < 4; q++) v -= st->R[a * 4 + q] * dp[q];
+            dp[a] = v / st->R[a * 4 + a];
+        }
+        for (int a = 0; a < 4; a++) x[(i64)j * m + N + a] = dp[a];
+    }
+    __syncthreads();
+    // 5. Bordered residual max, unscaled coefficients.
+    double rm = 0.0;
+    for (int r = tid; r < m; r += NT) {
+        double acc[4];
+        row_product(s, x, m, r, acc);
+        #pragma unroll
+        for (int j = 0; j < 4; j++) rm = fmax(rm, fabs((r < N ? s.y[r * 4 + j] : 0.0) - acc[j]));
+    }
+    for (int o = 32; o > 0; o >>= 1) rm = fmax(rm, __shfl_xor(rm, o, 64));
+    if ((tid & 63) == 0) L.v.red[tid >> 6] = rm;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NT / 64; w++) t = fmax(t, L.v.red[w]);
+        st->fit_rmax = t;
+    }
+    if (J.cf) {
+        for (int i = tid; i < N; i += NT) unscale_coef(x, N, J.scale, J.cf, i);
+        if (tid < 4) unscale_affine(x, N, J.scale, s.mu[0], s.mu[1], s.mu[2], J.d, tid);
+    }
+}
+
+__global__ __launch_bounds__(FIT_SMALL_THREADS) void k_fit_small(const SmallFitJob job) {
+    __shared__ SmallFitLds L;
+    fit_small_body(job, L);
+}
+
+__global__ __launch_bounds__(FIT_SMALL_THREADS) void k_fit_small_batch(const SmallFitJob* __restrict__ jobs) {
+    __shared__ SmallFitLds L;
+    const SmallFitJob job = jobs[blockIdx.x];
+    fit_small_body(job, L);
+}
+
+// Systems up to this many landmarks are fitted by one work-group.
+constexpr int FIT_SMALL_MAX = 256;
+
+// Host-side scaling of a system: centroid mu and half-extent s of the
+// centres; bmax = the largest |entry| of [1 targets] (the residual's scale).
+struct HostSys {
+    double mu[3], s, bmax;
+};
+bool host_sys(const float* centres, const float* targets, int N, HostSys& h) {
+    for (int a = 0; a < 3; a++) h.mu[a] = 0;
+    for (int i = 0; i < N; i++)
+        for (int a = 0; a < 3; a++) h.mu[a] += centres[3 * i + a];
+    for (int a = 0; a < 3; a++) h.mu[a] /= N;
+    h.s = 0;
+    for (int i = 0; i < N; i++)
+        for (int a = 0; a < 3; a++) h.s = std::max(h.s, std::fabs(centres[3 * i + a] - h.mu[a]));
+    h.bmax = 1.0;
+    for (int i = 0; i < 3 * N; i++) h.bmax = std::max(h.bmax, std::fabs((double)targets[i]));
+    return h.s > 0;
+}
+
+// Landmark pack of a system: N centres as float4, then N rows [1 target] (FP64).
+inline size_t pack_pts_bytes(int N) { return ((size_t)N * sizeof(float4) + 255) & ~(size_t)255; }
+inline size_t pack_y_bytes(int N) { return ((size_t)N * 4 * sizeof(double) + 255) & ~(size_t)255; }
+void pack_host(const float* centres, const float* targets, int N, uint8_t* out) {
+    float4* hp = (float4*)out;
+    double* hy = (double*)(out + pack_pts_bytes(N));
+    for (int i = 0; i < N; i++) {
+        hp[i] = make_float4(centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], 0.f);
+        hy[4 * i] = 1.0;
+        for (int a = 0; a < 3; a++) hy[4 * i + 1 + a] = targets[3 * i + a];
+    }
+}
+SysDesc sys_desc(const HostSys& h, const void* pack, int N, double lambda) {
+    SysDesc sd;
+    sd.pts = (const float4*)pack;
+    sd.y = (const double*)((const uint8_t*)pack + pack_pts_bytes(N));
+    sd.N = N;
+    sd.m = N + 4;
+    sd.inv_s = 1.0 / h.s;
+    for (int a = 0; a < 3; a++) sd.mu[a] = h.mu[a];
+    sd.lam = lambda / h.s;
+    return sd;
+}
+
 // Device results of one fit run (scaled coordinates: c = c'/s, d = T^-1 d').
 struct FitRun {
     double* x;        // m x 4 column-major: [c'; d'], device
@@ -1063,36 +1397,44 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
             FitRun& out) {
     int rc;
     const int N = K, n = N - 4, m = N + 4;
-    double mu[3] = {0, 0, 0};
-    for (int i = 0; i < N; i++)
-        for (int a = 0; a < 3; a++) mu[a] += centres[3 * i + a];
-    for (int a = 0; a < 3; a++) mu[a] /= N;
-    double s = 0;
-    for (int i = 0; i < N; i++)
-        for (int a = 0; a < 3; a++) s = std::max(s, std::fabs(centres[3 * i + a] - mu[a]));
-    if (!(s > 0)) return tw_fail(ctx, TW_E_SINGULAR, "all centres coincide");
+    HostSys hs;
+    if (!host_sys(centres, targets, K, hs)) return tw_fail(ctx, TW_E_SINGULAR, "all centres coincide");
+    const double s = hs.s;
+    const double* mu = hs.mu;
 
     // Landmark packs.
-    const size_t pb = (size_t)N * sizeof(float4), yb = (size_t)N * 4 * sizeof(double);
+    const size_t pb = pack_pts_bytes(N), yb = pack_y_bytes(N);
     void* pack = nullptr;
     if ((rc = tw_ws(ctx, WS_PTS, pb + yb, &pack))) return rc;
     std::vector<uint8_t> host(pb + yb);
-    float4* hp = (float4*)host.data();
-    double* hy = (double*)(host.data() + pb);
-    for (int i = 0; i < N; i++) {
-        hp[i] = make_float4(centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], 0.f);
-        hy[4 * i] = 1.0;
-        for (int a = 0; a < 3; a++) hy[4 * i + 1 + a] = targets[3 * i + a];
-    }
+    pack_host(centres, targets, N, host.data());
     TW_HIP(ctx, hipMemcpyAsync(pack, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
-    SysDesc sd;
-    sd.pts = (const float4*)pack;
-    sd.y = (const double*)((uint8_t*)pack + pb);
-    sd.N = N;
-    sd.m = m;
-    sd.inv_s = 1.0 / s;
-    for (int a = 0; a < 3; a++) sd.mu[a] = mu[a];
-    sd.lam = lambda / s;
+    const SysDesc sd = sys_desc(hs, pack, N, lambda);
+    out.pts = sd.pts;
+    out.s = s;
+    for (int a = 0; a < 3; a++) out.mu[a] = mu[a];
+    out.bmax = hs.bmax;
+    out.N = N;
+    out.m = m;
+    out.iters = 0;
+
+    if (K <= FIT_SMALL_MAX) {
+        // One work-group, FP64 (every precision form).
+        void *arena = nullptr, *aux = nullptr;
+        if ((rc = tw_ws(ctx, WS_SOLVE_B, small_ws_doubles(N) * sizeof(double), &arena))) return rc;
+        if ((rc = tw_ws(ctx, WS_SOLVE_AUX, sizeof(FitState), &aux))) return rc;
+        SmallFitJob job;
+        job.s = sd;
+        job.ws = (double*)arena;
+        job.st = (FitState*)aux;
+        job.cf = job.d = nullptr;
+        job.scale = s;
+        hipLaunchKernelGGL(k_fit_small, dim3(1), dim3(FIT_SMALL_THREADS), 0, ctx->stream, job);
+        TW_LAUNCH_CHECK(ctx);
+        out.x = (double*)arena + small_ws_x(N);
+        out.st = job.st;
+        return TW_OK;
+    }
 
     // Workspaces: B (n x n, FP64 or FP32); vectors in one FP64 arena;
     // state, FP32 vectors and the inverted diagonal blocks in aux.
@@ -1216,14 +1558,6 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
     TW_LAUNCH_CHECK(ctx);
     out.x = x;
     out.st = st;
-    out.pts = sd.pts;
-    out.s = s;
-    for (int a = 0; a < 3; a++) out.mu[a] = mu[a];
-    double bmax = 1.0;
-    for (int i = 0; i < 3 * N; i++) bmax = std::max(bmax, std::fabs((double)targets[i]));
-    out.bmax = bmax;
-    out.N = N;
-    out.m = m;
     out.iters = iters;
     return TW_OK;
 }
@@ -1240,27 +1574,6 @@ int fit_finish(tw_ctx* ctx, const FitState* st, double bmax, int iters) {
     ctx->refine_iters = iters;
     if (!std::isfinite(ctx->fit_residual)) return tw_fail(ctx, TW_E_SINGULAR, "non-finite TPS coefficients");
     return TW_OK;
-}
-
-// Grid-kernel coefficients from the scaled solution x (m x 4, column-major):
-// cf[3i + q] = c'[i][q+1] / s and d = T^-1 d' (d[0][j] = d'0j - (mu . d'[1..3][j]) / s,
-// d[a][j] = d'[a][j] / s), as the host unscaling of tw_tps_fit.
-__global__ void k_unscale(const double* __restrict__ x, int N, double s, double mu0, double mu1, double mu2,
-                          double* __restrict__ cf, double* __restrict__ d) {
-    const int m = N + 4;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < N)
-        #pragma unroll
-        for (int q = 0; q < 3; q++) cf[3 * i + q] = x[(i64)(q + 1) * m + i] / s;
-    if (blockIdx.x == 0 && threadIdx.x < 4) {
-        const int j = threadIdx.x;
-        const double d0 = x[(i64)j * m + N], d1 = x[(i64)j * m + N + 1], d2 = x[(i64)j * m + N + 2],
-                     d3 = x[(i64)j * m + N + 3];
-        d[0 * 4 + j] = d0 - (mu0 * d1 + mu1 * d2 + mu2 * d3) / s;
-        d[1 * 4 + j] = d1 / s;
-        d[2 * 4 + j] = d2 / s;
-        d[3 * 4 + j] = d3 / s;
-    }
 }
 
 }  // namespace
@@ -1327,4 +1640,121 @@ int tw_tps_fit_dev(tw_ctx* ctx, const float* centres, const float* targets, int 
 
 int tw_tps_fit_check(tw_ctx* ctx, const TwFitDev* fit) {
     return fit_finish(ctx, (const FitState*)fit->state, fit->bmax, ctx->refine_iters);
+}
+
+// ---- batches -----------------------------------------------------------------
+// Every system up to FIT_SMALL_MAX landmarks in one launch of the one-work-
+// group kernel (arithmetic identical to tw_tps_fit_dev's for the same
+// system); larger systems through fit_run one after another.  Layout of
+// WS_BATCH_FIT: landmark packs, small-system jobs, fit states, coefficients.
+namespace {
+constexpr size_t STATE_BYTES = (sizeof(FitState) + 255) & ~(size_t)255;
+inline size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+int tw_fit_batch(tw_ctx* ctx, TwFitItem* items, int n, double lambda, int precision) {
+    int rc;
+    std::vector<HostSys> hs((size_t)n);
+    std::vector<size_t> pack_off((size_t)n), ws_off((size_t)n), coef_off((size_t)n);
+    size_t pack_total = 0, ws_total = 0, coef_total = 0;
+    int nsmall = 0;
+    for (int i = 0; i < n; i++) {
+        const TwFitItem& it = items[i];
+        if ((rc = check_fit_args(ctx, it.centres, it.targets, it.K, lambda, precision)))
+            return tw_fail(ctx, rc, "item %d: %s", i, tw_last_error(ctx));
+        if (!host_sys(it.centres, it.targets, it.K, hs[i]))
+            return tw_fail(ctx, TW_E_SINGULAR, "item %d: all centres coincide", i);
+        pack_off[i] = pack_total;
+        pack_total += pack_pts_bytes(it.K) + pack_y_bytes(it.K);
+        coef_off[i] = coef_total;
+        coef_total += up256(((size_t)it.K * 3 + 16) * sizeof(double));
+        if (it.K <= FIT_SMALL_MAX) {
+            ws_off[i] = ws_total;
+            ws_total += up256(small_ws_doubles(it.K) * sizeof(double));
+            nsmall++;
+        }
+    }
+    const size_t jobs_off = pack_total, jobs_b = up256((size_t)nsmall * sizeof(SmallFitJob));
+    const size_t st_off = jobs_off + jobs_b, coefs_off = st_off + (size_t)n * STATE_BYTES;
+    void *base = nullptr, *wsb = nullptr;
+    if ((rc = tw_ws(ctx, WS_BATCH_FIT, coefs_off + coef_total, &base))) return rc;
+    if ((rc = tw_ws(ctx, WS_BATCH_SOLVE, ws_total, &wsb))) return rc;
+    uint8_t* dev = (uint8_t*)base;
+    uint8_t* host = tw_host_stage(ctx, st_off);
+    SmallFitJob* hjobs = (SmallFitJob*)(host + jobs_off);
+    int js = 0;
+    for (int i = 0; i < n; i++) {
+        TwFitItem& it = items[i];
+        pack_host(it.centres, it.targets, it.K, host + pack_off[i]);
+        it.pts = (const float4*)(dev + pack_off[i]);
+        it.cf = (double*)(dev + coefs_off + coef_off[i]);
+        it.d = it.cf + (size_t)it.K * 3;
+        it.state = dev + st_off + (size_t)i * STATE_BYTES;
+        it.bmax = hs[i].bmax;
+        if (it.K <= FIT_SMALL_MAX) {
+            SmallFitJob& J = hjobs[js++];
+            J.s = sys_desc(hs[i], dev + pack_off[i], it.K, lambda);
+            J.ws = (double*)((uint8_t*)wsb + ws_off[i]);
+            J.st = (FitState*)it.state;
+            J.cf = it.cf;
+            J.d = it.d;
+            J.scale = hs[i].s;
+        }
+    }
+    TW_HIP(ctx, hipMemcpyAsync(dev, host, st_off, hipMemcpyHostToDevice, ctx->stream));
+    if (nsmall > 0) {
+        hipLaunchKernelGGL(k_fit_small_batch, dim3((unsigned)nsmall), dim3(FIT_SMALL_THREADS), 0, ctx->stream,
+                           (const SmallFitJob*)(dev + jobs_off));
+        TW_LAUNCH_CHECK(ctx);
+    }
+    int iters = 0;
+    for (int i = 0; i < n; i++) {
+        TwFitItem& it = items[i];
+        if (it.K <= FIT_SMALL_MAX) continue;
+        FitRun r;
+        if ((rc = fit_run(ctx, it.centres, it.targets, it.K, lambda, precision, r)))
+            return tw_fail(ctx, rc, "item %d: %s", i, tw_last_error(ctx));
+        hipLaunchKernelGGL(k_unscale, dim3(nblk(it.K)), dim3(256), 0, ctx->stream, r.x, it.K, r.s, r.mu[0], r.mu[1],
+                           r.mu[2], it.cf, it.d);
+        TW_LAUNCH_CHECK(ctx);
+        TW_HIP(ctx, hipMemcpyAsync(it.state, r.st, sizeof(FitState), hipMemcpyDeviceToDevice, ctx->stream));
+        iters = std::max(iters, r.iters);
+    }
+    ctx->refine_iters = iters;
+    return TW_OK;
+}
+
+int tw_fit_batch_check(tw_ctx* ctx, const TwFitItem* items, int n, int* status) {
+    if (n <= 0) return TW_OK;
+    const uint8_t* st0 = (const uint8_t*)items[0].state;
+    std::vector<uint8_t> h((size_t)n * STATE_BYTES);
+    TW_HIP(ctx, hipMemcpy(h.data(), st0, h.size(), hipMemcpyDeviceToHost));
+    int first = TW_OK;
+    double worst = 0.0;
+    char msg[256] = {0};
+    for (int i = 0; i < n; i++) {
+        FitState hst;
+        memcpy(&hst, h.data() + (size_t)i * STATE_BYTES, sizeof hst);
+        int code = TW_OK;
+        if (hst.info == 1) {
+            code = TW_E_SINGULAR;
+            snprintf(msg, sizeof msg, "item %d: landmark centres are coplanar (rank-deficient X)", i);
+        } else if (hst.info >= 2) {
+            code = TW_E_SINGULAR;
+            snprintf(msg, sizeof msg, "item %d: projected TPS system not positive definite (column %d)", i,
+                     hst.info - 2);
+        } else {
+            const double res = hst.fit_rmax / items[i].bmax;
+            if (!std::isfinite(res)) {
+                code = TW_E_SINGULAR;
+                snprintf(msg, sizeof msg, "item %d: non-finite TPS coefficients", i);
+            } else {
+                worst = std::max(worst, res);
+            }
+        }
+        if (status) status[i] = code;
+        if (code != TW_OK && first == TW_OK) first = tw_fail(ctx, code, "%s", msg);
+    }
+    ctx->fit_residual = worst;
+    return first;
 }

@@ -20,6 +20,17 @@ int tw_bind(tw_ctx* ctx) {
     return TW_OK;
 }
 
+uint8_t* tw_host_stage(tw_ctx* ctx, size_t bytes) {
+    ctx->staging.emplace_back(bytes ? bytes : 1, (uint8_t)0);
+    return ctx->staging.back().data();
+}
+
+void tw_host_stage_reset(tw_ctx* ctx) {
+    if (ctx->staging.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->staging.clear();
+}
+
 int tw_ws(tw_ctx* ctx, int slot, size_t bytes, void** out) {
     if (bytes == 0) bytes = 256;
     if (ctx->ws_cap[slot] < bytes) {

@@ -29,6 +29,11 @@ enum TwSlot {
     WS_WEIGHTS,     // B-spline weight tables
     WS_AUX_BOUNDS,  // grid layer bounds
     WS_COEF,        // device-side grid coefficients of the chained warp
+    WS_BATCH_FIT,   // batch: landmark packs, fit jobs, fit states, coefficients
+    WS_BATCH_SOLVE, // batch: per-system solve workspaces
+    WS_BATCH_GRID,  // batch: grids (FP32 fields, FP16 planes, block tables)
+    WS_BATCH_GTAB,  // batch: grid job tables
+    WS_BATCH_RTAB,  // batch: resample job tables, queues, timestamps
     WS_COUNT
 };
 
@@ -53,7 +58,15 @@ struct tw_ctx {
     // weights_f (0: none yet); their host image must outlive the copy.
     int weights_f = 0;
     std::vector<uint8_t> weights_host;
+    // Host images of a batch's tables, kept until the next batch (the batch
+    // synchronises before it returns).
+    std::vector<std::vector<uint8_t>> staging;
 };
+
+// A zeroed host buffer of `bytes` that lives until tw_host_stage_reset.
+uint8_t* tw_host_stage(tw_ctx* ctx, size_t bytes);
+// Drop the staged host buffers (synchronises the stream first if any exist).
+void tw_host_stage_reset(tw_ctx* ctx);
 
 // Set the context error string and return code.
 int tw_fail(tw_ctx* ctx, int code, const char* fmt, ...);
@@ -120,3 +133,36 @@ int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const dou
                      tw_grid* grid);
 int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
                         int precision);
+
+// ---- batches ---------------------------------------------------------------
+// Fits of a batch: systems up to the small-system limit in one launch (one
+// work-group each), larger ones by the single-system kernel sequence.  The
+// batch fills each item's device centres (pts), grid-kernel coefficients
+// (cf, K x 3), affine block (d, 4 x 4) and fit state, all in WS_BATCH_FIT.
+struct TwFitItem {
+    const float* centres;  // host, K x 3
+    const float* targets;  // host, K x 3
+    int K;
+    double* cf;            // device out: grid-kernel coefficients (K x 3)
+    double* d;             // device out: affine block (4 x 4)
+    const float4* pts;     // device out: centres
+    void* state;           // device out: fit state
+    double bmax;           // out: residual scale
+};
+int tw_fit_batch(tw_ctx* ctx, TwFitItem* items, int n, double lambda, int precision);
+// Check the states of a batch's fits (one device-to-host copy; the caller has
+// synchronised).  status[i] = TW_OK or the item's error code; returns the
+// first error (message names the item) or TW_OK.
+int tw_fit_batch_check(tw_ctx* ctx, const TwFitItem* items, int n, int* status);
+
+// Grids of a batch: grid.gx..gz, f, ox..oz, storage (and, for FP16, data =
+// FP16 planes and qtab) describe the output; f32 holds the FP32 field.
+struct TwGridItem {
+    const float4* pts;
+    const double* cf;
+    const double* d;
+    int K;
+    tw_grid grid;
+    float* f32;
+};
+int tw_grid_eval_batch(tw_ctx* ctx, const TwGridItem* items, int n);

@@ -282,3 +282,42 @@ class Warper:
         self._check(self.lib.tw_warp_tps(self.ctx, ctypes.byref(vs), ctypes.byref(vd), _fptr(ce), _fptr(ta),
                                          ce.shape[0], f, lam, interp, precision))
         return dst
+
+    # ---- batches ---------------------------------------------------------
+    def warp_batch(self, items, f=8, lam=0.2, interp=LINEAR, precision=MIXED):
+        """Warp many device volumes in one launch per stage (tw_warp_batch).
+
+        items: dicts with "src" and "dst" (device tensors) and either
+        "centres" + "targets" ((K, 3) float32; a TPS warp) or "M" (a 3x4 or
+        4x4 target->subject matrix; an affine or rigid warp), optionally
+        "dst_origin".  Returns (status, done_ms): per-item status codes and the
+        device-clock times at which the items' outputs were complete.
+        Raises on the first failed item after the batch ran.
+        """
+        n = len(items)
+        arr = (L.TwWarpItem * n)()
+        keep = []
+        for i, it in enumerate(items):
+            w = arr[i]
+            w.src = _vol(it["src"])
+            w.dst = _vol(it["dst"], it.get("dst_origin", (0, 0, 0)))
+            if "M" in it:
+                w.kind = L.TW_WARP_AFFINE
+                M = np.ascontiguousarray(it["M"], np.float64).reshape(-1)[:12]
+                for k in range(12):
+                    w.M[k] = float(M[k])
+            else:
+                ce, ta = _pts(it["centres"]), _pts(it["targets"])
+                if ce.shape != ta.shape:
+                    raise ValueError(f"item {i}: centres and targets differ in shape")
+                keep += [ce, ta]
+                w.kind = L.TW_WARP_TPS
+                w.K = ce.shape[0]
+                w.centres = _fptr(ce)
+                w.targets = _fptr(ta)
+        status = (ctypes.c_int32 * n)()
+        done = np.zeros(n, np.float64)
+        rc = self.lib.tw_warp_batch(self.ctx, arr, n, f, lam, interp, precision, status, _dptr(done))
+        self._check(rc)
+        return [int(x) for x in status], done
+

@@ -1,0 +1,174 @@
+"""Batched warps (tw_warp_batch): uneven sizes, both dtypes, TPS / affine /
+rigid items in one call.
+
+Bars (stated per test):
+  every item      bit-identical to the single-volume call on the same
+                  arguments (tw_warp_tps / tw_resample_affine): the batch
+                  runs the same per-system, per-node and per-voxel arithmetic
+  TPS items       vs the FP64 oracle chain: within 1 LSB on >= 99.99 % of
+                  voxels (plain and mixed), plain bit-identical on >= 99.9 %
+  affine items    bit-identical to the oracle
+  a failed fit    reported for its item only; the other items are complete
+"""
+import numpy as np
+import pytest
+
+from terawarp_amd import synth, warp
+
+pytestmark = pytest.mark.gpu
+
+
+def _tensor(v):
+    import torch
+    a = v.view(np.int16) if v.dtype == np.uint16 else v
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _host(t, dtype):
+    a = t.cpu().numpy()
+    return a.view(np.uint16) if dtype == np.uint16 else a
+
+
+def make_items(seed, n, size_lo=24, size_hi=80, k_lo=8, k_hi=48, nc2=True):
+    """A seeded mix of TPS (60 %), affine (25 %) and rigid (15 %) items."""
+    rng = np.random.default_rng(seed)
+    items = []
+    for i in range(n):
+        size = tuple(int(s) for s in rng.integers(size_lo, size_hi + 1, 3))
+        dtype = np.uint8 if rng.random() < 0.5 else np.uint16
+        amp = (20, 200) if dtype == np.uint8 else (100, 4000)
+        vol = synth.blob_volume(size[::-1], dtype, n_blobs=12, amp=amp, seed=seed * 1000 + i)
+        if nc2 and i == 1:
+            vol = np.stack([vol, vol[::-1].copy()])
+        u = rng.random()
+        it = {"vol": vol, "size": size, "dtype": dtype}
+        if u < 0.6:
+            K = int(rng.integers(k_lo, k_hi + 1))
+            sub, tar = synth.landmarks(size, K, 4.0, seed=seed * 1000 + i)
+            it.update(kind="tps", centres=tar, targets=sub)
+        elif u < 0.85:
+            it.update(kind="affine", M=synth.random_affine(size, seed=seed * 1000 + i))
+        else:
+            ang = rng.uniform(-180, 180, 3)
+            tr = rng.integers(-32, 33, 3).astype(np.float64)
+            R = warp.mode_matrix(warp.ROTATION, ang, size)
+            T = warp.mode_matrix(warp.TRANSLATION, tr, size)
+            it.update(kind="rigid", M=R @ T)
+        items.append(it)
+    return items
+
+
+def run_batch(warper, items, f, precision, interp=warp.LINEAR):
+    import torch
+    srcs = [_tensor(it["vol"]) for it in items]
+    dsts = [torch.zeros_like(s) for s in srcs]
+    spec = []
+    for it, s, d in zip(items, srcs, dsts):
+        e = {"src": s, "dst": d}
+        if it["kind"] == "tps":
+            e.update(centres=it["centres"], targets=it["targets"])
+        else:
+            e["M"] = it["M"]
+        spec.append(e)
+    status, done = warper.warp_batch(spec, f=f, lam=0.2, interp=interp, precision=precision)
+    torch.cuda.synchronize()
+    return [_host(d, it["dtype"]) for d, it in zip(dsts, items)], status, done, srcs
+
+
+def single(warper, it, src, f, precision, interp=warp.LINEAR):
+    import torch
+    dst = torch.zeros_like(src)
+    if it["kind"] == "tps":
+        warper.warp_tps(src, it["centres"], it["targets"], f=f, lam=0.2, interp=interp, precision=precision, dst=dst)
+    else:
+        warper.resample_affine(src, it["M"], dst=dst, interp=interp)
+    torch.cuda.synchronize()
+    return _host(dst, it["dtype"])
+
+
+@pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
+@pytest.mark.parametrize("interp", [warp.LINEAR, warp.NN])
+def test_batch_equals_single_calls(warper, precision, interp):
+    items = make_items(seed=3 + interp, n=12)
+    f = 8
+    outs, status, done, srcs = run_batch(warper, items, f, precision, interp)
+    assert status == [0] * len(items)
+    assert np.all(done > 0) and np.all(np.isfinite(done))
+    for i, (it, out, src) in enumerate(zip(items, outs, srcs)):
+        ref = single(warper, it, src, f, precision, interp)
+        assert np.array_equal(out, ref), (i, it["kind"], it["size"], np.count_nonzero(out != ref))
+
+
+@pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
+def test_batch_matches_oracle(warper, oracle, precision):
+    items = make_items(seed=11, n=8, nc2=False)
+    f = 4
+    outs, status, _, _ = run_batch(warper, items, f, precision)
+    assert status == [0] * len(items)
+    for i, (it, out) in enumerate(zip(items, outs)):
+        if it["kind"] == "tps":
+            ref = oracle.warp_tps(it["vol"], it["centres"], it["targets"], g=f, lam=0.2)[0]
+            d = np.abs(out.astype(np.int64) - ref.astype(np.int64))
+            assert np.mean(d <= 1) >= 0.9999, (i, np.mean(d <= 1), d.max())
+            if precision == warp.PLAIN:  # FP64 blend: bit-identical on >= 99.9 %
+                assert np.mean(d == 0) >= 0.999, (i, np.mean(d == 0))
+        else:
+            ref = oracle.affine_resample(it["vol"], it["M"])
+            assert np.array_equal(out, ref), (i, it["kind"], np.count_nonzero(out != ref))
+
+
+def test_batch_uneven_large_and_small(warper):
+    """Sizes from one tile to several hundred tiles per axis in one launch,
+    K from 5 to 256 (the one-work-group fit) and 300 (the kernel sequence)."""
+    rng = np.random.default_rng(5)
+    items = []
+    for i, (size, K) in enumerate([((9, 7, 10), 5), ((300, 200, 40), 256), ((65, 5, 130), 300),
+                                   ((128, 128, 64), 64)]):
+        dtype = np.uint16 if i % 2 else np.uint8
+        vol = rng.integers(0, np.iinfo(dtype).max + 1, size[::-1]).astype(dtype)
+        sub, tar = synth.landmarks(size, K, 3.0, seed=40 + i)
+        items.append({"vol": vol, "size": size, "dtype": dtype, "kind": "tps", "centres": tar, "targets": sub})
+    outs, status, _, srcs = run_batch(warper, items, 4, warp.MIXED)
+    assert status == [0] * len(items)
+    for i, (it, out, src) in enumerate(zip(items, outs, srcs)):
+        assert np.array_equal(out, single(warper, it, src, 4, warp.MIXED)), i
+
+
+def test_batch_reports_failed_fit(warper):
+    """A coplanar landmark set fails its own item; the others complete."""
+    items = make_items(seed=17, n=5, nc2=False)
+    bad = next(i for i, it in enumerate(items) if it["kind"] == "tps")
+    P = items[bad]["centres"].copy()
+    P[:, 2] = 5.0
+    items[bad]["centres"] = P
+    import torch
+    srcs = [_tensor(it["vol"]) for it in items]
+    dsts = [torch.zeros_like(s) for s in srcs]
+    spec = [dict(src=s, dst=d, **({"M": it["M"]} if "M" in it else {"centres": it["centres"],
+                                                                      "targets": it["targets"]}))
+            for it, s, d in zip(items, srcs, dsts)]
+    with pytest.raises(warp.TeraWarpError) as e:
+        warper.warp_batch(spec, f=8, precision=warp.PLAIN)
+    assert f"item {bad}" in str(e.value)
+    torch.cuda.synchronize()
+    for i, (it, s, d) in enumerate(zip(items, srcs, dsts)):
+        if i != bad:
+            assert np.array_equal(_host(d, it["dtype"]), single(warper, it, s, 8, warp.PLAIN)), i
+
+
+def test_batch_rejects_bad_arguments(warper):
+    import torch
+    v = np.zeros((20, 20, 20), np.uint8)
+    src = _tensor(v)
+    dst = torch.zeros_like(src)
+    M = np.eye(4)[:3]
+    with pytest.raises(warp.TeraWarpError):  # host volume
+        warper.warp_batch([{"src": v, "dst": dst, "M": M}])
+    with pytest.raises(warp.TeraWarpError):  # dtype mismatch
+        warper.warp_batch([{"src": src, "dst": torch.zeros((20, 20, 20), dtype=torch.int16, device="cuda"),
+                            "M": M}])
+    with pytest.raises(warp.TeraWarpError):  # exact form is single-volume only
+        warper.warp_batch([{"src": src, "dst": dst, "M": M}], precision=warp.EXACT)
+    with pytest.raises(warp.TeraWarpError):  # spacing not below the image size
+        P = np.random.default_rng(0).uniform(0, 19, (10, 3)).astype(np.float32)
+        warper.warp_batch([{"src": src, "dst": dst, "centres": P, "targets": P}], f=32)
