@@ -233,6 +233,170 @@ def precision_run(args):
     print(json.dumps(out))
 
 
+N_B, SEED_B, G_B, DMAX_B = 256, 5, 8, 8.0
+
+
+def batch_specs(seed=SEED_B, n=N_B):
+    """The uneven-instance batch: axes U[64, 512], uint8 / uint16 50/50, and
+    60 % TPS (K U[16, 256], |d| <= DMAX_B), 25 % affine (random 3x4, det in
+    [0.8, 1.2]), 15 % rigid (rotation U[-180, 180] deg per axis, integer
+    translation U[-32, 32])."""
+    from terawarp_amd import synth, warp
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        size = tuple(int(v) for v in rng.integers(64, 513, 3))
+        dtype = np.uint8 if rng.random() < 0.5 else np.uint16
+        u = rng.random()
+        sp = {"size": size, "dtype": dtype, "seed": seed * 100000 + i}
+        if u < 0.6:
+            K = int(rng.integers(16, 257))
+            sub, tar = synth.landmarks(size, K, DMAX_B, seed=sp["seed"])
+            sp.update(kind="tps", centres=tar, targets=sub)
+        elif u < 0.85:
+            sp.update(kind="affine", M=synth.random_affine(size, seed=sp["seed"]))
+        else:
+            ang = rng.uniform(-180.0, 180.0, 3)
+            tr = rng.integers(-32, 33, 3).astype(np.float64)
+            sp.update(kind="rigid", M=warp.mode_matrix(warp.ROTATION, ang, size) @
+                      warp.mode_matrix(warp.TRANSLATION, tr, size))
+        specs.append(sp)
+    return specs
+
+
+def gpu_blob_volume(torch, size, dtype, seed, n_blobs=24):
+    """Seeded blob volume generated on the device (separable Gaussian blobs,
+    sigma 2-8 % of the smallest axis, plus uniform noise)."""
+    g = np.random.default_rng(seed)
+    nx, ny, nz = size
+    hi = 255.0 if dtype == np.uint8 else 4095.0
+    amp = (20.0, 200.0) if dtype == np.uint8 else (100.0, 4000.0)
+    ax = [torch.arange(n, device="cuda", dtype=torch.float32) for n in (nz, ny, nx)]
+    vol = torch.zeros((nz, ny, nx), device="cuda", dtype=torch.float32)
+    smin = min(size)
+    for _ in range(n_blobs):
+        c = g.uniform(0, 1, 3) * np.array([nz, ny, nx])
+        sg = g.uniform(0.02, 0.08) * smin
+        a = g.uniform(*amp)
+        e = [torch.exp(-((ax[k] - float(c[k])) ** 2) / (2 * sg * sg)) for k in range(3)]
+        vol += a * e[0][:, None, None] * e[1][None, :, None] * e[2][None, None, :]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(int(seed))
+    vol += (torch.rand(vol.shape, generator=gen, device="cuda") - 0.5) * 10.0
+    vol.clamp_(0, hi)
+    if dtype == np.uint8:
+        return vol.to(torch.uint8)
+    return vol.to(torch.int32).to(torch.int16)
+
+
+def batch_run(args):
+    """The batched uneven-instance configuration: 256 independent warps in one
+    tw_warp_batch call (one launch per stage), against the same warps as
+    single-volume calls one after another; per-instance latency from the
+    device clock; outputs checked bit-exact against the single calls on a
+    seeded sample and against the FP64 oracle on the smallest instances."""
+    import torch
+    from terawarp_amd import warp
+    from terawarp_amd.warp import Warper
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    specs = batch_specs()
+    log("generating", len(specs), "volumes on the device")
+    srcs = [gpu_blob_volume(torch, sp["size"], sp["dtype"], sp["seed"]) for sp in specs]
+    dsts = [torch.zeros_like(s) for s in srcs]
+    torch.cuda.synchronize()
+    items = []
+    for sp, s, d in zip(specs, srcs, dsts):
+        it = {"src": s, "dst": d}
+        if sp["kind"] == "tps":
+            it.update(centres=sp["centres"], targets=sp["targets"])
+        else:
+            it["M"] = sp["M"]
+        items.append(it)
+    vox = np.array([sp["size"][0] * sp["size"][1] * sp["size"][2] for sp in specs], np.float64)
+    bpv = np.array([1 if sp["dtype"] == np.uint8 else 2 for sp in specs], np.float64)
+    total_vox = float(vox.sum())
+    nodes = np.array([np.prod([warp.grid_dim(v, G_B) for v in sp["size"]]) if sp["kind"] == "tps" else 0
+                      for sp in specs], np.float64)
+    # Algorithmic bytes of the resample launch: one source read and one output
+    # write per voxel, the FP16 grid (6 B per node) of the TPS items.
+    b_alg = float((2 * bpv * vox).sum() + (6 * nodes).sum())
+    w = Warper(0, timing=True)
+    prec = warp.MIXED
+    log("batch: warm-up")
+    w.warp_batch(items, f=G_B, lam=0.2, precision=prec)
+    torch.cuda.synchronize()
+    walls, stages, done = [], [], None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, done = w.warp_batch(items, f=G_B, lam=0.2, precision=prec)
+        walls.append(time.perf_counter() - t0)
+        stages.append(w.timings())
+    wall = float(np.median(walls))
+    st = {k: float(np.median([t[k] for t in stages])) for k in ("fit_ms", "grid_ms", "quant_ms", "resample_ms",
+                                                                   "total_ms")}
+    out_batch = [d.clone() for d in dsts]
+    # The same warps as single-volume calls, one after another.
+    log("sequential single-volume calls")
+
+    def sequential():
+        for sp, s, d in zip(specs, srcs, dsts):
+            if sp["kind"] == "tps":
+                w.warp_tps(s, sp["centres"], sp["targets"], f=G_B, lam=0.2, precision=prec, dst=d)
+            else:
+                w.resample_affine(s, sp["M"], dst=d)
+        torch.cuda.synchronize()
+    sequential()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sequential()
+    seq_wall = (time.perf_counter() - t0) / args.steps
+    same = sum(bool(torch.equal(a, b)) for a, b in zip(out_batch, dsts))
+    # FP64 oracle on the smallest instances of each kind.
+    checks = []
+    try:
+        from oracle import oracle
+        order = np.argsort(vox)
+        picked = []
+        for kind, cnt in (("tps", 3), ("affine", 1), ("rigid", 1)):
+            picked += [int(i) for i in order if specs[i]["kind"] == kind][:cnt]
+        for i in picked:
+            sp = specs[i]
+            v = srcs[i].cpu().numpy()
+            v = v.view(np.uint16) if sp["dtype"] == np.uint16 else v
+            o = out_batch[i].cpu().numpy()
+            o = o.view(np.uint16) if sp["dtype"] == np.uint16 else o
+            if sp["kind"] == "tps":
+                ref = oracle.warp_tps(v, sp["centres"], sp["targets"], g=G_B, lam=0.2, nthreads=threads)[0]
+            else:
+                ref = oracle.affine_resample(v, sp["M"], nthreads=threads)
+            dd = np.abs(o.astype(np.int64) - ref.astype(np.int64))
+            checks.append({"item": i, "kind": sp["kind"], "size_xyz": list(sp["size"]),
+                           "lsb_exact_frac": float(np.mean(dd == 0)), "lsb_le1_frac": float(np.mean(dd <= 1)),
+                           "lsb_max": int(dd.max())})
+    except Exception as e:
+        log("oracle leg failed:", repr(e))
+        checks.append({"error": repr(e)})
+    kinds = [sp["kind"] for sp in specs]
+    line = {
+        "config": {"workload": "batched uneven instances: %d warps, axes U[64,512], uint8/uint16 50/50, "
+                               "60%% TPS (K U[16,256], |d|<=%g, grid %d) / 25%% affine / 15%% rigid "
+                               "(rotation U[-180,180] deg, translation U[-32,32])" % (N_B, DMAX_B, G_B),
+                   "instances": N_B, "tps": kinds.count("tps"), "affine": kinds.count("affine"),
+                   "rigid": kinds.count("rigid"), "voxels": total_vox, "precision": "mixed"},
+        "gvox_s": total_vox / wall / 1e9, "ms_per_batch": wall * 1e3, "steps": args.steps,
+        "stages_ms": st,
+        "resample": {"b_alg_bytes": b_alg, "gbs": b_alg / (st["resample_ms"] * 1e-3) / 1e9,
+                     "frac_of_8tbs": b_alg / (st["resample_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "latency_ms": {"p50": float(np.percentile(done, 50)), "p99": float(np.percentile(done, 99)),
+                       "max": float(np.max(done)), "from": "the batch's first kernel (device clock)"},
+        "sequential": {"gvox_s": total_vox / seq_wall / 1e9, "ms": seq_wall * 1e3},
+        "batch_equals_sequential": "%d/%d" % (same, len(specs)),
+        "oracle_checks": checks,
+    }
+    w.close()
+    print(json.dumps(line))
+
+
 def measure_traffic(vol):
     """HBM bytes per launch of the mixed resample kernel from two rocprofv3
     counter passes of this script (--profile), FETCH_SIZE then WRITE_SIZE, each
@@ -277,7 +441,7 @@ def measure_traffic(vol):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 10; --batch: 3)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -287,6 +451,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes for roofline.traffic")
     ap.add_argument("--precision", action="store_true",
                     help="the precision configuration (2048x2048x512, K=4096): per-stage errors vs the FP64 oracle")
+    ap.add_argument("--batch", action="store_true",
+                    help="the batched uneven-instance configuration (256 warps in one call)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -296,6 +462,11 @@ def main():
     if args.precision:
         precision_run(args)
         return
+    if args.batch:
+        args.steps = args.steps or 3
+        batch_run(args)
+        return
+    args.steps = args.steps or 10
 
     import torch
     from terawarp_amd import synth, warp

@@ -14,9 +14,10 @@
 // bit-identical to it for the same matrix.
 //
 // Roofline: 2*bpv algorithmic bytes per voxel (one subject read, one
-// output write); the FP64 weight algebra (~45 VALU ops per voxel) makes the
-// uint8 case VALU-bound on MI355X, the uint16 case near the HBM/VALU
-// balance point.
+// output write).  The fast path (tw_affine_dev.h) issues four pair loads per
+// voxel, like the fused TPS kernel, and ~90 VALU instructions (FP64 weights
+// and blend in the reference's order), so it is bound by the texture-address
+// unit and the VALU, not by HBM bandwidth.
 #include "tw_internal.h"
 
 #include <cmath>
@@ -35,8 +36,28 @@ __global__ __launch_bounds__(256) void k_affine(const AffineArgs a) {
     affine_voxel<T, INTERP>(a, x, y, z, 0, a.nc);
 }
 
+template <typename T, int INTERP>
+__global__ __launch_bounds__(FT) void k_affine_fast(const AffineArgs a, const AffGeom g) {
+    // XCD-major tile order, as the fused TPS kernel.
+    const int wg = blockIdx.x;
+    const int tile = (wg & 7) * g.per_xcd + (wg >> 3);
+    if (tile >= g.ntiles) return;
+    affine_tile_fast<T, INTERP>(a, g, tile, 0, (int)a.nc);
+}
+
 template <typename T>
 int launch_affine(tw_ctx* ctx, const AffineArgs& a, int interp) {
+    AffGeom g;
+    if (a.nc <= 1024 && !affine_geom<T>(a, g)) {
+        const dim3 grid((unsigned)(8 * g.per_xcd)), block(TX, FTY, 1);
+        if (interp == TW_INTERP_NN)
+            hipLaunchKernelGGL((k_affine_fast<T, TW_INTERP_NN>), grid, block, 0, ctx->stream, a, g);
+        else
+            hipLaunchKernelGGL((k_affine_fast<T, TW_INTERP_LINEAR>), grid, block, 0, ctx->stream, a, g);
+        TW_LAUNCH_CHECK(ctx);
+        return TW_OK;
+    }
+    // Windows beyond 32-bit offsets (or one voxel wide): 64-bit addressing.
     const dim3 block(AFF_BX, AFF_BY, 1);
     const i64 zmax = 65535;
     for (i64 z0 = 0; z0 < a.dnz; z0 += zmax) {

@@ -13,7 +13,8 @@
 // single-volume entry points, so each item's output equals theirs.
 //
 // Persistent resample: the work is the items' output tiles (TPS: the fused
-// kernel's 64x4xtz tiles; affine: 64x4x16 voxel tiles), per channel.  Each
+// kernel's 64x4xtz tiles; affine: the fast affine path's 64x4x16 tiles), per
+// channel.  Each
 // item's tiles are split in eight contiguous parts, part x served by the
 // work-groups of XCD x (work-groups are dispatched round-robin over the 8
 // XCDs), which walk their parts item after item with a static stride; all
@@ -33,29 +34,19 @@
 namespace {
 
 enum { BK_TPS = 0, BK_AFFINE = 1 };
-constexpr int AFF_TZ = 16;  // affine tile depth (voxels along z)
 
 struct BatchJob {
     int kind, dtype;
     int ntile;              // tiles per channel
-    int tiles_x, tiles_y;   // affine tile grid
     int pad;
     i64 src_cstride, dst_cstride;  // bytes between channels
     TpsArgs t;
     FastGeom g;
     AffineArgs af;
+    AffGeom ag;
 };
 
 #define TW_CONST __attribute__((address_space(4)))
-
-template <typename T, int INTERP>
-__device__ __forceinline__ void affine_tile(const AffineArgs& a, int tiles_x, int tiles_y, int tile, i64 ch) {
-    const int tix = tile % tiles_x, tiy = (tile / tiles_x) % tiles_y, tiz = tile / (tiles_x * tiles_y);
-    const i64 x = (i64)tix * TX + threadIdx.x, y = (i64)tiy * FTY + threadIdx.y;
-    if (x >= a.dnx || y >= a.dny) return;
-    const i64 z0 = (i64)tiz * AFF_TZ, z1 = z0 + AFF_TZ < a.dnz ? z0 + AFF_TZ : a.dnz;
-    for (i64 z = z0; z < z1; z++) affine_voxel<T, INTERP>(a, x, y, z, ch, ch + 1);
-}
 
 // parts: for XCD x at parts + x * (2 * njobs + 1): P[0..njobs] (prefix sums of
 // the part sizes) then S[0..njobs-1] (first tile of each part).  t_end[j]:
@@ -91,8 +82,9 @@ __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restric
                 tps_tile<uint16_t, MODE, F16, INTERP, false, false>(a, g, t, src, dst, lds, rmax_w);
         } else {
             const AffineArgs& af = *(const AffineArgs*)&J.af;
-            if (J.dtype == TW_U8) affine_tile<uint8_t, INTERP>(af, J.tiles_x, J.tiles_y, t, ch);
-            else affine_tile<uint16_t, INTERP>(af, J.tiles_x, J.tiles_y, t, ch);
+            const AffGeom& ag = *(const AffGeom*)&J.ag;
+            if (J.dtype == TW_U8) affine_tile_fast<uint8_t, INTERP>(af, ag, t, ch, ch + 1);
+            else affine_tile_fast<uint16_t, INTERP>(af, ag, t, ch, ch + 1);
         }
     }
     __syncthreads();
@@ -280,9 +272,9 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
             a.dx0 = it.dst.x0; a.dy0 = it.dst.y0; a.dz0 = it.dst.z0;
             a.nc = it.src.nc;
             for (int k = 0; k < 12; k++) a.m[k] = it.M[k];
-            J.tiles_x = (int)((a.dnx + TX - 1) / TX);
-            J.tiles_y = (int)((a.dny + FTY - 1) / FTY);
-            per_ch = (i64)J.tiles_x * J.tiles_y * ((a.dnz + AFF_TZ - 1) / AFF_TZ);
+            const char* why = it.src.dtype == TW_U8 ? affine_geom<uint8_t>(a, J.ag) : affine_geom<uint16_t>(a, J.ag);
+            if (why) return tw_fail(ctx, TW_E_UNSUPPORTED, "item %d: %s", i, why);
+            per_ch = J.ag.ntiles;
         }
         if (per_ch * it.src.nc >= (1ll << 30)) return tw_fail(ctx, TW_E_UNSUPPORTED, "item %d: too many tiles", i);
         J.ntile = (int)per_ch;

@@ -69,7 +69,8 @@ __device__ __forceinline__ double sys_entry(const SysDesc& s, int r, int c) {
     if (a == 0) return 1.0;
     const float4 p = s.pts[i];
     const double v = a == 1 ? (double)p.x : (a == 2 ? (double)p.y : (double)p.z);
-    return (v - s.mu[a - 1]) * s.inv_s;
+    const double mu = a == 1 ? s.mu[0] : (a == 2 ? s.mu[1] : s.mu[2]);  // no indexed access to s
+    return (v - mu) * s.inv_s;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -78,7 +79,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Sum over a work-group of up to 1024 threads; every thread gets the total.
-__device__ double block_sum(double v, double* red) {
+__device__ __forceinline__ double block_sum(double v, double* red) {
     v = wave_sum(v);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     __syncthreads();
@@ -104,7 +105,7 @@ __device__ __forceinline__ double bcast(double v, int lane) {
 // Sums of NV values over a work-group of up to 1024 threads; red must hold 17 * NV
 // doubles.  Every thread gets the totals in v.
 template <int NV>
-__device__ void block_sum_n(double* v, double* red) {
+__device__ __forceinline__ void block_sum_n(double* v, double* red) {
     #pragma unroll
     for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -140,7 +141,7 @@ struct FitState {
 
 // Householder QR of X' (N x 4) and the compact-WY factor T.  One work-group.
 template <int NT>
-__device__ void hh_x_body(const SysDesc& s, double* __restrict__ V, FitState* st, double* red, double* sh) {
+__device__ __forceinline__ void hh_x_body(const SysDesc& s, double* __restrict__ V, FitState* st, double* red, double* sh) {
     const int N = s.N, tid = threadIdx.x;
     // X' into V's storage (column-major N x 4).
     for (int i = tid; i < N; i += NT) {
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void k_av(const SysDesc s, const double* __res
 
 // S = V'P, M = T' S T, W = P T - 1/2 V M.  One work-group.
 template <int NT>
-__device__ void build_w_body(const double* __restrict__ V, const double* __restrict__ P, double* __restrict__ W,
+__device__ __forceinline__ void build_w_body(const double* __restrict__ V, const double* __restrict__ P, double* __restrict__ W,
                              const FitState* st, int N, double* red, double* M, double* Ts) {
     const int tid = threadIdx.x;
     double S[16];
@@ -851,7 +852,7 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
 
 // U (N x 4, column-major) <- Q' U (trans = 1) or Q U (trans = 0), Q = I - V T V'.
 template <int NT>
-__device__ void q_apply_body(const double* __restrict__ V, const FitState* st, double* __restrict__ U, int N,
+__device__ __forceinline__ void q_apply_body(const double* __restrict__ V, const FitState* st, double* __restrict__ U, int N,
                              int trans, double* red, double* H) {
     const int tid = threadIdx.x;
     double G[16];
@@ -970,7 +971,10 @@ __global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__
             }
     }
     block_sum_n<32>(G, red);
-    if (tid < 16) { G1[tid] = G[tid]; G2[tid] = G[16 + tid]; }
+    if (tid < 16) {  // the totals, from LDS (G indexed by tid would live in scratch)
+        G1[tid] = red[16 * 32 + tid];
+        G2[tid] = red[16 * 32 + 16 + tid];
+    }
     __syncthreads();
     double rmax = 0, bmax = 0;
     for (int i = tid; i < n; i += 1024)
@@ -1143,9 +1147,12 @@ __global__ void k_unscale(const double* __restrict__ x, int N, double s, double 
 // work-group of 256 threads with work-group barriers between them, so a
 // single small system costs one launch instead of a kernel sequence and a
 // batch of systems takes one launch with a work-group per system.  The
-// tile kernels' bodies are shared with the large-system path; the
-// matrix-free products go one row per thread.
+// landmarks and every N x 4 vector stay in LDS; B and the inverted diagonal
+// blocks live in the system's workspace.  The tile kernels' bodies are
+// shared with the large-system path; the matrix-free products go one row per
+// thread with the column index uniform across the wave (LDS broadcasts).
 constexpr int FIT_SMALL_THREADS = 256;
+constexpr int FIT_SMALL_MAX = 256;  // largest K fitted this way (one row per thread)
 
 struct SmallFitJob {
     SysDesc s;
@@ -1156,14 +1163,14 @@ struct SmallFitJob {
     double scale;     // s (the solution is in scaled coordinates)
 };
 
-// Workspace of one small system: V, P, W, U, U2 (N x 4), x (m x 4), z (n x 4),
-// B (n x n), the inverted diagonal blocks.
+// Workspace of one small system: x (m x 4), B (n x n, lower triangle), the
+// inverted diagonal blocks.
 inline size_t small_ws_doubles(int N) {
     const size_t n = (size_t)N - 4, m = (size_t)N + 4, nb = (n + NB - 1) / NB;
-    return 20 * (size_t)N + 4 * m + 4 * n + n * n + nb * NB * NB;
+    return 4 * m + n * n + nb * NB * NB;
 }
 // Offset of x in that workspace.
-inline size_t small_ws_x(int N) { return 20 * (size_t)N; }
+inline size_t small_ws_x(int) { return 0; }
 
 union SmallFitLds {
     PotrfLds<double> potrf;
@@ -1176,33 +1183,50 @@ union SmallFitLds {
     } v;
 };
 
-// acc = row i of M' x, x being len x 4 column-major: len = N takes the A'
-// block of the bordered matrix M', len = m the whole of it.
-__device__ __forceinline__ void row_product(const SysDesc& s, const double* __restrict__ x, int len, int i,
-                                            double* acc) {
+// The system's vectors (column-major N x 4 with stride N; x: m x 4, stride m).
+struct SmallVecs {
+    float4 pts[FIT_SMALL_MAX];
+    double y[4 * FIT_SMALL_MAX];  // rows [1 target]
+    double V[4 * FIT_SMALL_MAX], P[4 * FIT_SMALL_MAX], W[4 * FIT_SMALL_MAX], U[4 * FIT_SMALL_MAX],
+        U2[4 * FIT_SMALL_MAX];
+    double x[4 * (FIT_SMALL_MAX + 4)];
+    double z[4 * FIT_SMALL_MAX];
+};
+
+// acc = row i < N of A' x (x: N x 4 column-major, stride N); the entries of
+// A' as sys_entry forms them, four columns per step so that their LDS reads
+// are in flight together.
+__device__ __forceinline__ void a_row(const SysDesc& s, const double* __restrict__ x, int N, int i, double* acc) {
+    const float4 pi = s.pts[i];
     acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
-    for (int k = 0; k < len; k++) {
-        const double a = sys_entry(s, i, k);
+    #pragma unroll 4
+    for (int k = 0; k < N; k++) {
+        const float4 pk = s.pts[k];
+        const double dx = (float)(pi.x - pk.x), dy = (float)(pi.y - pk.y), dz = (float)(pi.z - pk.z);
+        double a = -sqrt(dx * dx + dy * dy + dz * dz) * s.inv_s;
+        if (k == i) a += s.lam;
         #pragma unroll
-        for (int j = 0; j < 4; j++) acc[j] += a * x[(i64)j * len + k];
+        for (int j = 0; j < 4; j++) acc[j] += a * x[j * N + k];
     }
 }
 
-__device__ void fit_small_body(const SmallFitJob& J, SmallFitLds& L) {
+__device__ __forceinline__ void fit_small_body(const SmallFitJob& J, SmallFitLds& L, SmallVecs& S) {
     constexpr int NT = FIT_SMALL_THREADS;
-    const SysDesc& s = J.s;
-    const int N = s.N, n = N - 4, m = N + 4, tid = threadIdx.x;
-    const size_t NV = (size_t)N * 4;
-    double* V = J.ws;
-    double* P = V + NV;
-    double* W = P + NV;
-    double* U = W + NV;
-    double* U2 = U + NV;
-    double* x = U2 + NV;
-    double* z = x + (size_t)m * 4;
-    double* B = z + (size_t)n * 4;
+    static_assert(FIT_SMALL_MAX <= NT, "one row per thread");
+    const int N = J.s.N, n = N - 4, m = N + 4, tid = threadIdx.x;
+    double* xg = J.ws;
+    double* B = xg + (size_t)m * 4;
     double* Dinv = B + (size_t)n * n;
     FitState* st = J.st;
+    if (tid < N) {
+        S.pts[tid] = J.s.pts[tid];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) S.y[4 * tid + k] = J.s.y[4 * tid + k];
+    }
+    SysDesc s = J.s;
+    s.pts = S.pts;
+    s.y = S.y;
+    double *V = S.V, *P = S.P, *W = S.W, *U = S.U, *U2 = S.U2, *x = S.x, *z = S.z;
     if (tid == 0) {
         st->ir_stop = 0;
         st->ir_prev = INFINITY;
@@ -1213,32 +1237,43 @@ __device__ void fit_small_body(const SmallFitJob& J, SmallFitLds& L) {
     // 1. QR of X', P = A' V, W.
     hh_x_body<NT>(s, V, st, L.v.red, L.v.sh);
     __syncthreads();
-    for (int i = tid; i < N; i += NT) {
+    if (tid < N) {
         double acc[4];
-        row_product(s, V, N, i, acc);
+        a_row(s, V, N, tid, acc);
         #pragma unroll
-        for (int k = 0; k < 4; k++) P[(size_t)k * N + i] = acc[k];
+        for (int k = 0; k < 4; k++) P[k * N + tid] = acc[k];
     }
     __syncthreads();
     build_w_body<NT>(V, P, W, st, N, L.v.red, L.v.M, L.v.Ts);
-    // 2. U = Q'Y; B; z = U[4:].
-    for (int i = tid; i < N; i += NT)
+    // 2. U = Q'Y; B (lower triangle, row i - 4 in thread i - 4); z = U[4:].
+    if (tid < N)
         #pragma unroll
-        for (int k = 0; k < 4; k++) U[(size_t)k * N + i] = s.y[i * 4 + k];
+        for (int k = 0; k < 4; k++) U[k * N + tid] = s.y[tid * 4 + k];
     __syncthreads();
     q_apply_body<NT>(V, st, U, N, 1, L.v.red, L.v.H);
     __syncthreads();
-    for (i64 e = tid; e < (i64)n * n; e += NT) {
-        const int i = (int)(e % n) + 4, j = (int)(e / n) + 4;
-        double v = sys_entry(s, i, j);
+    if (tid < n) {
+        const int i = tid + 4;
+        double wi[4], vi[4];
         #pragma unroll
-        for (int k = 0; k < 4; k++)
-            v -= W[(size_t)k * N + i] * V[(size_t)k * N + j] + V[(size_t)k * N + i] * W[(size_t)k * N + j];
-        B[e] = v;
+        for (int k = 0; k < 4; k++) {
+            wi[k] = W[k * N + i];
+            vi[k] = V[k * N + i];
+        }
+        const float4 pi = s.pts[i];
+        #pragma unroll 4
+        for (int j = 4; j <= i; j++) {
+            const float4 pj = s.pts[j];
+            const double dx = (float)(pi.x - pj.x), dy = (float)(pi.y - pj.y), dz = (float)(pi.z - pj.z);
+            double v = -sqrt(dx * dx + dy * dy + dz * dz) * s.inv_s;
+            if (i == j) v += s.lam;
+            #pragma unroll
+            for (int k = 0; k < 4; k++) v -= wi[k] * V[k * N + j] + vi[k] * W[k * N + j];
+            B[(size_t)(j - 4) * n + tid] = v;
+        }
+        #pragma unroll
+        for (int k = 0; k < 4; k++) z[k * n + tid] = U[k * N + i];
     }
-    for (int i = tid; i < n; i += NT)
-        #pragma unroll
-        for (int k = 0; k < 4; k++) z[(size_t)k * n + i] = U[(size_t)k * N + 4 + i];
     __syncthreads();
     // 3. Blocked Cholesky of B, tile by tile.
     for (int j0 = 0; j0 < n; j0 += NB) {
@@ -1259,44 +1294,76 @@ __device__ void fit_small_body(const SmallFitJob& J, SmallFitLds& L) {
     chol_solve_body<double, NT>(B, Dinv, n, z, L.solve);
     __syncthreads();
     // 4. c' = Q [0; z]; U2 = Q'(Y - A'c'); d' = R^-1 U2[0:4].
-    for (int i = tid; i < N; i += NT)
+    if (tid < N)
         #pragma unroll
-        for (int k = 0; k < 4; k++) U[(size_t)k * N + i] = i < 4 ? 0.0 : z[(size_t)k * n + i - 4];
+        for (int k = 0; k < 4; k++) U[k * N + tid] = tid < 4 ? 0.0 : z[k * n + tid - 4];
     __syncthreads();
     q_apply_body<NT>(V, st, U, N, 0, L.v.red, L.v.H);
     __syncthreads();
-    for (int i = tid; i < N; i += NT) {
+    if (tid < N) {
         double acc[4];
-        row_product(s, U, N, i, acc);
+        a_row(s, U, N, tid, acc);
         #pragma unroll
-        for (int k = 0; k < 4; k++) U2[(size_t)k * N + i] = s.y[i * 4 + k] - acc[k];
+        for (int k = 0; k < 4; k++) U2[k * N + tid] = s.y[tid * 4 + k] - acc[k];
     }
     __syncthreads();
     q_apply_body<NT>(V, st, U2, N, 1, L.v.red, L.v.H);
     __syncthreads();
-    for (int i = tid; i < N; i += NT)
+    if (tid < N)
         #pragma unroll
-        for (int j = 0; j < 4; j++) x[(i64)j * m + i] = U[(i64)j * N + i];
+        for (int j = 0; j < 4; j++) x[j * m + tid] = U[j * N + tid];
     if (tid < 4) {
         const int j = tid;
         double dp[4];
+        #pragma unroll
         for (int a = 3; a >= 0; a--) {
-            double v = U2[(i64)j * N + a];
+            double v = U2[j * N + a];
+            #pragma unroll
             for (int q = a + 1; q < 4; q++) v -= st->R[a * 4 + q] * dp[q];
             dp[a] = v / st->R[a * 4 + a];
         }
-        for (int a = 0; a < 4; a++) x[(i64)j * m + N + a] = dp[a];
+        #pragma unroll
+        for (int a = 0; a < 4; a++) x[j * m + N + a] = dp[a];
     }
     __syncthreads();
-    // 5. Bordered residual max, unscaled coefficients.
-    double rm = 0.0;
-    for (int r = tid; r < m; r += NT) {
-        double acc[4];
-        row_product(s, x, m, r, acc);
+    // 5. Bordered residual max: rows < N one per thread, the four border rows
+    //    (X'^T c') by a work-group sum; x and the unscaled coefficients out.
+    double rm = 0.0, bs[16];
+    #pragma unroll
+    for (int e = 0; e < 16; e++) bs[e] = 0.0;
+    if (tid < N) {
+        // Row tid of the bordered product: A' c' (stride m) then X' d'.
+        const float4 pi = s.pts[tid];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        #pragma unroll 4
+        for (int k = 0; k < N; k++) {
+            const float4 pk = s.pts[k];
+            const double dx = (float)(pi.x - pk.x), dy = (float)(pi.y - pk.y), dz = (float)(pi.z - pk.z);
+            double a = -sqrt(dx * dx + dy * dy + dz * dz) * s.inv_s;
+            if (k == tid) a += s.lam;
+            #pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] += a * x[j * m + k];
+        }
         #pragma unroll
-        for (int j = 0; j < 4; j++) rm = fmax(rm, fabs((r < N ? s.y[r * 4 + j] : 0.0) - acc[j]));
+        for (int a = 0; a < 4; a++) {
+            const double xa = sys_entry(s, tid, N + a);
+            #pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] += xa * x[j * m + N + a];
+        }
+        #pragma unroll
+        for (int j = 0; j < 4; j++) rm = fmax(rm, fabs(s.y[tid * 4 + j] - acc[j]));
+        #pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const double xa = sys_entry(s, N + a, tid);
+            #pragma unroll
+            for (int j = 0; j < 4; j++) bs[a * 4 + j] = xa * x[j * m + tid];
+        }
     }
+    block_sum_n<16>(bs, L.v.red);
+    #pragma unroll
+    for (int e = 0; e < 16; e++) rm = fmax(rm, fabs(bs[e]));
     for (int o = 32; o > 0; o >>= 1) rm = fmax(rm, __shfl_xor(rm, o, 64));
+    __syncthreads();
     if ((tid & 63) == 0) L.v.red[tid >> 6] = rm;
     __syncthreads();
     if (tid == 0) {
@@ -1304,25 +1371,25 @@ __device__ void fit_small_body(const SmallFitJob& J, SmallFitLds& L) {
         for (int w = 0; w < NT / 64; w++) t = fmax(t, L.v.red[w]);
         st->fit_rmax = t;
     }
+    for (int i = tid; i < 4 * m; i += NT) xg[i] = x[i];
     if (J.cf) {
-        for (int i = tid; i < N; i += NT) unscale_coef(x, N, J.scale, J.cf, i);
+        if (tid < N) unscale_coef(x, N, J.scale, J.cf, tid);
         if (tid < 4) unscale_affine(x, N, J.scale, s.mu[0], s.mu[1], s.mu[2], J.d, tid);
     }
 }
 
 __global__ __launch_bounds__(FIT_SMALL_THREADS) void k_fit_small(const SmallFitJob job) {
     __shared__ SmallFitLds L;
-    fit_small_body(job, L);
+    __shared__ SmallVecs S;
+    fit_small_body(job, L, S);
 }
 
 __global__ __launch_bounds__(FIT_SMALL_THREADS) void k_fit_small_batch(const SmallFitJob* __restrict__ jobs) {
     __shared__ SmallFitLds L;
+    __shared__ SmallVecs S;
     const SmallFitJob job = jobs[blockIdx.x];
-    fit_small_body(job, L);
+    fit_small_body(job, L, S);
 }
-
-// Systems up to this many landmarks are fitted by one work-group.
-constexpr int FIT_SMALL_MAX = 256;
 
 // Host-side scaling of a system: centroid mu and half-extent s of the
 // centres; bmax = the largest |entry| of [1 targets] (the residual's scale).

@@ -1,10 +1,14 @@
-// tw_affine_dev.h -- per-voxel affine / rigid resample in the reference's
-// FP64 operation order, shared by k_affine.hip and the batch kernel
+// tw_affine_dev.h -- affine / rigid resample in the reference's FP64
+// operation order, shared by k_affine.hip and the batch kernel
 // (k_batch.hip).  Reference and semantics: see k_affine.hip's header.
 #pragma once
 #include "tw_internal.h"
 
+#include <algorithm>
 #include <cmath>
+#include <cstring>
+
+#include "tw_tps_dev.h"  // buffer-load helpers and the tile shape
 
 namespace {
 
@@ -78,6 +82,169 @@ __device__ __forceinline__ void affine_voxel(const AffineArgs& a, i64 x, i64 y, 
         const double hi = tw * (lw * v000 + rw * v100) + bw * (lw * v010 + rw * v110);
         const double lo = tw * (lw * v001 + rw * v101) + bw * (lw * v011 + rw * v111);
         dst[c * dvol + o] = (T)(uw * hi + dw * lo);
+    }
+}
+
+// ---- fast path ---------------------------------------------------------------
+// The same per-voxel arithmetic with the fused TPS kernel's addressing:
+// 32-bit byte offsets through four buffer resources (+0, +row, +plane,
+// +row+plane), so a voxel's eight taps are four (x, x+1) pair loads, and four
+// z-slices per step.  The reference reads (x_s, x_b) with x_b = ceil(p); where
+// x_b = x_s its weight r = 1 - l is exactly 0, so the pair's x_s + 1 (or the
+// next row / plane, likewise under a zero weight) leaves the result
+// unchanged; a sample on the window's last column takes the pair (x_s - 1,
+// x_s) instead, so that no pair leaves the channel (a range check would zero
+// the whole access).  Bit-identical to affine_voxel.
+
+struct AffGeom {
+    int snx, sny, snz, sx0, sy0, sz0;   // source window
+    uint32_t row_b, plane_b, chan_b;    // source strides / channel bytes
+    uint32_t nrec01, nrec10, nrec11;    // channel bytes past +row, +plane, +row+plane
+    uint32_t dchan_b;                   // destination channel bytes
+    int tiles_x, tiles_y, ntiles, per_xcd;
+    double fmx, fmy, fmz;               // full extent - 1 (bounds test)
+    int fnx, fny, fnz;                  // full extent
+};
+
+constexpr int AFF_TZ = 16;  // tile depth (voxels along z)
+
+// Geometry of the fast path, or null if the window exceeds its limits (the
+// caller then takes affine_voxel).
+template <typename T>
+const char* affine_geom(const AffineArgs& a, AffGeom& g) {
+    const uint64_t schan = (uint64_t)a.snx * a.sny * a.snz * sizeof(T);
+    const uint64_t taps = schan + (uint64_t)a.snx * (a.sny + 1) * sizeof(T) + 16;
+    const uint64_t dchan = (uint64_t)a.dnx * a.dny * a.dnz * sizeof(T);
+    const i64 cmax = std::max({a.dx0 + a.dnx, a.dy0 + a.dny, a.dz0 + a.dnz, a.fnx, a.fny, a.fnz});
+    if (a.snx < 2 || taps > 0x7FFF0000ull || a.sny * a.snz >= (1ll << 24) || dchan > 0xF0000000ull ||
+        cmax > (1ll << 24))
+        return "window exceeds the fast affine path's 32-bit addressing";
+    memset(&g, 0, sizeof g);
+    g.snx = (int)a.snx; g.sny = (int)a.sny; g.snz = (int)a.snz;
+    g.sx0 = (int)a.sx0; g.sy0 = (int)a.sy0; g.sz0 = (int)a.sz0;
+    g.row_b = (uint32_t)(a.snx * sizeof(T));
+    g.plane_b = (uint32_t)(a.snx * a.sny * sizeof(T));
+    g.chan_b = (uint32_t)schan;
+    auto past = [&](uint64_t skip) { return (uint32_t)(schan > skip ? schan - skip : 0); };
+    g.nrec01 = past(g.row_b);
+    g.nrec10 = past(g.plane_b);
+    g.nrec11 = past((uint64_t)g.row_b + g.plane_b);
+    g.dchan_b = (uint32_t)dchan;
+    g.tiles_x = (int)((a.dnx + TX - 1) / TX);
+    g.tiles_y = (int)((a.dny + FTY - 1) / FTY);
+    const i64 nt = (i64)g.tiles_x * g.tiles_y * ((a.dnz + AFF_TZ - 1) / AFF_TZ);
+    if (nt > (1ll << 30)) return "too many tiles for one launch";
+    g.ntiles = (int)nt;
+    g.per_xcd = (int)((nt + 7) / 8);
+    g.fmx = (double)(a.fnx - 1); g.fmy = (double)(a.fny - 1); g.fmz = (double)(a.fnz - 1);
+    g.fnx = (int)a.fnx; g.fny = (int)a.fny; g.fnz = (int)a.fnz;
+    return nullptr;
+}
+
+struct ATap {
+    uint32_t off;   // byte offset of (x_s, y_s, z_s) (or of x_s - 1: shift), OOB_OFF when outside
+    bool ok;        // inside the volume and the window
+    bool shift;     // last column: the pair is (x_s - 1, x_s)
+    double lw, rw, tw, bw, uw, dw;
+};
+
+template <typename T, int INTERP>
+__device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g, double gx, double gy, double gz) {
+    ATap t;
+    const double p0 = a.m[0] * gx + a.m[1] * gy + a.m[2] * gz + a.m[3] * 1.0;
+    const double p1 = a.m[4] * gx + a.m[5] * gy + a.m[6] * gz + a.m[7] * 1.0;
+    const double p2 = a.m[8] * gx + a.m[9] * gy + a.m[10] * gz + a.m[11] * 1.0;
+    const bool in = !(p0 < 0 || p0 > g.fmx || p1 < 0 || p1 > g.fmy || p2 < 0 || p2 > g.fmz);
+    // Outside: evaluate at 0 (keeps the conversions in range); the result is 0.
+    const double q0 = in ? p0 : 0.0, q1 = in ? p1 : 0.0, q2 = in ? p2 : 0.0;
+    t.shift = false;
+    if (INTERP == TW_INTERP_NN) {
+        int qx = (int)(q0 + 0.5), qy = (int)(q1 + 0.5), qz = (int)(q2 + 0.5);
+        qx = (qx < g.fnx ? qx : g.fnx - 1) - g.sx0;
+        qy = (qy < g.fny ? qy : g.fny - 1) - g.sy0;
+        qz = (qz < g.fnz ? qz : g.fnz - 1) - g.sz0;
+        const bool w = qx >= 0 && qx < g.snx && qy >= 0 && qy < g.sny && qz >= 0 && qz < g.snz;
+        t.ok = in && w;
+        t.off = t.ok ? (__umul24((uint32_t)(qz * g.sny + qy), (uint32_t)g.snx) + (uint32_t)qx) * (uint32_t)sizeof(T)
+                     : OOB_OFF;
+        t.lw = t.rw = t.tw = t.bw = t.uw = t.dw = 0.0;
+        return t;
+    }
+    const int xs = (int)floor(q0), xb = (int)ceil(q0);
+    const int ys = (int)floor(q1), yb = (int)ceil(q1);
+    const int zs = (int)floor(q2), zb = (int)ceil(q2);
+    t.lw = 1.0 - (q0 - (double)xs);
+    t.rw = 1.0 - t.lw;
+    t.tw = 1.0 - (q1 - (double)ys);
+    t.bw = 1.0 - t.tw;
+    t.uw = 1.0 - (q2 - (double)zs);
+    t.dw = 1.0 - t.uw;
+    const int lxs = xs - g.sx0, lxb = xb - g.sx0, lys = ys - g.sy0, lyb = yb - g.sy0;
+    const int lzs = zs - g.sz0, lzb = zb - g.sz0;
+    const bool w = !(lxs < 0 || lxb >= g.snx || lys < 0 || lyb >= g.sny || lzs < 0 || lzb >= g.snz);
+    t.ok = in && w;
+    t.shift = lxs == g.snx - 1;
+    const uint32_t e = __umul24((uint32_t)(lzs * g.sny + lys), (uint32_t)g.snx) + (uint32_t)(lxs - (t.shift ? 1 : 0));
+    t.off = t.ok ? e * (uint32_t)sizeof(T) : OOB_OFF;
+    return t;
+}
+
+// The reference's blend of a tap (q_warp_affine_tps.cpp:1418-1433), truncated.
+template <typename T, int INTERP>
+__device__ __forceinline__ uint32_t affine_blend(const SrcRs& rs, const ATap& t) {
+    const uint32_t off = t.off;
+    __builtin_assume(off <= OOB_OFF);
+    if (INTERP == TW_INTERP_NN) return t.ok ? bld<T>(rs.r00, off) : 0u;
+    const uint32_t p00 = bld2<T>(rs.r00, off), p10 = bld2<T>(rs.r01, off);
+    const uint32_t p01 = bld2<T>(rs.r10, off), p11 = bld2<T>(rs.r11, off);
+    const double v000 = (double)(t.shift ? pair_hi<T>(p00) : pair_lo<T>(p00)), v100 = (double)pair_hi<T>(p00);
+    const double v010 = (double)(t.shift ? pair_hi<T>(p10) : pair_lo<T>(p10)), v110 = (double)pair_hi<T>(p10);
+    const double v001 = (double)(t.shift ? pair_hi<T>(p01) : pair_lo<T>(p01)), v101 = (double)pair_hi<T>(p01);
+    const double v011 = (double)(t.shift ? pair_hi<T>(p11) : pair_lo<T>(p11)), v111 = (double)pair_hi<T>(p11);
+    const double hi = t.tw * (t.lw * v000 + t.rw * v100) + t.bw * (t.lw * v010 + t.rw * v110);
+    const double lo = t.tw * (t.lw * v001 + t.rw * v101) + t.bw * (t.lw * v011 + t.rw * v111);
+    return t.ok ? (uint32_t)(T)(t.uw * hi + t.dw * lo) : 0u;
+}
+
+__device__ __forceinline__ SrcRs affine_src_rs(const void* src, const AffGeom& g) {
+    const char* p = (const char*)src;
+    SrcRs r;
+    r.r00 = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)g.chan_b, BUF_FLAGS);
+    r.r01 = __builtin_amdgcn_make_buffer_rsrc((void*)(p + g.row_b), (short)0, (int)g.nrec01, BUF_FLAGS);
+    r.r10 = __builtin_amdgcn_make_buffer_rsrc((void*)(p + g.plane_b), (short)0, (int)g.nrec10, BUF_FLAGS);
+    r.r11 = __builtin_amdgcn_make_buffer_rsrc((void*)(p + g.row_b + g.plane_b), (short)0, (int)g.nrec11, BUF_FLAGS);
+    return r;
+}
+
+// One 64 x 4 x AFF_TZ tile of the output (x-fastest tile index), channels
+// [c0, c1): each thread walks its (x, y) column four slices per step.
+template <typename T, int INTERP>
+__device__ __forceinline__ void affine_tile_fast(const AffineArgs& a, const AffGeom& g, int tile, int c0, int c1) {
+    const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
+    const int x = tix * TX + (int)threadIdx.x, y = tiy * FTY + (int)threadIdx.y;
+    const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
+    if (x >= dnx || y >= dny) return;
+    const int z0 = tiz * AFF_TZ, z1 = min(z0 + AFF_TZ, dnz);
+    const double gx = (double)(a.dx0 + x), gy = (double)(a.dy0 + y);
+    const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
+    for (int z = z0; z < z1; z += 4) {
+        const int nz = min(4, z1 - z);
+        ATap t[4];
+        #pragma unroll
+        for (int j = 0; j < 4; j++) t[j] = affine_tap<T, INTERP>(a, g, gx, gy, (double)(a.dz0 + z + min(j, nz - 1)));
+        const uint32_t o = (uint32_t)z * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x;
+        for (int c = c0; c < c1; c++) {
+            const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
+            const __amdgpu_buffer_rsrc_t rd =
+                __builtin_amdgcn_make_buffer_rsrc((uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b,
+                                                  BUF_FLAGS);
+            uint32_t v[4];
+            #pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = affine_blend<T, INTERP>(rs, t[j]);
+            #pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < nz) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
+        }
     }
 }
 

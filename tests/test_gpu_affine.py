@@ -91,3 +91,14 @@ def test_affine_plain_gpu_config_size(warper, oracle, dtype):
 
 def tdt_np(dtype):
     return np.uint8 if dtype == np.uint8 else np.int16
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("interp", [warp.LINEAR, warp.NN])
+def test_affine_identity_reproduces_every_voxel(warper, rng, dtype, interp):
+    """Identity matrix: every sample sits exactly on a voxel, the last one on
+    the channel's final voxel (x_b = x_s everywhere, pairs at the last column
+    shifted): the output is the input, channel by channel."""
+    v = _vol(rng, (2, 13, 7, 33), dtype)
+    I = np.eye(4)
+    assert np.array_equal(warper.resample_affine(v, I, interp=interp), v)
