@@ -234,6 +234,97 @@ typedef struct {
 int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int f, double lambda, int interp,
                   int precision, int32_t* status, double* done_ms);
 
+/* ---- streaming --------------------------------------------------------- */
+/* A volume larger than HBM (or than host memory), warped brick by brick.
+ * The reference's tera-block drivers (q_imgwarp_tps_quicksmallmemory.cpp:
+ * 1161-1994, q_warp_affine_tps.cpp:851-1329) read each block's window
+ * (probing 14 points of the field to guess its extent), warp it and write
+ * block files, one block at a time.  Here one fit, one grid over the whole
+ * volume (the bricks' grids are its sub-grids: brick origins lie on the node
+ * lattice), exact per-brick source windows from the field's bounds over each
+ * brick's nodes, and a pipeline of pinned host buffers: host threads fill
+ * brick windows through `read` while earlier bricks are copied to the device,
+ * resampled and copied back, and a writer thread hands finished bricks to
+ * `write`.  The output equals tw_warp_tps on the whole volume (same grid,
+ * same per-voxel arithmetic). */
+
+/* Fill the box [x0, x0+nx) x [y0, y0+ny) x [z0, z0+nz) of the subject volume
+ * into out ([z][y][x], x fastest).  Called concurrently from several threads
+ * for disjoint z-ranges of one window; returns TW_OK or an error code. */
+typedef int (*tw_read_fn)(void* user, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny, int64_t nz,
+                          void* out);
+/* Receive output brick `brick` (box origin and size, data [z][y][x]); called
+ * from one writer thread, in brick order; data is valid during the call. */
+typedef int (*tw_write_fn)(void* user, int64_t brick, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny,
+                           int64_t nz, const void* data);
+
+typedef struct {
+    int64_t      nx, ny, nz;      /* whole volume (one channel) */
+    int32_t      dtype;           /* TW_U8 / TW_U16 */
+    int32_t      brick;           /* output brick edge; a multiple of f (edge bricks are smaller) */
+    int32_t      f;               /* grid spacing */
+    int32_t      interp;
+    int32_t      precision;       /* TW_EXACT, TW_PLAIN or TW_MIXED */
+    int32_t      threads;         /* host threads filling windows */
+    int32_t      K;
+    int32_t      pad;
+    double       lambda;
+    const float* centres;         /* K x 3, host */
+    const float* targets;         /* K x 3, host */
+    tw_read_fn   read;
+    void*        read_user;
+    tw_write_fn  write;
+    void*        write_user;
+} tw_stream_desc;
+
+typedef struct {
+    int64_t bricks;
+    double  wall_s;               /* whole call */
+    double  setup_ms;             /* fit + grid + bounds */
+    double  resample_ms;          /* sum of the bricks' resample kernels (device events) */
+    double  read_s, write_s;      /* time spent in the callbacks (summed over bricks) */
+    double  h2d_bytes, d2h_bytes; /* bytes copied */
+    double  max_margin;           /* largest source margin used (voxels past the brick) */
+} tw_stream_stats;
+
+int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* desc, tw_stream_stats* stats);
+
+/* Procedural subject volume for streaming runs: uniform noise hashed from
+ * (seed, x, y, z) plus Gaussian blobs drawn per cell of `cell` voxels; any
+ * window holds the same values as the whole volume would. */
+typedef struct {
+    uint64_t seed;
+    int64_t  nx, ny, nz;
+    int32_t  dtype;
+    int32_t  cell;
+    int32_t  blobs_per_cell;
+    int32_t  pad;
+    float    sigma_lo, sigma_hi, amp_lo, amp_hi, noise;
+} tw_synth;
+
+int tw_synth_fill(const tw_synth* s, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny, int64_t nz,
+                  void* out);
+/* tw_read_fn over a tw_synth (user = const tw_synth*). */
+int tw_synth_read(void* user, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny, int64_t nz, void* out);
+
+/* Position-weighted 64-bit checksum of a buffer: the sum of its 64-bit
+ * little-endian words w_i (last one zero-padded) times (2i + 1), modulo 2^64,
+ * xor the byte count. */
+uint64_t tw_checksum64(const void* data, int64_t bytes);
+
+/* tw_write_fn that records each brick's tw_checksum64 in sums[brick] and
+ * copies the bricks listed in keep_ids to keep_ptrs (user = tw_checksum_sink*). */
+typedef struct {
+    uint64_t*       sums;
+    int64_t         nbricks;
+    int32_t         bytes_per_voxel;
+    int32_t         nkeep;
+    const int64_t*  keep_ids;
+    void* const*    keep_ptrs;
+} tw_checksum_sink;
+int tw_checksum_write(void* user, int64_t brick, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny,
+                      int64_t nz, const void* data);
+
 /* Grid node count per axis for a volume extent n and spacing f:
  * ceil(n / f) + 3 (q_imgwarp_tps_quicksmallmemory.cpp:592). */
 int64_t tw_grid_dim(int64_t n, int f);

@@ -42,11 +42,13 @@ def lib():
         L.or_bspline_basis.argtypes = [ctypes.c_int, _P, _P]
         L.or_tps_resample.argtypes = [_P, _P, ctypes.c_int, _P, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
                                       _P, _i64, _i64, ctypes.c_int]
+        L.or_tps_resample_window.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P, _i64, _i64, _i64, _P, ctypes.c_int,
+                                             ctypes.c_int, _P, _P, _P, ctypes.c_int]
         L.or_dense_df.argtypes = [_P, _P, _i64, _i64, _i64, ctypes.c_int, _P, ctypes.c_int]
         L.or_affine_fit.argtypes = [_P, _P, ctypes.c_int, _P]
         L.or_affine_resample.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _i64, _i64, ctypes.c_int]
         for f in ("or_stps_fit", "or_stps_fit_bordered", "or_grid_eval", "or_bspline_basis", "or_tps_resample",
-                  "or_dense_df", "or_affine_fit", "or_affine_resample", "or_max_threads"):
+                  "or_tps_resample_window", "or_dense_df", "or_affine_fit", "or_affine_resample", "or_max_threads"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -129,6 +131,23 @@ def tps_resample(src, df, n, interp=0, zblocks=(0, -1), dst=None, nthreads=0):
     _check(lib().or_tps_resample(_ptr(v), _ptr(sz), dt, _ptr(df), df.shape[3], df.shape[2], df.shape[1], n, interp,
                                  _ptr(out), zblocks[0], zblocks[1], nthreads), "tps_resample")
     return out if src.ndim == 4 else out[0]
+
+
+def tps_resample_window(src_win, src_origin, full_xyz, df, n, grid_origin, out_size_xyz, out_origin, interp=0,
+                        nthreads=0):
+    """Warp the output window [out_origin, out_origin + out_size) of a volume of
+    extent full_xyz from the source window src_win (z, y, x) at src_origin,
+    with a grid whose node (i, j, k) sits at (i-1)n + grid_origin."""
+    v = np.ascontiguousarray(src_win)
+    dt = 1 if v.dtype == np.uint8 else 2
+    df = np.ascontiguousarray(df, np.float32)
+    ssz, s0, fsz, g0, dsz, d0 = (np.array(list(a), np.int64) for a in (
+        (v.shape[2], v.shape[1], v.shape[0]), src_origin, full_xyz, grid_origin, out_size_xyz, out_origin))
+    out = np.zeros((out_size_xyz[2], out_size_xyz[1], out_size_xyz[0]), v.dtype)
+    _check(lib().or_tps_resample_window(_ptr(v), _ptr(ssz), _ptr(s0), _ptr(fsz), dt, _ptr(df), df.shape[3],
+                                        df.shape[2], df.shape[1], _ptr(g0), n, interp, _ptr(out), _ptr(dsz),
+                                        _ptr(d0), nthreads), "tps_resample_window")
+    return out
 
 
 def dense_df(size_xyz, df, n, nthreads=0):

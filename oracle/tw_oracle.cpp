@@ -21,6 +21,9 @@
 //                      ordering against a hand-built 4-node example.
 //   or_tps_resample    linear precision of the upsample, integer
 //                      positions, out-of-bounds strictness, brute force.
+//   or_tps_resample_window  equal to or_tps_resample on the whole volume and
+//                      on sub-windows whose source covers every tap; zero
+//                      where the taps leave the window.
 //   or_affine_fit      exact recovery of a known matrix, order invariance.
 //   or_affine_resample brute force on tiny volumes, identity/shift cases.
 // newmat's own rounding sequences (QRZ, .i(), SVD) cannot be reproduced
@@ -461,6 +464,90 @@ int tps_resample(const T* src, const i64* sz, const float* df, i64 gx, i64 gy, i
     return 0;
 }
 
+// Tera-block form of the warp (q_imgblockwarp_CYF,
+// q_imgwarp_tps_quicksmallmemory.cpp:2208-2304): the output window
+// [d0, d0 + dsz) of a volume of full extent fsz, read from the source window
+// [s0, s0 + ssz) of that volume, with a grid whose node (i, j, k) sits at
+// ((i-1)n + g0x, (j-1)n + g0y, (k-1)n + g0z).  A voxel's sample position is
+// the float sum of its global coordinate and the upsampled displacement (the
+// in-memory path's arithmetic, 106-108); a sample outside the full extent
+// gives 0 (2235-2242), and so does one whose taps leave the source window
+// (2243-2251).  Blocks are those of the grid (origin g0), visited only where
+// they hold output voxels.
+template <class T>
+int tps_resample_window(const T* src, const i64* ssz, const i64* s0, const i64* fsz, const float* df, i64 gx,
+                        i64 gy, i64 gz, const i64* g0, int n, int interp, T* dst, const i64* dsz, const i64* d0) {
+    std::vector<double> TB((size_t)n * 4), basis((size_t)n * n * n * 64);
+    or_bspline_basis(n, TB.data(), basis.data());
+    for (int a = 0; a < 3; a++)
+        if (d0[a] < g0[a] || dsz[a] < 1 || ssz[a] < 1) return -1;
+    // Block range (grid-relative) holding output voxels.
+    const i64 bx0 = (d0[0] - g0[0]) / n, bx1 = (d0[0] + dsz[0] - 1 - g0[0]) / n;
+    const i64 by0 = (d0[1] - g0[1]) / n, by1 = (d0[1] + dsz[1] - 1 - g0[1]) / n;
+    const i64 bz0 = (d0[2] - g0[2]) / n, bz1 = (d0[2] + dsz[2] - 1 - g0[2]) / n;
+    if (bx1 + 3 >= gx || by1 + 3 >= gy || bz1 + 3 >= gz) return -1;
+    const i64 ssx = ssz[0], ssxy = ssz[0] * ssz[1];
+    const i64 dsx = dsz[0], dsxy = dsz[0] * dsz[1];
+#pragma omp parallel for schedule(dynamic)
+    for (i64 bz = bz0; bz <= bz1; bz++) {
+        std::vector<float> blk((size_t)n * n * n * 3);
+        for (i64 by = by0; by <= by1; by++)
+            for (i64 bx = bx0; bx <= bx1; bx++) {
+                df_block(df, gx, gy, gz, basis.data(), n, bx, by, bz, blk.data());
+                for (int z = 0; z < n; z++)
+                    for (int y = 0; y < n; y++)
+                        for (int x = 0; x < n; x++) {
+                            const i64 X = g0[0] + bx * n + x, Y = g0[1] + by * n + y, Z = g0[2] + bz * n + z;
+                            const i64 lx = X - d0[0], ly = Y - d0[1], lz = Z - d0[2];
+                            if (lx < 0 || ly < 0 || lz < 0 || lx >= dsz[0] || ly >= dsz[1] || lz >= dsz[2]) continue;
+                            const float* dv = blk.data() + (((i64)z * n + y) * n + x) * 3;
+                            double p[3];
+                            p[0] = (float)X + dv[0];
+                            p[1] = (float)Y + dv[1];
+                            p[2] = (float)Z + dv[2];
+                            T& out = dst[lz * dsxy + ly * dsx + lx];
+                            if (p[0] < 0 || p[0] > fsz[0] - 1 || p[1] < 0 || p[1] > fsz[1] - 1 || p[2] < 0 ||
+                                p[2] > fsz[2] - 1) {
+                                out = 0;
+                                continue;
+                            }
+                            if (interp == 1) {
+                                i64 q[3];
+                                bool in = true;
+                                for (int a = 0; a < 3; a++) {
+                                    q[a] = (i64)(p[a] + 0.5);
+                                    q[a] = (q[a] < fsz[a] ? q[a] : fsz[a] - 1) - s0[a];
+                                    in = in && q[a] >= 0 && q[a] < ssz[a];
+                                }
+                                out = in ? src[q[2] * ssxy + q[1] * ssx + q[0]] : (T)0;
+                                continue;
+                            }
+                            const i64 xs = (i64)std::floor(p[0]), xb = (i64)std::ceil(p[0]);
+                            const i64 ys = (i64)std::floor(p[1]), yb = (i64)std::ceil(p[1]);
+                            const i64 zs = (i64)std::floor(p[2]), zb = (i64)std::ceil(p[2]);
+                            const double lw = 1.0 - (p[0] - xs), rw = 1.0 - lw;
+                            const double tw = 1.0 - (p[1] - ys), bw = 1.0 - tw;
+                            const double uw = 1.0 - (p[2] - zs), dw = 1.0 - uw;
+                            const i64 lxs = xs - s0[0], lxb = xb - s0[0], lys = ys - s0[1], lyb = yb - s0[1];
+                            const i64 lzs = zs - s0[2], lzb = zb - s0[2];
+                            if (lxs < 0 || lxb >= ssz[0] || lys < 0 || lyb >= ssz[1] || lzs < 0 || lzb >= ssz[2]) {
+                                out = 0;
+                                continue;
+                            }
+                            const double hi =
+                                tw * (lw * px(src, ssx, ssxy, lxs, lys, lzs) + rw * px(src, ssx, ssxy, lxb, lys, lzs)) +
+                                bw * (lw * px(src, ssx, ssxy, lxs, lyb, lzs) + rw * px(src, ssx, ssxy, lxb, lyb, lzs));
+                            const double lo =
+                                tw * (lw * px(src, ssx, ssxy, lxs, lys, lzb) + rw * px(src, ssx, ssxy, lxb, lys, lzb)) +
+                                bw * (lw * px(src, ssx, ssxy, lxs, lyb, lzb) + rw * px(src, ssx, ssxy, lxb, lyb, lzb));
+                            const double iv = uw * hi + dw * lo + 0.5;
+                            out = (T)iv;
+                        }
+            }
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -481,6 +568,22 @@ int or_tps_resample(const void* src, const i64* sz, int dtype, const float* df,
         return tps_resample((const uint8_t*)src, sz, df, gx, gy, gz, n, interp, (uint8_t*)dst, zb0, zb1);
     if (dtype == 2)
         return tps_resample((const uint16_t*)src, sz, df, gx, gy, gz, n, interp, (uint16_t*)dst, zb0, zb1);
+    return -1;
+}
+
+// Windowed form (tps_resample_window above).  dtype 1 = uint8, 2 = uint16;
+// ssz, s0, fsz, g0, dsz, d0: xyz triples.
+int or_tps_resample_window(const void* src, const i64* ssz, const i64* s0, const i64* fsz, int dtype,
+                           const float* df, i64 gx, i64 gy, i64 gz, const i64* g0, int n, int interp, void* dst,
+                           const i64* dsz, const i64* d0, int nthreads) {
+    if (!src || !ssz || !s0 || !fsz || !df || !g0 || !dst || !dsz || !d0 || n <= 0) return -1;
+    set_threads(nthreads);
+    if (dtype == 1)
+        return tps_resample_window((const uint8_t*)src, ssz, s0, fsz, df, gx, gy, gz, g0, n, interp, (uint8_t*)dst,
+                                   dsz, d0);
+    if (dtype == 2)
+        return tps_resample_window((const uint16_t*)src, ssz, s0, fsz, df, gx, gy, gz, g0, n, interp,
+                                   (uint16_t*)dst, dsz, d0);
     return -1;
 }
 

@@ -6,10 +6,10 @@ bindings, the seeded synthetic workloads and the build script.
 """
 from .warp import (  # noqa: F401
     EXACT, F16, F32, LINEAR, MIXED, NN, PLAIN, ROTATION, SCALING, TRANSLATION,
-    Grid, TeraWarpError, Warper, affine_fit, grid_dim, mode_matrix,
+    Grid, Synth, TeraWarpError, Warper, affine_fit, checksum64, grid_dim, mode_matrix,
 )
 
 __all__ = [
-    "Warper", "Grid", "TeraWarpError", "affine_fit", "mode_matrix", "grid_dim",
+    "Warper", "Grid", "Synth", "TeraWarpError", "affine_fit", "mode_matrix", "grid_dim", "checksum64",
     "PLAIN", "MIXED", "EXACT", "LINEAR", "NN", "F32", "F16", "ROTATION", "TRANSLATION", "SCALING",
 ]

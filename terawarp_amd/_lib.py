@@ -36,7 +36,8 @@ TW_WARP_AFFINE = 1
 EXPORTS = (
     "tw_create", "tw_destroy", "tw_set_stream", "tw_sync", "tw_set_timing", "tw_get_timings", "tw_last_error",
     "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
-    "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_warp_batch", "tw_grid_dim",
+    "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_warp_batch", "tw_warp_stream",
+    "tw_synth_fill", "tw_synth_read", "tw_checksum64", "tw_checksum_write", "tw_grid_dim",
 )
 # Every entry point include/terawarp_host.h declares.
 HOST_EXPORTS = (
@@ -74,6 +75,36 @@ class TwWarpItem(ctypes.Structure):
     _fields_ = [("src", TwVolume), ("dst", TwVolume), ("kind", _i32), ("K", _i32),
                 ("centres", ctypes.POINTER(ctypes.c_float)), ("targets", ctypes.POINTER(ctypes.c_float)),
                 ("M", ctypes.c_double * 12)]
+
+
+READ_FN = ctypes.CFUNCTYPE(ctypes.c_int, _P, _i64, _i64, _i64, _i64, _i64, _i64, _P)
+WRITE_FN = ctypes.CFUNCTYPE(ctypes.c_int, _P, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _P)
+
+
+class TwStreamDesc(ctypes.Structure):
+    _fields_ = [("nx", _i64), ("ny", _i64), ("nz", _i64), ("dtype", _i32), ("brick", _i32), ("f", _i32),
+                ("interp", _i32), ("precision", _i32), ("threads", _i32), ("K", _i32), ("pad", _i32),
+                ("lam", ctypes.c_double), ("centres", ctypes.POINTER(ctypes.c_float)),
+                ("targets", ctypes.POINTER(ctypes.c_float)), ("read", _P), ("read_user", _P),
+                ("write", _P), ("write_user", _P)]
+
+
+class TwStreamStats(ctypes.Structure):
+    _fields_ = [("bricks", _i64), ("wall_s", ctypes.c_double), ("setup_ms", ctypes.c_double),
+                ("resample_ms", ctypes.c_double), ("read_s", ctypes.c_double), ("write_s", ctypes.c_double),
+                ("h2d_bytes", ctypes.c_double), ("d2h_bytes", ctypes.c_double), ("max_margin", ctypes.c_double)]
+
+
+class TwSynth(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("nx", _i64), ("ny", _i64), ("nz", _i64), ("dtype", _i32),
+                ("cell", _i32), ("blobs_per_cell", _i32), ("pad", _i32), ("sigma_lo", ctypes.c_float),
+                ("sigma_hi", ctypes.c_float), ("amp_lo", ctypes.c_float), ("amp_hi", ctypes.c_float),
+                ("noise", ctypes.c_float)]
+
+
+class TwChecksumSink(ctypes.Structure):
+    _fields_ = [("sums", ctypes.POINTER(ctypes.c_uint64)), ("nbricks", _i64), ("bytes_per_voxel", _i32),
+                ("nkeep", _i32), ("keep_ids", ctypes.POINTER(_i64)), ("keep_ptrs", ctypes.POINTER(_P))]
 
 
 class TwWarpParams(ctypes.Structure):
@@ -118,6 +149,11 @@ def load():
                                        ctypes.c_int]),
         "tw_warp_batch": (ctypes.c_int, [_P, ctypes.POINTER(TwWarpItem), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(_i32), dp]),
+        "tw_warp_stream": (ctypes.c_int, [_P, ctypes.POINTER(TwStreamDesc), ctypes.POINTER(TwStreamStats)]),
+        "tw_synth_fill": (ctypes.c_int, [ctypes.POINTER(TwSynth), _i64, _i64, _i64, _i64, _i64, _i64, _P]),
+        "tw_synth_read": (ctypes.c_int, [_P, _i64, _i64, _i64, _i64, _i64, _i64, _P]),
+        "tw_checksum64": (ctypes.c_uint64, [_P, _i64]),
+        "tw_checksum_write": (ctypes.c_int, [_P, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _P]),
         "tw_grid_dim": (_i64, [_i64, ctypes.c_int]),
         "tw_v3draw_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),

@@ -321,3 +321,83 @@ class Warper:
         self._check(rc)
         return [int(x) for x in status], done
 
+    # ---- streaming ----------------------------------------------------------
+    def warp_stream(self, size_xyz, dtype, centres, targets, source, f=8, brick=512, lam=0.2, interp=LINEAR,
+                    precision=MIXED, threads=16, keep=()):
+        """Warp a volume brick by brick (tw_warp_stream).
+
+        source: a Synth (procedural volume, filled by the library's own
+        threads) or a Python callable read(x0, y0, z0, nx, ny, nz) returning
+        a (nz, ny, nx) array.  Returns (checksums, kept, stats): per-brick
+        tw_checksum64 values, {brick id: (z, y, x) output array} for the
+        bricks listed in keep, and the call's statistics.
+        """
+        ce, ta = _pts(centres), _pts(targets)
+        nx, ny, nz = size_xyz
+        dt = L.TW_U8 if np.dtype(dtype) == np.uint8 else L.TW_U16
+        nb = -(-nx // brick) * -(-ny // brick) * -(-nz // brick)
+        sums = np.zeros(nb, np.uint64)
+        keep = [int(k) for k in keep]
+        kept = {}
+        ids = (ctypes.c_int64 * max(1, len(keep)))(*keep)
+        ptrs = (ctypes.c_void_p * max(1, len(keep)))()
+        for i, b in enumerate(keep):
+            bx, by, bz = b % -(-nx // brick), (b // -(-nx // brick)) % -(-ny // brick), \
+                b // (-(-nx // brick) * -(-ny // brick))
+            shape = (min(brick, nz - bz * brick), min(brick, ny - by * brick), min(brick, nx - bx * brick))
+            kept[b] = np.zeros(shape, dtype)
+            ptrs[i] = kept[b].ctypes.data
+        sink = L.TwChecksumSink(sums.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), nb,
+                                np.dtype(dtype).itemsize, len(keep), ids, ptrs)
+        d = L.TwStreamDesc()
+        d.nx, d.ny, d.nz, d.dtype, d.brick, d.f = nx, ny, nz, dt, brick, f
+        d.interp, d.precision, d.threads, d.K, d.lam = interp, precision, threads, ce.shape[0], lam
+        d.centres, d.targets = _fptr(ce), _fptr(ta)
+        if isinstance(source, Synth):
+            d.read = ctypes.cast(self.lib.tw_synth_read, ctypes.c_void_p)
+            d.read_user = ctypes.cast(ctypes.byref(source.s), ctypes.c_void_p)
+            cb = None
+        else:
+            def _read(user, x0, y0, z0, bnx, bny, bnz, out):
+                try:
+                    a = np.ascontiguousarray(source(x0, y0, z0, bnx, bny, bnz), dtype)
+                    ctypes.memmove(out, a.ctypes.data, a.nbytes)
+                    return L.TW_OK
+                except Exception:
+                    return L.TW_E_IO
+            cb = L.READ_FN(_read)
+            d.read = ctypes.cast(cb, ctypes.c_void_p)
+        d.write = ctypes.cast(self.lib.tw_checksum_write, ctypes.c_void_p)
+        d.write_user = ctypes.cast(ctypes.byref(sink), ctypes.c_void_p)
+        st = L.TwStreamStats()
+        self._check(self.lib.tw_warp_stream(self.ctx, ctypes.byref(d), ctypes.byref(st)))
+        return sums, kept, {k: getattr(st, k) for k, _ in L.TwStreamStats._fields_}
+
+
+class Synth:
+    """Procedural subject volume (tw_synth): noise hashed from (seed, x, y, z)
+    plus Gaussian blobs per cell; any window of it is reproducible."""
+
+    def __init__(self, size_xyz, dtype=np.uint16, seed=1, cell=256, blobs_per_cell=16, sigma=(4.0, 16.0),
+                 amp=None, noise=None):
+        self.dtype = np.dtype(dtype)
+        u8 = self.dtype == np.uint8
+        amp = amp or ((20.0, 200.0) if u8 else (100.0, 4000.0))
+        self.s = L.TwSynth(seed, size_xyz[0], size_xyz[1], size_xyz[2], L.TW_U8 if u8 else L.TW_U16, cell,
+                           blobs_per_cell, 0, sigma[0], sigma[1], amp[0], amp[1],
+                           noise if noise is not None else (5.0 if u8 else 40.0))
+        self.size = tuple(size_xyz)
+
+    def fill(self, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nz, ny, nx), self.dtype)
+        rc = L.load().tw_synth_fill(ctypes.byref(self.s), x0, y0, z0, nx, ny, nz, out.ctypes.data)
+        if rc != L.TW_OK:
+            raise TeraWarpError(rc, "tw_synth_fill: box outside the volume")
+        return out
+
+
+def checksum64(a):
+    """tw_checksum64 of an array's bytes."""
+    a = np.ascontiguousarray(a)
+    return int(L.load().tw_checksum64(a.ctypes.data, a.nbytes))
+

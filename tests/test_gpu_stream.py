@@ -1,0 +1,104 @@
+"""Bricked streaming (tw_warp_stream) vs the in-memory warp and the oracle.
+
+Bars (stated per test):
+  TW_PLAIN / TW_MIXED   the assembled bricks are bit-identical to tw_warp_tps on
+                        the whole volume (same grid, same per-voxel arithmetic;
+                        exact source windows cover every tap)
+  TW_EXACT              every brick's checksum equals that of the same box of
+                        the FP64 oracle's in-memory warp
+  failures              an error in the read callback ends the call with an
+                        error (no hang); bad descriptors are rejected
+"""
+import numpy as np
+import pytest
+
+from terawarp_amd import Synth, checksum64, synth, warp
+
+pytestmark = pytest.mark.gpu
+
+
+def boxes(size, brick):
+    nbx, nby, nbz = (-(-n // brick) for n in size)
+    for b in range(nbx * nby * nbz):
+        bx, by, bz = b % nbx, (b // nbx) % nby, b // (nbx * nby)
+        yield b, (slice(bz * brick, min((bz + 1) * brick, size[2])), slice(by * brick, min((by + 1) * brick, size[1])),
+                  slice(bx * brick, min((bx + 1) * brick, size[0])))
+
+
+def in_memory(warper, vol, tar, sub, f, precision, interp):
+    import torch
+    src = torch.from_numpy(vol.view(np.int16) if vol.dtype == np.uint16 else vol).cuda()
+    dst = torch.zeros_like(src)
+    warper.warp_tps(src, tar, sub, f=f, lam=0.2, interp=interp, precision=precision, dst=dst)
+    torch.cuda.synchronize()
+    out = dst.cpu().numpy()
+    return out.view(np.uint16) if vol.dtype == np.uint16 else out
+
+
+@pytest.mark.parametrize("precision,interp", [(warp.PLAIN, warp.LINEAR), (warp.MIXED, warp.LINEAR),
+                                              (warp.MIXED, warp.NN)])
+def test_stream_equals_in_memory_warp(warper, precision, interp):
+    size, brick, f = (200, 150, 90), 64, 8          # 4 x 3 x 2 bricks, uneven on every axis
+    s = Synth(size, np.uint16, seed=4, cell=64, blobs_per_cell=4)
+    vol = s.fill(0, 0, 0, *size)
+    sub, tar = synth.landmarks(size, 48, 6.0, seed=4)
+    ref = in_memory(warper, vol, tar, sub, f, precision, interp)
+    nb = 4 * 3 * 2
+    sums, kept, st = warper.warp_stream(size, np.uint16, tar, sub, s, f=f, brick=brick, precision=precision,
+                                        interp=interp, threads=4, keep=range(nb))
+    assert st["bricks"] == nb
+    for b, box in boxes(size, brick):
+        assert kept[b].shape == ref[box].shape
+        assert np.array_equal(kept[b], ref[box]), (b, np.count_nonzero(kept[b] != ref[box]))
+        assert int(sums[b]) == checksum64(ref[box])
+    # Exact windows: no wider than the field's reach plus the rounding margin.
+    assert 0 < st["max_margin"] <= 6 + 4
+
+
+def test_stream_exact_checksums_match_oracle(warper, oracle):
+    size, brick, f = (130, 97, 70), 32, 4
+    s = Synth(size, np.uint8, seed=8, cell=32, blobs_per_cell=4)
+    vol = s.fill(0, 0, 0, *size)
+    sub, tar = synth.landmarks(size, 30, 4.0, seed=8)
+    ref = oracle.warp_tps(vol, tar, sub, g=f, lam=0.2)[0]
+    sums, _, st = warper.warp_stream(size, np.uint8, tar, sub, s, f=f, brick=brick, precision=warp.EXACT,
+                                     threads=3)
+    for b, box in boxes(size, brick):
+        assert int(sums[b]) == checksum64(ref[box]), b
+    assert st["h2d_bytes"] > 0 and st["d2h_bytes"] == vol.nbytes
+
+
+def test_stream_python_source_and_read_failure(warper):
+    size, brick, f = (96, 80, 48), 32, 8
+    s = Synth(size, np.uint8, seed=2, cell=32, blobs_per_cell=2)
+    vol = s.fill(0, 0, 0, *size)
+    sub, tar = synth.landmarks(size, 20, 3.0, seed=2)
+
+    def source(x0, y0, z0, nx, ny, nz):
+        return vol[z0:z0 + nz, y0:y0 + ny, x0:x0 + nx]
+
+    a, _, _ = warper.warp_stream(size, np.uint8, tar, sub, s, f=f, brick=brick, precision=warp.MIXED, threads=2)
+    b, _, _ = warper.warp_stream(size, np.uint8, tar, sub, source, f=f, brick=brick, precision=warp.MIXED, threads=2)
+    assert np.array_equal(a, b)
+
+    def failing(x0, y0, z0, nx, ny, nz):
+        if z0 >= 32:
+            raise IOError("read failed")
+        return source(x0, y0, z0, nx, ny, nz)
+
+    with pytest.raises(warp.TeraWarpError) as e:
+        warper.warp_stream(size, np.uint8, tar, sub, failing, f=f, brick=brick, precision=warp.MIXED, threads=2)
+    assert "read callback failed" in str(e.value)
+    # The context still works afterwards.
+    c, _, _ = warper.warp_stream(size, np.uint8, tar, sub, s, f=f, brick=brick, precision=warp.MIXED, threads=2)
+    assert np.array_equal(a, c)
+
+
+def test_stream_rejects_bad_descriptors(warper):
+    size = (64, 64, 32)
+    s = Synth(size, np.uint8)
+    sub, tar = synth.landmarks(size, 10, 2.0, seed=1)
+    with pytest.raises(warp.TeraWarpError):  # brick not a multiple of the spacing
+        warper.warp_stream(size, np.uint8, tar, sub, s, f=8, brick=20)
+    with pytest.raises(warp.TeraWarpError):  # spacing not below the image size
+        warper.warp_stream(size, np.uint8, tar, sub, s, f=32, brick=32)

@@ -9,6 +9,8 @@ import numpy as np
 import pytest
 from scipy import ndimage
 
+from terawarp_amd import synth
+
 LAM = 0.2
 
 
@@ -250,3 +252,27 @@ def test_resample_nearest_neighbour(oracle, rng):
     expect = np.zeros_like(v)
     expect[:-1, 1:, :-1] = v[:-1, 1:, 1:]
     assert np.array_equal(out, expect)
+
+
+def test_window_resample_matches_whole_volume(oracle):
+    """The tera-block form: the whole volume as its own window equals the
+    in-memory warp; an output window with its own grid (origin on the node
+    lattice) and a source window covering every tap equals that region of the
+    whole warp; a source window too small for the taps gives zeros there."""
+    size, g = (40, 33, 29), 4
+    vol = synth.blob_volume(size[::-1], np.uint16, n_blobs=10, amp=(100, 4000), seed=2)
+    sub, tar = synth.landmarks(size, 20, 3.0, seed=2)
+    ref, df, (c, d) = oracle.warp_tps(vol, tar, sub, g=g)
+    for interp in (0, 1):
+        whole = oracle.tps_resample(vol, df, g, interp=interp)
+        w = oracle.tps_resample_window(vol, (0, 0, 0), size, df, g, (0, 0, 0), size, (0, 0, 0), interp=interp)
+        assert np.array_equal(w, whole)
+    d0, dsz = (8, 4, 12), (16, 16, 17)
+    dfw = oracle.grid_eval(tar, c, d, tuple(int(np.ceil(n / g)) + 3 for n in dsz), g, origin=d0)
+    part = ref[12:29, 4:20, 8:24]
+    src = np.ascontiguousarray(vol[4:29, :, :])
+    assert np.array_equal(oracle.tps_resample_window(src, (0, 0, 4), size, dfw, g, d0, dsz, d0), part)
+    tight = np.ascontiguousarray(vol[14:20, 6:18, 10:22])
+    w = oracle.tps_resample_window(tight, (10, 6, 14), size, dfw, g, d0, dsz, d0)
+    changed = w != part
+    assert changed.any() and np.all(w[changed] == 0)
