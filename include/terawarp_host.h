@@ -44,6 +44,28 @@ int tw_markers_write(const char* path, const float* xyz, int n);
 
 const char* tw_io_last_error(void);
 
+/* ---- tera output layout ---------------------------------------------------------
+ * The tera-block driver's files (q_imgwarp_tps_quicksmallmemory.cpp:1239-1241,
+ * 1863-1875; Bigwarp.cpp:960-1000): one Vaa3D raw stack per output block,
+ *   <base>_stps_block/ZZZZZZ_YYYYYY_XXXXXX.v3draw  (offsets x 10, 6 digits),
+ * and one single-slice stack per z,
+ *   <base>_stps_slice/ZZZZZZZ.raw                   (z x 10, 7 digits). */
+typedef struct {
+    const char* base;
+    int32_t     dtype;  /* TW_U8 / TW_U16 */
+    int32_t     pad;
+} tw_block_files;
+int tw_tera_block_path(const char* base, int64_t x0, int64_t y0, int64_t z0, char* out, int cap);
+int tw_tera_slice_path(const char* base, int64_t z, char* out, int cap);
+/* tw_write_fn (include/terawarp.h) writing each brick as its block file
+ * (user = const tw_block_files*). */
+int tw_block_file_write(void* user, int64_t brick, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny,
+                        int64_t nz, const void* data);
+/* Read the block files of a volume of nx x ny x nz in bricks of `brick` and
+ * write its slices. */
+int tw_stitch_slices(const char* base, int64_t nx, int64_t ny, int64_t nz, int32_t brick, int32_t dtype);
+const char* tw_tera_last_error(void);
+
 /* ---- plugin surface ----------------------------------------------------------
  * plugin_Bigimagewarp.cpp:26-38 (menulist / funclist), 73-107 (dofunc),
  * 123-336 (bigimagewarp). */

@@ -44,6 +44,7 @@ HOST_EXPORTS = (
     "tw_v3draw_header", "tw_v3draw_read", "tw_v3draw_write", "tw_markers_read", "tw_markers_write",
     "tw_io_last_error", "tw_plugin_menu_count", "tw_plugin_menu_item", "tw_plugin_func_count",
     "tw_plugin_func_item", "tw_plugin_version", "tw_plugin_parse_params", "tw_plugin_dofunc", "tw_bigimagewarp",
+    "tw_tera_block_path", "tw_tera_slice_path", "tw_block_file_write", "tw_stitch_slices", "tw_tera_last_error",
 )
 
 _i64 = ctypes.c_int64
@@ -107,6 +108,10 @@ class TwChecksumSink(ctypes.Structure):
                 ("nkeep", _i32), ("keep_ids", ctypes.POINTER(_i64)), ("keep_ptrs", ctypes.POINTER(_P))]
 
 
+class TwBlockFiles(ctypes.Structure):
+    _fields_ = [("base", ctypes.c_char_p), ("dtype", _i32), ("pad", _i32)]
+
+
 class TwWarpParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("device_mode", ctypes.c_int), ("resize", ctypes.c_int),
                 ("downsample", ctypes.c_int), ("para", ctypes.c_double * 3), ("newsize", _i64 * 3),
@@ -155,6 +160,11 @@ def load():
         "tw_checksum64": (ctypes.c_uint64, [_P, _i64]),
         "tw_checksum_write": (ctypes.c_int, [_P, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _P]),
         "tw_grid_dim": (_i64, [_i64, ctypes.c_int]),
+        "tw_tera_block_path": (ctypes.c_int, [ctypes.c_char_p, _i64, _i64, _i64, ctypes.c_char_p, ctypes.c_int]),
+        "tw_tera_slice_path": (ctypes.c_int, [ctypes.c_char_p, _i64, ctypes.c_char_p, ctypes.c_int]),
+        "tw_block_file_write": (ctypes.c_int, [_P, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _P]),
+        "tw_stitch_slices": (ctypes.c_int, [ctypes.c_char_p, _i64, _i64, _i64, _i32, _i32]),
+        "tw_tera_last_error": (ctypes.c_char_p, []),
         "tw_v3draw_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "tw_v3draw_read": (ctypes.c_int, [ctypes.c_char_p, _P, _i64]),

@@ -323,14 +323,16 @@ class Warper:
 
     # ---- streaming ----------------------------------------------------------
     def warp_stream(self, size_xyz, dtype, centres, targets, source, f=8, brick=512, lam=0.2, interp=LINEAR,
-                    precision=MIXED, threads=16, keep=()):
+                    precision=MIXED, threads=16, keep=(), block_base=None):
         """Warp a volume brick by brick (tw_warp_stream).
 
         source: a Synth (procedural volume, filled by the library's own
         threads) or a Python callable read(x0, y0, z0, nx, ny, nz) returning
         a (nz, ny, nx) array.  Returns (checksums, kept, stats): per-brick
         tw_checksum64 values, {brick id: (z, y, x) output array} for the
-        bricks listed in keep, and the call's statistics.
+        bricks listed in keep, and the call's statistics.  With block_base,
+        the bricks are written as the tera driver's block files instead
+        (tw_block_file_write; checksums and kept bricks are then empty).
         """
         ce, ta = _pts(centres), _pts(targets)
         nx, ny, nz = size_xyz
@@ -367,8 +369,13 @@ class Warper:
                     return L.TW_E_IO
             cb = L.READ_FN(_read)
             d.read = ctypes.cast(cb, ctypes.c_void_p)
-        d.write = ctypes.cast(self.lib.tw_checksum_write, ctypes.c_void_p)
-        d.write_user = ctypes.cast(ctypes.byref(sink), ctypes.c_void_p)
+        if block_base is not None:
+            files = L.TwBlockFiles(str(block_base).encode(), dt, 0)
+            d.write = ctypes.cast(self.lib.tw_block_file_write, ctypes.c_void_p)
+            d.write_user = ctypes.cast(ctypes.byref(files), ctypes.c_void_p)
+        else:
+            d.write = ctypes.cast(self.lib.tw_checksum_write, ctypes.c_void_p)
+            d.write_user = ctypes.cast(ctypes.byref(sink), ctypes.c_void_p)
         st = L.TwStreamStats()
         self._check(self.lib.tw_warp_stream(self.ctx, ctypes.byref(d), ctypes.byref(st)))
         return sums, kept, {k: getattr(st, k) for k, _ in L.TwStreamStats._fields_}
@@ -400,4 +407,24 @@ def checksum64(a):
     """tw_checksum64 of an array's bytes."""
     a = np.ascontiguousarray(a)
     return int(L.load().tw_checksum64(a.ctypes.data, a.nbytes))
+
+
+def stitch_slices(base, size_xyz, brick, dtype):
+    """Write <base>_stps_slice/ZZZZZZZ.raw from the block files (tw_stitch_slices)."""
+    nx, ny, nz = size_xyz
+    dt = L.TW_U8 if np.dtype(dtype) == np.uint8 else L.TW_U16
+    lib = L.load()
+    rc = lib.tw_stitch_slices(str(base).encode(), nx, ny, nz, brick, dt)
+    if rc != L.TW_OK:
+        raise TeraWarpError(rc, lib.tw_tera_last_error().decode(errors="replace"))
+
+
+def tera_paths(base, x0=0, y0=0, z0=0, z=None):
+    """(block file path of the brick at (x0, y0, z0), slice file path of z)."""
+    lib = L.load()
+    b = ctypes.create_string_buffer(4096)
+    lib.tw_tera_block_path(str(base).encode(), x0, y0, z0, b, 4096)
+    s = ctypes.create_string_buffer(4096)
+    lib.tw_tera_slice_path(str(base).encode(), z if z is not None else z0, s, 4096)
+    return b.value.decode(), s.value.decode()
 

@@ -102,3 +102,24 @@ def test_stream_rejects_bad_descriptors(warper):
         warper.warp_stream(size, np.uint8, tar, sub, s, f=8, brick=20)
     with pytest.raises(warp.TeraWarpError):  # spacing not below the image size
         warper.warp_stream(size, np.uint8, tar, sub, s, f=32, brick=32)
+
+
+def test_stream_to_block_files_and_slices(warper, tmp_path):
+    """The tera driver's output layout: block files from the stream, stitched
+    into slices that equal the in-memory warp."""
+    from terawarp_amd import _lib as L
+    from terawarp_amd.warp import stitch_slices, tera_paths
+    size, brick, f = (100, 70, 40), 32, 8
+    s = Synth(size, np.uint8, seed=6, cell=32, blobs_per_cell=3)
+    vol = s.fill(0, 0, 0, *size)
+    sub, tar = synth.landmarks(size, 24, 4.0, seed=6)
+    ref = in_memory(warper, vol, tar, sub, f, warp.MIXED, warp.LINEAR)
+    base = str(tmp_path / "tw")
+    warper.warp_stream(size, np.uint8, tar, sub, s, f=f, brick=brick, precision=warp.MIXED, threads=2,
+                       block_base=base)
+    stitch_slices(base, size, brick, np.uint8)
+    lib = L.load()
+    for z in range(size[2]):
+        out = np.zeros((size[1], size[0]), np.uint8)
+        assert lib.tw_v3draw_read(tera_paths(base, z=z)[1].encode(), out.ctypes.data, out.nbytes) == 0
+        assert np.array_equal(out, ref[z]), z

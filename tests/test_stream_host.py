@@ -45,3 +45,37 @@ def test_checksum64_is_the_weighted_word_sum(n):
         b[:8], b[8:16] = a[8:16], a[:8]
         if not np.array_equal(a[:8], a[8:16]):
             assert checksum64(b) != checksum64(a)
+
+
+def test_tera_block_files_and_slices(tmp_path):
+    """Block files named as the reference's tera driver names them, written
+    through the stream's write callback, stitched back into per-z slices."""
+    import ctypes
+    from terawarp_amd import _lib as L
+    from terawarp_amd.warp import stitch_slices, tera_paths
+    b, s = tera_paths("/out/vol", 512, 1024, 0, z=37)
+    assert b == "/out/vol_stps_block/000000_010240_005120.v3draw"
+    assert s == "/out/vol_stps_slice/0000370.raw"
+    lib = L.load()
+    size, brick = (45, 30, 21), 16                     # uneven bricks on every axis
+    vol = np.random.default_rng(3).integers(0, 65536, size[::-1]).astype(np.uint16)
+    base = str(tmp_path / "w")
+    files = L.TwBlockFiles(base.encode(), L.TW_U16, 0)
+    nbx, nby, nbz = (-(-n // brick) for n in size)
+    for bid in range(nbx * nby * nbz):
+        bx, by, bz = bid % nbx, (bid // nbx) % nby, bid // (nbx * nby)
+        box = vol[bz * brick:(bz + 1) * brick, by * brick:(by + 1) * brick, bx * brick:(bx + 1) * brick]
+        box = np.ascontiguousarray(box)
+        rc = lib.tw_block_file_write(ctypes.byref(files), bid, bx * brick, by * brick, bz * brick, box.shape[2],
+                                     box.shape[1], box.shape[0], box.ctypes.data)
+        assert rc == 0
+    stitch_slices(base, size, brick, np.uint16)
+    for z in range(size[2]):
+        path = tera_paths(base, z=z)[1]
+        sz = (ctypes.c_int64 * 4)()
+        dt, sw, hl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        assert lib.tw_v3draw_header(path.encode(), sz, ctypes.byref(dt), ctypes.byref(sw), ctypes.byref(hl)) == 0
+        assert list(sz) == [size[0], size[1], 1, 1] and dt.value == 2
+        out = np.zeros((size[1], size[0]), np.uint16)
+        assert lib.tw_v3draw_read(path.encode(), out.ctypes.data, out.nbytes) == 0
+        assert np.array_equal(out, vol[z]), z
