@@ -211,6 +211,47 @@ def test_bricks_reassemble_the_full_output(warper, oracle):
             assert np.array_equal(brick, full[z0:z0 + nz, y0:y0 + ny, x0:x0 + nx])
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("interp", [warp.LINEAR, warp.NN])
+def test_zero_field_is_identity(warper, dtype, interp):
+    """A zero displacement samples every voxel at itself (all weights 0), so
+    the output equals the input exactly, the volume's last voxel included.
+    Odd extents put that voxel in a dword that runs past the channel's end."""
+    rng = np.random.default_rng(7)
+    g = 4
+    for size in [(67, 31, 23), (129, 17, 13), (64, 9, 6)]:
+        vol = rng.integers(1, np.iinfo(dtype).max + 1, size[::-1]).astype(dtype)
+        grid = warper.grid_from_array(np.zeros((3,) + tuple(warp.grid_dim(s, g) for s in size[::-1]), np.float32), g)
+        for precision in (warp.PLAIN, warp.MIXED):
+            out = warper.resample_tps(vol, grid, interp=interp, precision=precision)
+            assert np.array_equal(out, vol), (size, precision, np.argwhere(out != vol)[:4])
+
+
+@pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
+def test_source_window_equals_full_source(warper, oracle, precision):
+    """Output boxes resampled from a source window that holds every tap
+    equal the same boxes resampled from the whole volume (odd extents; one
+    box reaches the volume's far corner)."""
+    size = (97, 61, 37)
+    g = 4
+    vol, sub, tar = case(size, np.uint16, 24, 3.0, seed=43)
+    c, d = oracle.stps_fit(tar, sub)
+    grid = warper.new_grid(tuple(warp.grid_dim(s, g) for s in size), g, device=False)
+    warper.grid_eval(tar, c, d, grid, precision=warp.EXACT)
+    full = warper.resample_tps(vol, grid, precision=precision)
+    m = int(np.ceil(np.abs(grid.data).max())) + 2  # field bound plus the rounding margin
+    for (x0, y0, z0), (nx, ny, nz) in [((33, 21, 9), (64, 40, 28)), ((8, 4, 0), (31, 23, 17)),
+                                       ((40, 30, 20), (57, 31, 17))]:
+        lo = [max(0, o - m) for o in (x0, y0, z0)]
+        hi = [min(s, o + n + m) for o, n, s in zip((x0, y0, z0), (nx, ny, nz), size)]
+        win = np.ascontiguousarray(vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]])
+        dst = np.zeros((nz, ny, nx), np.uint16)
+        warper.resample_tps(win, grid, dst=dst, precision=precision, src_origin=tuple(lo), dst_origin=(x0, y0, z0),
+                            full=size)
+        ref = full[z0:z0 + nz, y0:y0 + ny, x0:x0 + nx]
+        assert np.array_equal(dst, ref), ((x0, y0, z0), np.count_nonzero(dst != ref))
+
+
 def test_resample_rejects_uncovered_grid(warper):
     vol = np.zeros((20, 20, 20), np.uint8)
     grid = warper.grid_from_array(np.zeros((3, 5, 5, 5), np.float32), 4)
