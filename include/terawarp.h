@@ -68,6 +68,24 @@ extern "C" {
 #define TW_DF_F32  0
 #define TW_DF_F16  1
 
+/* How a grid's node displacements are upsampled to voxels (tw_grid.basis).
+ * The reference declares both (i_interpmethod_df: 0 linear, 1 B-spline;
+ * q_imgwarp_tps_quicksmallmemory.h:14-17, .cpp:1158, 2021-2025) but only
+ * implements the B-spline (q_dfblcokinterp_bspline, .cpp:169-203; the
+ * linear q_dfblcokinterp_linear is declared and never defined).
+ * TW_BASIS_BSPLINE: voxel b*f + t of a block uses nodes b .. b+3 with the
+ *   uniform cubic B-spline weights TB[t] (q_nonrigid_ini_bsplinebasis_3D).
+ * TW_BASIS_LINEAR: the same voxel interpolates linearly between nodes b+1
+ *   and b+2 (the nodes at b*f and (b+1)*f) with weights 1 - t/f and t/f
+ *   per axis, i.e. TB[t] = {0, 1 - t/f, t/f, 0}. */
+#define TW_BASIS_BSPLINE  0
+#define TW_BASIS_LINEAR   1
+
+/* OR into the interp argument of the one-call warps (tw_warp_tps,
+ * tw_warp_batch, tw_warp_stream) to upsample their grids with
+ * TW_BASIS_LINEAR. */
+#define TW_DF_LINEAR      0x100
+
 typedef struct tw_ctx tw_ctx;
 
 /* A volume or a window of one.  full_* is the extent of the whole volume
@@ -98,7 +116,7 @@ typedef struct {
     int32_t storage;           /* TW_DF_F32 / TW_DF_F16 */
     int64_t ox, oy, oz;
     int32_t mem;               /* where data lives */
-    int32_t reserved;
+    int32_t basis;             /* TW_BASIS_BSPLINE (0) or TW_BASIS_LINEAR */
     float*  qtab;              /* F16: device table, per node block 6 floats
                                   (offset xyz, scale xyz); owned by library */
 } tw_grid;

@@ -40,14 +40,16 @@ def lib():
         L.or_grid_eval.argtypes = [_P, ctypes.c_int, _P, _P, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, _i64, _i64, _i64, _P, ctypes.c_int]
         L.or_bspline_basis.argtypes = [ctypes.c_int, _P, _P]
+        L.or_df_basis.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P]
         L.or_tps_resample.argtypes = [_P, _P, ctypes.c_int, _P, _i64, _i64, _i64, ctypes.c_int, ctypes.c_int,
                                       _P, _i64, _i64, ctypes.c_int]
         L.or_tps_resample_window.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P, _i64, _i64, _i64, _P, ctypes.c_int,
                                              ctypes.c_int, _P, _P, _P, ctypes.c_int]
-        L.or_dense_df.argtypes = [_P, _P, _i64, _i64, _i64, ctypes.c_int, _P, ctypes.c_int]
+        L.or_dense_df.argtypes = [_P, _P, _i64, _i64, _i64, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int]
         L.or_affine_fit.argtypes = [_P, _P, ctypes.c_int, _P]
         L.or_affine_resample.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _i64, _i64, ctypes.c_int]
-        for f in ("or_stps_fit", "or_stps_fit_bordered", "or_grid_eval", "or_bspline_basis", "or_tps_resample",
+        for f in ("or_stps_fit", "or_stps_fit_bordered", "or_grid_eval", "or_bspline_basis", "or_df_basis",
+                  "or_tps_resample",
                   "or_tps_resample_window", "or_dense_df", "or_affine_fit", "or_affine_resample", "or_max_threads"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
@@ -115,10 +117,22 @@ def grid_eval(centres, c, d, gshape_xyz, g, origin=(0, 0, 0), nthreads=0):
     return df
 
 
+# OR into a resample's interp: the linear DF upsample (df_basis kind 1).
+DF_LINEAR = 0x100
+
+
 def bspline_basis(n, full=True):
     TB = np.zeros((n, 4), np.float64)
     basis = np.zeros((n ** 3, 64), np.float64) if full else None
     _check(lib().or_bspline_basis(n, _ptr(TB), _ptr(basis) if full else None), "bspline_basis")
+    return TB, basis
+
+
+def df_basis(n, kind, full=True):
+    """Upsampling weights of kind 0 (cubic B-spline) or 1 (linear)."""
+    TB = np.zeros((n, 4), np.float64)
+    basis = np.zeros((n ** 3, 64), np.float64) if full else None
+    _check(lib().or_df_basis(n, kind, _ptr(TB), _ptr(basis) if full else None), "df_basis")
     return TB, basis
 
 
@@ -150,12 +164,12 @@ def tps_resample_window(src_win, src_origin, full_xyz, df, n, grid_origin, out_s
     return out
 
 
-def dense_df(size_xyz, df, n, nthreads=0):
+def dense_df(size_xyz, df, n, nthreads=0, kind=0):
     df = np.ascontiguousarray(df, np.float32)
     sz = np.array(list(size_xyz) + [1], np.int64)
     out = np.zeros((3, size_xyz[2], size_xyz[1], size_xyz[0]), np.float32)
-    _check(lib().or_dense_df(_ptr(sz), _ptr(df), df.shape[3], df.shape[2], df.shape[1], n, _ptr(out), nthreads),
-           "dense_df")
+    _check(lib().or_dense_df(_ptr(sz), _ptr(df), df.shape[3], df.shape[2], df.shape[1], n, _ptr(out), nthreads,
+                             kind), "dense_df")
     return out
 
 

@@ -44,6 +44,9 @@
 #endif
 
 typedef int64_t i64;
+// OR-ed into a resample's interp: upsample the grid with the linear DF basis
+// (or_df_basis kind 1) instead of the cubic B-spline.
+constexpr int DF_LINEAR_FLAG = 0x100;
 
 namespace {
 
@@ -327,20 +330,37 @@ int or_grid_eval(const float* centres, int K, const double* c, const double* d,
 // T(i,:) = [t^3 t^2 t 1], t = i/n (681-688), TB = T*B (691),
 // BxBxB = KP(KP(TB,TB),TB) (694-696).
 // TB: n x 4; basis: n^3 x 64 (row (i1*n+i2)*n+i3, col (j1*4+j2)*4+j3).
-int or_bspline_basis(int n, double* TB, double* basis) {
-    if (n <= 0 || !TB) return -1;
-    static const double Mi[4][4] = {{-1, 3, -3, 1}, {3, -6, 3, 0}, {-3, 0, 3, 0}, {1, 4, 1, 0}};
-    double Bm[4][4];
-    for (int a = 0; a < 4; a++)
-        for (int b = 0; b < 4; b++) Bm[a][b] = Mi[a][b] / 6.0;
+//
+// kind 1: the linear DF upsample.  The reference declares it
+// (i_interpmethod_df = 0, q_imgwarp_tps_quicksmallmemory.h:14-17,
+// .cpp:1158, 2021-2025) and never defines q_dfblcokinterp_linear; this
+// restatement defines it on the B-spline's block layout: voxel b*n + i uses
+// nodes b+1 and b+2 (at b*n and (b+1)*n) with weights 1 - t and t, t = i/n,
+// so TB(i,:) = [0, 1 - t, t, 0] and BxBxB is built as for the B-spline.
+int or_df_basis(int n, int kind, double* TB, double* basis) {
+    if (n <= 0 || !TB || (kind != 0 && kind != 1)) return -1;
     const double step = 1.0 / n;
-    for (int i = 0; i < n; i++) {
-        const double t = step * i;
-        const double T[4] = {std::pow(t, 3.0), std::pow(t, 2.0), std::pow(t, 1.0), std::pow(t, 0.0)};
-        for (int j = 0; j < 4; j++) {
-            double s = 0.0;
-            for (int m = 0; m < 4; m++) s += T[m] * Bm[m][j];
-            TB[i * 4 + j] = s;
+    if (kind == 1) {
+        for (int i = 0; i < n; i++) {
+            const double t = step * i;
+            TB[i * 4 + 0] = 0.0;
+            TB[i * 4 + 1] = 1.0 - t;
+            TB[i * 4 + 2] = t;
+            TB[i * 4 + 3] = 0.0;
+        }
+    } else {
+        static const double Mi[4][4] = {{-1, 3, -3, 1}, {3, -6, 3, 0}, {-3, 0, 3, 0}, {1, 4, 1, 0}};
+        double Bm[4][4];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) Bm[a][b] = Mi[a][b] / 6.0;
+        for (int i = 0; i < n; i++) {
+            const double t = step * i;
+            const double T[4] = {std::pow(t, 3.0), std::pow(t, 2.0), std::pow(t, 1.0), std::pow(t, 0.0)};
+            for (int j = 0; j < 4; j++) {
+                double s = 0.0;
+                for (int m = 0; m < 4; m++) s += T[m] * Bm[m][j];
+                TB[i * 4 + j] = s;
+            }
         }
     }
     if (basis) {
@@ -357,6 +377,8 @@ int or_bspline_basis(int n, double* TB, double* basis) {
     }
     return 0;
 }
+
+int or_bspline_basis(int n, double* TB, double* basis) { return or_df_basis(n, 0, TB, basis); }
 
 }  // extern "C"
 
@@ -448,7 +470,8 @@ template <class T>
 int tps_resample(const T* src, const i64* sz, const float* df, i64 gx, i64 gy, i64 gz, int n,
                  int interp, T* dst, i64 zb0, i64 zb1) {
     std::vector<double> TB((size_t)n * 4), basis((size_t)n * n * n * 64);
-    or_bspline_basis(n, TB.data(), basis.data());
+    or_df_basis(n, (interp & DF_LINEAR_FLAG) ? 1 : 0, TB.data(), basis.data());
+    interp &= ~DF_LINEAR_FLAG;
     const i64 nbx = (sz[0] + n - 1) / n, nby = (sz[1] + n - 1) / n, nbz = (sz[2] + n - 1) / n;
     if (nbx + 3 > gx || nby + 3 > gy || nbz + 3 > gz) return -1;
     if (zb1 < 0 || zb1 > nbz) zb1 = nbz;
@@ -478,7 +501,8 @@ template <class T>
 int tps_resample_window(const T* src, const i64* ssz, const i64* s0, const i64* fsz, const float* df, i64 gx,
                         i64 gy, i64 gz, const i64* g0, int n, int interp, T* dst, const i64* dsz, const i64* d0) {
     std::vector<double> TB((size_t)n * 4), basis((size_t)n * n * n * 64);
-    or_bspline_basis(n, TB.data(), basis.data());
+    or_df_basis(n, (interp & DF_LINEAR_FLAG) ? 1 : 0, TB.data(), basis.data());
+    interp &= ~DF_LINEAR_FLAG;
     for (int a = 0; a < 3; a++)
         if (d0[a] < g0[a] || dsz[a] < 1 || ssz[a] < 1) return -1;
     // Block range (grid-relative) holding output voxels.
@@ -588,14 +612,15 @@ int or_tps_resample_window(const void* src, const i64* ssz, const i64* s0, const
 }
 
 // Dense per-voxel DF of the B-spline upsample (the pppDFBlock values of
-// q_dfblcokinterp_bspline for every voxel), three float planes of
-// sz0*sz1*sz2 each.  Used to state the displacement error of each stage.
+// q_dfblcokinterp_bspline for every voxel; kind 1: the linear upsample of
+// or_df_basis), three float planes of sz0*sz1*sz2 each.  Used to state the
+// displacement error of each stage.
 int or_dense_df(const i64* sz, const float* df, i64 gx, i64 gy, i64 gz, int n, float* out,
-                int nthreads) {
+                int nthreads, int kind) {
     if (!sz || !df || !out || n <= 0) return -1;
     set_threads(nthreads);
     std::vector<double> TB((size_t)n * 4), basis((size_t)n * n * n * 64);
-    or_bspline_basis(n, TB.data(), basis.data());
+    if (or_df_basis(n, kind, TB.data(), basis.data())) return -1;
     const i64 nbx = (sz[0] + n - 1) / n, nby = (sz[1] + n - 1) / n, nbz = (sz[2] + n - 1) / n;
     if (nbx + 3 > gx || nby + 3 > gy || nbz + 3 > gz) return -1;
     const i64 sx = sz[0], sxy = sz[0] * sz[1], vol = sxy * sz[2];

@@ -27,6 +27,9 @@ TW_MIXED = 1
 TW_EXACT = 2
 TW_INTERP_LINEAR = 0
 TW_INTERP_NN = 1
+TW_DF_LINEAR = 0x100
+TW_BASIS_BSPLINE = 0
+TW_BASIS_LINEAR = 1
 TW_DF_F32 = 0
 TW_DF_F16 = 1
 TW_WARP_TPS = 0
@@ -61,7 +64,7 @@ class TwVolume(ctypes.Structure):
 
 class TwGrid(ctypes.Structure):
     _fields_ = [("data", _P), ("gx", _i64), ("gy", _i64), ("gz", _i64), ("f", _i32), ("storage", _i32),
-                ("ox", _i64), ("oy", _i64), ("oz", _i64), ("mem", _i32), ("reserved", _i32),
+                ("ox", _i64), ("oy", _i64), ("oz", _i64), ("mem", _i32), ("basis", _i32),
                 ("qtab", _P)]
 
 

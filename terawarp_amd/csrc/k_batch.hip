@@ -132,8 +132,10 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
                              int precision, int32_t* status, double* done_ms) {
     if (!ctx) return TW_E_ARG;
     if (!items || n < 1) return tw_fail(ctx, TW_E_ARG, "empty batch");
+    const int basis = (interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
+    interp &= ~TW_DF_LINEAR;
     if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
-        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN");
+        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN (optionally | TW_DF_LINEAR)");
     if (precision != TW_PLAIN && precision != TW_MIXED)
         return tw_fail(ctx, TW_E_ARG, "a batch runs TW_PLAIN or TW_MIXED (got %d)", precision);
     if (f < 1 || f > 64) return tw_fail(ctx, TW_E_ARG, "grid spacing must be in [1, 64]");
@@ -251,6 +253,7 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
             a.gx = g.gx; a.gy = g.gy; a.gz = g.gz;
             a.ox = g.ox; a.oy = g.oy; a.oz = g.oz;
             a.f = f;
+            a.basis = basis;
             a.tz = tz;
             a.qtab = g.qtab;
             a.qbx = (g.gx + 7) / 8;

@@ -160,20 +160,21 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
 // FP64 and FP32 B-spline weight tables of spacing f on the device, cached
 // per context: uploaded only when f changes (ordered on the stream after
 // every earlier use of the old tables).
-int upload_weights(tw_ctx* ctx, int f, const float** w32, const double** w64) {
+int upload_weights(tw_ctx* ctx, int f, int basis, const float** w32, const double** w64) {
     const size_t n = (size_t)f * 4, b64 = n * sizeof(double), b32 = n * sizeof(float);
     void* p = nullptr;
     int rc = tw_ws(ctx, WS_WEIGHTS, b64 + b32, &p);
     if (rc) return rc;
-    if (ctx->weights_f != f || ctx->weights_host.size() != b64 + b32) {
+    if (ctx->weights_f != f || ctx->weights_basis != basis || ctx->weights_host.size() != b64 + b32) {
         std::vector<double> tb(n);
-        tw_bspline_tb(f, tb.data());
+        tw_df_tb(f, basis, tb.data());
         ctx->weights_host.assign(b64 + b32, 0);
         memcpy(ctx->weights_host.data(), tb.data(), b64);
         float* h32 = (float*)(ctx->weights_host.data() + b64);
         for (size_t i = 0; i < n; i++) h32[i] = (float)tb[i];
         TW_HIP(ctx, hipMemcpyAsync(p, ctx->weights_host.data(), b64 + b32, hipMemcpyHostToDevice, ctx->stream));
         ctx->weights_f = f;
+        ctx->weights_basis = basis;
     }
     *w64 = (const double*)p;
     *w32 = (const float*)((uint8_t*)p + b64);
@@ -183,6 +184,8 @@ int upload_weights(tw_ctx* ctx, int f, const float** w32, const double** w64) {
 int check_grid_cover(tw_ctx* ctx, const tw_grid* g, i64 x0, i64 y0, i64 z0, i64 nx, i64 ny, i64 nz) {
     if (!g || !g->data) return tw_fail(ctx, TW_E_ARG, "null grid");
     if (g->f < 1) return tw_fail(ctx, TW_E_ARG, "grid spacing must be >= 1");
+    if (g->basis != TW_BASIS_BSPLINE && g->basis != TW_BASIS_LINEAR)
+        return tw_fail(ctx, TW_E_ARG, "grid basis must be TW_BASIS_BSPLINE or TW_BASIS_LINEAR");
     if (g->mem != TW_MEM_DEVICE && g->mem != TW_MEM_HOST) return tw_fail(ctx, TW_E_ARG, "bad grid mem");
     if (g->storage == TW_DF_F16 && (g->mem != TW_MEM_DEVICE || !g->qtab))
         return tw_fail(ctx, TW_E_ARG, "an FP16 grid must be device-resident with its block table");
@@ -215,6 +218,7 @@ void fill_grid_args(TpsArgs& a, const tw_grid* g, const void* gdev) {
     a.gx = g->gx; a.gy = g->gy; a.gz = g->gz;
     a.ox = g->ox; a.oy = g->oy; a.oz = g->oz;
     a.f = g->f;
+    a.basis = g->basis;
     a.qtab = g->qtab;
     a.qbx = (g->gx + 7) / 8;
     a.qby = (g->gy + 7) / 8;
@@ -353,7 +357,7 @@ int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const
     if ((rc = tw_stage_in(ctx, src, WS_SRC, &sdev))) return rc;
     if ((rc = tw_stage_out_buffer(ctx, dst, WS_DST, &ddev))) return rc;
     if ((rc = stage_grid(ctx, grid, &gdev))) return rc;
-    if ((rc = upload_weights(ctx, grid->f, &a.w32, &a.w64))) return rc;
+    if ((rc = upload_weights(ctx, grid->f, grid->basis, &a.w32, &a.w64))) return rc;
     a.src = sdev;
     a.dst = ddev;
     a.snx = src->nx; a.sny = src->ny; a.snz = src->nz;
@@ -396,7 +400,7 @@ extern "C" int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int6
     memset(&a, 0, sizeof a);
     const void* gdev = nullptr;
     if ((rc = stage_grid(ctx, grid, &gdev))) return rc;
-    if ((rc = upload_weights(ctx, grid->f, &a.w32, &a.w64))) return rc;
+    if ((rc = upload_weights(ctx, grid->f, grid->basis, &a.w32, &a.w64))) return rc;
     const size_t bytes = (size_t)nx * ny * nz * 3 * sizeof(float);
     float* dout = out;
     if (mem == TW_MEM_HOST) {

@@ -20,8 +20,10 @@ extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, co
     if ((rc = tw_check_volume(ctx, src, "src")) || (rc = tw_check_volume(ctx, dst, "dst"))) return rc;
     if (src->dtype != dst->dtype || src->nc != dst->nc)
         return tw_fail(ctx, TW_E_ARG, "src and dst must share dtype and channel count");
+    const int basis = (interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
+    interp &= ~TW_DF_LINEAR;
     if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
-        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN");
+        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN (optionally | TW_DF_LINEAR)");
     if (precision != TW_EXACT && precision != TW_PLAIN && precision != TW_MIXED)
         return tw_fail(ctx, TW_E_ARG, "unknown precision %d", precision);
     if (f < 1) return tw_fail(ctx, TW_E_ARG, "grid spacing must be >= 1");
@@ -39,6 +41,7 @@ extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, co
     // Grid over the output window (global coordinates of dst).
     tw_grid g = {};
     g.f = f;
+    g.basis = basis;
     g.ox = dst->x0;
     g.oy = dst->y0;
     g.oz = dst->z0;

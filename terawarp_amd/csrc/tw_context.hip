@@ -137,6 +137,21 @@ void tw_bspline_tb(int n, double* TB) {
     }
 }
 
+void tw_df_tb(int n, int basis, double* TB) {
+    if (basis != TW_BASIS_LINEAR) {
+        tw_bspline_tb(n, TB);
+        return;
+    }
+    const double step = 1.0 / n;
+    for (int i = 0; i < n; i++) {
+        const double t = step * i;
+        TB[i * 4 + 0] = 0.0;
+        TB[i * 4 + 1] = 1.0 - t;
+        TB[i * 4 + 2] = t;
+        TB[i * 4 + 3] = 0.0;
+    }
+}
+
 extern "C" {
 
 const char* tw_version(void) { return "terawarp-mi355x 0.1 (gfx950)"; }

@@ -57,6 +57,7 @@ struct tw_ctx {
     // B-spline weight tables on the device (WS_WEIGHTS) are for spacing
     // weights_f (0: none yet); their host image must outlive the copy.
     int weights_f = 0;
+    int weights_basis = 0;
     std::vector<uint8_t> weights_host;
     // Host images of a batch's tables, kept until the next batch (the batch
     // synchronises before it returns).
@@ -109,6 +110,9 @@ int tw_stage_out_finish(tw_ctx* ctx, const tw_volume* v, const void* dev);
 // Host-side FP64 cubic B-spline weights TB (n x 4), the reference's
 // q_nonrigid_ini_bsplinebasis_3D T*B product (q_imgwarp_tps_quicksmallmemory.cpp:663-691).
 void tw_bspline_tb(int n, double* TB);
+// The n x 4 upsampling weights of a basis (TW_BASIS_BSPLINE: tw_bspline_tb;
+// TW_BASIS_LINEAR: {0, 1 - t, t, 0}, t = i/n).
+void tw_df_tb(int n, int basis, double* TB);
 
 // Internal launchers shared between translation units.
 int tw_grid_quantize_f16(tw_ctx* ctx, const float* df32, tw_grid* grid);

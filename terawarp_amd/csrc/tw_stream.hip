@@ -164,8 +164,9 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
         return tw_fail(ctx, TW_E_ARG, "the brick edge must be a positive multiple of the grid spacing");
     if (d->f >= d->nx || d->f >= d->ny || d->f >= d->nz)
         return tw_fail(ctx, TW_E_ARG, "block size should be smaller than the image size");
-    if (d->interp != TW_INTERP_LINEAR && d->interp != TW_INTERP_NN)
-        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN");
+    const int interp = d->interp & ~TW_DF_LINEAR;
+    if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
+        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN (optionally | TW_DF_LINEAR)");
     if (d->precision != TW_EXACT && d->precision != TW_PLAIN && d->precision != TW_MIXED)
         return tw_fail(ctx, TW_E_ARG, "unknown precision %d", d->precision);
     if (!d->read || !d->write) return tw_fail(ctx, TW_E_ARG, "null read or write callback");
@@ -182,6 +183,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
     if ((rc = tw_tps_fit_dev(ctx, d->centres, d->targets, d->K, d->lambda, fit_prec, &fit))) return rc;
     tw_grid g = {};
     g.f = d->f;
+    g.basis = (d->interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
     g.gx = tw_grid_dim(d->nx, d->f);
     g.gy = tw_grid_dim(d->ny, d->f);
     g.gz = tw_grid_dim(d->nz, d->f);
@@ -345,7 +347,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
                             d->dtype, TW_MEM_DEVICE};
             tw_volume vd = {P.ddst[s], k.nx, k.ny, k.nz, 1, k.x0, k.y0, k.z0, 0, 0, 0, d->dtype, TW_MEM_DEVICE};
             TW_HIP(ctx, hipEventRecord(P.ev_beg[s], ctx->stream));
-            int r = tw_resample_tps_run(ctx, &vs, &vd, &g, d->interp, d->precision);
+            int r = tw_resample_tps_run(ctx, &vs, &vd, &g, interp, d->precision);
             if (r != TW_OK) return r;
             TW_HIP(ctx, hipEventRecord(P.ev_end[s], ctx->stream));
             // The pinned brick of slot s is free once the writer took brick b - NSLOT.

@@ -24,6 +24,7 @@ struct TpsArgs {
     const void* grid;                  // 3 planes, float or half
     i64 gx, gy, gz, ox, oy, oz;
     int f;
+    int basis;                         // TW_BASIS_BSPLINE / TW_BASIS_LINEAR (weight tables)
     int tz;                            // voxels per work-group along z
     const float* qtab;                 // FP16: per 8^3 node block (off xyz, scale xyz)
     i64 qbx, qby;
@@ -548,10 +549,10 @@ inline int pick_tz(int f) {
     return tz;
 }
 
-// FP32 B-spline weights of spacing f into the geometry block.
-inline void fill_weights(FastGeom& g, int f) {
+// FP32 upsampling weights of spacing f and a basis into the geometry block.
+inline void fill_weights(FastGeom& g, int f, int basis) {
     std::vector<double> tb((size_t)f * 4);
-    tw_bspline_tb(f, tb.data());
+    tw_df_tb(f, basis, tb.data());
     for (size_t i = 0; i < tb.size(); i++) g.wt[i] = (float)tb[i];
 }
 
@@ -593,7 +594,7 @@ const char* fused_geom(const TpsArgs& a, FastGeom& g, size_t& lds) {
     g.imx = (int)(a.fnx - 1); g.imy = (int)(a.fny - 1); g.imz = (int)(a.fnz - 1);
     g.invf = 1.0f / (float)a.f;
     g.dchan_b = (uint32_t)(a.dnx * a.dny * a.dnz * sizeof(T));
-    fill_weights(g, a.f);
+    fill_weights(g, a.f, a.basis);
     g.tiles_x = (int)((a.dnx + TX - 1) / TX);
     g.tiles_y = (int)((a.dny + FTY - 1) / FTY);
     const i64 ntiles = (i64)g.tiles_x * g.tiles_y * nbz;

@@ -20,6 +20,10 @@ from . import _lib as L
 PLAIN, MIXED, EXACT = L.TW_PLAIN, L.TW_MIXED, L.TW_EXACT
 LINEAR, NN = L.TW_INTERP_LINEAR, L.TW_INTERP_NN
 F32, F16 = L.TW_DF_F32, L.TW_DF_F16
+# Grid upsampling basis (Grid.basis); DF_LINEAR is OR-ed into the interp
+# argument of the one-call warps to select BASIS_LINEAR for their grids.
+BASIS_BSPLINE, BASIS_LINEAR = L.TW_BASIS_BSPLINE, L.TW_BASIS_LINEAR
+DF_LINEAR = L.TW_DF_LINEAR
 
 
 class TeraWarpError(RuntimeError):
@@ -124,7 +128,7 @@ def mode_matrix(mode, para, size_xyz):
 class Grid:
     """A displacement grid (3 planes) in host (numpy) or device (torch) memory."""
 
-    def __init__(self, warper, data, gshape_xyz, f, origin, storage):
+    def __init__(self, warper, data, gshape_xyz, f, origin, storage, basis=BASIS_BSPLINE):
         self._w = warper
         self.data = data
         self.g = L.TwGrid()
@@ -133,6 +137,7 @@ class Grid:
         self.g.f = f
         self.g.ox, self.g.oy, self.g.oz = origin
         self.g.storage = storage
+        self.g.basis = basis
         if _is_torch(data):
             self.g.data = data.data_ptr()
             self.g.mem = L.TW_MEM_DEVICE
@@ -229,7 +234,7 @@ class Warper:
         self._check(self.lib.tw_tps_fit(self.ctx, _fptr(ce), _fptr(ta), K, lam, precision, _dptr(c), _dptr(d)))
         return c, d
 
-    def new_grid(self, gshape_xyz, f, origin=(0, 0, 0), storage=F32, device=True):
+    def new_grid(self, gshape_xyz, f, origin=(0, 0, 0), storage=F32, device=True, basis=BASIS_BSPLINE):
         gx, gy, gz = gshape_xyz
         if device:
             torch = _torch()
@@ -239,7 +244,7 @@ class Warper:
             if storage != F32:
                 raise ValueError("host grids are FP32")
             data = np.zeros((3, gz, gy, gx), np.float32)
-        return Grid(self, data, gshape_xyz, f, origin, storage)
+        return Grid(self, data, gshape_xyz, f, origin, storage, basis)
 
     def grid_eval(self, centres, c, d, grid, precision=PLAIN):
         ce = _pts(centres)
@@ -249,10 +254,10 @@ class Warper:
                                               ctypes.byref(grid.g)))
         return grid
 
-    def grid_from_array(self, df, f, origin=(0, 0, 0)):
+    def grid_from_array(self, df, f, origin=(0, 0, 0), basis=BASIS_BSPLINE):
         """Wrap an existing (3, gz, gy, gx) float32 numpy array or device tensor."""
         shape = tuple(df.shape)
-        return Grid(self, df, (shape[3], shape[2], shape[1]), f, origin, F32)
+        return Grid(self, df, (shape[3], shape[2], shape[1]), f, origin, F32, basis)
 
     def resample_tps(self, src, grid, dst=None, interp=LINEAR, precision=PLAIN, src_origin=(0, 0, 0),
                      dst_origin=(0, 0, 0), full=None):

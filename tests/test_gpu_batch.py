@@ -99,6 +99,17 @@ def test_batch_equals_single_calls(warper, precision, interp):
         assert np.array_equal(out, ref), (i, it["kind"], it["size"], np.count_nonzero(out != ref))
 
 
+def test_batch_linear_df_equals_single_calls(warper):
+    """interp | TW_DF_LINEAR: every TPS item upsampled linearly, as its single call."""
+    items = make_items(seed=21, n=8)
+    interp = warp.LINEAR | warp.DF_LINEAR
+    outs, status, _, srcs = run_batch(warper, items, 8, warp.MIXED, interp)
+    assert status == [0] * len(items)
+    for i, (it, out, src) in enumerate(zip(items, outs, srcs)):
+        ref = single(warper, it, src, 8, warp.MIXED, interp if it["kind"] == "tps" else warp.LINEAR)
+        assert np.array_equal(out, ref), (i, it["kind"], np.count_nonzero(out != ref))
+
+
 @pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
 def test_batch_matches_oracle(warper, oracle, precision):
     items = make_items(seed=11, n=8, nc2=False)

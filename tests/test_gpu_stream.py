@@ -36,7 +36,7 @@ def in_memory(warper, vol, tar, sub, f, precision, interp):
 
 
 @pytest.mark.parametrize("precision,interp", [(warp.PLAIN, warp.LINEAR), (warp.MIXED, warp.LINEAR),
-                                              (warp.MIXED, warp.NN)])
+                                              (warp.MIXED, warp.NN), (warp.MIXED, warp.LINEAR | warp.DF_LINEAR)])
 def test_stream_equals_in_memory_warp(warper, precision, interp):
     size, brick, f = (200, 150, 90), 64, 8          # 4 x 3 x 2 bricks, uneven on every axis
     s = Synth(size, np.uint16, seed=4, cell=64, blobs_per_cell=4)

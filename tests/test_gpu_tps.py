@@ -252,6 +252,50 @@ def test_source_window_equals_full_source(warper, oracle, precision):
         assert np.array_equal(dst, ref), ((x0, y0, z0), np.count_nonzero(dst != ref))
 
 
+def test_linear_df_upsample_and_resample(warper, oracle):
+    """TW_BASIS_LINEAR grids (the reference's declared linear DF mode): the
+    FP64 reference-order form is bit-identical to the oracle's linear upsample
+    and warp; the plain form is within 1e-5 voxel / 1 LSB."""
+    size = (131, 77, 45)
+    g = 4
+    vol, sub, tar = case(size, np.uint16, 40, 5.0, seed=23)
+    c, d = oracle.stps_fit(tar, sub)
+    gsh = tuple(warp.grid_dim(s, g) for s in size)
+    df = oracle.grid_eval(tar, c, d, gsh, g)
+    grid = warper.grid_from_array(df, g, basis=warp.BASIS_LINEAR)
+    ref_df = oracle.dense_df(size, df, g, kind=1)
+    ex = warper.upsample_df(grid, size, precision=warp.EXACT)
+    assert np.array_equal(ex, ref_df), np.abs(ex - ref_df).max()
+    pl = warper.upsample_df(grid, size, precision=warp.PLAIN)
+    assert np.abs(pl - ref_df).max() < 1e-5
+    for interp in (warp.LINEAR, warp.NN):
+        ref = oracle.tps_resample(vol, df, g, interp=interp | oracle.DF_LINEAR)
+        got = warper.resample_tps(vol, grid, interp=interp, precision=warp.EXACT)
+        assert np.array_equal(got, ref), (interp, np.count_nonzero(got != ref))
+    ref = oracle.tps_resample(vol, df, g, interp=oracle.DF_LINEAR)
+    got = warper.resample_tps(vol, grid, precision=warp.PLAIN)
+    diff = np.abs(got.astype(np.int64) - ref.astype(np.int64))
+    assert diff.max() <= 1 and np.mean(diff == 0) >= 0.999
+    # The B-spline reading of the same grid is a different field.
+    spl = warper.resample_tps(vol, warper.grid_from_array(df, g), precision=warp.EXACT)
+    assert not np.array_equal(spl, ref)
+
+
+def test_linear_df_chain_vs_oracle(warper, oracle):
+    """One-call warp with interp | TW_DF_LINEAR against the oracle chain."""
+    size, g = (300, 260, 70), 8
+    vol, sub, tar = case(size, np.uint16, 128, 20.0, seed=31, n_blobs=120)
+    ref = oracle.warp_tps(vol, tar, sub, g=g, interp=oracle.DF_LINEAR)[0]
+    out = warper.warp_tps(vol, tar, sub, f=g, interp=warp.LINEAR | warp.DF_LINEAR, precision=warp.PLAIN)
+    diff = np.abs(out.astype(np.int64) - ref.astype(np.int64))
+    assert np.mean(diff <= 1) >= 0.9999 and np.mean(diff == 0) >= 0.999, (diff.max(), np.mean(diff == 0))
+    outm = warper.warp_tps(vol, tar, sub, f=g, interp=warp.LINEAR | warp.DF_LINEAR, precision=warp.MIXED)
+    dm_ = np.abs(outm.astype(np.int64) - ref.astype(np.int64))
+    assert np.mean(dm_ <= 1) >= 0.99
+    with pytest.raises(warp.TeraWarpError):
+        warper.warp_tps(vol, tar, sub, f=g, interp=2 | warp.DF_LINEAR)
+
+
 def test_resample_rejects_uncovered_grid(warper):
     vol = np.zeros((20, 20, 20), np.uint8)
     grid = warper.grid_from_array(np.zeros((3, 5, 5, 5), np.float32), 4)

@@ -276,3 +276,89 @@ def test_window_resample_matches_whole_volume(oracle):
     w = oracle.tps_resample_window(tight, (10, 6, 14), size, dfw, g, d0, dsz, d0)
     changed = w != part
     assert changed.any() and np.all(w[changed] == 0)
+
+
+# ---- linear DF upsample (the reference's i_interpmethod_df = 0) ---------------
+# The reference declares the mode but never defines it; the oracle defines it on
+# the B-spline's block layout (voxel b*n + t between nodes b+1 and b+2).  These
+# pins fix that definition by its closed form, its interpolation of the nodes,
+# linear precision and a brute-force trilinear evaluation.
+
+def test_linear_basis_closed_form(oracle):
+    for n in (1, 2, 3, 4, 5, 8):
+        TB, basis = oracle.df_basis(n, 1)
+        t = np.arange(n) * (1.0 / n)  # t = step * i, as the B-spline's T rows
+        ref = np.stack([0 * t, 1 - t, t, 0 * t], 1)
+        assert np.array_equal(TB, ref)
+        assert np.allclose(basis.sum(1), 1.0, atol=1e-15)
+        assert np.count_nonzero(basis, axis=1).max() <= 8  # a trilinear stencil
+    # kind 0 is the B-spline
+    assert np.array_equal(oracle.df_basis(4, 0)[0], oracle.bspline_basis(4)[0])
+
+
+def test_linear_df_interpolates_the_nodes(oracle, rng):
+    """Voxel (i n, j n, k n) sits on node (i+1, j+1, k+1): its DF is that node's."""
+    n = 4
+    size = (21, 13, 9)
+    gsh = tuple(int(np.ceil(s / n)) + 3 for s in size)
+    df = rng.uniform(-5, 5, (3,) + gsh[::-1]).astype(np.float32)
+    dense = oracle.dense_df(size, df, n, kind=1)
+    for k in range(0, size[2], n):
+        for j in range(0, size[1], n):
+            for i in range(0, size[0], n):
+                assert np.array_equal(dense[:, k, j, i], df[:, k // n + 1, j // n + 1, i // n + 1])
+
+
+def test_linear_df_linear_precision(oracle, rng):
+    n = 4
+    size = (23, 19, 14)
+    gsh = tuple(int(np.ceil(s / n)) + 3 for s in size)
+    a = rng.uniform(-3, 3, 3)
+    B = rng.uniform(-0.05, 0.05, (3, 3))
+    k, j, i = np.meshgrid(np.arange(gsh[2]), np.arange(gsh[1]), np.arange(gsh[0]), indexing="ij")
+    node = np.stack([(i - 1) * n, (j - 1) * n, (k - 1) * n], 0).astype(np.float64)
+    df = (np.einsum("ab,bzyx->azyx", B, node) + a[:, None, None, None]).astype(np.float32)
+    dense = oracle.dense_df(size, df, n, kind=1)
+    z, y, x = np.meshgrid(np.arange(size[2]), np.arange(size[1]), np.arange(size[0]), indexing="ij")
+    vox = np.stack([x, y, z], 0).astype(np.float64)
+    expect = np.einsum("ab,bzyx->azyx", B, vox) + a[:, None, None, None]
+    assert np.abs(dense - expect).max() < 2e-6
+
+
+def test_linear_df_matches_brute_force_trilinear(oracle, rng):
+    """Every voxel against a direct trilinear evaluation between its two
+    surrounding nodes per axis (node i at (i - 1) n)."""
+    n = 3
+    size = (10, 8, 7)
+    gsh = tuple(int(np.ceil(s / n)) + 3 for s in size)
+    df = rng.uniform(-4, 4, (3,) + gsh[::-1]).astype(np.float32)
+    dense = oracle.dense_df(size, df, n, kind=1)
+    for z in range(size[2]):
+        for y in range(size[1]):
+            for x in range(size[0]):
+                w = []
+                for v in (x, y, z):
+                    b, t = v // n, (v % n) / n
+                    w.append(((b + 1, 1 - t), (b + 2, t)))
+                s = np.zeros(3)
+                for (iz, wz) in w[2]:
+                    for (iy, wy) in w[1]:
+                        for (ix, wx) in w[0]:
+                            s += wz * wy * wx * df[:, iz, iy, ix].astype(np.float64)
+                assert np.allclose(dense[:, z, y, x], s, rtol=0, atol=4e-6), (x, y, z)
+
+
+def test_linear_df_resample(oracle, rng):
+    """The resample with the linear DF flag: a zero field is the identity, and
+    a field that is constant on the nodes shifts like the B-spline's (both
+    reproduce constants)."""
+    size = (18, 14, 9)
+    v = rng.integers(0, 65536, size[::-1]).astype(np.uint16)
+    flag = oracle.DF_LINEAR
+    assert np.array_equal(oracle.tps_resample(v, _const_grid(size, 4, (0, 0, 0)), 4, interp=flag), v)
+    g = _const_grid(size, 4, (0.25, -0.5, 0.75))
+    assert np.array_equal(oracle.tps_resample(v, g, 4, interp=flag), oracle.tps_resample(v, g, 4))
+    # a non-constant field differs between the two bases
+    gsh = tuple(int(np.ceil(s / 4)) + 3 for s in size)
+    df = rng.uniform(-2, 2, (3,) + gsh[::-1]).astype(np.float32)
+    assert not np.array_equal(oracle.tps_resample(v, df, 4, interp=flag), oracle.tps_resample(v, df, 4))
