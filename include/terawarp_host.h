@@ -47,9 +47,11 @@ const char* tw_io_last_error(void);
 /* ---- tera output layout ---------------------------------------------------------
  * The tera-block driver's files (q_imgwarp_tps_quicksmallmemory.cpp:1239-1241,
  * 1863-1875; Bigwarp.cpp:960-1000): one Vaa3D raw stack per output block,
- *   <base>_stps_block/ZZZZZZ_YYYYYY_XXXXXX.v3draw  (offsets x 10, 6 digits),
+ *   <base>_block/ZZZZZZ_YYYYYY_XXXXXX.v3draw  (offsets x 10, 6 digits),
  * and one single-slice stack per z,
- *   <base>_stps_slice/ZZZZZZZ.raw                   (z x 10, 7 digits). */
+ *   <base>_slice/ZZZZZZZ.raw                   (z x 10, 7 digits),
+ * where <base> carries the pass's suffix (<out>/<name>_stps, _affine,
+ * <...>_downsample; q_warp_affine_tps.cpp:914-916, 1545-1547). */
 typedef struct {
     const char* base;
     int32_t     dtype;  /* TW_U8 / TW_U16 */
@@ -64,6 +66,26 @@ int tw_block_file_write(void* user, int64_t brick, int64_t x0, int64_t y0, int64
 /* Read the block files of a volume of nx x ny x nz in bricks of `brick` and
  * write its slices. */
 int tw_stitch_slices(const char* base, int64_t nx, int64_t ny, int64_t nz, int32_t brick, int32_t dtype);
+/* ---- slice-directory input ------------------------------------------------------
+ * A volume stored as one single-slice Vaa3D raw stack per z in a directory:
+ * every entry except ".", "..", "mdata.bin" and ".iim.format", sorted by
+ * name, is one slice (read_2Dtif_dir, dir_make.cpp:175-217; the slices'
+ * header read as loadMetadata_cyf, RawFmtMngr.cpp:332-446).  All slices must
+ * share the first one's extent and datatype (uint8 / uint16). */
+typedef struct tw_slice_dir tw_slice_dir;
+int  tw_slice_dir_open(const char* dir, tw_slice_dir** out);
+int  tw_slice_dir_size(const tw_slice_dir* d, int64_t* nx, int64_t* ny, int64_t* nz, int32_t* dtype);
+/* tw_read_fn (include/terawarp.h) over an open slice directory (user =
+ * tw_slice_dir*); safe to call from several threads at once. */
+int  tw_slice_dir_read(void* user, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny, int64_t nz,
+                       void* out);
+void tw_slice_dir_close(tw_slice_dir* d);
+
+/* Downsample factor of the preview pass (q_warp_affine_tps.cpp:1477-1479):
+ * with m the largest extent, m / 600 rounded half up; 0 when m < 300 (the
+ * reference then divides by zero). */
+int  tw_downsample_factor(int64_t nx, int64_t ny, int64_t nz);
+
 const char* tw_tera_last_error(void);
 
 /* ---- plugin surface ----------------------------------------------------------

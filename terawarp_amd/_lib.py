@@ -48,6 +48,7 @@ HOST_EXPORTS = (
     "tw_io_last_error", "tw_plugin_menu_count", "tw_plugin_menu_item", "tw_plugin_func_count",
     "tw_plugin_func_item", "tw_plugin_version", "tw_plugin_parse_params", "tw_plugin_dofunc", "tw_bigimagewarp",
     "tw_tera_block_path", "tw_tera_slice_path", "tw_block_file_write", "tw_stitch_slices", "tw_tera_last_error",
+    "tw_slice_dir_open", "tw_slice_dir_size", "tw_slice_dir_read", "tw_slice_dir_close", "tw_downsample_factor",
 )
 
 _i64 = ctypes.c_int64
@@ -168,6 +169,12 @@ def load():
         "tw_block_file_write": (ctypes.c_int, [_P, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _P]),
         "tw_stitch_slices": (ctypes.c_int, [ctypes.c_char_p, _i64, _i64, _i64, _i32, _i32]),
         "tw_tera_last_error": (ctypes.c_char_p, []),
+        "tw_slice_dir_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+        "tw_slice_dir_size": (ctypes.c_int, [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                             ctypes.POINTER(_i32)]),
+        "tw_slice_dir_read": (ctypes.c_int, [_P, _i64, _i64, _i64, _i64, _i64, _i64, _P]),
+        "tw_slice_dir_close": (None, [_P]),
+        "tw_downsample_factor": (ctypes.c_int, [_i64, _i64, _i64]),
         "tw_v3draw_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "tw_v3draw_read": (ctypes.c_int, [ctypes.c_char_p, _P, _i64]),

@@ -5,6 +5,8 @@
 // and bigimagewarp (123-336: marker check and read, subject image read,
 // output naming <out>/<basename>_affine|_stps.v3draw, dispatch to the
 // affine or TPS driver).  The compute runs on the GPU through terawarp.h.
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -60,6 +62,212 @@ struct Ctx {
         if (c) tw_destroy(c);
     }
 };
+
+struct SliceDir {
+    tw_slice_dir* d = nullptr;
+    ~SliceDir() {
+        if (d) tw_slice_dir_close(d);
+    }
+};
+
+// Output blocks of the tera drivers: 100^3 for the warps
+// (q_imgwarp_tps_quicksmallmemory.cpp:1256, q_warp_affine_tps.cpp:896),
+// 500^3 for the downsample pass (q_warp_affine_tps.cpp:1521).
+constexpr int kTeraBlock = 100, kDownBlock = 500;
+
+// Affine / rigid warp of a slice directory block by block
+// (q_imagewarp_affine_cyf, q_warp_affine_tps.cpp:851-1329): each output
+// block reads the source window that its image under M spans (the bounding
+// box of the eight transformed corners, one voxel of margin for the
+// rounding of the positions), resamples it on the GPU and is written as a
+// block file.
+int affine_blocks(tw_ctx* ctx, tw_slice_dir* sd, const double* M, int64_t nx, int64_t ny, int64_t nz, int32_t dtype,
+                  const std::string& base) {
+    const int bpv = dtype == TW_U8 ? 1 : 2;
+    const tw_block_files bf = {base.c_str(), dtype, 0};
+    const int64_t nbx = (nx + kTeraBlock - 1) / kTeraBlock, nby = (ny + kTeraBlock - 1) / kTeraBlock,
+                  nbz = (nz + kTeraBlock - 1) / kTeraBlock;
+    const int64_t full[3] = {nx, ny, nz};
+    std::vector<uint8_t> win, out;
+    int64_t brick = 0;
+    for (int64_t bz = 0; bz < nbz; bz++)
+        for (int64_t by = 0; by < nby; by++)
+            for (int64_t bx = 0; bx < nbx; bx++, brick++) {
+                const int64_t o[3] = {bx * kTeraBlock, by * kTeraBlock, bz * kTeraBlock};
+                const int64_t n[3] = {std::min<int64_t>(kTeraBlock, nx - o[0]), std::min<int64_t>(kTeraBlock, ny - o[1]),
+                                      std::min<int64_t>(kTeraBlock, nz - o[2])};
+                double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                for (int c = 0; c < 8; c++) {
+                    const double p[3] = {(double)(o[0] + ((c & 1) ? n[0] - 1 : 0)),
+                                         (double)(o[1] + ((c & 2) ? n[1] - 1 : 0)),
+                                         (double)(o[2] + ((c & 4) ? n[2] - 1 : 0))};
+                    for (int a = 0; a < 3; a++) {
+                        const double q = M[4 * a] * p[0] + M[4 * a + 1] * p[1] + M[4 * a + 2] * p[2] + M[4 * a + 3];
+                        lo[a] = std::min(lo[a], q);
+                        hi[a] = std::max(hi[a], q);
+                    }
+                }
+                int64_t w0[3], wn[3];
+                bool empty = false;
+                for (int a = 0; a < 3; a++) {
+                    const double l = std::floor(lo[a]) - 1, h = std::ceil(hi[a]) + 2;  // [l, h)
+                    w0[a] = (int64_t)std::max(0.0, std::min(l, (double)full[a]));
+                    const int64_t e = (int64_t)std::max(0.0, std::min(h, (double)full[a]));
+                    wn[a] = e - w0[a];
+                    empty |= wn[a] < 1;
+                }
+                out.assign((size_t)(n[0] * n[1] * n[2] * bpv), 0);
+                if (!empty) {
+                    win.resize((size_t)(wn[0] * wn[1] * wn[2] * bpv));
+                    int rc = tw_slice_dir_read(sd, w0[0], w0[1], w0[2], wn[0], wn[1], wn[2], win.data());
+                    if (rc) return fail(std::string("slice read failed: ") + tw_tera_last_error()), rc;
+                    tw_volume vs = {}, vd = {};
+                    vs.data = win.data();
+                    vs.nx = wn[0]; vs.ny = wn[1]; vs.nz = wn[2]; vs.nc = 1;
+                    vs.x0 = w0[0]; vs.y0 = w0[1]; vs.z0 = w0[2];
+                    vs.full_nx = nx; vs.full_ny = ny; vs.full_nz = nz;
+                    vs.dtype = dtype;
+                    vs.mem = TW_MEM_HOST;
+                    vd = vs;
+                    vd.data = out.data();
+                    vd.nx = n[0]; vd.ny = n[1]; vd.nz = n[2];
+                    vd.x0 = o[0]; vd.y0 = o[1]; vd.z0 = o[2];
+                    vd.full_nx = vd.full_ny = vd.full_nz = 0;
+                    if ((rc = tw_resample_affine(ctx, &vs, &vd, M, TW_INTERP_LINEAR)))
+                        return fail(std::string("warp failed: ") + tw_last_error(ctx)), rc;
+                }
+                int rc = tw_block_file_write((void*)&bf, brick, o[0], o[1], o[2], n[0], n[1], n[2], out.data());
+                if (rc) return fail(tw_tera_last_error()), rc;
+            }
+    return TW_OK;
+}
+
+// Preview pass (q_imagewarp_affine_cyf_downsample, q_warp_affine_tps.cpp:
+// 1474-2029): the warped slices of <base>_slice downsampled by the factor f
+// of tw_downsample_factor to extent n / f, written as 500^3 block files and
+// slices under <base>_downsample.  The reference estimates the matrix by
+// its DLT from the target markers and their f-scaled copies, which is
+// diag(f, f, f) up to ~1e-12 rounding; the exact matrix is used here, so an
+// output voxel is the warped voxel at f times its coordinates.  The slices
+// f*k are gathered slab by slab and resampled on the GPU with
+// diag(f, f, 1).
+int downsample_pass(tw_ctx* ctx, const std::string& base, int64_t nx, int64_t ny, int64_t nz, int32_t dtype) {
+    const int f = tw_downsample_factor(nx, ny, nz);
+    if (f < 1) return fail("downsample: the largest extent is below 300 voxels (factor 0)"), TW_E_ARG;
+    const int64_t onx = nx / f, ony = ny / f, onz = nz / f;
+    if (onx < 1 || ony < 1 || onz < 1) return fail("downsample: an extent is below the factor"), TW_E_ARG;
+    SliceDir sd;
+    int rc = tw_slice_dir_open((base + "_slice").c_str(), &sd.d);
+    if (rc) return fail(tw_tera_last_error()), rc;
+    int64_t snx, sny, snz;
+    int32_t sdt;
+    tw_slice_dir_size(sd.d, &snx, &sny, &snz, &sdt);
+    if (snx != nx || sny != ny || snz != nz || sdt != dtype) return fail("downsample: warped slices differ"), TW_E_IO;
+    const int bpv = dtype == TW_U8 ? 1 : 2;
+    std::vector<uint8_t> vol((size_t)(onx * ony * onz * bpv)), slab;
+    const int64_t ks_max = std::max<int64_t>(1, std::min<int64_t>(onz, (1ll << 30) / (nx * ny * bpv)));
+    const double M[12] = {(double)f, 0, 0, 0, 0, (double)f, 0, 0, 0, 0, 1, 0};
+    for (int64_t k0 = 0; k0 < onz; k0 += ks_max) {
+        const int64_t ks = std::min(ks_max, onz - k0);
+        slab.resize((size_t)(nx * ny * ks * bpv));
+        for (int64_t k = 0; k < ks; k++)
+            if ((rc = tw_slice_dir_read(sd.d, 0, 0, (k0 + k) * f, nx, ny, 1, slab.data() + (size_t)(k * nx * ny * bpv))))
+                return fail(tw_tera_last_error()), rc;
+        tw_volume vs = {}, vd = {};
+        vs.data = slab.data();
+        vs.nx = nx; vs.ny = ny; vs.nz = ks; vs.nc = 1;
+        vs.dtype = dtype;
+        vs.mem = TW_MEM_HOST;
+        vd = vs;
+        vd.data = vol.data() + (size_t)(k0 * onx * ony * bpv);
+        vd.nx = onx; vd.ny = ony; vd.nz = ks;
+        if ((rc = tw_resample_affine(ctx, &vs, &vd, M, TW_INTERP_LINEAR)))
+            return fail(std::string("downsample failed: ") + tw_last_error(ctx)), rc;
+    }
+    const std::string dbase = base + "_downsample";
+    const tw_block_files bf = {dbase.c_str(), dtype, 0};
+    std::vector<uint8_t> blk;
+    int64_t brick = 0;
+    for (int64_t z0 = 0; z0 < onz; z0 += kDownBlock)
+        for (int64_t y0 = 0; y0 < ony; y0 += kDownBlock)
+            for (int64_t x0 = 0; x0 < onx; x0 += kDownBlock, brick++) {
+                const int64_t bnx = std::min<int64_t>(kDownBlock, onx - x0), bny = std::min<int64_t>(kDownBlock, ony - y0),
+                              bnz = std::min<int64_t>(kDownBlock, onz - z0);
+                blk.resize((size_t)(bnx * bny * bnz * bpv));
+                for (int64_t z = 0; z < bnz; z++)
+                    for (int64_t y = 0; y < bny; y++)
+                        std::memcpy(blk.data() + (size_t)(((z * bny) + y) * bnx * bpv),
+                                    vol.data() + (size_t)((((z0 + z) * ony + (y0 + y)) * onx + x0) * bpv),
+                                    (size_t)(bnx * bpv));
+                if ((rc = tw_block_file_write((void*)&bf, brick, x0, y0, z0, bnx, bny, bnz, blk.data())))
+                    return fail(tw_tera_last_error()), rc;
+            }
+    if ((rc = tw_stitch_slices(dbase.c_str(), onx, ony, onz, kDownBlock, dtype))) return fail(tw_tera_last_error()), rc;
+    return TW_OK;
+}
+
+// Tera-scale branch of bigimagewarp (plugin_Bigimagewarp.cpp:211-330): the
+// subject is a directory of slices; the warp writes <out>/<dir><suffix>_block
+// files and <out>/<dir><suffix>_slice slices, then, with downsample_mode,
+// the preview pass.  TPS runs through the streaming driver (one fit, one
+// grid, exact per-brick source windows); affine and rigid warps block by
+// block.
+int tera_branch(const std::string& dir, const char* out_dir, const tw_warp_params* p, const std::vector<float>& tar,
+                const std::vector<float>& sub, int nm, const char* suffix, char* out_path, int out_cap) {
+    SliceDir sd;
+    int rc = tw_slice_dir_open(dir.c_str(), &sd.d);
+    if (rc) return fail(tw_tera_last_error()), rc;
+    int64_t nx, ny, nz;
+    int32_t dtype;
+    tw_slice_dir_size(sd.d, &nx, &ny, &nz, &dtype);
+    std::string name = dir;
+    while (name.size() > 1 && name.back() == '/') name.pop_back();
+    name = name.substr(name.find_last_of('/') == std::string::npos ? 0 : name.find_last_of('/') + 1);
+    const std::string base = std::string(out_dir) + "/" + name + suffix;
+    Ctx ctx;
+    if ((rc = tw_create(0, &ctx.c))) return fail("no usable GPU (terawarp runs on the MI355X only)"), rc;
+    const int precision = p->device_mode == 0 ? TW_EXACT : (p->device_mode == 1 ? TW_PLAIN : TW_MIXED);
+    if (p->kind == 1) {
+        // imgwarp_smallmemory_CYF(sub, tar, 4, 4, 4, ...) (294): centres are the
+        // target-space markers.
+        const tw_block_files bf = {base.c_str(), dtype, 0};
+        tw_stream_desc d = {};
+        d.nx = nx; d.ny = ny; d.nz = nz;
+        d.dtype = dtype;
+        d.brick = kTeraBlock;
+        d.f = 4;
+        d.interp = TW_INTERP_LINEAR;
+        d.precision = precision;
+        d.threads = 8;
+        d.K = nm;
+        d.lambda = 0.2;
+        d.centres = tar.data();
+        d.targets = sub.data();
+        d.read = tw_slice_dir_read;
+        d.read_user = sd.d;
+        d.write = tw_block_file_write;
+        d.write_user = (void*)&bf;
+        if ((rc = tw_warp_stream(ctx.c, &d, nullptr))) return fail(std::string("warp failed: ") + tw_last_error(ctx.c)), rc;
+    } else {
+        double M[16];
+        if (p->kind == 0) rc = tw_affine_fit(tar.data(), sub.data(), nm, M);
+        else rc = tw_mode_matrix(p->kind == 2 ? 2 : (p->kind == 3 ? 0 : 1), p->para, nx, ny, nz, M);
+        if (rc) return fail("cannot build the transform matrix (degenerate markers or parameters)"), rc;
+        if ((rc = affine_blocks(ctx.c, sd.d, M, nx, ny, nz, dtype, base))) return rc;
+    }
+    if ((rc = tw_stitch_slices(base.c_str(), nx, ny, nz, kTeraBlock, dtype))) return fail(tw_tera_last_error()), rc;
+    if (p->downsample && (rc = downsample_pass(ctx.c, base, nx, ny, nz, dtype))) return rc;
+    if (out_path && out_cap > 0) {
+        std::strncpy(out_path, base.c_str(), (size_t)out_cap - 1);
+        out_path[out_cap - 1] = '\0';
+    }
+    return TW_OK;
+}
+
+bool is_dir(const std::string& path) {
+    struct stat st;
+    return stat(path.c_str(), &st) == 0 && S_ISDIR(st.st_mode);
+}
 
 }  // namespace
 
@@ -152,10 +360,12 @@ int tw_bigimagewarp(const char* img_path, const char* marker_sub, const char* ma
         tw_markers_read(marker_sub, sub.data(), nsub);
         tw_markers_read(marker_tar, tar.data(), ntar);
     }
-    // 2. subject image (only the in-memory .v3draw path; 155-210)
+    static const char* const suffix[] = {"_affine", "_stps", "_scaling", "_rotation", "_translation"};
+    // 2. subject image: a directory of slices takes the tera branch (211-330),
+    // a .v3draw stack the in-memory one (155-210).
+    if (is_dir(img)) return tera_branch(img, out_dir, p, tar, sub, nsub, suffix[p->kind], out_path, out_cap);
     if (!ends_with(img, ".v3draw"))
-        return fail("slice-directory (tera-scale) input is not part of this build; pass a .v3draw stack"),
-               TW_E_UNSUPPORTED;
+        return fail("the subject must be a .v3draw stack or a directory of slices"), TW_E_ARG;
     int64_t sz[4];
     int dt = 0, sw = 0, hl = 0;
     if (tw_v3draw_header(img_path, sz, &dt, &sw, &hl)) return fail(tw_io_last_error()), TW_E_IO;
@@ -171,7 +381,6 @@ int tw_bigimagewarp(const char* img_path, const char* marker_sub, const char* ma
     const size_t slash = img.find_last_of('/');
     const size_t b0 = slash == std::string::npos ? 0 : slash + 1;
     const std::string base = img.substr(b0, img.find_last_of('.') - b0);
-    static const char* const suffix[] = {"_affine", "_stps", "_scaling", "_rotation", "_translation"};
     const std::string out = std::string(out_dir) + "/" + base + suffix[p->kind] + ".v3draw";
 
     Ctx ctx;

@@ -81,3 +81,62 @@ def dofunc(func_name, inputs, paras, outputs):
 
 def last_error():
     return _err()
+
+
+# ---- slice directories (the tera driver's input and output) -----------------
+
+def write_slices(directory, vol, name="{:07d}.raw"):
+    """Write a (z, y, x) volume as one single-slice Vaa3D raw stack per z."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    for z in range(vol.shape[0]):
+        write_v3draw(os.path.join(directory, name.format(z * 10)), vol[z:z + 1])
+
+
+def _terr():
+    return L.load().tw_tera_last_error().decode(errors="replace")
+
+
+class SliceDir:
+    """An open slice directory (tw_slice_dir): size, windows, and a read
+    callback for the streaming driver."""
+
+    def __init__(self, directory):
+        self.lib = L.load()
+        self.h = ctypes.c_void_p()
+        if self.lib.tw_slice_dir_open(str(directory).encode(), ctypes.byref(self.h)) != 0:
+            raise HostError(_terr())
+        nx, ny, nz, dt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        self.lib.tw_slice_dir_size(self.h, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz), ctypes.byref(dt))
+        self.size = (nx.value, ny.value, nz.value)
+        self.dtype = np.dtype(np.uint8 if dt.value == L.TW_U8 else np.uint16)
+
+    def read(self, x0, y0, z0, nx, ny, nz):
+        out = np.empty((nz, ny, nx), self.dtype)
+        if self.lib.tw_slice_dir_read(self.h, x0, y0, z0, nx, ny, nz, out.ctypes.data) != 0:
+            raise HostError(_terr())
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.tw_slice_dir_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_slices(directory):
+    d = SliceDir(directory)
+    try:
+        nx, ny, nz = d.size
+        return d.read(0, 0, 0, nx, ny, nz)
+    finally:
+        d.close()
+
+
+def downsample_factor(nx, ny, nz):
+    return int(L.load().tw_downsample_factor(nx, ny, nz))

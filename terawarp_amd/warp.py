@@ -360,9 +360,14 @@ class Warper:
         d.nx, d.ny, d.nz, d.dtype, d.brick, d.f = nx, ny, nz, dt, brick, f
         d.interp, d.precision, d.threads, d.K, d.lam = interp, precision, threads, ce.shape[0], lam
         d.centres, d.targets = _fptr(ce), _fptr(ta)
+        from .host import SliceDir
         if isinstance(source, Synth):
             d.read = ctypes.cast(self.lib.tw_synth_read, ctypes.c_void_p)
             d.read_user = ctypes.cast(ctypes.byref(source.s), ctypes.c_void_p)
+            cb = None
+        elif isinstance(source, SliceDir):
+            d.read = ctypes.cast(self.lib.tw_slice_dir_read, ctypes.c_void_p)
+            d.read_user = source.h
             cb = None
         else:
             def _read(user, x0, y0, z0, bnx, bny, bnz, out):
@@ -415,7 +420,7 @@ def checksum64(a):
 
 
 def stitch_slices(base, size_xyz, brick, dtype):
-    """Write <base>_stps_slice/ZZZZZZZ.raw from the block files (tw_stitch_slices)."""
+    """Write <base>_slice/ZZZZZZZ.raw from the block files <base>_block/ (tw_stitch_slices)."""
     nx, ny, nz = size_xyz
     dt = L.TW_U8 if np.dtype(dtype) == np.uint8 else L.TW_U16
     lib = L.load()

@@ -95,6 +95,46 @@ def test_dofunc_argument_errors(tmp_path):
     assert "marker" in host.last_error()
     assert not host.dofunc("bigimagewarp", [img, m, m2], ten, [str(tmp_path)])
     assert "control points" in host.last_error()
+    # A directory is a slice series (the tera branch): these files are not slices.
     assert not host.dofunc("bigimagewarp", [str(tmp_path), m, m], ten, [str(tmp_path)])
-    assert "slice-directory" in host.last_error()
+    assert "not a Vaa3D raw stack" in host.last_error()
     assert host.dofunc("help", [], [], [])
+
+
+def test_slice_dir_reads_windows(tmp_path, rng):
+    """A directory of single-slice stacks (sorted by name; mdata.bin and
+    .iim.format skipped) reads back any window; big-endian slices are swapped."""
+    vol = rng.integers(0, 65536, (7, 13, 17)).astype(np.uint16)
+    d = tmp_path / "slices"
+    host.write_slices(str(d), vol)
+    (d / "mdata.bin").write_bytes(b"x")
+    (d / ".iim.format").write_bytes(b"y")
+    # slice 3 rewritten big-endian
+    with open(d / "0000030.raw", "wb") as f:
+        f.write(b"raw_image_stack_by_hpeng" + b"B" + struct.pack(">h4i", 2, 17, 13, 1, 1))
+        f.write(vol[3].astype(">u2").tobytes())
+    sd = host.SliceDir(str(d))
+    assert sd.size == (17, 13, 7) and sd.dtype == np.uint16
+    assert np.array_equal(sd.read(0, 0, 0, 17, 13, 7), vol)
+    for (x0, y0, z0, nx, ny, nz) in [(3, 2, 1, 9, 5, 4), (16, 12, 6, 1, 1, 1), (0, 5, 2, 17, 3, 5)]:
+        assert np.array_equal(sd.read(x0, y0, z0, nx, ny, nz), vol[z0:z0 + nz, y0:y0 + ny, x0:x0 + nx])
+    with pytest.raises(host.HostError):
+        sd.read(10, 0, 0, 8, 1, 1)  # past the row
+    sd.close()
+
+
+def test_slice_dir_rejects_mixed_slices(tmp_path):
+    d = tmp_path / "s"
+    host.write_slices(str(d), np.zeros((3, 4, 5), np.uint8))
+    host.write_v3draw(str(d / "0000030.raw"), np.zeros((1, 4, 6), np.uint8))
+    with pytest.raises(host.HostError):
+        host.SliceDir(str(d))
+
+
+def test_downsample_factor():
+    # m / 600 rounded half up, m the largest extent (q_warp_affine_tps.cpp:1477-1479)
+    assert host.downsample_factor(299, 10, 10) == 0
+    assert host.downsample_factor(300, 10, 10) == 1
+    assert host.downsample_factor(10, 899, 10) == 1
+    assert host.downsample_factor(10, 10, 900) == 2
+    assert host.downsample_factor(8192, 8192, 2048) == 14

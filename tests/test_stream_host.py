@@ -53,7 +53,7 @@ def test_tera_block_files_and_slices(tmp_path):
     import ctypes
     from terawarp_amd import _lib as L
     from terawarp_amd.warp import stitch_slices, tera_paths
-    b, s = tera_paths("/out/vol", 512, 1024, 0, z=37)
+    b, s = tera_paths("/out/vol_stps", 512, 1024, 0, z=37)
     assert b == "/out/vol_stps_block/000000_010240_005120.v3draw"
     assert s == "/out/vol_stps_slice/0000370.raw"
     lib = L.load()
