@@ -135,7 +135,7 @@ struct FitState {
     double rnorm, bnorm;
     double ir_prev; // last relative residual
     int ir_iters;   // corrections applied
-    int pad;
+    int sync_timeout;  // a bounded wait between work-groups ran out
     double fit_rmax;  // max |residual| of the bordered system
 };
 
@@ -823,13 +823,139 @@ __global__ __launch_bounds__(STEP_THREADS) void k_solve_step(const T* __restrict
     }
 }
 
+// Pipelined block substitution: one work-group per 64-row block, all
+// resident at once.  Forward: work-group j subtracts L_jk y_k for k < j as
+// each y_k is published (per block a flag, stored with release order after
+// y_k), forms y_j = X_j r_j (X_j the inverted diagonal block) and publishes
+// it.  Backward: x_j = X_j' (y_j - sum_{k>j} L_kj' x_k), published the same
+// way.  The blocks of L do not depend on the solution: each is loaded before
+// the wait for the vector it multiplies.  The critical path is one block
+// product and one flag hand-over per block instead of the single work-group's
+// stream of the whole factor through one CU.  Flags hold the call's epoch
+// (no reset between calls).  Every wait is bounded: one that runs out marks
+// the fit state (reported as a failed fit) and stops waiting.
+constexpr int PIPE_THREADS = 256, PIPE_MAX_BLOCKS = 128;
+
+template <typename T>
+__global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict__ L, const T* __restrict__ Dinv, int n,
+                                                             T* __restrict__ b, T* __restrict__ y, int* flags,
+                                                             int epoch, FitState* st, const int* skip) {
+    if (skip && *skip) return;
+    __shared__ T vs[NB][4];       // the published block being applied (y_k or x_k)
+    __shared__ T part[4][NB][4];  // partial sums of the four quarters
+    __shared__ T accs[NB][4];     // the block's reduced right-hand side
+    __shared__ T ys[NB][4];       // y_j, the backward pass's start
+    __shared__ int timed_out;
+    const int tid = threadIdx.x, r = tid & (NB - 1), q = tid / NB;
+    const int nb = (n + NB - 1) / NB, j = blockIdx.x, j0 = j * NB, ib = min(NB, n - j0);
+    int* fwd = flags;
+    int* bwd = flags + nb;
+    if (tid == 0) timed_out = 0;
+    __syncthreads();
+    auto wait_for = [&](int* f) {
+        if (tid == 0 && !timed_out) {
+            // Bounded by the device clock (100 MHz): 20 ms per wait.
+            const unsigned long long t0 = wall_clock64();
+            while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                if (wall_clock64() - t0 > 2000000ull) {
+                    timed_out = 1;
+                    st->sync_timeout = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    };
+    auto publish = [&](int* f) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(f, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // acc (thread r, quarter q) -> accs[r][c] = sum over the quarters.
+    auto reduce = [&](const T* acc) {
+        #pragma unroll
+        for (int c = 0; c < 4; c++) part[q][r][c] = acc[c];
+        __syncthreads();
+        accs[r][q] = (part[0][r][q] + part[1][r][q]) + (part[2][r][q] + part[3][r][q]);
+        __syncthreads();
+    };
+    const T* X = Dinv + (i64)j * NB * NB;  // row-major; rows past ib are the identity's
+    T acc[4], l[NB / 4];
+    // ---- forward: y_j = X_j (b_j - sum_{k<j} L_jk y_k)
+    #pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = (q == 0 && r < ib) ? b[(i64)c * n + j0 + r] : (T)0;
+    for (int k = 0; k < j; k++) {
+        const int k0 = k * NB;
+        #pragma unroll
+        for (int m = 0; m < NB / 4; m++) l[m] = r < ib ? L[(i64)(k0 + 16 * q + m) * n + j0 + r] : (T)0;
+        wait_for(&fwd[k]);
+        vs[r][q] = y[(i64)q * n + k0 + r];  // block k is full (k < j)
+        __syncthreads();
+        #pragma unroll
+        for (int m = 0; m < NB / 4; m++)
+            #pragma unroll
+            for (int c = 0; c < 4; c++) acc[c] = fma(-l[m], vs[16 * q + m][c], acc[c]);
+        __syncthreads();
+    }
+    reduce(acc);
+    {
+        T s = 0;
+        #pragma unroll 8
+        for (int m = 0; m < NB; m++) s = fma(X[r * NB + m], accs[m][q], s);
+        ys[r][q] = s;
+        if (r < ib) y[(i64)q * n + j0 + r] = s;
+    }
+    publish(&fwd[j]);
+    // ---- backward: x_j = X_j' (y_j - sum_{k>j} L_kj' x_k)
+    #pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = q == 0 ? ys[r][c] : (T)0;
+    for (int k = nb - 1; k > j; k--) {
+        const int k0 = k * NB, ibk = min(NB, n - k0);
+        #pragma unroll
+        for (int m = 0; m < NB / 4; m++)
+            l[m] = (r < ib && 16 * q + m < ibk) ? L[(i64)(j0 + r) * n + k0 + 16 * q + m] : (T)0;
+        wait_for(&bwd[k]);
+        vs[r][q] = r < ibk ? b[(i64)q * n + k0 + r] : (T)0;
+        __syncthreads();
+        #pragma unroll
+        for (int m = 0; m < NB / 4; m++)
+            #pragma unroll
+            for (int c = 0; c < 4; c++) acc[c] = fma(-l[m], vs[16 * q + m][c], acc[c]);
+        __syncthreads();
+    }
+    reduce(acc);
+    {
+        T s = 0;
+        #pragma unroll 8
+        for (int m = 0; m < NB; m++) s = fma(X[m * NB + r], accs[m][q], s);
+        if (r < ib) b[(i64)q * n + j0 + r] = s;
+    }
+    publish(&bwd[j]);
+}
+
 // Systems above this order use the per-block launches.
 constexpr int SOLVE_MULTI_MIN = 1025;
 
-// L L' x = b (n x 4 column-major, in place); y: n x 4 scratch for large n.
+// L L' x = b (n x 4 column-major, in place); y: n x 4 scratch.  Up to
+// PIPE_MAX_BLOCKS blocks the pipelined kernel runs (one launch); above, one
+// launch per block.
 template <typename T>
-int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip) {
-    if (n < SOLVE_MULTI_MIN) {
+int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip, FitState* st) {
+    const int nb = (n + NB - 1) / NB;
+    if (nb <= PIPE_MAX_BLOCKS) {
+        if (!ctx->solve_flags) {
+            void* p = nullptr;
+            TW_HIP(ctx, hipMalloc(&p, 2 * PIPE_MAX_BLOCKS * sizeof(int)));
+            TW_HIP(ctx, hipMemset(p, 0, 2 * PIPE_MAX_BLOCKS * sizeof(int)));
+            ctx->solve_flags = (int*)p;
+        }
+        if (++ctx->solve_epoch == 0) ctx->solve_epoch = 1;  // a new value per call, never the initial 0
+        const int epoch = (int)ctx->solve_epoch;
+        hipLaunchKernelGGL(k_solve_pipe<T>, dim3(nb), dim3(PIPE_THREADS), 0, ctx->stream, L, Dinv, n, b, y,
+                           ctx->solve_flags, epoch, st, skip);
+    } else if (n < SOLVE_MULTI_MIN) {
         hipLaunchKernelGGL(k_chol_solve<T>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, L, Dinv, n, b, skip);
     } else {
         const int nb = (n + NB - 1) / NB;
@@ -1026,6 +1152,7 @@ __global__ void k_ir_init(FitState* st) {
     st->ir_prev = INFINITY;
     st->ir_iters = 0;
     st->info = 0;
+    st->sync_timeout = 0;
 }
 
 __global__ void k_axpy_f32(double* __restrict__ x, const float* __restrict__ d, i64 n, const int* skip) {
@@ -1232,6 +1359,7 @@ __device__ __forceinline__ void fit_small_body(const SmallFitJob& J, SmallFitLds
         st->ir_prev = INFINITY;
         st->ir_iters = 0;
         st->info = 0;
+        st->sync_timeout = 0;
     }
     __syncthreads();
     // 1. QR of X', P = A' V, W.
@@ -1564,7 +1692,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
         cholesky<double>(ctx, B, (double*)dinv, n, st);
         hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
-        return chol_solve<double>(ctx, B, (const double*)dinv, n, z, (double*)ys, nullptr);
+        return chol_solve<double>(ctx, B, (const double*)dinv, n, z, (double*)ys, nullptr, st);
     };
 
     int iters = 0;
@@ -1576,7 +1704,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         hipLaunchKernelGGL(k_build_b<float>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
         cholesky<float>(ctx, B, Df, n, st);
         hipLaunchKernelGGL(k_tail<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, b2, f32a, N);
-        if ((rc = chol_solve<float>(ctx, B, (const float*)Df, n, f32a, (float*)ys, nullptr))) return rc;
+        if ((rc = chol_solve<float>(ctx, B, (const float*)Df, n, f32a, (float*)ys, nullptr, st))) return rc;
         hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, f32a, z,
                            (i64)n * 4);
         TW_LAUNCH_CHECK(ctx);
@@ -1591,7 +1719,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
                 hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U2, P, 0, stop);  // -A'[0;z]
                 hipLaunchKernelGGL(k_ir_residual, dim3(1), dim3(1024), 0, ctx->stream, V, W, U2, P, b2, f32b, st,
                                    N);
-                if ((rc = chol_solve<float>(ctx, B, (const float*)Df, n, f32b, (float*)ys, stop))) return rc;
+                if ((rc = chol_solve<float>(ctx, B, (const float*)Df, n, f32b, (float*)ys, stop, st))) return rc;
                 hipLaunchKernelGGL(k_axpy_f32, dim3(nblk((i64)n * 4)), dim3(256), 0, ctx->stream, z, f32b,
                                    (i64)n * 4, stop);
             }
@@ -1637,6 +1765,7 @@ int fit_finish(tw_ctx* ctx, const FitState* st, double bmax, int iters) {
     if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
     if (hst.info >= 2)
         return tw_fail(ctx, TW_E_SINGULAR, "projected TPS system not positive definite (column %d)", hst.info - 2);
+    if (hst.sync_timeout) return tw_fail(ctx, TW_E_DEVICE, "the solve's work-groups did not meet (wait bound reached)");
     ctx->fit_residual = hst.fit_rmax / bmax;
     ctx->refine_iters = iters;
     if (!std::isfinite(ctx->fit_residual)) return tw_fail(ctx, TW_E_SINGULAR, "non-finite TPS coefficients");

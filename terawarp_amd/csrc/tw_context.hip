@@ -192,6 +192,7 @@ int tw_destroy(tw_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < WS_COUNT; i++)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
+    if (ctx->solve_flags) (void)hipFree(ctx->solve_flags);
     for (int i = 0; i < ST_COUNT; i++) {
         if (ctx->ev_begin[i]) (void)hipEventDestroy(ctx->ev_begin[i]);
         if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);

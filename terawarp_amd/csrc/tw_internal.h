@@ -58,6 +58,10 @@ struct tw_ctx {
     // weights_f (0: none yet); their host image must outlive the copy.
     int weights_f = 0;
     int weights_basis = 0;
+    // Per-block flags of the pipelined solve (2 * 128 ints, zeroed once) and
+    // the epoch its calls count up.
+    int* solve_flags = nullptr;
+    unsigned solve_epoch = 0;
     std::vector<uint8_t> weights_host;
     // Host images of a batch's tables, kept until the next batch (the batch
     // synchronises before it returns).
