@@ -622,6 +622,106 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
     syrk_body<T>(B, n, j0, jb, blockIdx.x, blockIdx.y, S);
 }
 
+// One right-looking step of the blocked Cholesky in one launch, for small
+// systems (latency-bound): after the diagonal block k (rows j0 .. j0+jb-1) is
+// factored and inverted, work-group (bx, by) (by <= bx) of the trailing tiles
+// forms its two panel blocks L_R = B_Rk X_k' and L_C = B_Ck X_k' itself (no
+// separate panel launch; the panel is computed once per tile instead of once
+// per row block), writes L_R to the panel store Lp when by == 0, updates its
+// tile B_RC -= L_R L_C', and tile (0, 0) then factors and inverts the next
+// diagonal block.  The panel input B_Rk is only read in a step (the factor's
+// off-diagonal blocks go to Lp, not over B), so tiles of one row block never
+// race.  Lp: n x n column-major like B; the diagonal blocks stay in B and
+// their inverses in Dinv.
+template <typename T>
+struct StepLds {
+    union {
+        struct {
+            T Xs[NB][NB];      // X_k = L_kk^-1, row-major
+            T As[NB][NB + 1];  // As[c][r] = L_R[r][c]
+            T Bs[NB][NB + 1];  // Bs[c][r] = L_C[r][c]
+        } u;
+        PotrfLds<T> potrf;
+    };
+};
+
+template <typename T>
+__device__ __forceinline__ void step_panel(const T* __restrict__ B, int n, int j0, int jb, int R, const T (*Xs)[NB],
+                                           T (*out)[NB + 1], T* __restrict__ Lp) {
+    const int lane = threadIdx.x & 63;
+    const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = R + lane;
+    const bool ok = i < n;
+    T a[NB];
+    #pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = (ok && j < jb) ? B[(i64)(j0 + j) * n + i] : (T)0;
+    #pragma unroll
+    for (int t = 0; t < NB / 4; t++) {
+        const T* xr = &Xs[4 * t + q][0];
+        T s = 0;
+        #pragma unroll
+        for (int j = 0; j < 4 * t + 4; j++) {
+            s = fma(a[j], xr[j], s);
+            if ((j & 7) == 7) asm volatile("" : "+v"(s)::"memory");  // bound the LDS reads in flight
+        }
+        out[4 * t + q][lane] = s;
+        if (Lp && ok && 4 * t + q < jb) Lp[(i64)(j0 + 4 * t + q) * n + i] = s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_step(T* __restrict__ B, T* __restrict__ Lp, T* __restrict__ Dinv,
+                                                   int n, int j0, int jb, FitState* st) {
+    const int bx = blockIdx.x, by = blockIdx.y;
+    if (by > bx) return;
+    __shared__ StepLds<T> S;
+    auto& Xs = S.u.Xs;
+    auto& As = S.u.As;
+    auto& Bs = S.u.Bs;
+    const int tid = threadIdx.x;
+    const int r0 = j0 + jb, R = r0 + bx * NB, C = r0 + by * NB;
+    const T* D = Dinv + (i64)(j0 / NB) * NB * NB;
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) (&Xs[0][0])[tid + 256 * u] = D[tid + 256 * u];
+    __syncthreads();
+    step_panel<T>(B, n, j0, jb, R, Xs, As, by == 0 ? Lp : nullptr);
+    if (by != bx) step_panel<T>(B, n, j0, jb, C, Xs, Bs, nullptr);
+    __syncthreads();
+    T (*Cs)[NB + 1] = by == bx ? As : Bs;
+    const int tx = tid % 16, ty = tid / 16;
+    T acc[4][4];
+    #pragma unroll
+    for (int i = 0; i < 4; i++)
+        #pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+    #pragma unroll 8
+    for (int k = 0; k < NB; k++) {
+        T av[4], bv[4];
+        #pragma unroll
+        for (int i = 0; i < 4; i++) av[i] = As[k][tx + 16 * i];
+        #pragma unroll
+        for (int j = 0; j < 4; j++) bv[j] = Cs[k][ty + 16 * j];
+        #pragma unroll
+        for (int i = 0; i < 4; i++)
+            #pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    #pragma unroll
+    for (int j = 0; j < 4; j++)
+        #pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = R + tx + 16 * i, c = C + ty + 16 * j;
+            if (r < n && c < n && c <= r) B[(i64)c * n + r] -= acc[i][j];
+        }
+    if (bx == 0 && by == 0) {
+        __syncthreads();  // the updated tile is in B; the step's LDS is free
+        potrf_diag_body<T>(B, Dinv, n, r0, min(NB, n - r0), st, S.potrf);
+    }
+}
+
+// Systems of at most this many blocks factor with the fused steps.
+constexpr int STEP_MAX_BLOCKS = 16;
+
 constexpr int SOLVE_THREADS = 512;
 constexpr int BUF_FLAGS = 0x00020000;  // raw buffer, gfx9-family data format
 
@@ -1226,6 +1326,17 @@ __global__ __launch_bounds__(1024) void k_res_max(const double* __restrict__ res
 
 inline unsigned nblk(i64 n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// Blocked Cholesky with the fused steps (small systems): the off-diagonal
+// blocks of the factor go to Lp.
+template <typename T>
+void cholesky_steps(tw_ctx* ctx, T* B, T* Lp, T* Dinv, int n, FitState* st) {
+    hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, Dinv, n, 0, std::min(NB, n), st);
+    for (int j0 = 0; j0 + NB < n; j0 += NB) {
+        const unsigned nt = nblk(n - j0 - NB, NB);
+        hipLaunchKernelGGL(k_chol_step<T>, dim3(nt, nt), dim3(256), 0, ctx->stream, B, Lp, Dinv, n, j0, NB, st);
+    }
+}
+
 template <typename T>
 void cholesky(tw_ctx* ctx, T* B, T* Dinv, int n, FitState* st) {
     for (int j0 = 0; j0 < n; j0 += NB) {
@@ -1690,9 +1801,18 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
             B = (double*)Bw;
         }
         hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
-        cholesky<double>(ctx, B, (double*)dinv, n, st);
+        const double* Lf = B;  // where the solve reads the factor's off-diagonal blocks
+        if ((n + NB - 1) / NB <= STEP_MAX_BLOCKS) {
+            void* lp = nullptr;
+            int r2 = tw_ws(ctx, WS_SOLVE_L, (size_t)n * n * sizeof(double), &lp);
+            if (r2) return r2;
+            cholesky_steps<double>(ctx, B, (double*)lp, (double*)dinv, n, st);
+            Lf = (const double*)lp;
+        } else {
+            cholesky<double>(ctx, B, (double*)dinv, n, st);
+        }
         hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
-        return chol_solve<double>(ctx, B, (const double*)dinv, n, z, (double*)ys, nullptr, st);
+        return chol_solve<double>(ctx, Lf, (const double*)dinv, n, z, (double*)ys, nullptr, st);
     };
 
     int iters = 0;

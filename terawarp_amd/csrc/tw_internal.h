@@ -24,6 +24,7 @@ enum TwSlot {
     WS_SOLVE_B,     // TPS right-hand side / solution
     WS_SOLVE_AUX,   // pivots, landmark packs
     WS_SOLVE_AUX2,  // refinement vectors
+    WS_SOLVE_L,     // the factor's off-diagonal panels (fused Cholesky steps)
     WS_PTS,         // landmark / coefficient packs for the grid kernel
     WS_DF,          // dense DF staging
     WS_WEIGHTS,     // B-spline weight tables
