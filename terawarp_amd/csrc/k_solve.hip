@@ -36,6 +36,7 @@
 #include "tw_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 
@@ -328,22 +329,22 @@ __device__ __forceinline__ void pin16(T* v) {
                  "+v"(v[15]));
 }
 
-// Value of the quad neighbour lane ^ 1 / lane ^ 2 (DPP quad permutes).
-__device__ __forceinline__ float quad_swap1(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+// Lane s's value in all four lanes of each quad (DPP quad permute; s is a
+// constant once the caller's loop is unrolled).
+__device__ __forceinline__ int quad_bcast32(int v, int s) {
+    switch (s) {
+        case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, true);
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, true);
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, true);
+        default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, true);
+    }
 }
-__device__ __forceinline__ float quad_swap2(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
-}
-__device__ __forceinline__ double quad_dpp(double v, int ctrl) {
+__device__ __forceinline__ float quad_bcast(float v, int s) { return __int_as_float(quad_bcast32(__float_as_int(v), s)); }
+__device__ __forceinline__ double quad_bcast(double v, int s) {
     const long long b = __double_as_longlong(v);
-    int lo = (int)(b & 0xffffffff), hi = (int)(b >> 32);
-    lo = ctrl == 1 ? __builtin_amdgcn_mov_dpp(lo, 0xB1, 0xF, 0xF, true) : __builtin_amdgcn_mov_dpp(lo, 0x4E, 0xF, 0xF, true);
-    hi = ctrl == 1 ? __builtin_amdgcn_mov_dpp(hi, 0xB1, 0xF, 0xF, true) : __builtin_amdgcn_mov_dpp(hi, 0x4E, 0xF, 0xF, true);
+    const int lo = quad_bcast32((int)(b & 0xffffffff), s), hi = quad_bcast32((int)(b >> 32), s);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ double quad_swap1(double v) { return quad_dpp(v, 1); }
-__device__ __forceinline__ double quad_swap2(double v) { return quad_dpp(v, 2); }
 
 // 1/d in a few instructions: the hardware reciprocal and one Newton step
 // (d > 0 and normal here; the result is within an ulp of 1/d).
@@ -358,6 +359,16 @@ __device__ __forceinline__ float recip(float d) {
     return fmaf(r, e, r);
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// D += A B for a 16 x 4 A and a 4 x 16 B on one wave's FP64 matrix core:
+// lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and item m of
+// the result holds D[4m + l / 16][l % 16], so item s is lane l's B operand
+// for rows 4s..4s+3 of a following product.
+__device__ __forceinline__ v4d mfma16x4(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 // Factor and invert the diagonal block at j0.  Four waves; lane r owns row
 // r; the 16 blocks of 4 columns are dealt round-robin, wave w holding
 // blocks b = 4t + w (columns 16t + 4w + j).  The factor is built as
@@ -365,8 +376,14 @@ __device__ __forceinline__ float recip(float d) {
 // block step the owner factors its 4 columns locally (pivots through
 // readlane), publishes them through LDS, and every wave applies the 4
 // rank-1 updates to its columns right of the block: 16 barriers in all.
-// L = L~ D^-1/2 at the end.  Then X = L^-1 by columns: lane c of wave 0
-// substitutes column c alone, reading the rows of L from LDS.
+// L = L~ D^-1/2 at the end (lane r forms d_r^-1/2 once; column c takes
+// lane c's value).  Then X = L^-1 by columns, right-looking: column
+// c = 16w + r/4 belongs to the quad r/4 of wave w, whose lane p keeps the
+// running sums acc_i = sum_(j<q) L[4i+p][j] x[j] of rows 4i + p.  At row q
+// the lane holding row q forms x[q] = (delta_qc - acc) / L[q][q], the quad
+// shares it (DPP), and every lane adds L[4i+p][q] x[q] to its rows' sums,
+// so a row's critical path is a subtract, a multiply, the exchange and one
+// FMA.
 constexpr int LS = NB + 2;  // row stride of the diagonal block's L (16-byte aligned rows)
 
 template <typename T>
@@ -374,11 +391,74 @@ struct PotrfLds {
     T colb[2][4][NB];                  // L~[r][k] (0 for r < k), by step parity
     __align__(16) T colp[2][4][NB];    // L~[c][k] / d_k for columns c right of the block, else 0
     T dsh[NB];                         // pivots d_k (non-positive ones replaced by 1)
-    // Lr[q][16p + i] = L[q][4i + p]: lane p of a quad reads its terms contiguously.
-    __align__(16) T Lr[NB][LS];        // L, row-major
+    // Lc[q][16p + i] = L[4i + p][q]: lane p of a quad reads its rows' terms contiguously.
+    __align__(16) T Lc[NB][LS];        // L by columns
+    __align__(16) T Xs[NB][LS];        // X = L^-1, row-major (FP64 blocked inverse)
     T rsqsh[NB];                       // 1 / L[c][c] = d_c^-1/2
     int bad;
 };
+
+// X = L^-1 in 16 x 16 blocks (FP64), L in Lc (by columns, see PotrfLds)
+// and 1 / L[q][q] in rsq.  The diagonal blocks X_II = L_II^-1 by
+// substitution, wave I: column 16I + r/4 in the quad r/4, lane p keeping
+// rows 4i + p, the quad sharing each solved row through DPP.  Then
+// X_IJ = -X_II sum_(K=J..I-1) L_IK X_KJ on the matrix cores, by distance
+// I - J: (1,0) (2,1) (3,2), then (2,0) (3,1), then (3,0), whose first two
+// terms wave 3 forms during the earlier passes.  X goes to Xs (for the
+// later passes) and to D (row-major, the upper blocks zero).
+__device__ __forceinline__ void inverse_blocked(const double (*Lc)[LS], double (*Xs)[LS], const double* rsq,
+                                                double* __restrict__ D, int r, int w) {
+    {
+        const int cc = r >> 2, p = r & 3;
+        double acc[4] = {0, 0, 0, 0}, xr[4] = {0, 0, 0, 0};
+        #pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int iq = q >> 2, pq = q & 3;
+            const double xq = quad_bcast(((q == cc ? 1.0 : 0.0) - acc[iq]) * rsq[16 * w + q], pq);
+            xr[iq] = p == pq ? xq : xr[iq];
+            const double* lq = &Lc[16 * w + q][16 * p + 4 * w];  // L[16w + 4i + p][16w + q] at i
+            #pragma unroll
+            for (int i = iq & ~1; i < 4; i++) acc[i] = fma(lq[i], xq, acc[i]);
+        }
+        #pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = 16 * w + 4 * i + p;
+            Xs[row][16 * w + cc] = xr[i];
+            D[row * NB + 16 * w + cc] = xr[i];
+            for (int J = w + 1; J < 4; J++) D[row * NB + 16 * J + cc] = 0;
+        }
+    }
+    __syncthreads();
+    const int lr = r & 15, lk = r >> 4;
+    auto lx = [&](int I, int K, int J, v4d y) {  // y + L_IK X_KJ
+        #pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int kk = 16 * K + 4 * s + lk;
+            y = mfma16x4(Lc[kk][16 * (lr & 3) + 4 * I + (lr >> 2)], Xs[kk][16 * J + lr], y);
+        }
+        return y;
+    };
+    auto finish = [&](int I, int J, v4d y) {  // X_IJ = -X_II y
+        v4d o = {0, 0, 0, 0};
+        #pragma unroll
+        for (int s = 0; s < 4; s++) o = mfma16x4(Xs[16 * I + lr][16 * I + 4 * s + lk], y[s], o);
+        #pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int row = 16 * I + 4 * m + lk;
+            Xs[row][16 * J + lr] = -o[m];
+            D[row * NB + 16 * J + lr] = -o[m];
+        }
+    };
+    const v4d z = {0, 0, 0, 0};
+    v4d y30 = z;
+    if (w < 3) finish(w + 1, w, lx(w + 1, w, w, z));
+    else y30 = lx(3, 0, 0, z);
+    __syncthreads();
+    if (w < 2) finish(w + 2, w, lx(w + 2, w + 1, w, lx(w + 2, w, w, z)));
+    else if (w == 3) y30 = lx(3, 1, 0, y30);
+    __syncthreads();
+    if (w == 3) finish(3, 0, lx(3, 2, 0, y30));
+}
 
 template <typename T>
 __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
@@ -386,7 +466,7 @@ __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict
     auto& colb = S.colb;
     auto& colp = S.colp;
     auto& dsh = S.dsh;
-    auto& Lr = S.Lr;
+    auto& Lc = S.Lc;
     auto& rsqsh = S.rsqsh;
     int& bad = S.bad;
     const int r = threadIdx.x & 63;
@@ -444,15 +524,15 @@ __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict
         pin16(a);
     }
     // L = L~ D^-1/2 (lower part; the rest is zero).
+    const T rsr = (T)1 / sqrt(dsh[r]);
+    if (w == 0) rsqsh[r] = rsr;
     #pragma unroll
     for (int t = 0; t < 4; t++)
         #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int c = 16 * t + 4 * w + j;
-            const T rs = (T)1 / sqrt(dsh[c]);
-            a[4 * t + j] = r >= c ? a[4 * t + j] * rs : (T)0;
-            Lr[r][16 * (c & 3) + (c >> 2)] = a[4 * t + j];
-            if (r == 0) rsqsh[c] = rs;
+            a[4 * t + j] = r >= c ? a[4 * t + j] * bcast(rsr, c) : (T)0;
+            Lc[c][16 * (r & 3) + (r >> 2)] = a[4 * t + j];
         }
     {
         T* col = B + (i64)j0 * n + j0 + r;
@@ -466,34 +546,44 @@ __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict
     }
     __syncthreads();
     if (threadIdx.x == 0 && bad >= 0 && st->info == 0) st->info = 2 + j0 + bad;
-    // X = L^-1 by columns: column c = 16w + r/4 of X belongs to the quad
-    // r/4 of wave w; lane p of the quad keeps x[4i + p] and sums those terms
-    // of x[q] = (delta_qc - sum_{j<q} L[q][j] x[j]) / L[q][q], the quad
-    // combining its four partial sums with two DPP exchanges per row.
-    const int p = r & 3, c = 16 * w + (r >> 2);
-    T xr[NB / 4];
-    #pragma unroll
-    for (int i = 0; i < NB / 4; i++) xr[i] = 0;  // x[j] not yet formed reads as 0
-    #pragma unroll
-    for (int q = 0; q < NB; q++) {
-        const T* lq = &Lr[q][16 * p];
-        T s0 = 0, s1 = 0;
+    if constexpr (std::is_same<T, double>::value) {
+        inverse_blocked(S.Lc, S.Xs, rsqsh, Dinv + (i64)(j0 / NB) * NB * NB, r, w);
+    } else {
+        const int p = r & 3, c = 16 * w + (r >> 2);
+        T acc[NB / 4], xr[NB / 4];  // xr[i] = X[4i + p][c]
         #pragma unroll
-        for (int i = 0; 4 * i < q; i++) {  // terms j = 4i + p; those with j >= q are 0
-            if (i & 1) s1 = fma(lq[i], xr[i], s1);
-            else s0 = fma(lq[i], xr[i], s0);
+        for (int i = 0; i < NB / 4; i++) acc[i] = xr[i] = 0;
+        // Row q's terms L[4i+p][q] (the rows below q: i >= q/4, from an even i
+        // for 16-byte reads) and 1 / L[q][q], read one row ahead.
+        T cur[NB / 4], nxt[NB / 4], rc, rn;
+        #pragma unroll
+        for (int i = 0; i < NB / 4; i++) nxt[i] = 0;
+        auto fetch = [&](int q, T* v, T& rq) {
+            const T* lq = &Lc[q][16 * p];
+            #pragma unroll
+            for (int i = (q >> 2) & ~1; i < NB / 4; i++) v[i] = lq[i];
+            rq = rsqsh[q];
+        };
+        fetch(0, cur, rc);
+        #pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int iq = q >> 2, pq = q & 3;
+            if (q + 1 < NB) fetch(q + 1, nxt, rn);
+            asm volatile("" ::: "memory");  // the next row's reads are issued here, before this row's work
+            const T xq = quad_bcast(((q == c ? (T)1 : (T)0) - acc[iq]) * rc, pq);
+            xr[iq] = p == pq ? xq : xr[iq];
+            // Rows at or above q take harmless updates: their sums are no longer read.
+            #pragma unroll
+            for (int i = iq & ~1; i < NB / 4; i++) acc[i] = fma(cur[i], xq, acc[i]);
+            #pragma unroll
+            for (int i = 0; i < NB / 4; i++) cur[i] = nxt[i];
+            rc = rn;
         }
-        T sum = s0 + s1;
-        sum += quad_swap1(sum);
-        sum += quad_swap2(sum);
-        const T xq = ((q == c ? (T)1 : (T)0) - sum) * rsqsh[q];
-        if ((q & 3) == p) xr[q >> 2] = xq;
-        asm volatile("" : "+v"(xr[q >> 2])::"memory");  // one row's LDS reads at a time
+        // Dinv row-major: D[row][c].
+        T* D = Dinv + (i64)(j0 / NB) * NB * NB + c;
+        #pragma unroll
+        for (int i = 0; i < NB / 4; i++) D[(4 * i + p) * NB] = xr[i];
     }
-    // xr[i] = X[4i + p][c]; Dinv row-major: D[row][c].
-    T* D = Dinv + (i64)(j0 / NB) * NB * NB + c;
-    #pragma unroll
-    for (int i = 0; i < NB / 4; i++) D[(4 * i + p) * NB] = xr[i];
 }
 
 template <typename T>
