@@ -722,12 +722,13 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
 // diagonal block.  The panel input B_Rk is only read in a step (the factor's
 // off-diagonal blocks go to Lp, not over B), so tiles of one row block never
 // race.  Lp: n x n column-major like B; the diagonal blocks stay in B and
-// their inverses in Dinv.
+// their inverses in Dinv.  Both products run on the FP64 matrix cores, wave w
+// owning rows 16w .. 16w + 15 of the tile (FP64 only: jb = NB).
 template <typename T>
 struct StepLds {
     union {
         struct {
-            T Xs[NB][NB];      // X_k = L_kk^-1, row-major
+            T Xs[NB][NB + 1];  // X_k = L_kk^-1, row-major
             T As[NB][NB + 1];  // As[c][r] = L_R[r][c]
             T Bs[NB][NB + 1];  // Bs[c][r] = L_C[r][c]
         } u;
@@ -735,73 +736,92 @@ struct StepLds {
     };
 };
 
-template <typename T>
-__device__ __forceinline__ void step_panel(const T* __restrict__ B, int n, int j0, int jb, int R, const T (*Xs)[NB],
-                                           T (*out)[NB + 1], T* __restrict__ Lp) {
-    const int lane = threadIdx.x & 63;
-    const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = R + lane;
-    const bool ok = i < n;
-    T a[NB];
+// Rows 16w .. 16w + 15 of L_R = B_Rk X_k': a[s] is the lane's A operand
+// B_Rk[16w + l%16][4s + l/16]; output tile (w, c) sums the chunks s <= 4c + 3
+// (X_k' is upper triangular), the four tiles' chains interleaved.
+// out[c][r] = L_R[r][c]; Lp, when given, gets L_R.
+__device__ __forceinline__ void step_panel(const double (&a)[NB / 4], const double (*Xs)[NB + 1],
+                                           double (*out)[NB + 1], double* __restrict__ Lp, int n, int j0, int R,
+                                           int w, int lr, int lk) {
+    v4d acc[4];
     #pragma unroll
-    for (int j = 0; j < NB; j++) a[j] = (ok && j < jb) ? B[(i64)(j0 + j) * n + i] : (T)0;
+    for (int c = 0; c < 4; c++) acc[c] = v4d{0, 0, 0, 0};
     #pragma unroll
-    for (int t = 0; t < NB / 4; t++) {
-        const T* xr = &Xs[4 * t + q][0];
-        T s = 0;
+    for (int s = 0; s < NB / 4; s++)
         #pragma unroll
-        for (int j = 0; j < 4 * t + 4; j++) {
-            s = fma(a[j], xr[j], s);
-            if ((j & 7) == 7) asm volatile("" : "+v"(s)::"memory");  // bound the LDS reads in flight
+        for (int c = s / 4; c < 4; c++) acc[c] = mfma16x4(a[s], Xs[16 * c + lr][4 * s + lk], acc[c]);
+    #pragma unroll
+    for (int c = 0; c < 4; c++)
+        #pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int row = 16 * w + 4 * m + lk, col = 16 * c + lr;
+            out[col][row] = acc[c][m];
+            if (Lp && R + row < n) Lp[(i64)(j0 + col) * n + R + row] = acc[c][m];
         }
-        out[4 * t + q][lane] = s;
-        if (Lp && ok && 4 * t + q < jb) Lp[(i64)(j0 + 4 * t + q) * n + i] = s;
-    }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_step(T* __restrict__ B, T* __restrict__ Lp, T* __restrict__ Dinv,
                                                    int n, int j0, int jb, FitState* st) {
+    static_assert(std::is_same<T, double>::value, "the fused step runs on the FP64 matrix cores");
     const int bx = blockIdx.x, by = blockIdx.y;
     if (by > bx) return;
     __shared__ StepLds<T> S;
     auto& Xs = S.u.Xs;
     auto& As = S.u.As;
     auto& Bs = S.u.Bs;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lr = tid & 15, lk = (tid >> 4) & 3;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int r0 = j0 + jb, R = r0 + bx * NB, C = r0 + by * NB;
+    // The wave's A operands of both panels and its part of the tile B_RC, in
+    // flight while X_k is staged (rows and columns past n are clamped for the
+    // load and zeroed or not stored).
+    T ar[NB / 4], ac[NB / 4], old[4][4];
+    {
+        const int rr = R + 16 * w + lr, rc = C + 16 * w + lr;
+        const int rrc = min(rr, n - 1), rcc = min(rc, n - 1);
+        #pragma unroll
+        for (int s = 0; s < NB / 4; s++) {
+            const i64 col = (i64)(j0 + 4 * s + lk) * n;
+            const T va = B[col + rrc], vc = B[col + rcc];
+            ar[s] = rr < n ? va : (T)0;
+            ac[s] = rc < n ? vc : (T)0;
+        }
+        #pragma unroll
+        for (int J = 0; J < 4; J++)
+            #pragma unroll
+            for (int m = 0; m < 4; m++)
+                old[J][m] = B[(i64)min(C + 16 * J + lr, n - 1) * n + min(R + 16 * w + 4 * m + lk, n - 1)];
+    }
     const T* D = Dinv + (i64)(j0 / NB) * NB * NB;
     #pragma unroll
-    for (int u = 0; u < NB * NB / 256; u++) (&Xs[0][0])[tid + 256 * u] = D[tid + 256 * u];
+    for (int u = 0; u < NB * NB / 256; u++) {
+        const int e = tid + 256 * u;
+        Xs[e / NB][e % NB] = D[e];
+    }
     __syncthreads();
-    step_panel<T>(B, n, j0, jb, R, Xs, As, by == 0 ? Lp : nullptr);
-    if (by != bx) step_panel<T>(B, n, j0, jb, C, Xs, Bs, nullptr);
+    step_panel(ar, Xs, As, by == 0 ? Lp : nullptr, n, j0, R, w, lr, lk);
+    if (by != bx) step_panel(ac, Xs, Bs, nullptr, n, j0, C, w, lr, lk);
     __syncthreads();
     T (*Cs)[NB + 1] = by == bx ? As : Bs;
-    const int tx = tid % 16, ty = tid / 16;
-    T acc[4][4];
+    // B_RC -= L_R L_C': tiles (w, J).  A diagonal tile needs only J <= w; the
+    // waves form all four anyway (wave 3 needs them, so the others cost
+    // nothing) and the store keeps the lower part.
+    v4d acc[4];
     #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int J = 0; J < 4; J++) acc[J] = v4d{0, 0, 0, 0};
+    #pragma unroll
+    for (int s = 0; s < NB / 4; s++) {
+        const T a = As[4 * s + lk][16 * w + lr];
         #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0;
-    #pragma unroll 8
-    for (int k = 0; k < NB; k++) {
-        T av[4], bv[4];
-        #pragma unroll
-        for (int i = 0; i < 4; i++) av[i] = As[k][tx + 16 * i];
-        #pragma unroll
-        for (int j = 0; j < 4; j++) bv[j] = Cs[k][ty + 16 * j];
-        #pragma unroll
-        for (int i = 0; i < 4; i++)
-            #pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+        for (int J = 0; J < 4; J++) acc[J] = mfma16x4(a, Cs[4 * s + lk][16 * J + lr], acc[J]);
     }
     #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int J = 0; J < 4; J++)
         #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int r = R + tx + 16 * i, c = C + ty + 16 * j;
-            if (r < n && c < n && c <= r) B[(i64)c * n + r] -= acc[i][j];
+        for (int m = 0; m < 4; m++) {
+            const int r = R + 16 * w + 4 * m + lk, c = C + 16 * J + lr;
+            if (r < n && c < n && c <= r) B[(i64)c * n + r] = old[J][m] - acc[J][m];
         }
     if (bx == 0 && by == 0) {
         __syncthreads();  // the updated tile is in B; the step's LDS is free
