@@ -390,12 +390,12 @@ template <typename T>
 struct PotrfLds {
     T colb[2][4][NB];                  // L~[r][k] (0 for r < k), by step parity
     __align__(16) T colp[2][4][NB];    // L~[c][k] / d_k for columns c right of the block, else 0
-    T dsh[NB];                         // pivots d_k (non-positive ones replaced by 1)
+    T dsh[NB];                         // pivots d_k
     // Lc[q][16p + i] = L[4i + p][q]: lane p of a quad reads its rows' terms contiguously.
     __align__(16) T Lc[NB][LS];        // L by columns
     __align__(16) T Xs[NB][LS];        // X = L^-1, row-major (FP64 blocked inverse)
     T rsqsh[NB];                       // 1 / L[c][c] = d_c^-1/2
-    int bad;
+    int badw[4];                       // first non-positive pivot each wave met (NB: none)
 };
 
 // X = L^-1 in 16 x 16 blocks (FP64), L in Lc (by columns, see PotrfLds)
@@ -468,7 +468,6 @@ __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict
     auto& dsh = S.dsh;
     auto& Lc = S.Lc;
     auto& rsqsh = S.rsqsh;
-    int& bad = S.bad;
     const int r = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const bool rin = r < jb;
@@ -483,29 +482,57 @@ __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict
                 a[4 * t + j] = (rin && c < jb) ? col[(i64)c * n] : (T)(r == c ? 1 : 0);
             }
     }
-    if (threadIdx.x == 0) bad = -1;
+    int mybad = NB;  // first non-positive pivot this wave met (wave-uniform)
     #pragma unroll
     for (int b = 0; b < NB / 4; b++) {
         const int buf = b & 1, tb = b >> 2, k0 = 4 * b;
         if (w == (b & 3)) {
-            #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int k = k0 + j;
-                T d = bcast(a[4 * tb + j], k);
-                if (!(d > (T)0)) {
-                    if (r == 0 && bad < 0) bad = k;
-                    d = (T)1;
-                }
-                const T rd = recip(d);
-                const T lrk = r >= k ? a[4 * tb + j] : (T)0;
-                colb[buf][j][r] = lrk;
-                colp[buf][j][r] = r > k0 + 3 ? lrk * rd : (T)0;
-                if (r == 0) dsh[k] = d;
-                // the block's later columns
-                #pragma unroll
-                for (int jj = j + 1; jj < 4; jj++)
-                    a[4 * tb + jj] = fma(-lrk * rd, bcast(a[4 * tb + j], k0 + jj), a[4 * tb + jj]);
-            }
+            // The block's 4 x 4 diagonal part, read once and eliminated on
+            // uniform values with the same operations the columns take below,
+            // so each pivot's reciprocal waits only on the one before.
+            T* ab = &a[4 * tb];
+            const T s00 = bcast(ab[0], k0), s10 = bcast(ab[0], k0 + 1), s20 = bcast(ab[0], k0 + 2);
+            const T s30 = bcast(ab[0], k0 + 3), s11 = bcast(ab[1], k0 + 1), s21 = bcast(ab[1], k0 + 2);
+            const T s31 = bcast(ab[1], k0 + 3), s22 = bcast(ab[2], k0 + 2), s32 = bcast(ab[2], k0 + 3);
+            const T s33 = bcast(ab[3], k0 + 3);
+            const T r0 = recip(s00);
+            const T l10 = s10 * r0, l20 = s20 * r0, l30 = s30 * r0;
+            const T d1 = fma(-l10, s10, s11), t21 = fma(-l20, s10, s21), t31 = fma(-l30, s10, s31);
+            const T t22 = fma(-l20, s20, s22), t32 = fma(-l30, s20, s32), t33 = fma(-l30, s30, s33);
+            const T r1 = recip(d1);
+            const T l21 = t21 * r1, l31 = t31 * r1;
+            const T d2 = fma(-l21, t21, t22), u32 = fma(-l31, t21, t32), u33 = fma(-l31, t31, t33);
+            const T r2 = recip(d2);
+            const T l32 = u32 * r2;
+            const T d3 = fma(-l32, u32, u33);
+            const T r3 = recip(d3);
+            // A non-positive pivot marks the factor as failed; what follows
+            // it is not used.
+            mybad = min(mybad, s00 > (T)0 ? NB : k0);
+            mybad = min(mybad, d1 > (T)0 ? NB : k0 + 1);
+            mybad = min(mybad, d2 > (T)0 ? NB : k0 + 2);
+            mybad = min(mybad, d3 > (T)0 ? NB : k0 + 3);
+            // The columns: L~[r][k] for r >= k (0 above), m_rk = L~[r][k] / d_k.
+            const T lr0 = r >= k0 ? ab[0] : (T)0, m0 = lr0 * r0;
+            ab[1] = fma(-m0, s10, ab[1]);
+            ab[2] = fma(-m0, s20, ab[2]);
+            ab[3] = fma(-m0, s30, ab[3]);
+            const T lr1 = r >= k0 + 1 ? ab[1] : (T)0, m1 = lr1 * r1;
+            ab[2] = fma(-m1, t21, ab[2]);
+            ab[3] = fma(-m1, t31, ab[3]);
+            const T lr2 = r >= k0 + 2 ? ab[2] : (T)0, m2 = lr2 * r2;
+            ab[3] = fma(-m2, u32, ab[3]);
+            const T lr3 = r >= k0 + 3 ? ab[3] : (T)0, m3 = lr3 * r3;
+            const bool below = r > k0 + 3;
+            colb[buf][0][r] = lr0;
+            colb[buf][1][r] = lr1;
+            colb[buf][2][r] = lr2;
+            colb[buf][3][r] = lr3;
+            colp[buf][0][r] = below ? m0 : (T)0;
+            colp[buf][1][r] = below ? m1 : (T)0;
+            colp[buf][2][r] = below ? m2 : (T)0;
+            colp[buf][3][r] = below ? m3 : (T)0;
+            if (r < 4) dsh[k0 + r] = r == 0 ? s00 : r == 1 ? d1 : r == 2 ? d2 : d3;
         }
         __syncthreads();
         // Only columns right of the block change: the wave's groups t > tb,
@@ -544,8 +571,12 @@ __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict
                 if (rin && c < jb && c <= r) col[(i64)c * n] = a[4 * t + j];
             }
     }
+    if (r == 0) S.badw[w] = mybad;
     __syncthreads();
-    if (threadIdx.x == 0 && bad >= 0 && st->info == 0) st->info = 2 + j0 + bad;
+    if (threadIdx.x == 0) {
+        const int bad = min(min(S.badw[0], S.badw[1]), min(S.badw[2], S.badw[3]));
+        if (bad < NB && st->info == 0) st->info = 2 + j0 + bad;
+    }
     if constexpr (std::is_same<T, double>::value) {
         inverse_blocked(S.Lc, S.Xs, rsqsh, Dinv + (i64)(j0 / NB) * NB * NB, r, w);
     } else {
@@ -1055,6 +1086,7 @@ __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict
     __shared__ T part[4][NB][4];  // partial sums of the four quarters
     __shared__ T accs[NB][4];     // the block's reduced right-hand side
     __shared__ T ys[NB][4];       // y_j, the backward pass's start
+    __shared__ T Xs[NB][NB + 1];  // X_j, staged before the first wait
     __shared__ int timed_out;
     const int tid = threadIdx.x, r = tid & (NB - 1), q = tid / NB;
     const int nb = (n + NB - 1) / NB, j = blockIdx.x, j0 = j * NB, ib = min(NB, n - j0);
@@ -1091,7 +1123,14 @@ __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict
         accs[r][q] = (part[0][r][q] + part[1][r][q]) + (part[2][r][q] + part[3][r][q]);
         __syncthreads();
     };
-    const T* X = Dinv + (i64)j * NB * NB;  // row-major; rows past ib are the identity's
+    {
+        const T* X = Dinv + (i64)j * NB * NB;  // row-major; rows past ib are the identity's
+        #pragma unroll
+        for (int u = 0; u < NB * NB / PIPE_THREADS; u++) {
+            const int e = tid + PIPE_THREADS * u;
+            Xs[e / NB][e % NB] = X[e];
+        }
+    }
     T acc[4], l[NB / 4];
     // ---- forward: y_j = X_j (b_j - sum_{k<j} L_jk y_k)
     #pragma unroll
@@ -1113,7 +1152,7 @@ __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict
     {
         T s = 0;
         #pragma unroll 8
-        for (int m = 0; m < NB; m++) s = fma(X[r * NB + m], accs[m][q], s);
+        for (int m = 0; m < NB; m++) s = fma(Xs[r][m], accs[m][q], s);
         ys[r][q] = s;
         if (r < ib) y[(i64)q * n + j0 + r] = s;
     }
@@ -1139,7 +1178,7 @@ __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict
     {
         T s = 0;
         #pragma unroll 8
-        for (int m = 0; m < NB; m++) s = fma(X[m * NB + r], accs[m][q], s);
+        for (int m = 0; m < NB; m++) s = fma(Xs[m][r], accs[m][q], s);
         if (r < ib) b[(i64)q * n + j0 + r] = s;
     }
     publish(&bwd[j]);
