@@ -48,29 +48,35 @@ def b_alg_per_voxel(bpv, s_df, nodes, voxels):
     return 2 * bpv + 3 * s_df * nodes / voxels
 
 
-def run_form(w, torch, src, dst, tar, sub, precision, steps, warmup):
+def run_form(w, wp, torch, src, dst, tar, sub, precision, steps, warmup):
     """Time `steps` whole warps through the library's one-call chain
-    (tw_warp_tps: fit, grid, resample on the device, one synchronisation);
-    per-stage kernel times come from the chain's HIP events."""
-    from terawarp_amd import warp
+    (tw_warp_tps: fit, grid, resample on the device, one synchronisation) on
+    the production context `wp` (no stage events); the per-stage kernel
+    times come from the same number of warps on the timing context `w`,
+    whose HIP events bracket each stage."""
     keys = ("fit_ms", "grid_ms", "quant_ms", "resample_ms", "total_ms", "refine_iters")
     rec = {k: [] for k in keys}
 
-    def step(record):
-        w.warp_tps(src, tar, sub, f=G, lam=0.2, precision=precision, dst=dst)
+    def step(ctx, record):
+        ctx.warp_tps(src, tar, sub, f=G, lam=0.2, precision=precision, dst=dst)
         if record:
-            t = w.timings()
+            t = ctx.timings()
             for k in keys:
                 rec[k].append(t[k])
 
     for _ in range(warmup):
-        step(False)
+        step(wp, False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        step(True)
+        step(wp, False)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    for _ in range(max(1, warmup)):
+        step(w, False)
+    for _ in range(steps):
+        step(w, True)
+    torch.cuda.synchronize()
     vox = SIZE[0] * SIZE[1] * SIZE[2]
     res = float(np.mean(rec["resample_ms"]))
     return {
@@ -570,9 +576,10 @@ def main():
     torch.cuda.synchronize()
 
     w = Warper(0, timing=True)
+    wp = Warper(0)
     if args.profile:
-        run_form(w, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
-        run_form(w, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
+        run_form(w, wp, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
+        run_form(w, wp, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
         # Calibration copy of a known size: 1 GiB read + 1 GiB written.
         a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)
@@ -580,12 +587,13 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile": "done", "copy_bytes": 1 << 30}))
         w.close()
+        wp.close()
         return
     log("plain form")
-    plain = run_form(w, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
+    plain = run_form(w, wp, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
     out_plain = dst.cpu().numpy().view(np.uint16)
     log("mixed form")
-    mixed = run_form(w, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
+    mixed = run_form(w, wp, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
     out_mixed = dst.cpu().numpy().view(np.uint16)
     gp = accuracy_grid(w, tar, sub, warp.PLAIN)
     gm = accuracy_grid(w, tar, sub, warp.MIXED)
@@ -666,6 +674,7 @@ def main():
     }
     print(json.dumps(line))
     w.close()
+    wp.close()
 
 
 if __name__ == "__main__":

@@ -1861,9 +1861,11 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
     const size_t pb = pack_pts_bytes(N), yb = pack_y_bytes(N);
     void* pack = nullptr;
     if ((rc = tw_ws(ctx, WS_PTS, pb + yb, &pack))) return rc;
-    std::vector<uint8_t> host(pb + yb);
-    pack_host(centres, targets, N, host.data());
-    TW_HIP(ctx, hipMemcpyAsync(pack, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+    void* host = nullptr;
+    if ((rc = tw_pinned(ctx, pb + yb, &host))) return rc;
+    pack_host(centres, targets, N, (uint8_t*)host);
+    TW_HIP(ctx, hipMemcpyAsync(pack, host, pb + yb, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = tw_pinned_issued(ctx))) return rc;
     const SysDesc sd = sys_desc(hs, pack, N, lambda);
     out.pts = sd.pts;
     out.s = s;
@@ -1927,8 +1929,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
 
     FitState hst;
     auto fetch_state = [&]() -> int {
-        TW_HIP(ctx, hipMemcpyAsync(&hst, st, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
-        TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = tw_read_small(ctx, &hst, st, sizeof hst))) return rc;
         return TW_OK;
     };
 
@@ -2029,8 +2030,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
 // Fit state after a run: errors, residual, refinement count (synchronises).
 int fit_finish(tw_ctx* ctx, const FitState* st, double bmax, int iters) {
     FitState hst;
-    TW_HIP(ctx, hipMemcpyAsync(&hst, st, sizeof hst, hipMemcpyDeviceToHost, ctx->stream));
-    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc = tw_read_small(ctx, &hst, st, sizeof hst)) return rc;
     if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
     if (hst.info >= 2)
         return tw_fail(ctx, TW_E_SINGULAR, "projected TPS system not positive definite (column %d)", hst.info - 2);

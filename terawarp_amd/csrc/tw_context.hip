@@ -54,6 +54,52 @@ int tw_ws(tw_ctx* ctx, int slot, size_t bytes, void** out) {
     return TW_OK;
 }
 
+int tw_pinned(tw_ctx* ctx, size_t bytes, void** out) {
+    if (ctx->pin_busy) {
+        TW_HIP(ctx, hipEventSynchronize(ctx->pin_ev));
+        ctx->pin_busy = false;
+    }
+    if (ctx->pin_cap < bytes) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_cap = 0;
+        const size_t cap = (bytes + 0xFFFF) & ~size_t(0xFFFF);
+        hipError_t e = hipHostMalloc(&ctx->pin, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ctx->pin = nullptr;
+            (void)hipGetLastError();
+            return tw_fail(ctx, TW_E_NOMEM, "pinned host allocation of %zu bytes failed: %s", cap,
+                           hipGetErrorString(e));
+        }
+        ctx->pin_cap = cap;
+    }
+    *out = ctx->pin;
+    return TW_OK;
+}
+
+int tw_pinned_issued(tw_ctx* ctx) {
+    if (!ctx->pin_ev) TW_HIP(ctx, hipEventCreateWithFlags(&ctx->pin_ev, hipEventDisableTiming));
+    TW_HIP(ctx, hipEventRecord(ctx->pin_ev, ctx->stream));
+    ctx->pin_busy = true;
+    return TW_OK;
+}
+
+int tw_read_small(tw_ctx* ctx, void* out, const void* dev, size_t bytes) {
+    if (bytes > 4096) return tw_fail(ctx, TW_E_ARG, "tw_read_small: %zu bytes", bytes);
+    if (!ctx->pin_state) {
+        hipError_t e = hipHostMalloc(&ctx->pin_state, 4096, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ctx->pin_state = nullptr;
+            (void)hipGetLastError();
+            return tw_fail(ctx, TW_E_NOMEM, "pinned host allocation failed: %s", hipGetErrorString(e));
+        }
+    }
+    TW_HIP(ctx, hipMemcpyAsync(ctx->pin_state, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(out, ctx->pin_state, bytes);
+    return TW_OK;
+}
+
 void tw_stages_reset(tw_ctx* ctx) {
     for (int i = 0; i < ST_COUNT; i++) ctx->ev_used[i] = false;
     ctx->refine_iters = 0;
@@ -193,6 +239,9 @@ int tw_destroy(tw_ctx* ctx) {
     for (int i = 0; i < WS_COUNT; i++)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     if (ctx->solve_flags) (void)hipFree(ctx->solve_flags);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->pin_state) (void)hipHostFree(ctx->pin_state);
+    if (ctx->pin_ev) (void)hipEventDestroy(ctx->pin_ev);
     for (int i = 0; i < ST_COUNT; i++) {
         if (ctx->ev_begin[i]) (void)hipEventDestroy(ctx->ev_begin[i]);
         if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);

@@ -64,6 +64,14 @@ struct tw_ctx {
     int* solve_flags = nullptr;
     unsigned solve_epoch = 0;
     std::vector<uint8_t> weights_host;
+    // Pinned host staging for small uploads (landmark packs): pin_ev marks
+    // the last upload from it, waited on before the buffer is refilled.
+    // pin_state: a pinned mirror for reading a fit's state back.
+    void* pin = nullptr;
+    size_t pin_cap = 0;
+    hipEvent_t pin_ev = nullptr;
+    bool pin_busy = false;
+    void* pin_state = nullptr;
     // Host images of a batch's tables, kept until the next batch (the batch
     // synchronises before it returns).
     std::vector<std::vector<uint8_t>> staging;
@@ -73,6 +81,15 @@ struct tw_ctx {
 uint8_t* tw_host_stage(tw_ctx* ctx, size_t bytes);
 // Drop the staged host buffers (synchronises the stream first if any exist).
 void tw_host_stage_reset(tw_ctx* ctx);
+
+// A pinned host buffer of at least `bytes` to fill and upload from; waits
+// for the previous upload from it first.  Call tw_pinned_issued after the
+// upload is enqueued.
+int tw_pinned(tw_ctx* ctx, size_t bytes, void** out);
+int tw_pinned_issued(tw_ctx* ctx);
+// Copy `bytes` (at most 4096) of device memory to `out` through the pinned
+// state mirror and synchronise the stream.
+int tw_read_small(tw_ctx* ctx, void* out, const void* dev, size_t bytes);
 
 // Set the context error string and return code.
 int tw_fail(tw_ctx* ctx, int code, const char* fmt, ...);
