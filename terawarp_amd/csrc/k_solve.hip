@@ -1396,13 +1396,16 @@ __global__ __launch_bounds__(1024) void k_ir_residual(const double* __restrict__
     }
 }
 
-__global__ void k_ir_init(FitState* st) {
+__device__ __forceinline__ void fit_state_init(FitState* st) {
     st->ir_stop = 0;
     st->ir_prev = INFINITY;
     st->ir_iters = 0;
     st->info = 0;
     st->sync_timeout = 0;
+    st->fit_rmax = 0.0;
 }
+
+__global__ void k_ir_init(FitState* st) { fit_state_init(st); }
 
 __global__ void k_axpy_f32(double* __restrict__ x, const float* __restrict__ d, i64 n, const int* skip) {
     if (skip && *skip) return;
@@ -1416,7 +1419,8 @@ __global__ void k_convert(const Src* __restrict__ a, Dst* __restrict__ b, i64 n)
     if (i < n) b[i] = (Dst)a[i];
 }
 
-// FP64 residual of the bordered system  r = [Y; 0] - M' x  (x is m x 4).
+// FP64 residual of the bordered system  r = [Y; 0] - M' x  (x is m x 4),
+// one row per work-group (k_unscale takes the largest |r|).
 __global__ __launch_bounds__(256) void k_residual(const SysDesc s, const double* __restrict__ x,
                                                   double* __restrict__ res) {
     __shared__ double red[4][4];
@@ -1458,18 +1462,286 @@ __global__ void k_finish(const double* __restrict__ U, const double* __restrict_
     }
 }
 
-// st->fit_rmax = max |res| over cnt entries.  One work-group of 1024.
-__global__ __launch_bounds__(1024) void k_res_max(const double* __restrict__ res, i64 cnt, FitState* st) {
-    __shared__ double red[16];
-    double m = 0;
-    for (i64 i = threadIdx.x; i < cnt; i += 1024) m = fmax(m, fabs(res[i]));
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0;
-        for (int w = 0; w < 16; w++) t = fmax(t, red[w]);
-        st->fit_rmax = t;
+// ---- the fit's one-work-group kernels for N <= QR_FAST_MAX -----------------
+// NT threads (the QR: one wave up to N = 1024; the products, whose time is
+// global-load latency, four waves); rows tid, tid + NT,
+// ... of an N x 4 operand belong to thread tid, and each phase's sums go
+// through one group reduction (DPP and readlane within one wave, no
+// barrier).  The small 4 x 4 products are formed by every thread (no
+// broadcast barrier).
+constexpr int QR_FAST_MAX = 4096, QR_WAVE_MAX = 1024, QRPT = 16;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over one wave, the same value in every lane: quad exchanges, then the
+// half-row and row mirrors leave each row of 16 with its sum (a + b == b + a,
+// so every lane of a row holds the same bits), then the four rows in order.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_d<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    return (bcast(v, 0) + bcast(v, 16)) + (bcast(v, 32) + bcast(v, 48));
+}
+
+template <int NT, int NV>
+__device__ __forceinline__ void group_sum(double* v, double* red) {
+    if constexpr (NT == 64) {
+        #pragma unroll
+        for (int i = 0; i < NV; i++) v[i] = wave_sum_dpp(v[i]);
+    } else {
+        block_sum_n<NV>(v, red);
+    }
+}
+
+// State init, X' = [1 p'] (scaled) and its Householder QR (V, R, tau) and
+// compact-WY factor T; X' stays in registers.  Same steps as hh_x_body,
+// with the four column reductions of a step fused into two.
+template <int QT>
+__global__ __launch_bounds__(QT) void k_qr_fast(const SysDesc s, double* __restrict__ V, FitState* st) {
+    __shared__ double red[17 * 10];
+    const int N = s.N, tid = threadIdx.x;
+    if (tid == 0) fit_state_init(st);
+    double x[QRPT][4];
+    #pragma unroll
+    for (int u = 0; u < QRPT; u++) {
+        const int i = tid + QT * u;
+        x[u][0] = x[u][1] = x[u][2] = x[u][3] = 0.0;
+        if (QT * u < N && i < N) {
+            const float4 p = s.pts[i];
+            x[u][0] = 1.0;
+            x[u][1] = ((double)p.x - s.mu[0]) * s.inv_s;
+            x[u][2] = ((double)p.y - s.mu[1]) * s.inv_s;
+            x[u][3] = ((double)p.z - s.mu[2]) * s.inv_s;
+        }
+    }
+    double R[16] = {0};
+    double tau[4];
+    double scale0 = 0.0;
+    #pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double nv[2] = {0.0, 0.0};  // |x_k below row k|^2, x_k[k]
+        #pragma unroll
+        for (int u = 0; u < QRPT; u++) {
+            const int i = tid + QT * u;
+            if (QT * u < N) {
+                if (i > k) nv[0] += x[u][k] * x[u][k];
+                else if (i == k) nv[1] = x[u][k];
+            }
+        }
+        group_sum<QT, 2>(nv, red);
+        const double xnorm2 = nv[0], alpha = nv[1];
+        double beta, t, scale;
+        if (xnorm2 == 0.0) {
+            beta = alpha; t = 0.0; scale = 0.0;
+        } else {
+            beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        tau[k] = t;
+        R[k * 4 + k] = beta;
+        if (k == 0) scale0 = fabs(beta);
+        double d[6] = {0, 0, 0, 0, 0, 0};  // v_k . x_j and x_j[k] for j = k+1..3
+        #pragma unroll
+        for (int u = 0; u < QRPT; u++) {
+            const int i = tid + QT * u;
+            x[u][k] = i < k ? 0.0 : (i == k ? 1.0 : x[u][k] * scale);
+            if (QT * u < N)
+                #pragma unroll
+                for (int j = k + 1; j < 4; j++) {
+                    if (i >= k) d[j - k - 1] += x[u][k] * x[u][j];
+                    if (i == k) d[3 + j - k - 1] = x[u][j];
+                }
+        }
+        group_sum<QT, 6>(d, red);
+        #pragma unroll
+        for (int j = k + 1; j < 4; j++) {
+            const double w = d[j - k - 1];
+            R[k * 4 + j] = d[3 + j - k - 1] - t * w;
+            #pragma unroll
+            for (int u = 0; u < QRPT; u++) x[u][j] -= t * x[u][k] * w;
+        }
+    }
+    double G[10];  // V'V, upper triangle by rows
+    {
+        int g = 0;
+        #pragma unroll
+        for (int a = 0; a < 4; a++)
+            #pragma unroll
+            for (int b = a; b < 4; b++) {
+                double p = 0;
+                #pragma unroll
+                for (int u = 0; u < QRPT; u++)
+                    if (QT * u < N) p += x[u][a] * x[u][b];
+                G[g++] = p;
+            }
+    }
+    group_sum<QT, 10>(G, red);
+    #pragma unroll
+    for (int u = 0; u < QRPT; u++) {
+        const int i = tid + QT * u;
+        if (QT * u < N && i < N)
+            #pragma unroll
+            for (int k = 0; k < 4; k++) V[(size_t)k * N + i] = x[u][k];
+    }
+    if (tid == 0) {
+        auto gv = [&](int a, int b) { return a <= b ? G[a * 4 - a * (a - 1) / 2 + (b - a)] : G[b * 4 - b * (b - 1) / 2 + (a - b)]; };
+        double T[16] = {0};
+        for (int k = 0; k < 4; k++) {
+            T[k * 4 + k] = tau[k];
+            for (int i = 0; i < k; i++) {
+                double acc = 0;
+                for (int j = i; j < k; j++) acc += T[i * 4 + j] * gv(j, k);
+                T[i * 4 + k] = -tau[k] * acc;
+            }
+        }
+        for (int i = 0; i < 16; i++) { st->R[i] = R[i]; st->T[i] = T[i]; }
+        for (int k = 0; k < 4; k++) st->tau[k] = tau[k];
+        int bad = 0;
+        for (int k = 1; k < 4; k++)
+            if (!(fabs(R[k * 4 + k]) > 1e-9 * scale0)) bad = 1;
+        st->info = bad;
+    }
+}
+
+// H = T' G (trans) or T G, every thread.
+__device__ __forceinline__ void small_tg(const double* __restrict__ T, const double* G, int trans, double* H) {
+    #pragma unroll
+    for (int a = 0; a < 4; a++)
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double acc = 0;
+            #pragma unroll
+            for (int k = 0; k < 4; k++) acc += (trans ? T[k * 4 + a] : T[a * 4 + k]) * G[k * 4 + b];
+            H[a * 4 + b] = acc;
+        }
+}
+
+// W = P T - 1/2 V M (M = T' (V'P) T) and U = Q'Y = Y - V T' (V'Y), one pass
+// of sums for both.
+template <int QT>
+__global__ __launch_bounds__(QT) void k_w_qy_fast(const SysDesc s, const double* __restrict__ V,
+                                                  const double* __restrict__ P, double* __restrict__ W,
+                                                  double* __restrict__ U, const FitState* __restrict__ st) {
+    __shared__ double red[17 * 32];
+    const int N = s.N, tid = threadIdx.x;
+    double S[32];
+    #pragma unroll
+    for (int i = 0; i < 32; i++) S[i] = 0;
+    for (int i = tid; i < N; i += QT) {
+        double v[4], p[4], y[4];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) { v[k] = V[(size_t)k * N + i]; p[k] = P[(size_t)k * N + i]; y[k] = s.y[i * 4 + k]; }
+        #pragma unroll
+        for (int a = 0; a < 4; a++)
+            #pragma unroll
+            for (int b = 0; b < 4; b++) {
+                S[a * 4 + b] += v[a] * p[b];
+                S[16 + a * 4 + b] += v[a] * y[b];
+            }
+    }
+    group_sum<QT, 32>(S, red);
+    double Ts[16], TS[16], M[16], H[16];
+    #pragma unroll
+    for (int i = 0; i < 16; i++) Ts[i] = st->T[i];
+    small_tg(Ts, S, 1, TS);  // T' S
+    #pragma unroll
+    for (int a = 0; a < 4; a++)  // (T' S) T
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double acc = 0;
+            #pragma unroll
+            for (int k = 0; k < 4; k++) acc += TS[a * 4 + k] * Ts[k * 4 + b];
+            M[a * 4 + b] = acc;
+        }
+    small_tg(Ts, S + 16, 1, H);
+    for (int i = tid; i < N; i += QT) {
+        double v[4], p[4], y[4];
+        #pragma unroll
+        for (int k = 0; k < 4; k++) { v[k] = V[(size_t)k * N + i]; p[k] = P[(size_t)k * N + i]; y[k] = s.y[i * 4 + k]; }
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double acc = 0, q = 0;
+            #pragma unroll
+            for (int k = 0; k < 4; k++) {
+                acc += p[k] * Ts[k * 4 + b] - 0.5 * v[k] * M[k * 4 + b];
+                q += v[k] * H[k * 4 + b];
+            }
+            W[(size_t)b * N + i] = acc;
+            U[(size_t)b * N + i] = y[b] - q;
+        }
+    }
+}
+
+// U <- Q'U (trans) or QU for an N x 4 U.  With z, U starts as [0; z] (z:
+// n x 4, n = N - 4).  With x, the result's first four rows give
+// d' = R^-1 [Q'U]_(0:4) and x (m x 4, m = N + 4) = [Uc; d'].
+template <int QT>
+__global__ __launch_bounds__(QT) void k_q_fast(const double* __restrict__ V, const FitState* __restrict__ st,
+                                               double* __restrict__ U, int N, int trans, const double* __restrict__ z,
+                                               const double* __restrict__ Uc, double* __restrict__ x) {
+    __shared__ double red[17 * 16];
+    __shared__ double top[4][4];
+    const int tid = threadIdx.x, n = N - 4, m = N + 4;
+    auto row = [&](int i, double* u) {
+        #pragma unroll
+        for (int k = 0; k < 4; k++) u[k] = z ? (i < 4 ? 0.0 : z[(size_t)k * n + i - 4]) : U[(size_t)k * N + i];
+    };
+    double G[16];
+    #pragma unroll
+    for (int i = 0; i < 16; i++) G[i] = 0;
+    for (int i = tid; i < N; i += QT) {
+        double v[4], u[4];
+        row(i, u);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = V[(size_t)k * N + i];
+        #pragma unroll
+        for (int a = 0; a < 4; a++)
+            #pragma unroll
+            for (int b = 0; b < 4; b++) G[a * 4 + b] += v[a] * u[b];
+    }
+    group_sum<QT, 16>(G, red);
+    double Ts[16], H[16];
+    #pragma unroll
+    for (int i = 0; i < 16; i++) Ts[i] = st->T[i];
+    small_tg(Ts, G, trans, H);
+    for (int i = tid; i < N; i += QT) {
+        double v[4], u[4];
+        row(i, u);
+        #pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = V[(size_t)k * N + i];
+        #pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double acc = 0;
+            #pragma unroll
+            for (int k = 0; k < 4; k++) acc += v[k] * H[k * 4 + b];
+            const double r = u[b] - acc;
+            U[(size_t)b * N + i] = r;
+            if (x) {
+                if (i < 4) top[i][b] = r;
+                x[(i64)b * m + i] = Uc[(size_t)b * N + i];
+            }
+        }
+    }
+    if (x) {
+        __syncthreads();
+        if (tid < 4) {
+            const int j = tid;
+            double dp[4];
+            for (int a = 3; a >= 0; a--) {
+                double v = top[a][j];
+                for (int q = a + 1; q < 4; q++) v -= st->R[a * 4 + q] * dp[q];
+                dp[a] = v / st->R[a * 4 + a];
+            }
+            for (int a = 0; a < 4; a++) x[(i64)j * m + N + a] = dp[a];
+        }
     }
 }
 
@@ -1521,11 +1793,30 @@ __device__ __forceinline__ void unscale_affine(const double* __restrict__ x, int
     d[3 * 4 + j] = d3 / s;
 }
 
-__global__ void k_unscale(const double* __restrict__ x, int N, double s, double mu0, double mu1, double mu2,
-                          double* __restrict__ cf, double* __restrict__ d) {
+// st->fit_rmax = max |res| over cnt entries (256 threads).
+__device__ __forceinline__ void res_max_body(const double* __restrict__ res, int cnt, FitState* st) {
+    __shared__ double red[4];
+    double m = 0;
+    for (int k = threadIdx.x; k < cnt; k += 256) m = fmax(m, fabs(res[k]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) st->fit_rmax = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void k_res_max(const double* __restrict__ res, int cnt, FitState* st) {
+    res_max_body(res, cnt, st);
+}
+
+// With res, work-group 0 also sets st->fit_rmax = max |res| over cnt entries.
+__global__ __launch_bounds__(256) void k_unscale(const double* __restrict__ x, int N, double s, double mu0,
+                                                 double mu1, double mu2, double* __restrict__ cf,
+                                                 double* __restrict__ d, const double* __restrict__ res, int cnt,
+                                                 FitState* st) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < N) unscale_coef(x, N, s, cf, i);
     if (blockIdx.x == 0 && threadIdx.x < 4) unscale_affine(x, N, s, mu0, mu1, mu2, d, threadIdx.x);
+    if (blockIdx.x == 0 && res) res_max_body(res, cnt, st);
 }
 
 // ---- small systems: the whole fit in one work-group ------------------------
@@ -1825,6 +2116,7 @@ SysDesc sys_desc(const HostSys& h, const void* pack, int N, double lambda) {
 struct FitRun {
     double* x;        // m x 4 column-major: [c'; d'], device
     FitState* st;     // device
+    const double* res = nullptr;  // bordered residual (m x 4) whose max k_unscale takes; none: set already
     const float4* pts;  // centres, device
     double s, mu[3];
     double bmax;      // largest |entry| of [1 targets]
@@ -1934,13 +2226,22 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
     };
 
     // 1. QR of X', W, B.
-    hipLaunchKernelGGL(k_ir_init, dim3(1), dim3(1), 0, ctx->stream, st);
-    hipLaunchKernelGGL(k_hh_x, dim3(1), dim3(1024), 0, ctx->stream, sd, V, st);
-    hipLaunchKernelGGL(k_av, dim3(N), dim3(256), 0, ctx->stream, sd, V, P);
-    hipLaunchKernelGGL(k_build_w, dim3(1), dim3(1024), 0, ctx->stream, V, P, W, st, N);
-    // 2. Q'Y -> b2
-    hipLaunchKernelGGL(k_load_y, dim3(nblk(N)), dim3(256), 0, ctx->stream, sd, U);
-    hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 1);
+    // 2. Q'Y -> U (b2 = rows 4.. of U)
+    const bool qfast = N <= QR_FAST_MAX, qwave = N <= QR_WAVE_MAX;
+    if (qfast) {
+        if (qwave) hipLaunchKernelGGL(k_qr_fast<64>, dim3(1), dim3(64), 0, ctx->stream, sd, V, st);
+        else hipLaunchKernelGGL(k_qr_fast<256>, dim3(1), dim3(256), 0, ctx->stream, sd, V, st);
+        hipLaunchKernelGGL(k_av, dim3(N), dim3(256), 0, ctx->stream, sd, V, P);
+        hipLaunchKernelGGL(k_w_qy_fast<256>, dim3(1), dim3(256), 0, ctx->stream, sd, V, P, W, U,
+                           (const FitState*)st);
+    } else {
+        hipLaunchKernelGGL(k_ir_init, dim3(1), dim3(1), 0, ctx->stream, st);
+        hipLaunchKernelGGL(k_hh_x, dim3(1), dim3(1024), 0, ctx->stream, sd, V, st);
+        hipLaunchKernelGGL(k_av, dim3(N), dim3(256), 0, ctx->stream, sd, V, P);
+        hipLaunchKernelGGL(k_build_w, dim3(1), dim3(1024), 0, ctx->stream, V, P, W, st, N);
+        hipLaunchKernelGGL(k_load_y, dim3(nblk(N)), dim3(256), 0, ctx->stream, sd, U);
+        hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 1);
+    }
     TW_LAUNCH_CHECK(ctx);
 
     auto factor_solve_fp64 = [&]() -> int {
@@ -2013,15 +2314,23 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
     }
     // 3. c' = Q [0; z];  t = Y - A' c';  Q' t (first 4 rows give q1' t);
     //    d' = R^-1 (q1' t); FP64 residual of the bordered system.
-    hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U, N, (const int*)nullptr);
-    hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 0);
-    hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U, U2, 1, (const int*)nullptr);
-    hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U2, N, 1);
-    hipLaunchKernelGGL(k_finish, dim3(nblk(N)), dim3(256), 0, ctx->stream, U, U2, st, x, N);
+    if (qfast) {
+        hipLaunchKernelGGL(k_q_fast<256>, dim3(1), dim3(256), 0, ctx->stream, V, (const FitState*)st, U, N, 0,
+                           (const double*)z, (const double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U, U2, 1, (const int*)nullptr);
+        hipLaunchKernelGGL(k_q_fast<256>, dim3(1), dim3(256), 0, ctx->stream, V, (const FitState*)st, U2, N, 1,
+                           (const double*)nullptr, (const double*)U, x);
+    } else {
+        hipLaunchKernelGGL(k_embed, dim3(nblk(N)), dim3(256), 0, ctx->stream, z, U, N, (const int*)nullptr);
+        hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U, N, 0);
+        hipLaunchKernelGGL(k_matvec, dim3(N), dim3(256), 0, ctx->stream, sd, U, U2, 1, (const int*)nullptr);
+        hipLaunchKernelGGL(k_q_apply, dim3(1), dim3(1024), 0, ctx->stream, V, st, U2, N, 1);
+        hipLaunchKernelGGL(k_finish, dim3(nblk(N)), dim3(256), 0, ctx->stream, U, U2, st, x, N);
+    }
     hipLaunchKernelGGL(k_residual, dim3(m), dim3(256), 0, ctx->stream, sd, x, res);
-    hipLaunchKernelGGL(k_res_max, dim3(1), dim3(1024), 0, ctx->stream, res, (i64)m * 4, st);
     TW_LAUNCH_CHECK(ctx);
     out.x = x;
+    out.res = res;
     out.st = st;
     out.iters = iters;
     return TW_OK;
@@ -2056,6 +2365,7 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     FitRun r;
     if ((rc = fit_run(ctx, centres, targets, K, lambda, precision, r))) return rc;
     const int N = r.N, m = r.m;
+    if (r.res) hipLaunchKernelGGL(k_res_max, dim3(1), dim3(256), 0, ctx->stream, r.res, m * 4, r.st);
     std::vector<double> hx((size_t)m * 4);
     TW_HIP(ctx, hipMemcpyAsync(hx.data(), r.x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     tw_stage_end(ctx, ST_FIT);
@@ -2090,7 +2400,7 @@ int tw_tps_fit_dev(tw_ctx* ctx, const float* centres, const float* targets, int 
     double* cf = (double*)p;
     double* d = cf + (size_t)K * 3;
     hipLaunchKernelGGL(k_unscale, dim3(nblk(K)), dim3(256), 0, ctx->stream, r.x, K, r.s, r.mu[0], r.mu[1], r.mu[2],
-                       cf, d);
+                       cf, d, r.res, (K + 4) * 4, r.st);
     TW_LAUNCH_CHECK(ctx);
     tw_stage_end(ctx, ST_FIT);
     out->pts = r.pts;
@@ -2180,7 +2490,7 @@ int tw_fit_batch(tw_ctx* ctx, TwFitItem* items, int n, double lambda, int precis
         if ((rc = fit_run(ctx, it.centres, it.targets, it.K, lambda, precision, r)))
             return tw_fail(ctx, rc, "item %d: %s", i, tw_last_error(ctx));
         hipLaunchKernelGGL(k_unscale, dim3(nblk(it.K)), dim3(256), 0, ctx->stream, r.x, it.K, r.s, r.mu[0], r.mu[1],
-                           r.mu[2], it.cf, it.d);
+                           r.mu[2], it.cf, it.d, r.res, (it.K + 4) * 4, r.st);
         TW_LAUNCH_CHECK(ctx);
         TW_HIP(ctx, hipMemcpyAsync(it.state, r.st, sizeof(FitState), hipMemcpyDeviceToDevice, ctx->stream));
         iters = std::max(iters, r.iters);
