@@ -762,6 +762,8 @@ struct StepLds {
             T Xs[NB][NB + 1];  // X_k = L_kk^-1, row-major
             T As[NB][NB + 1];  // As[c][r] = L_R[r][c]
             T Bs[NB][NB + 1];  // Bs[c][r] = L_C[r][c]
+            T rk[NB][4];       // forward substitution: r_k, then y_k
+            T yk[NB][4];
         } u;
         PotrfLds<T> potrf;
     };
@@ -793,7 +795,8 @@ __device__ __forceinline__ void step_panel(const double (&a)[NB / 4], const doub
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_step(T* __restrict__ B, T* __restrict__ Lp, T* __restrict__ Dinv,
-                                                   int n, int j0, int jb, FitState* st) {
+                                                   int n, int j0, int jb, FitState* st, T* __restrict__ rhs,
+                                                   T* __restrict__ yout) {
     static_assert(std::is_same<T, double>::value, "the fused step runs on the FP64 matrix cores");
     const int bx = blockIdx.x, by = blockIdx.y;
     if (by > bx) return;
@@ -854,6 +857,35 @@ __global__ __launch_bounds__(256) void k_chol_step(T* __restrict__ B, T* __restr
             const int r = R + 16 * w + 4 * m + lk, c = C + 16 * J + lr;
             if (r < n && c < n && c <= r) B[(i64)c * n + r] = old[J][m] - acc[J][m];
         }
+    // The forward substitution rides on the steps (rhs: n x 4, column-major):
+    // y_k = X_k r_k (r_k took every earlier step's update), then each trailing
+    // row block drops L_Rk y_k.  Work-groups (bx, 0), bx >= 1, take rows R,
+    // (1, 0) also rows r0 (its L_C) and writes y_k; with one trailing block
+    // (0, 0) does both before it factors.  These work-groups would otherwise
+    // idle while (0, 0) factors the next diagonal block.
+    const int nt = gridDim.x;
+    if (rhs && by == 0 && (bx >= 1 || nt == 1)) {
+        auto& rk = S.u.rk;
+        auto& yk = S.u.yk;
+        const int i = tid & (NB - 1), q = tid >> 6;
+        rk[i][q] = rhs[(i64)q * n + j0 + i];  // block k is full (jb = NB)
+        __syncthreads();
+        T y = 0;
+        #pragma unroll 8
+        for (int m = 0; m < NB; m++) y = fma(Xs[i][m], rk[m][q], y);
+        yk[i][q] = y;
+        if (bx == 1 || nt == 1) yout[(i64)q * n + j0 + i] = y;
+        __syncthreads();
+        auto drop = [&](const T (*Ls)[NB + 1], int R0) {
+            if (R0 + i >= n) return;
+            T v = rhs[(i64)q * n + R0 + i];
+            #pragma unroll 8
+            for (int c = 0; c < NB; c++) v = fma(-Ls[c][i], yk[c][q], v);
+            rhs[(i64)q * n + R0 + i] = v;
+        };
+        drop(As, R);
+        if (bx == 1) drop(Bs, C);
+    }
     if (bx == 0 && by == 0) {
         __syncthreads();  // the updated tile is in B; the step's LDS is free
         potrf_diag_body<T>(B, Dinv, n, r0, min(NB, n - r0), st, S.potrf);
@@ -1080,7 +1112,7 @@ constexpr int PIPE_THREADS = 256, PIPE_MAX_BLOCKS = 128;
 template <typename T>
 __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict__ L, const T* __restrict__ Dinv, int n,
                                                              T* __restrict__ b, T* __restrict__ y, int* flags,
-                                                             int epoch, FitState* st, const int* skip) {
+                                                             int epoch, FitState* st, const int* skip, int fwd_done) {
     if (skip && *skip) return;
     __shared__ T vs[NB][4];       // the published block being applied (y_k or x_k)
     __shared__ T part[4][NB][4];  // partial sums of the four quarters
@@ -1132,31 +1164,38 @@ __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict
         }
     }
     T acc[4], l[NB / 4];
-    // ---- forward: y_j = X_j (b_j - sum_{k<j} L_jk y_k)
-    #pragma unroll
-    for (int c = 0; c < 4; c++) acc[c] = (q == 0 && r < ib) ? b[(i64)c * n + j0 + r] : (T)0;
-    for (int k = 0; k < j; k++) {
-        const int k0 = k * NB;
+    // ---- forward: y_j = X_j (b_j - sum_{k<j} L_jk y_k).  With fwd_done the
+    // factor steps have done it for every block but the last (y holds y_j,
+    // b the last block's reduced right-hand side).
+    if (!fwd_done || j == nb - 1) {
         #pragma unroll
-        for (int m = 0; m < NB / 4; m++) l[m] = r < ib ? L[(i64)(k0 + 16 * q + m) * n + j0 + r] : (T)0;
-        wait_for(&fwd[k]);
-        vs[r][q] = y[(i64)q * n + k0 + r];  // block k is full (k < j)
-        __syncthreads();
-        #pragma unroll
-        for (int m = 0; m < NB / 4; m++)
+        for (int c = 0; c < 4; c++) acc[c] = (q == 0 && r < ib) ? b[(i64)c * n + j0 + r] : (T)0;
+        for (int k = 0; k < (fwd_done ? 0 : j); k++) {
+            const int k0 = k * NB;
             #pragma unroll
-            for (int c = 0; c < 4; c++) acc[c] = fma(-l[m], vs[16 * q + m][c], acc[c]);
-        __syncthreads();
+            for (int m = 0; m < NB / 4; m++) l[m] = r < ib ? L[(i64)(k0 + 16 * q + m) * n + j0 + r] : (T)0;
+            wait_for(&fwd[k]);
+            vs[r][q] = y[(i64)q * n + k0 + r];  // block k is full (k < j)
+            __syncthreads();
+            #pragma unroll
+            for (int m = 0; m < NB / 4; m++)
+                #pragma unroll
+                for (int c = 0; c < 4; c++) acc[c] = fma(-l[m], vs[16 * q + m][c], acc[c]);
+            __syncthreads();
+        }
+        reduce(acc);
+        {
+            T s = 0;
+            #pragma unroll 8
+            for (int m = 0; m < NB; m++) s = fma(Xs[r][m], accs[m][q], s);
+            ys[r][q] = s;
+            if (r < ib) y[(i64)q * n + j0 + r] = s;
+        }
+        if (!fwd_done) publish(&fwd[j]);
+    } else {
+        ys[r][q] = r < ib ? y[(i64)q * n + j0 + r] : (T)0;
     }
-    reduce(acc);
-    {
-        T s = 0;
-        #pragma unroll 8
-        for (int m = 0; m < NB; m++) s = fma(Xs[r][m], accs[m][q], s);
-        ys[r][q] = s;
-        if (r < ib) y[(i64)q * n + j0 + r] = s;
-    }
-    publish(&fwd[j]);
+    __syncthreads();
     // ---- backward: x_j = X_j' (y_j - sum_{k>j} L_kj' x_k)
     #pragma unroll
     for (int c = 0; c < 4; c++) acc[c] = q == 0 ? ys[r][c] : (T)0;
@@ -1190,8 +1229,10 @@ constexpr int SOLVE_MULTI_MIN = 1025;
 // L L' x = b (n x 4 column-major, in place); y: n x 4 scratch.  Up to
 // PIPE_MAX_BLOCKS blocks the pipelined kernel runs (one launch); above, one
 // launch per block.
+// fwd_done: the factor steps ran the forward substitution (pipelined path only).
 template <typename T>
-int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip, FitState* st) {
+int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip, FitState* st,
+               int fwd_done = 0) {
     const int nb = (n + NB - 1) / NB;
     if (nb <= PIPE_MAX_BLOCKS) {
         if (!ctx->solve_flags) {
@@ -1203,7 +1244,7 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
         if (++ctx->solve_epoch == 0) ctx->solve_epoch = 1;  // a new value per call, never the initial 0
         const int epoch = (int)ctx->solve_epoch;
         hipLaunchKernelGGL(k_solve_pipe<T>, dim3(nb), dim3(PIPE_THREADS), 0, ctx->stream, L, Dinv, n, b, y,
-                           ctx->solve_flags, epoch, st, skip);
+                           ctx->solve_flags, epoch, st, skip, fwd_done);
     } else if (n < SOLVE_MULTI_MIN) {
         hipLaunchKernelGGL(k_chol_solve<T>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, L, Dinv, n, b, skip);
     } else {
@@ -1749,12 +1790,16 @@ inline unsigned nblk(i64 n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // Blocked Cholesky with the fused steps (small systems): the off-diagonal
 // blocks of the factor go to Lp.
+// With rhs (n x 4), the steps also run the forward substitution: yout gets
+// y = L^-1 rhs for every block but the last, and rhs the last block's
+// reduced right-hand side (the solve finishes with fwd_done).
 template <typename T>
-void cholesky_steps(tw_ctx* ctx, T* B, T* Lp, T* Dinv, int n, FitState* st) {
+void cholesky_steps(tw_ctx* ctx, T* B, T* Lp, T* Dinv, int n, FitState* st, T* rhs, T* yout) {
     hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, Dinv, n, 0, std::min(NB, n), st);
     for (int j0 = 0; j0 + NB < n; j0 += NB) {
         const unsigned nt = nblk(n - j0 - NB, NB);
-        hipLaunchKernelGGL(k_chol_step<T>, dim3(nt, nt), dim3(256), 0, ctx->stream, B, Lp, Dinv, n, j0, NB, st);
+        hipLaunchKernelGGL(k_chol_step<T>, dim3(nt, nt), dim3(256), 0, ctx->stream, B, Lp, Dinv, n, j0, NB, st, rhs,
+                           yout);
     }
 }
 
@@ -2253,17 +2298,19 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         }
         hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
         const double* Lf = B;  // where the solve reads the factor's off-diagonal blocks
-        if ((n + NB - 1) / NB <= STEP_MAX_BLOCKS) {
+        // The right-hand side first: the fused steps run the forward substitution.
+        hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
+        const bool fused = (n + NB - 1) / NB <= STEP_MAX_BLOCKS;
+        if (fused) {
             void* lp = nullptr;
             int r2 = tw_ws(ctx, WS_SOLVE_L, (size_t)n * n * sizeof(double), &lp);
             if (r2) return r2;
-            cholesky_steps<double>(ctx, B, (double*)lp, (double*)dinv, n, st);
+            cholesky_steps<double>(ctx, B, (double*)lp, (double*)dinv, n, st, z, (double*)ys);
             Lf = (const double*)lp;
         } else {
             cholesky<double>(ctx, B, (double*)dinv, n, st);
         }
-        hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
-        return chol_solve<double>(ctx, Lf, (const double*)dinv, n, z, (double*)ys, nullptr, st);
+        return chol_solve<double>(ctx, Lf, (const double*)dinv, n, z, (double*)ys, nullptr, st, fused ? 1 : 0);
     };
 
     int iters = 0;
