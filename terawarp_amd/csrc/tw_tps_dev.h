@@ -253,16 +253,11 @@ __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, 
     return t;
 }
 
-// The eight taps of a voxel as four (x, x+1) pairs: y+1 / z+1 through the
-// shifted resources.  Pairs of zero weight may land in the next plane or
-// past the channel, where the range check returns 0.
-template <typename T, int MODE, int INTERP>
-__device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
-    const uint32_t off = t.off;
-    __builtin_assume(off <= OOB_OFF);
-    if (INTERP == TW_INTERP_NN) return bld<T>(rs.r00, off);
-    const uint32_t p00 = bld2<T>(rs.r00, off), p10 = bld2<T>(rs.r01, off);
-    const uint32_t p01 = bld2<T>(rs.r10, off), p11 = bld2<T>(rs.r11, off);
+// Trilinear blend of one voxel from its four (x, x+1) pairs: p00 / p10 the
+// lower plane's rows y / y+1, p01 / p11 the upper plane's.
+template <typename T, int MODE>
+__device__ __forceinline__ uint32_t blend_pairs(uint32_t p00, uint32_t p10, uint32_t p01, uint32_t p11,
+                                                const Tap& t) {
     const uint32_t u000 = pair_lo<T>(p00), u100 = pair_hi<T>(p00);
     const uint32_t u010 = pair_lo<T>(p10), u110 = pair_hi<T>(p10);
     const uint32_t u001 = pair_lo<T>(p01), u101 = pair_hi<T>(p01);
@@ -286,6 +281,48 @@ __device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
     const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
     const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
     return cvt_rpi(__builtin_fmaf(t.wz, b1 - b0, b0));
+}
+
+// The eight taps of a voxel as four (x, x+1) pairs: y+1 / z+1 through the
+// shifted resources.  Pairs of zero weight may land in the next plane or
+// past the channel, where the range check returns 0.
+template <typename T, int MODE, int INTERP>
+__device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
+    const uint32_t off = t.off;
+    __builtin_assume(off <= OOB_OFF);
+    if (INTERP == TW_INTERP_NN) return bld<T>(rs.r00, off);
+    return blend_pairs<T, MODE>(bld2<T>(rs.r00, off), bld2<T>(rs.r01, off), bld2<T>(rs.r10, off),
+                                bld2<T>(rs.r11, off), t);
+}
+
+// Four consecutive slices of one column (trilinear, every tap in bounds):
+// where slice j's offset is slice j-1's plus one plane, its lower plane is
+// slice j-1's upper plane (the same bytes), so those two pairs are not
+// loaded again; the other lanes load them under an exec mask.  Results are
+// those of blend.
+template <typename T, int MODE>
+__device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v) {
+    uint32_t up0[4], up1[4], lo0[4], lo1[4];
+    #pragma unroll
+    for (int j = 0; j < 4; j++) {
+        __builtin_assume(tp[j].off <= OOB_OFF);
+        up0[j] = bld2<T>(rs.r10, tp[j].off);
+        up1[j] = bld2<T>(rs.r11, tp[j].off);
+    }
+    lo0[0] = bld2<T>(rs.r00, tp[0].off);
+    lo1[0] = bld2<T>(rs.r01, tp[0].off);
+    #pragma unroll
+    for (int j = 1; j < 4; j++) {
+        uint32_t a = up0[j - 1], b = up1[j - 1];
+        if (tp[j].off != tp[j - 1].off + plane_b) {
+            a = bld2<T>(rs.r00, tp[j].off);
+            b = bld2<T>(rs.r01, tp[j].off);
+        }
+        lo0[j] = a;
+        lo1[j] = b;
+    }
+    #pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = blend_pairs<T, MODE>(lo0[j], lo1[j], up0[j], up1[j], tp[j]);
 }
 
 // floor(i / d) for 0 <= i < 2^21 as trunc((i + 1/2) / d) in FP32, with
@@ -485,8 +522,12 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
             tp[j] = tap_setup<T, INTERP, WIN, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2]);
         }
         uint32_t v[4];
-        #pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
+        if constexpr (!CHECK && INTERP != TW_INTERP_NN) {
+            blend4_shared<T, MODE>(rs, tp, g.plane_b, v);
+        } else {
+            #pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
+        }
         #pragma unroll
         for (int j = 0; j < 4; j++)
             if (FULL || j < nh) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
