@@ -298,10 +298,12 @@ __device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
 // Four consecutive slices of one column (trilinear, every tap in bounds):
 // where slice j's offset is slice j-1's plus one plane, its lower plane is
 // slice j-1's upper plane (the same bytes), so those two pairs are not
-// loaded again; the other lanes load them under an exec mask.  Results are
-// those of blend.
+// loaded again; the other lanes load them under an exec mask.  (have, poff,
+// pu0, pu1) carry the previous group's last slice the same way.  Results
+// are those of blend.
 template <typename T, int MODE>
-__device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v) {
+__device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v,
+                                              bool& have, uint32_t& poff, uint32_t& pu0, uint32_t& pu1) {
     uint32_t up0[4], up1[4], lo0[4], lo1[4];
     #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -309,8 +311,15 @@ __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, ui
         up0[j] = bld2<T>(rs.r10, tp[j].off);
         up1[j] = bld2<T>(rs.r11, tp[j].off);
     }
-    lo0[0] = bld2<T>(rs.r00, tp[0].off);
-    lo1[0] = bld2<T>(rs.r01, tp[0].off);
+    {
+        uint32_t a = pu0, b = pu1;
+        if (!(have && tp[0].off == poff + plane_b)) {
+            a = bld2<T>(rs.r00, tp[0].off);
+            b = bld2<T>(rs.r01, tp[0].off);
+        }
+        lo0[0] = a;
+        lo1[0] = b;
+    }
     #pragma unroll
     for (int j = 1; j < 4; j++) {
         uint32_t a = up0[j - 1], b = up1[j - 1];
@@ -321,6 +330,10 @@ __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, ui
         lo0[j] = a;
         lo1[j] = b;
     }
+    have = true;
+    poff = tp[3].off;
+    pu0 = up0[3];
+    pu1 = up1[3];
     #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = blend_pairs<T, MODE>(lo0[j], lo1[j], up0[j], up1[j], tp[j]);
 }
@@ -509,7 +522,10 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
     const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
     // Cell by cell: the node-layer window moves once per cell; inside a
     // cell four slices per iteration keep 32 taps in flight (a short last
-    // group repeats its last slice and stores only the real ones).
+    // group repeats its last slice and stores only the real ones).  The
+    // last slice's upper-plane pairs carry to the next group.
+    bool have = false;
+    uint32_t poff = 0, pu0 = 0, pu1 = 0;
     auto group = [&](auto check, auto full, int nh) {
         constexpr bool CHECK = decltype(check)::value, FULL = decltype(full)::value;
         Tap tp[4];
@@ -523,10 +539,11 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
         }
         uint32_t v[4];
         if constexpr (!CHECK && INTERP != TW_INTERP_NN) {
-            blend4_shared<T, MODE>(rs, tp, g.plane_b, v);
+            blend4_shared<T, MODE>(rs, tp, g.plane_b, v, have, poff, pu0, pu1);
         } else {
             #pragma unroll
             for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
+            have = false;
         }
         #pragma unroll
         for (int j = 0; j < 4; j++)
