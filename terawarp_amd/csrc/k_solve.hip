@@ -1666,11 +1666,12 @@ __device__ __forceinline__ void small_tg(const double* __restrict__ T, const dou
 }
 
 // W = P T - 1/2 V M (M = T' (V'P) T) and U = Q'Y = Y - V T' (V'Y), one pass
-// of sums for both.
+// of sums for both; z (n x 4) gets rows 4.. of U (the FP64 right-hand side).
 template <int QT>
 __global__ __launch_bounds__(QT) void k_w_qy_fast(const SysDesc s, const double* __restrict__ V,
                                                   const double* __restrict__ P, double* __restrict__ W,
-                                                  double* __restrict__ U, const FitState* __restrict__ st) {
+                                                  double* __restrict__ U, const FitState* __restrict__ st,
+                                                  double* __restrict__ z) {
     __shared__ double red[17 * 32];
     const int N = s.N, tid = threadIdx.x;
     double S[32];
@@ -1717,6 +1718,7 @@ __global__ __launch_bounds__(QT) void k_w_qy_fast(const SysDesc s, const double*
             }
             W[(size_t)b * N + i] = acc;
             U[(size_t)b * N + i] = y[b] - q;
+            if (i >= 4) z[(size_t)b * (N - 4) + i - 4] = y[b] - q;
         }
     }
 }
@@ -2278,7 +2280,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         else hipLaunchKernelGGL(k_qr_fast<256>, dim3(1), dim3(256), 0, ctx->stream, sd, V, st);
         hipLaunchKernelGGL(k_av, dim3(N), dim3(256), 0, ctx->stream, sd, V, P);
         hipLaunchKernelGGL(k_w_qy_fast<256>, dim3(1), dim3(256), 0, ctx->stream, sd, V, P, W, U,
-                           (const FitState*)st);
+                           (const FitState*)st, z);
     } else {
         hipLaunchKernelGGL(k_ir_init, dim3(1), dim3(1), 0, ctx->stream, st);
         hipLaunchKernelGGL(k_hh_x, dim3(1), dim3(1024), 0, ctx->stream, sd, V, st);
@@ -2298,8 +2300,11 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         }
         hipLaunchKernelGGL(k_build_b<double>, dim3(nblk((i64)n * n)), dim3(256), 0, ctx->stream, sd, V, W, B);
         const double* Lf = B;  // where the solve reads the factor's off-diagonal blocks
-        // The right-hand side first: the fused steps run the forward substitution.
-        hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
+        // The right-hand side first (k_w_qy_fast wrote it unless the FP32
+        // refinement has used z since): the fused steps run the forward
+        // substitution.
+        if (!qfast || mixed)
+            hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
         const bool fused = (n + NB - 1) / NB <= STEP_MAX_BLOCKS;
         if (fused) {
             void* lp = nullptr;
