@@ -42,6 +42,43 @@ def test_fit_large_system_matches_oracle(warper, oracle):
         assert warper.timings()["fit_residual"] < 1e-10
 
 
+@pytest.mark.parametrize("K", [257, 1024, 1028, 1029])
+def test_fit_path_boundaries_match_oracle(warper, oracle, K):
+    """Fits at the boundaries of the large-system paths: K = 257 (fewest
+    landmarks on the kernel sequence; a partial last 64-block), 1024 (the
+    one-wave QR at its largest N; 16 blocks, the last partial, in the fused
+    factor steps that also run the forward substitution), 1028 (16 full
+    blocks; four-wave QR) and 1029 (17 blocks: per-block-column factor and
+    the two-pass pipelined solve).  Coefficients match the oracle's bordered
+    FP64 solve."""
+    sub, tar = synth.landmarks((512, 512, 256), K, 10.0, seed=K)
+    c_ref, d_ref = oracle.stps_fit(tar, sub, bordered=True)
+    c, d = warper.tps_fit(tar, sub, precision=warp.PLAIN)
+    assert rel_err(c, c_ref) < 1e-8, rel_err(c, c_ref)
+    assert np.abs(d - d_ref).max() < 1e-8
+    assert warper.timings()["fit_residual"] < 1e-10
+
+
+def test_fit_four_wave_qr_at_its_largest(warper):
+    """K = 4096: the four-wave QR kernels at their largest N (16 rows per
+    thread).  The device's FP64 residual of the bordered system (formed from
+    the landmarks, independent of the QR) certifies the solution, and the
+    chain's affine part reproduces an exact affine map (pure translation
+    targets give d = identity + shift, c = 0)."""
+    sub, tar = synth.landmarks((2048, 2048, 512), 4096, 20.0, seed=5)
+    c, d = warper.tps_fit(tar, sub, precision=warp.PLAIN)
+    assert warper.timings()["fit_residual"] < 1e-10
+    # Integer centres and shift: the shifted centres are exact in FP32.
+    p = np.unique(np.round(sub), axis=0)[:4096].astype(np.float32)
+    assert len(p) == 4096
+    shift = np.array([3.0, -2.0, 5.0], np.float32)
+    c2, d2 = warper.tps_fit(p + shift, p, precision=warp.PLAIN)
+    assert warper.timings()["fit_residual"] < 1e-10
+    assert np.abs(c2).max() < 1e-9, np.abs(c2).max()
+    assert np.abs(d2[0, 1:] + shift).max() < 1e-6, d2[0]
+    assert np.abs(d2[1:, 1:] - np.eye(3)).max() < 1e-9, d2[1:, 1:]
+
+
 def test_fit_fp32_refinement_large(warper):
     """Above 4096 projected unknowns TW_MIXED factors in FP32 and refines the
     solution with FP64 residuals (an FP64 refactor if refinement stalls);
