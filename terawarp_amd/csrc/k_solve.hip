@@ -391,8 +391,12 @@ struct PotrfLds {
     T colb[2][4][NB];                  // L~[r][k] (0 for r < k), by step parity
     __align__(16) T colp[2][4][NB];    // L~[c][k] / d_k for columns c right of the block, else 0
     T dsh[NB];                         // pivots d_k
-    // Lc[q][16p + i] = L[4i + p][q]: lane p of a quad reads its rows' terms contiguously.
-    __align__(16) T Lc[NB][LS];        // L by columns
+    T rdsh[NB];                        // 1 / d_k (FP64 factor)
+    union {
+        // Lc[q][16p + i] = L[4i + p][q]: lane p of a quad reads its rows' terms contiguously.
+        __align__(16) T Lc[NB][LS];    // L by columns
+        __align__(16) T Am[NB][LS];    // FP64 factor: the working block, row-major
+    };
     __align__(16) T Xs[NB][LS];        // X = L^-1, row-major (FP64 blocked inverse)
     T rsqsh[NB];                       // 1 / L[c][c] = d_c^-1/2
     int badw[4];                       // first non-positive pivot each wave met (NB: none)
@@ -460,9 +464,149 @@ __device__ __forceinline__ void inverse_blocked(const double (*Lc)[LS], double (
     if (w == 3) finish(3, 0, lx(3, 2, 0, y30));
 }
 
+// FP64 diagonal block with contiguous ownership: wave s owns columns
+// 16s .. 16s+15 of the block (staged row-major in LDS) and factors them in
+// four 4-column steps inside the wave, with no work-group barrier: each
+// step eliminates its 4 x 4 diagonal part on wave-uniform values (each
+// pivot's reciprocal waits only on the previous one), forms the columns
+// lane-parallel, writes L~ back into the block, and drops the step from the
+// wave's later columns (the multipliers go through a wave-private LDS row,
+// read in order by the same wave).  Then the trailing 16 x 16 tiles of the
+// later sub-blocks take the rank-16 update on the matrix cores (A = the
+// sub-block's L~, B = L~ / d), dealt over the four waves: four barriers per
+// block instead of sixteen, and no broadcast reads of pivot rows.  L, the
+// pivots and X = L^-1 follow as in potrf_diag_body.
+__device__ __forceinline__ void potrf_diag_fp64(double* __restrict__ B, double* __restrict__ Dinv, int n, int j0,
+                                                int jb, FitState* st, PotrfLds<double>& S) {
+    auto& Am = S.Am;
+    auto& dsh = S.dsh;
+    auto& rdsh = S.rdsh;
+    auto& mw = S.colp[0];
+    const int tid = threadIdx.x, r = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lr = r & 15, lk = r >> 4;
+    const bool rin = r < jb;
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) {  // column-major B: consecutive threads take consecutive rows
+        const int e = tid + 256 * u, c = e >> 6, rr = e & 63;
+        Am[rr][c] = (rr < jb && c < jb) ? B[(i64)(j0 + c) * n + j0 + rr] : (rr == c ? 1.0 : 0.0);
+    }
+    int mybad = NB;  // first non-positive pivot this wave met (wave-uniform)
+    #pragma unroll
+    for (int s = 0; s < 4; s++) {
+        __syncthreads();
+        if (w == s) {
+            double a[16];  // column 16s + c at row r
+            #pragma unroll
+            for (int c = 0; c < 16; c++) a[c] = Am[r][16 * s + c];
+            #pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k0 = 16 * s + 4 * t;
+                double* ab = &a[4 * t];
+                const double s00 = bcast(ab[0], k0), s10 = bcast(ab[0], k0 + 1), s20 = bcast(ab[0], k0 + 2);
+                const double s30 = bcast(ab[0], k0 + 3), s11 = bcast(ab[1], k0 + 1), s21 = bcast(ab[1], k0 + 2);
+                const double s31 = bcast(ab[1], k0 + 3), s22 = bcast(ab[2], k0 + 2), s32 = bcast(ab[2], k0 + 3);
+                const double s33 = bcast(ab[3], k0 + 3);
+                const double r0 = recip(s00);
+                const double l10 = s10 * r0, l20 = s20 * r0, l30 = s30 * r0;
+                const double d1 = fma(-l10, s10, s11), t21 = fma(-l20, s10, s21), t31 = fma(-l30, s10, s31);
+                const double t22 = fma(-l20, s20, s22), t32 = fma(-l30, s20, s32), t33 = fma(-l30, s30, s33);
+                const double r1 = recip(d1);
+                const double l21 = t21 * r1, l31 = t31 * r1;
+                const double d2 = fma(-l21, t21, t22), u32 = fma(-l31, t21, t32), u33 = fma(-l31, t31, t33);
+                const double r2 = recip(d2);
+                const double l32 = u32 * r2;
+                const double d3 = fma(-l32, u32, u33);
+                const double r3 = recip(d3);
+                mybad = min(mybad, s00 > 0.0 ? NB : k0);
+                mybad = min(mybad, d1 > 0.0 ? NB : k0 + 1);
+                mybad = min(mybad, d2 > 0.0 ? NB : k0 + 2);
+                mybad = min(mybad, d3 > 0.0 ? NB : k0 + 3);
+                const double lr0 = r >= k0 ? ab[0] : 0.0, m0 = lr0 * r0;
+                ab[1] = fma(-m0, s10, ab[1]);
+                ab[2] = fma(-m0, s20, ab[2]);
+                ab[3] = fma(-m0, s30, ab[3]);
+                const double lr1 = r >= k0 + 1 ? ab[1] : 0.0, m1 = lr1 * r1;
+                ab[2] = fma(-m1, t21, ab[2]);
+                ab[3] = fma(-m1, t31, ab[3]);
+                const double lr2 = r >= k0 + 2 ? ab[2] : 0.0, m2 = lr2 * r2;
+                ab[3] = fma(-m2, u32, ab[3]);
+                const double lr3 = r >= k0 + 3 ? ab[3] : 0.0, m3 = lr3 * r3;
+                Am[r][k0 + 0] = lr0;
+                Am[r][k0 + 1] = lr1;
+                Am[r][k0 + 2] = lr2;
+                Am[r][k0 + 3] = lr3;
+                if (r < 4) {
+                    dsh[k0 + r] = r == 0 ? s00 : r == 1 ? d1 : r == 2 ? d2 : d3;
+                    rdsh[k0 + r] = r == 0 ? r0 : r == 1 ? r1 : r == 2 ? r2 : r3;
+                }
+                if (t < 3) {
+                    mw[0][r] = m0;
+                    mw[1][r] = m1;
+                    mw[2][r] = m2;
+                    mw[3][r] = m3;
+                    #pragma unroll
+                    for (int c = 4 * t + 4; c < 16; c++) {
+                        const int cr = 16 * s + c;
+                        double v = fma(-lr0, mw[0][cr], a[c]);
+                        v = fma(-lr1, mw[1][cr], v);
+                        v = fma(-lr2, mw[2][cr], v);
+                        a[c] = fma(-lr3, mw[3][cr], v);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (s < 3) {
+            // Tiles (I, J), s < J <= I <= 3, dealt to the waves in order.
+            int p = 0;
+            #pragma unroll
+            for (int J = s + 1; J < 4; J++)
+                #pragma unroll
+                for (int I = J; I < 4; I++, p++) {
+                    if ((p & 3) != w) continue;
+                    v4d c, d = {0, 0, 0, 0};
+                    #pragma unroll
+                    for (int m = 0; m < 4; m++) c[m] = Am[16 * I + 4 * m + lk][16 * J + lr];
+                    #pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int k = 16 * s + 4 * q + lk;
+                        d = mfma16x4(Am[16 * I + lr][k], Am[16 * J + lr][k] * rdsh[k], d);
+                    }
+                    #pragma unroll
+                    for (int m = 0; m < 4; m++) Am[16 * I + 4 * m + lk][16 * J + lr] = c[m] - d[m];
+                }
+        }
+    }
+    // L = L~ D^-1/2 (lower part; the rest is zero), row r, columns 16w + c.
+    if (r == 0) S.badw[w] = mybad;
+    const double rsr = 1.0 / sqrt(dsh[r]);
+    if (w == 0) S.rsqsh[r] = rsr;
+    double Lv[16];
+    #pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const int cc = 16 * w + c;
+        Lv[c] = r >= cc ? Am[r][cc] * bcast(rsr, cc) : 0.0;
+        if (rin && cc < jb && cc <= r) B[(i64)(j0 + cc) * n + j0 + r] = Lv[c];
+    }
+    __syncthreads();  // the block is read: Lc takes its storage
+    #pragma unroll
+    for (int c = 0; c < 16; c++) S.Lc[16 * w + c][16 * (r & 3) + (r >> 2)] = Lv[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int bad = min(min(S.badw[0], S.badw[1]), min(S.badw[2], S.badw[3]));
+        if (bad < NB && st->info == 0) st->info = 2 + j0 + bad;
+    }
+    inverse_blocked(S.Lc, S.Xs, S.rsqsh, Dinv + (i64)(j0 / NB) * NB * NB, r, w);
+}
+
 template <typename T>
 __device__ __forceinline__ void potrf_diag_body(T* __restrict__ B, T* __restrict__ Dinv, int n, int j0, int jb,
                                                 FitState* st, PotrfLds<T>& S) {
+    if constexpr (std::is_same<T, double>::value) {
+        potrf_diag_fp64(B, Dinv, n, j0, jb, st, S);
+        return;
+    }
     auto& colb = S.colb;
     auto& colp = S.colp;
     auto& dsh = S.dsh;
