@@ -476,8 +476,9 @@ __device__ __forceinline__ void inverse_blocked(const double (*Lc)[LS], double (
 // sub-block's L~, B = L~ / d), dealt over the four waves: four barriers per
 // block instead of sixteen, and no broadcast reads of pivot rows.  L, the
 // pivots and X = L^-1 follow as in potrf_diag_body.
+// staged: the caller has written the block into S.Am (identity past jb).
 __device__ __forceinline__ void potrf_diag_fp64(double* __restrict__ B, double* __restrict__ Dinv, int n, int j0,
-                                                int jb, FitState* st, PotrfLds<double>& S) {
+                                                int jb, FitState* st, PotrfLds<double>& S, bool staged = false) {
     auto& Am = S.Am;
     auto& dsh = S.dsh;
     auto& rdsh = S.rdsh;
@@ -486,10 +487,18 @@ __device__ __forceinline__ void potrf_diag_fp64(double* __restrict__ B, double* 
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lr = r & 15, lk = r >> 4;
     const bool rin = r < jb;
-    #pragma unroll
-    for (int u = 0; u < NB * NB / 256; u++) {  // column-major B: consecutive threads take consecutive rows
-        const int e = tid + 256 * u, c = e >> 6, rr = e & 63;
-        Am[rr][c] = (rr < jb && c < jb) ? B[(i64)(j0 + c) * n + j0 + rr] : (rr == c ? 1.0 : 0.0);
+    if (!staged) {
+        double v[NB * NB / 256];  // clamped loads, all in flight at once
+        #pragma unroll
+        for (int u = 0; u < NB * NB / 256; u++) {  // column-major B: consecutive threads take consecutive rows
+            const int e = tid + 256 * u, c = e >> 6, rr = e & 63;
+            v[u] = B[(i64)(j0 + min(c, jb - 1)) * n + j0 + min(rr, jb - 1)];
+        }
+        #pragma unroll
+        for (int u = 0; u < NB * NB / 256; u++) {
+            const int e = tid + 256 * u, c = e >> 6, rr = e & 63;
+            Am[rr][c] = (rr < jb && c < jb) ? v[u] : (rr == c ? 1.0 : 0.0);
+        }
     }
     int mybad = NB;  // first non-positive pivot this wave met (wave-uniform)
     #pragma unroll
@@ -1031,8 +1040,19 @@ __global__ __launch_bounds__(256) void k_chol_step(T* __restrict__ B, T* __restr
         if (bx == 1) drop(Bs, C);
     }
     if (bx == 0 && by == 0) {
-        __syncthreads();  // the updated tile is in B; the step's LDS is free
-        potrf_diag_body<T>(B, Dinv, n, r0, min(NB, n - r0), st, S.potrf);
+        // The updated tile is the next diagonal block: it goes from the
+        // registers into the factor's LDS block (identity past its extent)
+        // once the step's LDS is free.
+        const int jb1 = min(NB, n - r0);
+        __syncthreads();
+        #pragma unroll
+        for (int J = 0; J < 4; J++)
+            #pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int rr = 16 * w + 4 * m + lk, cc = 16 * J + lr;
+                S.potrf.Am[rr][cc] = (rr < jb1 && cc < jb1) ? old[J][m] - acc[J][m] : (rr == cc ? 1.0 : 0.0);
+            }
+        potrf_diag_fp64(B, Dinv, n, r0, jb1, st, S.potrf, true);
     }
 }
 
