@@ -127,6 +127,7 @@ __device__ __forceinline__ void grid_tiled_body(const GridArgs& a, int bx, int b
         __syncthreads();
         if (live) {
             const float* sr = syz[row];
+            #pragma unroll 16
             for (int n = 0; n < cnt; n++) {
                 const int m = n0 + n;
                 const float dx = Xf - a.pts[m].x;
