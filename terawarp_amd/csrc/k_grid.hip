@@ -181,22 +181,23 @@ __device__ __forceinline__ void quantize_body(const float* __restrict__ df, __ha
             mx[c] = fmaxf(mx[c], v);
         }
     }
+    // Min / max over the work-group: within each wave by shuffles, then the
+    // four waves' results (order-independent, so the same bits as any tree).
     for (int c = 0; c < 3; c++) {
-        red[0][c][threadIdx.x] = mn[c];
-        red[1][c][threadIdx.x] = mx[c];
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[c] = fminf(mn[c], __shfl_xor(mn[c], o, 64));
+            mx[c] = fmaxf(mx[c], __shfl_xor(mx[c], o, 64));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[0][c][threadIdx.x >> 6] = mn[c];
+            red[1][c][threadIdx.x >> 6] = mx[c];
+        }
     }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int c = 0; c < 3; c++) {
-                red[0][c][threadIdx.x] = fminf(red[0][c][threadIdx.x], red[0][c][threadIdx.x + s]);
-                red[1][c][threadIdx.x] = fmaxf(red[1][c][threadIdx.x], red[1][c][threadIdx.x + s]);
-            }
-        __syncthreads();
-    }
     if (threadIdx.x < 3) {
         const int c = threadIdx.x;
-        const float lo = red[0][c][0], hi = red[1][c][0];
+        const float lo = fminf(fminf(red[0][c][0], red[0][c][1]), fminf(red[0][c][2], red[0][c][3]));
+        const float hi = fmaxf(fmaxf(red[1][c][0], red[1][c][1]), fmaxf(red[1][c][2], red[1][c][3]));
         const float off = 0.5f * (lo + hi);
         float sc = fmaxf(hi - off, off - lo);
         if (!(sc > 0.f)) sc = 1.f;
