@@ -171,14 +171,24 @@ __device__ __forceinline__ void quantize_body(const float* __restrict__ df, __ha
     const i64 plane = gx * gy * gz;
     const i64 bx = b % qbx, by = (b / qbx) % qby, bz = b / (qbx * qby);
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int t = threadIdx.x; t < QB * QB * QB; t += 256) {
+    // A thread's two nodes stay in registers for the quantising pass.
+    constexpr int NPT = QB * QB * QB / 256;
+    float v[NPT][3];
+    i64 idx[NPT];
+    bool in[NPT];
+    #pragma unroll
+    for (int u = 0; u < NPT; u++) {
+        const int t = threadIdx.x + 256 * u;
         const i64 i = bx * QB + (t % QB), j = by * QB + (t / QB) % QB, k = bz * QB + t / (QB * QB);
-        if (i >= gx || j >= gy || k >= gz) continue;
-        const i64 idx = (k * gy + j) * gx + i;
+        in[u] = i < gx && j < gy && k < gz;
+        idx[u] = (k * gy + j) * gx + i;
+        #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float v = df[c * plane + idx];
-            mn[c] = fminf(mn[c], v);
-            mx[c] = fmaxf(mx[c], v);
+            v[u][c] = in[u] ? df[c * plane + idx[u]] : 0.f;
+            if (in[u]) {
+                mn[c] = fminf(mn[c], v[u][c]);
+                mx[c] = fmaxf(mx[c], v[u][c]);
+            }
         }
     }
     // Min / max over the work-group: within each wave by shuffles, then the
@@ -207,15 +217,14 @@ __device__ __forceinline__ void quantize_body(const float* __restrict__ df, __ha
         qtab[b * 6 + 3 + c] = sc;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < QB * QB * QB; t += 256) {
-        const i64 i = bx * QB + (t % QB), j = by * QB + (t / QB) % QB, k = bz * QB + t / (QB * QB);
-        if (i >= gx || j >= gy || k >= gz) continue;
-        const i64 idx = (k * gy + j) * gx + i;
-        for (int c = 0; c < 3; c++) {
-            const float h = (df[c * plane + idx] - prm[c]) / prm[3 + c];
-            out[c * plane + idx] = __float2half_rn(fminf(1.f, fmaxf(-1.f, h)));
-        }
-    }
+    #pragma unroll
+    for (int u = 0; u < NPT; u++)
+        if (in[u])
+            #pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float h = (v[u][c] - prm[c]) / prm[3 + c];
+                out[c * plane + idx[u]] = __float2half_rn(fminf(1.f, fmaxf(-1.f, h)));
+            }
 }
 
 __global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ df, __half* __restrict__ out,
