@@ -1,8 +1,9 @@
 """ctypes view of the FP64 CPU oracle (oracle/tw_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The terawarp_amd product never
-imports this module and has no CPU fallback.
+cpu_baseline leg of bench.py, as the reference the GPU results are compared
+with.  The terawarp_amd product never imports this module and has no CPU
+fallback.
 
 Array conventions (numpy):
   points          (K, 3) float32, xyz

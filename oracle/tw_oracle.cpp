@@ -1,8 +1,9 @@
 // tw_oracle.cpp -- FP64 CPU restatement of TeraWarp's warp path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by the
-// tests, by __graft_entry__.smoke() and by the CPU-baseline leg of bench.py,
-// never by the terawarp_amd product library, which has no CPU fallback.
+// TEST INFRASTRUCTURE ONLY.  This file is what the GPU results are compared
+// with: it is loaded by the tests, by __graft_entry__.smoke() and by the
+// CPU-baseline leg of bench.py, never by the terawarp_amd product library,
+// which has no CPU fallback.
 //
 // Every function restates one function of the reference's CPU path
 // (q_imgwarp_tps_quicksmallmemory.cpp, q_warp_affine_tps.cpp) with plain
