@@ -20,10 +20,13 @@
 //  * B is symmetric positive definite (-|r| is conditionally positive
 //    definite of order 1 and q2 is orthogonal to the constants), so it is
 //    factored by blocked Cholesky with no pivoting: a 64x64 diagonal block
-//    in LDS, a TRSM of the panel below it, a SYRK-style update of the
-//    lower trailing tiles;
-//  * z = B^-1 [Q' Y]_(4:) by forward / backward substitution (one
-//    work-group), c' = Q [0; z], d' = R^-1 [Q' (Y - A' c')]_(0:4);
+//    factored and inverted in LDS (FP64 on the matrix cores), the panel below
+//    it as a product with that inverse, an update of the lower trailing
+//    tiles; up to 16 blocks each block column is one fused launch that also
+//    runs the forward substitution;
+//  * z = B^-1 [Q' Y]_(4:) by block substitution with the inverted diagonal
+//    blocks (resident work-groups handing blocks on through flags),
+//    c' = Q [0; z], d' = R^-1 [Q' (Y - A' c')]_(0:4);
 //  * TW_MIXED factors B in FP32 and refines z with FP64 residuals
 //    r = [Q'Y]_(4:) - B z, B applied matrix-free in FP64; a stall of the
 //    refinement refactors in FP64.
