@@ -93,24 +93,26 @@ __global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
     }
 }
 
-// FP32-distance grid kernel, tiled: a work-group owns 16 x 16 nodes of one
-// z-plane (wave w: rows 4w..4w+3, 16 nodes each).  For every chunk of 256
-// landmarks it stages (Y - py)^2 + (Z - pz)^2 per row in LDS, so a node-
-// landmark pair costs dx, one FMA, the square root and the three FP64
-// accumulations.
-constexpr int GT = 16, GCH = 256, GRS = GCH + 4;  // tile side, landmark chunk, LDS row stride
+// FP32-distance grid kernel, tiled: a work-group owns 32 x 16 nodes of one
+// z-plane (wave w: rows 4w..4w+3, 32 nodes each, a thread nodes i and
+// i + 16 of one row, so each landmark's scalar loads and LDS entry serve two
+// nodes).  For every chunk of 256 landmarks it stages (Y - py)^2 + (Z - pz)^2
+// per row in LDS, so a node-landmark pair costs dx, one FMA, the square root
+// and the three FP64 accumulations.
+constexpr int GT = 16, GCH = 256, GRS = GCH + 4;  // tile rows, landmark chunk, LDS row stride
+constexpr int GTX = 2 * GT;  // tile columns: a thread owns nodes i and i + GT of one row
 
 __device__ __forceinline__ void grid_tiled_body(const GridArgs& a, int bx, int by, int bz, float (*syz)[GRS]) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const i64 i = (i64)bx * GT + (lane & 15);
+    const i64 i = (i64)bx * GTX + (lane & 15);
     const i64 j = (i64)by * GT + 4 * w + (lane >> 4);
     const i64 k = bz;
-    const bool live = i < a.gx && j < a.gy;
+    const bool live = i < a.gx && j < a.gy, live2 = live && i + GT < a.gx;
     const i64 X = (i - 1) * a.f + a.ox, Z = (k - 1) * a.f + a.oz;
-    const float Xf = (float)X, Zf = (float)Z;
+    const float Xf = (float)X, X2f = (float)(X + GT * a.f), Zf = (float)Z;
     const int row = 4 * w + (lane >> 4);
     const float Y0 = (float)(((i64)by * GT - 1) * a.f + a.oy);  // row 0 of the tile
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
     for (int n0 = 0; n0 < a.K; n0 += GCH) {
         const int cnt = min(GCH, a.K - n0);
         __syncthreads();
@@ -126,27 +128,41 @@ __device__ __forceinline__ void grid_tiled_body(const GridArgs& a, int bx, int b
         }
         __syncthreads();
         if (live) {
+            // The second node's terms are formed whether or not it exists
+            // (same row, so the same landmark data and LDS entry); only live
+            // nodes are stored.
             const float* sr = syz[row];
             #pragma unroll 16
             for (int n = 0; n < cnt; n++) {
                 const int m = n0 + n;
-                const float dx = Xf - a.pts[m].x;
-                const double r = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, sr[n]));
-                s1 = fma(-r, a.cf[3 * m + 0], s1);
-                s2 = fma(-r, a.cf[3 * m + 1], s2);
-                s3 = fma(-r, a.cf[3 * m + 2], s3);
+                const float px = a.pts[m].x, syzn = sr[n];
+                const float dx = Xf - px, dx2 = X2f - px;
+                const double r = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, syzn));
+                const double r2 = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx2, dx2, syzn));
+                const double c1 = a.cf[3 * m + 0], c2 = a.cf[3 * m + 1], c3 = a.cf[3 * m + 2];
+                s1 = fma(-r, c1, s1);
+                s2 = fma(-r, c2, s2);
+                s3 = fma(-r, c3, s3);
+                u1 = fma(-r2, c1, u1);
+                u2 = fma(-r2, c2, u2);
+                u3 = fma(-r2, c3, u3);
             }
         }
     }
     if (!live) return;
     const i64 plane = a.gx * a.gy * a.gz, idx = (k * a.gy + j) * a.gx + i;
-    const double Xd = (double)X, Yd = (double)((j - 1) * a.f + a.oy), Zd = (double)Z;
-    const double l1 = a.d[1] + Xd * (a.d[5] - 1.0) + Yd * a.d[9] + Zd * a.d[13];
-    const double l2 = a.d[2] + Xd * a.d[6] + Yd * (a.d[10] - 1.0) + Zd * a.d[14];
-    const double l3 = a.d[3] + Xd * a.d[7] + Yd * a.d[11] + Zd * (a.d[15] - 1.0);
-    a.out[idx] = (float)(l1 + s1);
-    a.out[plane + idx] = (float)(l2 + s2);
-    a.out[2 * plane + idx] = (float)(l3 + s3);
+    const double Yd = (double)((j - 1) * a.f + a.oy), Zd = (double)Z;
+    auto put = [&](i64 x, i64 o, double t1, double t2, double t3) {
+        const double Xd = (double)x;
+        const double l1 = a.d[1] + Xd * (a.d[5] - 1.0) + Yd * a.d[9] + Zd * a.d[13];
+        const double l2 = a.d[2] + Xd * a.d[6] + Yd * (a.d[10] - 1.0) + Zd * a.d[14];
+        const double l3 = a.d[3] + Xd * a.d[7] + Yd * a.d[11] + Zd * (a.d[15] - 1.0);
+        a.out[o] = (float)(l1 + t1);
+        a.out[plane + o] = (float)(l2 + t2);
+        a.out[2 * plane + o] = (float)(l3 + t3);
+    };
+    put(X, idx, s1, s2, s3);
+    if (live2) put(X + GT * a.f, idx + GT, u1, u2, u3);
 }
 
 __global__ __launch_bounds__(256) void k_grid_tiled(const GridArgs a) {
@@ -340,7 +356,7 @@ int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const dou
     } else {
         if (grid->gz > 65535 || (grid->gy + GT - 1) / GT > 65535)
             return tw_fail(ctx, TW_E_UNSUPPORTED, "grid too large for one launch");
-        const dim3 g((unsigned)((grid->gx + GT - 1) / GT), (unsigned)((grid->gy + GT - 1) / GT), (unsigned)grid->gz);
+        const dim3 g((unsigned)((grid->gx + GTX - 1) / GTX), (unsigned)((grid->gy + GT - 1) / GT), (unsigned)grid->gz);
         hipLaunchKernelGGL(k_grid_tiled, g, dim3(256), 0, ctx->stream, a);
     }
     TW_LAUNCH_CHECK(ctx);
@@ -418,7 +434,7 @@ int tw_grid_eval_batch(tw_ctx* ctx, const TwGridItem* items, int n) {
         J.a.ox = g.ox; J.a.oy = g.oy; J.a.oz = g.oz;
         J.a.d = it.d;
         J.a.out = it.f32;
-        J.tiles_x = (int)((g.gx + GT - 1) / GT);
+        J.tiles_x = (int)((g.gx + GTX - 1) / GTX);
         J.tiles_y = (int)((g.gy + GT - 1) / GT);
         J.qbx = (g.gx + QB - 1) / QB;
         J.qby = (g.gy + QB - 1) / QB;
