@@ -149,7 +149,6 @@ struct FastGeom {
     int imx, imy, imz;                  // the same as integers
     float invf;                         // 1 / f
     uint32_t dchan_b;                   // destination channel stride in bytes
-    int share_chk;                      // no in-range tap offset plus one plane reaches OOB_OFF
     int tiles_x, tiles_y, ntiles, per_xcd;  // tile grid and XCD-major order
     int nxm, nym, nzm;                  // node-tile extents (maxima over the launch's tiles)
     float wt[4 * 64];                   // FP32 B-spline weights TB[t][k], f <= 64
@@ -296,14 +295,12 @@ __device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
                                 bld2<T>(rs.r11, off), t);
 }
 
-// Four consecutive slices of one column (trilinear): where slice j's
-// offset is slice j-1's plus one plane, its lower plane is slice j-1's
-// upper plane (the same bytes through resources that end at the same
-// byte), so those two pairs are not loaded again; the other lanes load
-// them under an exec mask.  (have, poff, pu0, pu1) carry the previous
-// group's last slice the same way.  Results are those of blend, also for
-// samples out of bounds (OOB_OFF) as long as no in-range offset plus one
-// plane equals OOB_OFF (FastGeom::share_chk).
+// Four consecutive slices of one column (trilinear, every tap in bounds):
+// where slice j's offset is slice j-1's plus one plane, its lower plane is
+// slice j-1's upper plane (the same bytes), so those two pairs are not
+// loaded again; the other lanes load them under an exec mask.  (have, poff,
+// pu0, pu1) carry the previous group's last slice the same way.  Results
+// are those of blend.
 template <typename T, int MODE>
 __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v,
                                               bool& have, uint32_t& poff, uint32_t& pu0, uint32_t& pu1) {
@@ -541,7 +538,7 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
             tp[j] = tap_setup<T, INTERP, WIN, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2]);
         }
         uint32_t v[4];
-        if (INTERP != TW_INTERP_NN && (!CHECK || g.share_chk)) {
+        if constexpr (!CHECK && INTERP != TW_INTERP_NN) {
             blend4_shared<T, MODE>(rs, tp, g.plane_b, v, have, poff, pu0, pu1);
         } else {
             #pragma unroll
@@ -655,8 +652,6 @@ const char* fused_geom(const TpsArgs& a, FastGeom& g, size_t& lds) {
     g.imx = (int)(a.fnx - 1); g.imy = (int)(a.fny - 1); g.imz = (int)(a.fnz - 1);
     g.invf = 1.0f / (float)a.f;
     g.dchan_b = (uint32_t)(a.dnx * a.dny * a.dnz * sizeof(T));
-    // Tap offsets lie below the channel's bytes, OOB_OFF at or past them.
-    g.share_chk = schan + g.plane_b <= (uint64_t)OOB_OFF;
     fill_weights(g, a.f, a.basis);
     g.tiles_x = (int)((a.dnx + TX - 1) / TX);
     g.tiles_y = (int)((a.dny + FTY - 1) / FTY);
