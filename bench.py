@@ -139,6 +139,33 @@ def cpu_baseline(vol, tar, sub, threads):
     }, out
 
 
+def cpu_faithful(vol, tar, sub, threaded_out):
+    """The same oracle on one thread, as the reference's in-memory TPS path
+    runs (no OpenMP in q_imgwarp_tps_quicksmallmemory.cpp:377-661, 1997-2162),
+    on a smaller sample: the full fit, the grid nodes of one block row and its
+    resample (8 z-slices, 8.4 M voxels).  Its output must equal the
+    multi-threaded run's on those slices."""
+    from oracle import oracle
+    zb0, zb1 = 10, 11
+    t0 = time.perf_counter()
+    c, d = oracle.stps_fit(tar, sub, nthreads=1)
+    gx, gy = (oracle.grid_shape(SIZE, G)[0], oracle.grid_shape(SIZE, G)[1])
+    df_sl = oracle.grid_eval(tar, c, d, (gx, gy, zb1 - zb0 + 3), G, origin=(0, 0, zb0 * G), nthreads=1)
+    t1 = time.perf_counter()
+    out = oracle_slab_resample(oracle, vol, df_sl, zb0, zb1, 1)
+    t2 = time.perf_counter()
+    vox = SIZE[0] * SIZE[1] * (zb1 - zb0) * G
+    sl = slice(zb0 * G, zb1 * G)
+    return {
+        "value": vox / ((t1 - t0) * (zb1 - zb0) * G / SIZE[2] + (t2 - t1)) / 1e9,
+        "unit": "Gvox/s",
+        "cores": 1,
+        "sample": f"FP64 oracle on one thread: full K={K} fit ({t1 - t0:.2f} s incl. grid slab, prorated) + "
+                  f"resample of z-slices [{zb0 * G},{zb1 * G}) = {vox / 1e6:.1f} Mvox ({t2 - t1:.2f} s)",
+        "equals_threaded_output": bool(np.array_equal(out[sl], threaded_out[sl])),
+    }
+
+
 def oracle_slab_resample(oracle, vol, df_slab, zb0, zb1, threads):
     """Oracle resample of block rows [zb0, zb1) using a node slab whose
     first layer is node zb0 of the full grid: embed it in a zero grid."""
@@ -633,7 +660,8 @@ def main():
             acc[name + "_lsb_le1_frac"] = float(np.mean(diff <= 1))
             acc[name + "_lsb_max"] = int(diff.max())
         if not args.no_cpu:
-            cpu, _ = cpu_baseline(vol, tar, sub, threads)
+            cpu, cpu_out = cpu_baseline(vol, tar, sub, threads)
+            cpu["one_thread"] = cpu_faithful(vol, tar, sub, cpu_out)
     except Exception as e:  # the accuracy leg must not hide the GPU numbers
         log("oracle leg failed:", repr(e))
         acc["error"] = repr(e)
