@@ -93,26 +93,37 @@ __global__ __launch_bounds__(256) void k_grid_eval(const GridArgs a) {
     }
 }
 
-// FP32-distance grid kernel, tiled: a work-group owns 32 x 16 nodes of one
-// z-plane (wave w: rows 4w..4w+3, 32 nodes each, a thread nodes i and
-// i + 16 of one row, so each landmark's scalar loads and LDS entry serve two
-// nodes).  For every chunk of 256 landmarks it stages (Y - py)^2 + (Z - pz)^2
+// FP32-distance grid kernel, tiled: a work-group owns 16 GNP x 16 nodes of
+// one z-plane (wave w: rows 4w..4w+3, 16 GNP nodes each, a thread GNP nodes
+// 16 apart in one row, so each landmark's scalar loads and LDS entry serve
+// GNP nodes).  For every chunk of 256 landmarks it stages (Y - py)^2 + (Z - pz)^2
 // per row in LDS, so a node-landmark pair costs dx, one FMA, the square root
 // and the three FP64 accumulations.
 constexpr int GT = 16, GCH = 256, GRS = GCH + 4;  // tile rows, landmark chunk, LDS row stride
-constexpr int GTX = 2 * GT;  // tile columns: a thread owns nodes i and i + GT of one row
+// A thread owns nodes i + GT q, q < GNP, of one row: GNP = 2 for a single
+// large grid (more nodes per thread cost occupancy there: K = 4096 at
+// 259x259x67 nodes took 4.93 ms at 4 against 4.60 ms at 2), 4 for the
+// batch's many small grids (0.79 against 1.19 ms for 158 grids).
+constexpr int GNP_ONE = 2, GNP_BATCH = 4;
 
+template <int GNP>
 __device__ __forceinline__ void grid_tiled_body(const GridArgs& a, int bx, int by, int bz, float (*syz)[GRS]) {
+    constexpr int GTX = GNP * GT;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const i64 i = (i64)bx * GTX + (lane & 15);
     const i64 j = (i64)by * GT + 4 * w + (lane >> 4);
     const i64 k = bz;
-    const bool live = i < a.gx && j < a.gy, live2 = live && i + GT < a.gx;
+    const bool live = i < a.gx && j < a.gy;
     const i64 X = (i - 1) * a.f + a.ox, Z = (k - 1) * a.f + a.oz;
-    const float Xf = (float)X, X2f = (float)(X + GT * a.f), Zf = (float)Z;
+    float Xf[GNP];
+    #pragma unroll
+    for (int q = 0; q < GNP; q++) Xf[q] = (float)(X + q * GT * a.f);
+    const float Zf = (float)Z;
     const int row = 4 * w + (lane >> 4);
     const float Y0 = (float)(((i64)by * GT - 1) * a.f + a.oy);  // row 0 of the tile
-    double s1 = 0.0, s2 = 0.0, s3 = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
+    double s[GNP][3];
+    #pragma unroll
+    for (int q = 0; q < GNP; q++) s[q][0] = s[q][1] = s[q][2] = 0.0;
     for (int n0 = 0; n0 < a.K; n0 += GCH) {
         const int cnt = min(GCH, a.K - n0);
         __syncthreads();
@@ -128,46 +139,47 @@ __device__ __forceinline__ void grid_tiled_body(const GridArgs& a, int bx, int b
         }
         __syncthreads();
         if (live) {
-            // The second node's terms are formed whether or not it exists
-            // (same row, so the same landmark data and LDS entry); only live
-            // nodes are stored.
+            // Every node's terms are formed whether or not it exists (same
+            // row, so the same landmark data and LDS entry); only live nodes
+            // are stored.
             const float* sr = syz[row];
-            #pragma unroll 16
+            constexpr int UNROLL = 32 / GNP;
+            #pragma unroll(UNROLL)
             for (int n = 0; n < cnt; n++) {
                 const int m = n0 + n;
                 const float px = a.pts[m].x, syzn = sr[n];
-                const float dx = Xf - px, dx2 = X2f - px;
-                const double r = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, syzn));
-                const double r2 = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx2, dx2, syzn));
                 const double c1 = a.cf[3 * m + 0], c2 = a.cf[3 * m + 1], c3 = a.cf[3 * m + 2];
-                s1 = fma(-r, c1, s1);
-                s2 = fma(-r, c2, s2);
-                s3 = fma(-r, c3, s3);
-                u1 = fma(-r2, c1, u1);
-                u2 = fma(-r2, c2, u2);
-                u3 = fma(-r2, c3, u3);
+                #pragma unroll
+                for (int q = 0; q < GNP; q++) {
+                    const float dx = Xf[q] - px;
+                    const double r = (double)__builtin_amdgcn_sqrtf(__builtin_fmaf(dx, dx, syzn));
+                    s[q][0] = fma(-r, c1, s[q][0]);
+                    s[q][1] = fma(-r, c2, s[q][1]);
+                    s[q][2] = fma(-r, c3, s[q][2]);
+                }
             }
         }
     }
     if (!live) return;
     const i64 plane = a.gx * a.gy * a.gz, idx = (k * a.gy + j) * a.gx + i;
     const double Yd = (double)((j - 1) * a.f + a.oy), Zd = (double)Z;
-    auto put = [&](i64 x, i64 o, double t1, double t2, double t3) {
-        const double Xd = (double)x;
+    #pragma unroll
+    for (int q = 0; q < GNP; q++) {
+        if (i + q * GT >= a.gx) break;
+        const double Xd = (double)(X + q * GT * a.f);
         const double l1 = a.d[1] + Xd * (a.d[5] - 1.0) + Yd * a.d[9] + Zd * a.d[13];
         const double l2 = a.d[2] + Xd * a.d[6] + Yd * (a.d[10] - 1.0) + Zd * a.d[14];
         const double l3 = a.d[3] + Xd * a.d[7] + Yd * a.d[11] + Zd * (a.d[15] - 1.0);
-        a.out[o] = (float)(l1 + t1);
-        a.out[plane + o] = (float)(l2 + t2);
-        a.out[2 * plane + o] = (float)(l3 + t3);
-    };
-    put(X, idx, s1, s2, s3);
-    if (live2) put(X + GT * a.f, idx + GT, u1, u2, u3);
+        const i64 o = idx + q * GT;
+        a.out[o] = (float)(l1 + s[q][0]);
+        a.out[plane + o] = (float)(l2 + s[q][1]);
+        a.out[2 * plane + o] = (float)(l3 + s[q][2]);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_grid_tiled(const GridArgs a) {
     __shared__ __align__(16) float syz[GT][GRS];
-    grid_tiled_body(a, blockIdx.x, blockIdx.y, blockIdx.z, syz);
+    grid_tiled_body<GNP_ONE>(a, blockIdx.x, blockIdx.y, blockIdx.z, syz);
 }
 
 // FP16 offset-scaled storage: per block of 8x8x8 nodes and component,
@@ -279,7 +291,7 @@ __global__ __launch_bounds__(256) void k_grid_batch(const GridJob* __restrict__ 
     const GridJob& J = jobs[j];
     const int t = w - pref[j];
     const int bx = t % J.tiles_x, by = (t / J.tiles_x) % J.tiles_y, bz = t / (J.tiles_x * J.tiles_y);
-    grid_tiled_body(J.a, bx, by, bz, syz);
+    grid_tiled_body<GNP_BATCH>(J.a, bx, by, bz, syz);
 }
 
 __global__ __launch_bounds__(256) void k_quantize_batch(const GridJob* __restrict__ jobs, const int* __restrict__ pref,
@@ -356,7 +368,7 @@ int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const dou
     } else {
         if (grid->gz > 65535 || (grid->gy + GT - 1) / GT > 65535)
             return tw_fail(ctx, TW_E_UNSUPPORTED, "grid too large for one launch");
-        const dim3 g((unsigned)((grid->gx + GTX - 1) / GTX), (unsigned)((grid->gy + GT - 1) / GT), (unsigned)grid->gz);
+        const dim3 g((unsigned)((grid->gx + GNP_ONE * GT - 1) / (GNP_ONE * GT)), (unsigned)((grid->gy + GT - 1) / GT), (unsigned)grid->gz);
         hipLaunchKernelGGL(k_grid_tiled, g, dim3(256), 0, ctx->stream, a);
     }
     TW_LAUNCH_CHECK(ctx);
@@ -434,7 +446,7 @@ int tw_grid_eval_batch(tw_ctx* ctx, const TwGridItem* items, int n) {
         J.a.ox = g.ox; J.a.oy = g.oy; J.a.oz = g.oz;
         J.a.d = it.d;
         J.a.out = it.f32;
-        J.tiles_x = (int)((g.gx + GTX - 1) / GTX);
+        J.tiles_x = (int)((g.gx + GNP_BATCH * GT - 1) / (GNP_BATCH * GT));
         J.tiles_y = (int)((g.gy + GT - 1) / GT);
         J.qbx = (g.gx + QB - 1) / QB;
         J.qby = (g.gy + QB - 1) / QB;
