@@ -276,6 +276,15 @@ typedef int (*tw_read_fn)(void* user, int64_t x0, int64_t y0, int64_t z0, int64_
 typedef int (*tw_write_fn)(void* user, int64_t brick, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny,
                            int64_t nz, const void* data);
 
+/* tw_stream_desc.flags: with TW_EXACT, each sample position is the float sum
+ * (p - o + d) + o with o the brick's origin, the order of the reference's
+ * tera block warp, whose blocks are the bricks (pos_warp block-local plus
+ * the displacement, then plus x_offset: q_imgblockwarp_CYF,
+ * q_imgwarp_tps_quicksmallmemory.cpp:2229-2235); without it, p + d as in the
+ * in-memory warp.  TW_PLAIN / TW_MIXED form positions in FP32 from the
+ * global coordinate either way. */
+#define TW_STREAM_BLOCK_ORIGIN 1
+
 typedef struct {
     int64_t      nx, ny, nz;      /* whole volume (one channel) */
     int32_t      dtype;           /* TW_U8 / TW_U16 */
@@ -285,7 +294,7 @@ typedef struct {
     int32_t      precision;       /* TW_EXACT, TW_PLAIN or TW_MIXED */
     int32_t      threads;         /* host threads filling windows */
     int32_t      K;
-    int32_t      pad;
+    int32_t      flags;           /* TW_STREAM_BLOCK_ORIGIN or 0 */
     double       lambda;
     const float* centres;         /* K x 3, host */
     const float* targets;         /* K x 3, host */

@@ -46,12 +46,14 @@ def lib():
                                       _P, _i64, _i64, ctypes.c_int]
         L.or_tps_resample_window.argtypes = [_P, _P, _P, _P, ctypes.c_int, _P, _i64, _i64, _i64, _P, ctypes.c_int,
                                              ctypes.c_int, _P, _P, _P, ctypes.c_int]
+        L.or_tps_warp_tera.argtypes = [_P, _P, ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_int, _i64,
+                                       ctypes.c_double, ctypes.c_int, _P, ctypes.c_int]
         L.or_dense_df.argtypes = [_P, _P, _i64, _i64, _i64, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int]
         L.or_affine_fit.argtypes = [_P, _P, ctypes.c_int, _P]
         L.or_affine_resample.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _i64, _i64, ctypes.c_int]
         for f in ("or_stps_fit", "or_stps_fit_bordered", "or_grid_eval", "or_bspline_basis", "or_df_basis",
                   "or_tps_resample",
-                  "or_tps_resample_window", "or_dense_df", "or_affine_fit", "or_affine_resample", "or_max_threads"):
+                  "or_tps_resample_window", "or_tps_warp_tera", "or_dense_df", "or_affine_fit", "or_affine_resample", "or_max_threads"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -162,6 +164,21 @@ def tps_resample_window(src_win, src_origin, full_xyz, df, n, grid_origin, out_s
     _check(lib().or_tps_resample_window(_ptr(v), _ptr(ssz), _ptr(s0), _ptr(fsz), dt, _ptr(df), df.shape[3],
                                         df.shape[2], df.shape[1], _ptr(g0), n, interp, _ptr(out), _ptr(dsz),
                                         _ptr(d0), nthreads), "tps_resample_window")
+    return out
+
+
+def warp_tps_tera(src, tar_markers, sub_markers, g=4, block=1000, lam=0.2, interp=0, nthreads=0):
+    """The tera-block TPS driver of the reference's CPU path on a volume in
+    memory: markers scaled by 4, output blocks of `block` voxels, each with
+    its own grid and the block-local float order of the sample position."""
+    v = np.ascontiguousarray(src)
+    assert v.ndim == 3, "one channel"
+    dt = 1 if v.dtype == np.uint8 else 2
+    tar, sub = _pts(tar_markers), _pts(sub_markers)
+    sz = np.array([v.shape[2], v.shape[1], v.shape[0]], np.int64)
+    out = np.zeros_like(v)
+    _check(lib().or_tps_warp_tera(_ptr(v), _ptr(sz), dt, _ptr(tar), _ptr(sub), tar.shape[0], g, block, lam, interp,
+                                  _ptr(out), nthreads), "tps_warp_tera")
     return out
 
 

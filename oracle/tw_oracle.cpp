@@ -25,6 +25,11 @@
 //   or_tps_resample_window  equal to or_tps_resample on the whole volume and
 //                      on sub-windows whose source covers every tap; zero
 //                      where the taps leave the window.
+//   or_tps_warp_tera   one block over the volume = the in-memory chain of the
+//                      4x markers; a block's grid = the volume grid's
+//                      sub-grid; with small blocks, voxels differ from the
+//                      in-memory warp only where the block-local float order
+//                      of the position rounds differently.
 //   or_affine_fit      exact recovery of a known matrix, order invariance.
 //   or_affine_resample brute force on tiny volumes, identity/shift cases.
 // newmat's own rounding sequences (QRZ, .i(), SVD) cannot be reproduced
@@ -573,9 +578,122 @@ int tps_resample_window(const T* src, const i64* ssz, const i64* s0, const i64* 
     return 0;
 }
 
+// Tera-block TPS driver: the CPU branch of imgwarp_smallmemory_CYF
+// (q_imgwarp_tps_quicksmallmemory.cpp:1161-1994) over a volume held in
+// memory, after the fit (c, d of the 4x-scaled centres cen):
+//  * output blocks of `block` voxels stepped z, then y, then x from 0
+//    (per_block_siza, 1264, 1309-1317), edge blocks clipped;
+//  * each block gets its own grid at spacing n with the block's offset
+//    (compute_df_stps_subsampled_volume_4bspline_block, 1084-1152:
+//    ceil(size/n) + 3 nodes per axis, node i at (i-1)n + offset);
+//  * per n^3 sub-block (substart < gsz - 3, STPS_interpolate_CYF, 66-79) the
+//    B-spline upsample and q_imgblockwarp_CYF (2208-2304): the sample
+//    position is (pos_warp + sx) + x_offset with pos_warp block-local, each
+//    addition rounded to float (2229-2235), tested against the whole
+//    volume's extent (2237-2242); then floor/ceil, 1 - frac weights, the
+//    t/b, l/r, u/d tree, +0.5 and truncation (2266-2296).
+// The reference reads a block's source from a window it estimates from 14
+// probes of the field with a 60-voxel margin (1279, 1403-1711) and zeroes
+// samples outside it (2243-2250).  A position minus the window's integer
+// origin is an exact float difference for every sample inside the window,
+// so there the arithmetic equals this whole-volume form; samples the
+// estimate misses are not reproduced.  Linear image interpolation only:
+// the reference's nearest-neighbour branch clamps window-relative indices
+// against the block extent (2254-2263).
+template <class T>
+int tps_warp_tera(const T* src, const i64* sz, const float* cen, const double* c, const double* d, int K, int n,
+                  i64 block, int kind, T* dst) {
+    std::vector<double> TB((size_t)n * 4), basis((size_t)n * n * n * 64);
+    or_df_basis(n, kind, TB.data(), basis.data());
+    const i64 sx = sz[0], sxy = sz[0] * sz[1];
+    for (i64 oz = 0; oz < sz[2]; oz += block)
+        for (i64 oy = 0; oy < sz[1]; oy += block)
+            for (i64 ox = 0; ox < sz[0]; ox += block) {
+                const i64 bs[3] = {std::min(block, sz[0] - ox), std::min(block, sz[1] - oy),
+                                   std::min(block, sz[2] - oz)};
+                const i64 gx = (bs[0] + n - 1) / n + 3, gy = (bs[1] + n - 1) / n + 3, gz = (bs[2] + n - 1) / n + 3;
+                std::vector<float> df((size_t)(3 * gx * gy * gz));
+                if (or_grid_eval(cen, K, c, d, gx, gy, gz, n, n, n, ox, oy, oz, df.data(), 0)) return -1;
+#pragma omp parallel for schedule(dynamic)
+                for (i64 bz = 0; bz < gz - 3; bz++) {
+                    std::vector<float> blk((size_t)n * n * n * 3);
+                    for (i64 by = 0; by < gy - 3; by++)
+                        for (i64 bx = 0; bx < gx - 3; bx++) {
+                            df_block(df.data(), gx, gy, gz, basis.data(), n, bx, by, bz, blk.data());
+                            for (int z = 0; z < n; z++)
+                                for (int y = 0; y < n; y++)
+                                    for (int x = 0; x < n; x++) {
+                                        const i64 w[3] = {bx * n + x, by * n + y, bz * n + z};
+                                        if (w[0] >= bs[0] || w[1] >= bs[1] || w[2] >= bs[2]) continue;
+                                        const float* dv = blk.data() + (((i64)z * n + y) * n + x) * 3;
+                                        const i64 off[3] = {ox, oy, oz};
+                                        double p[3];
+                                        for (int a = 0; a < 3; a++) {
+                                            const float local = (float)w[a] + dv[a];
+                                            p[a] = local + (float)off[a];
+                                        }
+                                        T& out = dst[(oz + w[2]) * sxy + (oy + w[1]) * sx + (ox + w[0])];
+                                        if (p[0] < 0 || p[0] > sz[0] - 1 || p[1] < 0 || p[1] > sz[1] - 1 ||
+                                            p[2] < 0 || p[2] > sz[2] - 1) {
+                                            out = 0;
+                                            continue;
+                                        }
+                                        const i64 xs = (i64)std::floor(p[0]), xb = (i64)std::ceil(p[0]);
+                                        const i64 ys = (i64)std::floor(p[1]), yb = (i64)std::ceil(p[1]);
+                                        const i64 zs = (i64)std::floor(p[2]), zb = (i64)std::ceil(p[2]);
+                                        const double lw = 1.0 - (p[0] - xs), rw = 1.0 - lw;
+                                        const double tw = 1.0 - (p[1] - ys), bw = 1.0 - tw;
+                                        const double uw = 1.0 - (p[2] - zs), dw = 1.0 - uw;
+                                        const double hi =
+                                            tw * (lw * px(src, sx, sxy, xs, ys, zs) + rw * px(src, sx, sxy, xb, ys, zs)) +
+                                            bw * (lw * px(src, sx, sxy, xs, yb, zs) + rw * px(src, sx, sxy, xb, yb, zs));
+                                        const double lo =
+                                            tw * (lw * px(src, sx, sxy, xs, ys, zb) + rw * px(src, sx, sxy, xb, ys, zb)) +
+                                            bw * (lw * px(src, sx, sxy, xs, yb, zb) + rw * px(src, sx, sxy, xb, yb, zb));
+                                        const double iv = uw * hi + dw * lo + 0.5;
+                                        out = (T)iv;
+                                    }
+                        }
+                }
+            }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+// The tera-block TPS driver (tps_warp_tera above) from the marker files'
+// coordinates: both marker sets are scaled by 4 (x_para = y_para = z_para =
+// 4, q_imgwarp_tps_quicksmallmemory.cpp:1194-1204; int times float), the
+// target-space markers are the TPS centres (the plugin calls
+// imgwarp_smallmemory_CYF(sub, tar, ...) into parameters (tar, sub),
+// plugin_Bigimagewarp.cpp:294), then or_stps_fit with lambda and the block
+// warp.  tar, sub: K x 3 float; sz = {x, y, z}; one channel; dtype 1 / 2;
+// interp 0 (linear) or DF_LINEAR_FLAG (linear DF basis).  Returns 0, -1 bad
+// arguments, -2 singular fit.
+int or_tps_warp_tera(const void* src, const i64* sz, int dtype, const float* tar, const float* sub, int K,
+                     int n, i64 block, double lambda, int interp, void* dst, int nthreads) {
+    if (!src || !sz || !tar || !sub || !dst || K < 1 || n <= 0 || block < n || block % n) return -1;
+    if ((interp & ~DF_LINEAR_FLAG) != 0) return -1;
+    set_threads(nthreads);
+    std::vector<float> cen((size_t)K * 3), tgt((size_t)K * 3);
+    for (int i = 0; i < 3 * K; i++) {
+        cen[(size_t)i] = 4 * tar[i];
+        tgt[(size_t)i] = 4 * sub[i];
+    }
+    std::vector<double> c((size_t)K * 4), d(16);
+    const int rc = or_stps_fit(cen.data(), tgt.data(), K, lambda, c.data(), d.data(), nthreads);
+    if (rc) return rc;
+    const int kind = (interp & DF_LINEAR_FLAG) ? 1 : 0;
+    if (dtype == 1)
+        return tps_warp_tera((const uint8_t*)src, sz, cen.data(), c.data(), d.data(), K, n, block, kind,
+                             (uint8_t*)dst);
+    if (dtype == 2)
+        return tps_warp_tera((const uint16_t*)src, sz, cen.data(), c.data(), d.data(), K, n, block, kind,
+                             (uint16_t*)dst);
+    return -1;
+}
 
 // In-memory TPS warp, CPU branch of imgwarp_smallmemory_image
 // (q_imgwarp_tps_quicksmallmemory.cpp:2119-2133): for every n^3 block,

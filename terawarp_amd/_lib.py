@@ -28,6 +28,7 @@ TW_EXACT = 2
 TW_INTERP_LINEAR = 0
 TW_INTERP_NN = 1
 TW_DF_LINEAR = 0x100
+TW_STREAM_BLOCK_ORIGIN = 1
 TW_BASIS_BSPLINE = 0
 TW_BASIS_LINEAR = 1
 TW_DF_F32 = 0
@@ -88,7 +89,7 @@ WRITE_FN = ctypes.CFUNCTYPE(ctypes.c_int, _P, _i64, _i64, _i64, _i64, _i64, _i64
 
 class TwStreamDesc(ctypes.Structure):
     _fields_ = [("nx", _i64), ("ny", _i64), ("nz", _i64), ("dtype", _i32), ("brick", _i32), ("f", _i32),
-                ("interp", _i32), ("precision", _i32), ("threads", _i32), ("K", _i32), ("pad", _i32),
+                ("interp", _i32), ("precision", _i32), ("threads", _i32), ("K", _i32), ("flags", _i32),
                 ("lam", ctypes.c_double), ("centres", ctypes.POINTER(ctypes.c_float)),
                 ("targets", ctypes.POINTER(ctypes.c_float)), ("read", _P), ("read_user", _P),
                 ("write", _P), ("write_user", _P)]
@@ -132,6 +133,12 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run `python -m terawarp_amd.build` (no CPU fallback exists)")
+    # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 under the
+    # same sonames as /opt/rocm's, and the first one loaded serves the whole
+    # process.  Import torch first so that both torch and this library run on
+    # torch's runtime (loaded the other way round, torch's device discovery
+    # and the library's both fail).
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     V, G, T = ctypes.POINTER(TwVolume), ctypes.POINTER(TwGrid), ctypes.POINTER(TwTimings)
     dp = ctypes.POINTER(ctypes.c_double)

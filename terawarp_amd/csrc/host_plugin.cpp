@@ -70,10 +70,18 @@ struct SliceDir {
     }
 };
 
-// Output blocks of the tera drivers: 100^3 for the warps
-// (q_imgwarp_tps_quicksmallmemory.cpp:1256, q_warp_affine_tps.cpp:896),
-// 500^3 for the downsample pass (q_warp_affine_tps.cpp:1521).
-constexpr int kTeraBlock = 100, kDownBlock = 500;
+// The tera drivers step their output blocks by per_block_siza = 1000 voxels
+// and clip the last block of each axis: the TPS warp
+// (q_imgwarp_tps_quicksmallmemory.cpp:1264, 1309-1317), the affine warp
+// (q_warp_affine_tps.cpp:902, 966-974) and the downsample pass (1531,
+// 1595-1602).  (The block-size arrays start at 100 or 500, 1256 / 896 /
+// 1521, and are overwritten from per_block_siza before use.)
+constexpr int kTeraBlock = 1000;
+// Both tera warps multiply the marker coordinates by 4 before fitting
+// (x_para, q_imgwarp_tps_quicksmallmemory.cpp:1194-1204; x_power,
+// q_warp_affine_tps.cpp:866-872): the markers are placed on a view
+// downsampled 4x per axis.  The in-memory warps use them as they are.
+constexpr float kTeraMarkerScale = 4.0f;
 
 // Affine / rigid warp of a slice directory block by block
 // (q_imagewarp_affine_cyf, q_warp_affine_tps.cpp:851-1329): each output
@@ -144,8 +152,8 @@ int affine_blocks(tw_ctx* ctx, tw_slice_dir* sd, const double* M, int64_t nx, in
 
 // Preview pass (q_imagewarp_affine_cyf_downsample, q_warp_affine_tps.cpp:
 // 1474-2029): the warped slices of <base>_slice downsampled by the factor f
-// of tw_downsample_factor to extent n / f, written as 500^3 block files and
-// slices under <base>_downsample.  The reference estimates the matrix by
+// of tw_downsample_factor to extent n / f, written as block files on the
+// 1000-voxel lattice and slices under <base>_downsample.  The reference estimates the matrix by
 // its DLT from the target markers and their f-scaled copies, which is
 // diag(f, f, f) up to ~1e-12 rounding; the exact matrix is used here, so an
 // output voxel is the warped voxel at f times its coordinates.  The slices
@@ -188,11 +196,11 @@ int downsample_pass(tw_ctx* ctx, const std::string& base, int64_t nx, int64_t ny
     const tw_block_files bf = {dbase.c_str(), dtype, 0};
     std::vector<uint8_t> blk;
     int64_t brick = 0;
-    for (int64_t z0 = 0; z0 < onz; z0 += kDownBlock)
-        for (int64_t y0 = 0; y0 < ony; y0 += kDownBlock)
-            for (int64_t x0 = 0; x0 < onx; x0 += kDownBlock, brick++) {
-                const int64_t bnx = std::min<int64_t>(kDownBlock, onx - x0), bny = std::min<int64_t>(kDownBlock, ony - y0),
-                              bnz = std::min<int64_t>(kDownBlock, onz - z0);
+    for (int64_t z0 = 0; z0 < onz; z0 += kTeraBlock)
+        for (int64_t y0 = 0; y0 < ony; y0 += kTeraBlock)
+            for (int64_t x0 = 0; x0 < onx; x0 += kTeraBlock, brick++) {
+                const int64_t bnx = std::min<int64_t>(kTeraBlock, onx - x0), bny = std::min<int64_t>(kTeraBlock, ony - y0),
+                              bnz = std::min<int64_t>(kTeraBlock, onz - z0);
                 blk.resize((size_t)(bnx * bny * bnz * bpv));
                 for (int64_t z = 0; z < bnz; z++)
                     for (int64_t y = 0; y < bny; y++)
@@ -202,7 +210,7 @@ int downsample_pass(tw_ctx* ctx, const std::string& base, int64_t nx, int64_t ny
                 if ((rc = tw_block_file_write((void*)&bf, brick, x0, y0, z0, bnx, bny, bnz, blk.data())))
                     return fail(tw_tera_last_error()), rc;
             }
-    if ((rc = tw_stitch_slices(dbase.c_str(), onx, ony, onz, kDownBlock, dtype))) return fail(tw_tera_last_error()), rc;
+    if ((rc = tw_stitch_slices(dbase.c_str(), onx, ony, onz, kTeraBlock, dtype))) return fail(tw_tera_last_error()), rc;
     return TW_OK;
 }
 
@@ -210,10 +218,13 @@ int downsample_pass(tw_ctx* ctx, const std::string& base, int64_t nx, int64_t ny
 // subject is a directory of slices; the warp writes <out>/<dir><suffix>_block
 // files and <out>/<dir><suffix>_slice slices, then, with downsample_mode,
 // the preview pass.  TPS runs through the streaming driver (one fit, one
-// grid, exact per-brick source windows); affine and rigid warps block by
-// block.
-int tera_branch(const std::string& dir, const char* out_dir, const tw_warp_params* p, const std::vector<float>& tar,
-                const std::vector<float>& sub, int nm, const char* suffix, char* out_path, int out_cap) {
+// grid, exact per-brick source windows, bricks = the reference's blocks,
+// positions in its block-local order); affine and rigid warps block by
+// block.  The markers of both fitted warps are scaled by kTeraMarkerScale.
+int tera_branch(const std::string& dir, const char* out_dir, const tw_warp_params* p, std::vector<float> tar,
+                std::vector<float> sub, int nm, const char* suffix, char* out_path, int out_cap) {
+    for (float& v : tar) v *= kTeraMarkerScale;
+    for (float& v : sub) v *= kTeraMarkerScale;
     SliceDir sd;
     int rc = tw_slice_dir_open(dir.c_str(), &sd.d);
     if (rc) return fail(tw_tera_last_error()), rc;
@@ -239,6 +250,7 @@ int tera_branch(const std::string& dir, const char* out_dir, const tw_warp_param
         d.interp = TW_INTERP_LINEAR;
         d.precision = precision;
         d.threads = 8;
+        d.flags = TW_STREAM_BLOCK_ORIGIN;
         d.K = nm;
         d.lambda = 0.2;
         d.centres = tar.data();

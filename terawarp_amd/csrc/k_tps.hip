@@ -347,7 +347,7 @@ extern "C" int tw_resample_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst
 // The resample of tw_resample_tps on validated arguments, without a stage
 // reset (the chained warp records its stages in one set).
 int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
-                        int precision) {
+                        int precision, const i64* porigin) {
     int rc;
     TpsArgs a;
     memset(&a, 0, sizeof a);
@@ -368,6 +368,11 @@ int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const
     a.dnx = dst->nx; a.dny = dst->ny; a.dnz = dst->nz;
     a.dx0 = dst->x0; a.dy0 = dst->y0; a.dz0 = dst->z0;
     a.nc = src->nc;
+    if (porigin) {
+        a.px0 = porigin[0];
+        a.py0 = porigin[1];
+        a.pz0 = porigin[2];
+    }
     a.src_cstride = src->nx * src->ny * src->nz * (i64)dtype_size(src->dtype);
     a.dst_cstride = dst->nx * dst->ny * dst->nz * (i64)dtype_size(dst->dtype);
     fill_grid_args(a, grid, gdev);

@@ -157,8 +157,11 @@ int tw_tps_fit_dev(tw_ctx* ctx, const float* centres, const float* targets, int 
 int tw_tps_fit_check(tw_ctx* ctx, const TwFitDev* fit);
 int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const double* d, int K, int precision,
                      tw_grid* grid);
+// porigin (optional, xyz): the TW_EXACT sample position is the float sum
+// (p - porigin + d) + porigin, the tera block warp's order with porigin the
+// block's offset; null: p + d.
 int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
-                        int precision);
+                        int precision, const i64* porigin = nullptr);
 
 // ---- batches ---------------------------------------------------------------
 // Fits of a batch: systems up to the small-system limit in one launch (one

@@ -34,6 +34,12 @@
 namespace {
 
 constexpr int NSLOT = 3;
+// Resample events per brick: pair b % NEV, read last by the writer of brick
+// b.  The issuer records brick b's pair after its iteration for brick
+// b - NSLOT, which ended by waiting for the writer to take brick b - 2 NSLOT,
+// the previous user of the pair; so no pair is re-recorded before the
+// writer has read it.
+constexpr int NEV = 2 * NSLOT;
 
 // Brick b (x fastest): min / max of each displacement component over the
 // nodes X0/f .. (X1-1)/f + 3 (per axis) that reach its voxels.
@@ -97,7 +103,8 @@ struct Pipe {
     void* dsrc[NSLOT] = {};
     void* ddst[NSLOT] = {};
     hipStream_t h2d = nullptr, d2h = nullptr;
-    hipEvent_t ev_h2d[NSLOT] = {}, ev_beg[NSLOT] = {}, ev_end[NSLOT] = {}, ev_d2h[NSLOT] = {};
+    hipEvent_t ev_h2d[NSLOT] = {}, ev_d2h[NSLOT] = {};
+    hipEvent_t ev_beg[NEV] = {}, ev_end[NEV] = {};
     ~Pipe() {
         (void)hipSetDevice(device);
         if (h2d) (void)hipStreamSynchronize(h2d);
@@ -108,8 +115,12 @@ struct Pipe {
             if (hdst[s]) (void)hipHostFree(hdst[s]);
             if (dsrc[s]) (void)hipFree(dsrc[s]);
             if (ddst[s]) (void)hipFree(ddst[s]);
-            for (hipEvent_t e : {ev_h2d[s], ev_beg[s], ev_end[s], ev_d2h[s]})
+            for (hipEvent_t e : {ev_h2d[s], ev_d2h[s]})
                 if (e) (void)hipEventDestroy(e);
+        }
+        for (int e = 0; e < NEV; e++) {
+            if (ev_beg[e]) (void)hipEventDestroy(ev_beg[e]);
+            if (ev_end[e]) (void)hipEventDestroy(ev_end[e]);
         }
         if (h2d) (void)hipStreamDestroy(h2d);
         if (d2h) (void)hipStreamDestroy(d2h);
@@ -170,6 +181,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
     if (d->precision != TW_EXACT && d->precision != TW_PLAIN && d->precision != TW_MIXED)
         return tw_fail(ctx, TW_E_ARG, "unknown precision %d", d->precision);
     if (!d->read || !d->write) return tw_fail(ctx, TW_E_ARG, "null read or write callback");
+    if (d->flags & ~TW_STREAM_BLOCK_ORIGIN) return tw_fail(ctx, TW_E_ARG, "unknown stream flags 0x%x", d->flags);
     const int threads = d->threads < 1 ? 1 : (d->threads > 256 ? 256 : d->threads);
     int rc;
     if ((rc = tw_bind(ctx))) return rc;
@@ -247,11 +259,9 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
             k.sx0 = std::min(k.sx0, d->nx - 2);
             k.snx = 2;
         }
+        // Windows past the fused kernel's 32-bit offsets are resampled in
+        // z-slabs by tw_resample_tps_run (TW_EXACT addresses in 64 bits).
         const uint64_t sb = (uint64_t)k.snx * k.sny * k.snz * bpv;
-        if (sb + (uint64_t)k.snx * (k.sny + 1) * bpv + 16 > 0x7FFF0000ull || k.sny * k.snz >= (1ll << 24))
-            return tw_fail(ctx, TW_E_UNSUPPORTED,
-                           "brick %lld needs a %lld x %lld x %lld source window: use smaller bricks",
-                           (long long)b, (long long)k.snx, (long long)k.sny, (long long)k.snz);
         max_src = std::max<size_t>(max_src, sb);
         max_dst = std::max<size_t>(max_dst, (size_t)(k.nx * k.ny * k.nz * bpv));
     }
@@ -267,8 +277,10 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
         TW_HIP(ctx, hipMalloc(&P.ddst[s], max_dst));
         TW_HIP(ctx, hipEventCreateWithFlags(&P.ev_h2d[s], hipEventDisableTiming));
         TW_HIP(ctx, hipEventCreateWithFlags(&P.ev_d2h[s], hipEventDisableTiming));
-        TW_HIP(ctx, hipEventCreate(&P.ev_beg[s]));
-        TW_HIP(ctx, hipEventCreate(&P.ev_end[s]));
+    }
+    for (int e = 0; e < NEV; e++) {
+        TW_HIP(ctx, hipEventCreate(&P.ev_beg[e]));
+        TW_HIP(ctx, hipEventCreate(&P.ev_end[e]));
     }
     TW_HIP(ctx, hipStreamCreateWithFlags(&P.h2d, hipStreamNonBlocking));
     TW_HIP(ctx, hipStreamCreateWithFlags(&P.d2h, hipStreamNonBlocking));
@@ -314,7 +326,10 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
             if (!S.wait([&] { return S.d2h_issued >= b; })) return;
             if (hipEventSynchronize(P.ev_d2h[s]) != hipSuccess) return S.fail(TW_E_DEVICE, "download failed");
             float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, P.ev_beg[s], P.ev_end[s]);
+            const hipError_t te = hipEventElapsedTime(&ms, P.ev_beg[b % NEV], P.ev_end[b % NEV]);
+            if (te != hipSuccess)
+                return S.fail(TW_E_DEVICE, std::string("resample timing of brick ") + std::to_string(b) + ": " +
+                                               hipGetErrorString(te));
             const Brick& k = bricks[(size_t)b];
             const double t0 = now_s();
             const int r = d->write(d->write_user, b, k.x0, k.y0, k.z0, k.nx, k.ny, k.nz, P.hdst[s]);
@@ -336,7 +351,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
             if (!S.wait([&] { return S.ready >= b; })) return S.err;
             const size_t sb = (size_t)(k.snx * k.sny * k.snz * bpv), db = (size_t)(k.nx * k.ny * k.nz * bpv);
             // The device window of slot s is free once brick b - NSLOT's resample ended.
-            if (b >= NSLOT) TW_HIP(ctx, hipStreamWaitEvent(P.h2d, P.ev_end[s], 0));
+            if (b >= NSLOT) TW_HIP(ctx, hipStreamWaitEvent(P.h2d, P.ev_end[(b - NSLOT) % NEV], 0));
             TW_HIP(ctx, hipMemcpyAsync(P.dsrc[s], P.hsrc[s], sb, hipMemcpyHostToDevice, P.h2d));
             TW_HIP(ctx, hipEventRecord(P.ev_h2d[s], P.h2d));
             S.set([&] { S.h2d_issued[s] = b; });
@@ -346,13 +361,15 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
             tw_volume vs = {P.dsrc[s], k.snx, k.sny, k.snz, 1, k.sx0, k.sy0, k.sz0, d->nx, d->ny, d->nz,
                             d->dtype, TW_MEM_DEVICE};
             tw_volume vd = {P.ddst[s], k.nx, k.ny, k.nz, 1, k.x0, k.y0, k.z0, 0, 0, 0, d->dtype, TW_MEM_DEVICE};
-            TW_HIP(ctx, hipEventRecord(P.ev_beg[s], ctx->stream));
-            int r = tw_resample_tps_run(ctx, &vs, &vd, &g, interp, d->precision);
+            const i64 po[3] = {k.x0, k.y0, k.z0};
+            const bool block_origin = (d->flags & TW_STREAM_BLOCK_ORIGIN) != 0;
+            TW_HIP(ctx, hipEventRecord(P.ev_beg[b % NEV], ctx->stream));
+            int r = tw_resample_tps_run(ctx, &vs, &vd, &g, interp, d->precision, block_origin ? po : nullptr);
             if (r != TW_OK) return r;
-            TW_HIP(ctx, hipEventRecord(P.ev_end[s], ctx->stream));
+            TW_HIP(ctx, hipEventRecord(P.ev_end[b % NEV], ctx->stream));
             // The pinned brick of slot s is free once the writer took brick b - NSLOT.
             if (b >= NSLOT && !S.wait([&] { return S.written >= b - NSLOT; })) return S.err;
-            TW_HIP(ctx, hipStreamWaitEvent(P.d2h, P.ev_end[s], 0));
+            TW_HIP(ctx, hipStreamWaitEvent(P.d2h, P.ev_end[b % NEV], 0));
             TW_HIP(ctx, hipMemcpyAsync(P.hdst[s], P.ddst[s], db, hipMemcpyDeviceToHost, P.d2h));
             TW_HIP(ctx, hipEventRecord(P.ev_d2h[s], P.d2h));
             S.set([&] { S.d2h_issued = b; });

@@ -32,6 +32,7 @@ struct TpsArgs {
     const double* w64;                 // f x 4 FP64 weights TB
     float* dfout;                      // WRITE_DF: 3 planes of dnx*dny*dnz
     i64 zbase;                         // first z block of this launch (exact kernel)
+    i64 px0, py0, pz0;                 // exact kernel: origin of the sample-position sum
 };
 
 constexpr int TX = 64, TY = 4;  // reference-order kernel block
@@ -54,12 +55,18 @@ __device__ __forceinline__ double ld(const T* s, i64 i) {
 }
 
 // Reference blend in FP64 (q_imgwarp_tps_quicksmallmemory.cpp:133-160).
+// The sample position is a float sum.  In-memory (origin 0) it is
+// pos_warp + sx (106-108); the tera block warp adds the block's offset
+// afterwards, (pos_warp + sx) + x_offset with pos_warp block-local, each
+// step rounded to float (q_imgblockwarp_CYF, 2229-2235).  Adding the 0
+// origin changes nothing, so one expression serves both.
 template <typename T, int INTERP>
 __device__ __forceinline__ void resample_fp64(const TpsArgs& a, i64 X, i64 Y, i64 Z, float d0, float d1, float d2,
                                               i64 o, i64 dvol) {
     const T* __restrict__ src = (const T*)a.src;
     T* __restrict__ dst = (T*)a.dst;
-    const double p0 = (float)X + d0, p1 = (float)Y + d1, p2 = (float)Z + d2;
+    const float q0 = (float)(X - a.px0) + d0, q1 = (float)(Y - a.py0) + d1, q2 = (float)(Z - a.pz0) + d2;
+    const double p0 = q0 + (float)a.px0, p1 = q1 + (float)a.py0, p2 = q2 + (float)a.pz0;
     if (p0 < 0 || p0 > (double)(a.fnx - 1) || p1 < 0 || p1 > (double)(a.fny - 1) || p2 < 0 ||
         p2 > (double)(a.fnz - 1)) {
         for (i64 c = 0; c < a.nc; c++) dst[c * dvol + o] = 0;

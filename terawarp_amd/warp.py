@@ -328,7 +328,7 @@ class Warper:
 
     # ---- streaming ----------------------------------------------------------
     def warp_stream(self, size_xyz, dtype, centres, targets, source, f=8, brick=512, lam=0.2, interp=LINEAR,
-                    precision=MIXED, threads=16, keep=(), block_base=None):
+                    precision=MIXED, threads=16, keep=(), block_base=None, block_origin=False):
         """Warp a volume brick by brick (tw_warp_stream).
 
         source: a Synth (procedural volume, filled by the library's own
@@ -338,6 +338,8 @@ class Warper:
         bricks listed in keep, and the call's statistics.  With block_base,
         the bricks are written as the tera driver's block files instead
         (tw_block_file_write; checksums and kept bricks are then empty).
+        block_origin: TW_STREAM_BLOCK_ORIGIN (EXACT positions in the tera
+        block warp's block-local order, the bricks being its blocks).
         """
         ce, ta = _pts(centres), _pts(targets)
         nx, ny, nz = size_xyz
@@ -360,6 +362,7 @@ class Warper:
         d.nx, d.ny, d.nz, d.dtype, d.brick, d.f = nx, ny, nz, dt, brick, f
         d.interp, d.precision, d.threads, d.K, d.lam = interp, precision, threads, ce.shape[0], lam
         d.centres, d.targets = _fptr(ce), _fptr(ta)
+        d.flags = L.TW_STREAM_BLOCK_ORIGIN if block_origin else 0
         from .host import SliceDir
         if isinstance(source, Synth):
             d.read = ctypes.cast(self.lib.tw_synth_read, ctypes.c_void_p)

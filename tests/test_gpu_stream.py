@@ -68,6 +68,26 @@ def test_stream_exact_checksums_match_oracle(warper, oracle):
     assert st["h2d_bytes"] > 0 and st["d2h_bytes"] == vol.nbytes
 
 
+def test_stream_block_origin_exact_equals_tera_oracle(warper, oracle):
+    """TW_STREAM_BLOCK_ORIGIN with TW_EXACT: bricks are the tera driver's
+    blocks and positions take its block-local float order; the assembled
+    output equals the oracle's restatement of the tera TPS driver (whose
+    marker files carry a quarter of the coordinates) bit for bit, uint16,
+    bricks uneven on every axis."""
+    size, brick, f = (150, 97, 70), 40, 4
+    s = Synth(size, np.uint16, seed=11, cell=32, blobs_per_cell=4)
+    vol = s.fill(0, 0, 0, *size)
+    sub, tar = synth.landmarks(size, 30, 6.0, seed=11)
+    sub4, tar4 = sub / 4, tar / 4
+    ref = oracle.warp_tps_tera(vol, tar4, sub4, g=f, block=brick)
+    nb = 4 * 3 * 2
+    _, kept, st = warper.warp_stream(size, np.uint16, 4 * tar4, 4 * sub4, s, f=f, brick=brick,
+                                     precision=warp.EXACT, threads=3, keep=range(nb), block_origin=True)
+    assert st["bricks"] == nb
+    for b, box in boxes(size, brick):
+        assert np.array_equal(kept[b], ref[box]), (b, np.count_nonzero(kept[b] != ref[box]))
+
+
 def test_stream_python_source_and_read_failure(warper):
     size, brick, f = (96, 80, 48), 32, 8
     s = Synth(size, np.uint8, seed=2, cell=32, blobs_per_cell=2)

@@ -13,6 +13,8 @@ import pytest
 
 from terawarp_amd import synth, warp
 
+from .parity import check_fast_form
+
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 DF_TOL = 1e-2  # voxels
@@ -216,9 +218,9 @@ def test_chain_plain_and_mixed_vs_oracle(warper, oracle, dtype, size, g, K, dmax
     dense16 = warper.upsample_df(g16, size, precision=warp.MIXED)
     assert np.abs(dense16 - dense_ref).max() < DF_TOL
     outm = warper.warp_tps(vol, tar, sub, f=g, precision=warp.MIXED)
-    dm_ = np.abs(outm.astype(np.int64) - ref.astype(np.int64))
-    # Intensity errors follow from the <= 1e-2 voxel displacement bound.
-    assert np.mean(dm_ <= 1) >= 0.99
+    # Intensity errors follow from the <= 1e-2 voxel displacement bound:
+    # one level away from the faces, bounds-test flips counted at them.
+    check_fast_form(outm, ref, dense_ref, size)
     g16.release()
 
 
@@ -327,8 +329,8 @@ def test_linear_df_chain_vs_oracle(warper, oracle):
     diff = np.abs(out.astype(np.int64) - ref.astype(np.int64))
     assert np.mean(diff <= 1) >= 0.9999 and np.mean(diff == 0) >= 0.999, (diff.max(), np.mean(diff == 0))
     outm = warper.warp_tps(vol, tar, sub, f=g, interp=warp.LINEAR | warp.DF_LINEAR, precision=warp.MIXED)
-    dm_ = np.abs(outm.astype(np.int64) - ref.astype(np.int64))
-    assert np.mean(dm_ <= 1) >= 0.99
+    _, df, _ = oracle.warp_tps(vol, tar, sub, g=g)
+    check_fast_form(outm, ref, oracle.dense_df(size, df, g, kind=1), size)
     with pytest.raises(warp.TeraWarpError):
         warper.warp_tps(vol, tar, sub, f=g, interp=2 | warp.DF_LINEAR)
 

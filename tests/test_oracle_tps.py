@@ -278,6 +278,65 @@ def test_window_resample_matches_whole_volume(oracle):
     assert changed.any() and np.all(w[changed] == 0)
 
 
+# ---- tera-block driver (imgwarp_smallmemory_CYF) -------------------------------
+
+def test_tera_driver_one_block_is_the_in_memory_warp_of_4x_markers(oracle):
+    """With one block covering the volume (offset 0) the tera driver is the
+    in-memory chain on the markers scaled by 4: pins the x4 scale and the
+    roles (centres = target-space markers) of the tera restatement."""
+    size = (48, 40, 24)
+    vol = synth.blob_volume(size[::-1], np.uint8, n_blobs=20, amp=(20, 200), seed=5)
+    sub, tar = synth.landmarks(size, 20, 3.0, seed=5)
+    sub4, tar4 = sub / 4, tar / 4  # marker files on the 4x-downsampled view
+    ref, _, _ = oracle.warp_tps(vol, 4 * tar4, 4 * sub4, g=4)
+    assert np.array_equal(oracle.warp_tps_tera(vol, tar4, sub4, g=4, block=1000), ref)
+    # Scaling the other way round, or swapping the roles, changes the output.
+    assert not np.array_equal(oracle.warp_tps_tera(vol, tar, sub, g=4, block=1000), ref)
+    assert not np.array_equal(oracle.warp_tps_tera(vol, sub4, tar4, g=4, block=1000), ref)
+
+
+def test_block_grid_is_a_subgrid_of_the_volume_grid(oracle):
+    """A block's own grid (ceil(size/n) + 3 nodes at (i-1)n + offset) equals
+    the whole volume's grid from node offset/n on, when the offset is on the
+    node lattice: the tera driver's per-block grids and one grid over the
+    volume give the same field."""
+    size, g = (90, 60, 30), 4
+    sub, tar = synth.landmarks(size, 16, 3.0, seed=6)
+    c, d = oracle.stps_fit(tar, sub)
+    whole = oracle.grid_eval(tar, c, d, oracle.grid_shape(size, g), g)
+    off, bs = (40, 20, 8), (50, 40, 22)
+    gsh = tuple(int(np.ceil(n / g)) + 3 for n in bs)
+    blk = oracle.grid_eval(tar, c, d, gsh, g, origin=off)
+    i0 = [o // g for o in off]
+    sub_grid = whole[:, i0[2]:i0[2] + gsh[2], i0[1]:i0[1] + gsh[1], i0[0]:i0[0] + gsh[0]]
+    assert np.array_equal(blk, sub_grid)
+
+
+def test_tera_driver_differs_only_by_the_block_local_position_order(oracle):
+    """Blocks of 40 voxels: every voxel equals the in-memory warp except where
+    the block-local float order of the position, ((X - o) + d) + o, rounds
+    differently from X + d; such voxels exist, and differ by one level."""
+    size, g, B = (96, 72, 40), 4, 40
+    vol = synth.blob_volume(size[::-1], np.uint16, n_blobs=30, amp=(100, 4000), seed=4)
+    sub, tar = synth.landmarks(size, 24, 6.0, seed=4)
+    tera = oracle.warp_tps_tera(vol, tar / 4, sub / 4, g=g, block=B)
+    ref, df, _ = oracle.warp_tps(vol, tar, sub, g=g)
+    dense = oracle.dense_df(size, df, g)
+    X = [np.arange(n, dtype=np.int64) for n in size]
+    pos_mem, pos_blk = [], []
+    for a, shape in ((0, (1, 1, -1)), (1, (1, -1, 1)), (2, (-1, 1, 1))):
+        x = X[a].reshape(shape)
+        o = (x // B) * B
+        dv = dense[a]
+        pos_mem.append(np.float32(x) + dv)
+        pos_blk.append((np.float32(x - o) + dv) + np.float32(o))
+    same_pos = np.logical_and.reduce([pm == pb for pm, pb in zip(pos_mem, pos_blk)])
+    diff = tera.astype(np.int64) - ref.astype(np.int64)
+    assert not np.any(diff[same_pos]), "voxels with identical positions must agree"
+    assert (~same_pos).sum() > 1000
+    assert np.abs(diff).max() <= 1
+
+
 # ---- linear DF upsample (the reference's i_interpmethod_df = 0) ---------------
 # The reference declares the mode but never defines it; the oracle defines it on
 # the B-spline's block layout (voxel b*n + t between nodes b+1 and b+2).  These
