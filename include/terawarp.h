@@ -215,6 +215,21 @@ int tw_upsample_df(tw_ctx* ctx, const tw_grid* grid, int64_t nx, int64_t ny, int
 int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
                 const float* targets, int K, int f, double lambda, int interp, int precision);
 
+/* The chained warp of tw_warp_tps without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and outputs as tw_warp_tps; src and dst must be device
+ * buffers.  Every workspace comes from a static 64 MiB device array that each
+ * GPU holds from its first call until the process ends, host staging from a
+ * static host array; no hipMalloc, hipHostMalloc or hipFree is called.
+ * Limits: K <= 1024 landmarks, a source below 2 GiB, and the grid plus the
+ * fit within the 64 MiB (about 2.5 M grid nodes for TW_MIXED at K = 1024,
+ * e.g. 1024 x 1024 x 256 at f = 8 uses 0.6 M); larger requests are rejected
+ * with TW_E_UNSUPPORTED before any work.  Calls on one GPU share its array:
+ * calls that overlap on one GPU race. */
+int tw_warp_tps_noalloc(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
+                        const float* targets, int K, int f, double lambda, int interp, int precision);
+
 /* ---- batches ------------------------------------------------------------- */
 #define TW_WARP_TPS     0
 #define TW_WARP_AFFINE  1

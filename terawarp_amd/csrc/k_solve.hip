@@ -1401,7 +1401,16 @@ template <typename T>
 int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip, FitState* st,
                int fwd_done = 0) {
     const int nb = (n + NB - 1) / NB;
-    if (nb <= PIPE_MAX_BLOCKS) {
+    if (nb <= PIPE_MAX_BLOCKS && ctx->arena) {
+        // Allocation-free mode: the flags come from the arena and are cleared
+        // by a kernel before each solve (the arena keeps earlier calls' data).
+        void* p = nullptr;
+        int rc;
+        if ((rc = tw_ws(ctx, WS_SOLVE_FLAGS, 2 * PIPE_MAX_BLOCKS * sizeof(int), &p))) return rc;
+        if ((rc = tw_fill_zero_i32(ctx, (int*)p, 2 * PIPE_MAX_BLOCKS))) return rc;
+        hipLaunchKernelGGL(k_solve_pipe<T>, dim3(nb), dim3(PIPE_THREADS), 0, ctx->stream, L, Dinv, n, b, y, (int*)p,
+                           1, st, skip, fwd_done);
+    } else if (nb <= PIPE_MAX_BLOCKS) {
         if (!ctx->solve_flags) {
             void* p = nullptr;
             TW_HIP(ctx, hipMalloc(&p, 2 * PIPE_MAX_BLOCKS * sizeof(int)));

@@ -165,6 +165,22 @@ int upload_weights(tw_ctx* ctx, int f, int basis, const float** w32, const doubl
     void* p = nullptr;
     int rc = tw_ws(ctx, WS_WEIGHTS, b64 + b32, &p);
     if (rc) return rc;
+    if (ctx->arena) {
+        // Allocation-free mode: the tables go to the arena on every call,
+        // from the static host staging (not the context's cached copy).
+        void* h = nullptr;
+        if ((rc = tw_arena_host(ctx, b64 + b32, &h))) return rc;
+        double tb[4 * 64];
+        if (f > 64) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d above 64 is not supported", f);
+        tw_df_tb(f, basis, tb);
+        memcpy(h, tb, b64);
+        float* h32 = (float*)((uint8_t*)h + b64);
+        for (size_t i = 0; i < n; i++) h32[i] = (float)tb[i];
+        TW_HIP(ctx, hipMemcpyAsync(p, h, b64 + b32, hipMemcpyHostToDevice, ctx->stream));
+        *w64 = (const double*)p;
+        *w32 = (const float*)((uint8_t*)p + b64);
+        return TW_OK;
+    }
     if (ctx->weights_f != f || ctx->weights_basis != basis || ctx->weights_host.size() != b64 + b32) {
         std::vector<double> tb(n);
         tw_df_tb(f, basis, tb.data());
@@ -273,6 +289,10 @@ bool fits_fast(i64 nx, i64 ny, i64 nz, int bpv) {
 // fit the fused kernel; used when the whole source does not.
 template <typename T>
 int resample_slabs(tw_ctx* ctx, TpsArgs a, const tw_grid* grid, int interp, int precision, bool f16) {
+    if (ctx->arena)
+        return tw_fail(ctx, TW_E_UNSUPPORTED,
+                       "a source window past 2 GiB is resampled in slabs, which the allocation-free entry point "
+                       "does not take");
     const i64 gz = grid->gz;
     void* bp = nullptr;
     int rc = tw_ws(ctx, WS_AUX_BOUNDS, (size_t)gz * 6 * sizeof(float), &bp);

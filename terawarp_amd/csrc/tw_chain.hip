@@ -13,14 +13,11 @@
 #include <cmath>
 #include <vector>
 
-extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
-                           const float* targets, int K, int f, double lambda, int interp, int precision) {
-    if (!ctx) return TW_E_ARG;
+int tw_warp_tps_check(tw_ctx* ctx, const tw_volume* src, const tw_volume* dst, int f, int interp, int precision) {
     int rc;
     if ((rc = tw_check_volume(ctx, src, "src")) || (rc = tw_check_volume(ctx, dst, "dst"))) return rc;
     if (src->dtype != dst->dtype || src->nc != dst->nc)
         return tw_fail(ctx, TW_E_ARG, "src and dst must share dtype and channel count");
-    const int basis = (interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
     interp &= ~TW_DF_LINEAR;
     if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
         return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN (optionally | TW_DF_LINEAR)");
@@ -30,7 +27,23 @@ extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, co
     // Block-size checks of imgwarp_smallmemory_image (2011-2020).
     if (f >= src->nx || f >= src->ny || f >= src->nz)
         return tw_fail(ctx, TW_E_ARG, "block size should be smaller than the image size");
+    return TW_OK;
+}
+
+extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres,
+                           const float* targets, int K, int f, double lambda, int interp, int precision) {
+    if (!ctx) return TW_E_ARG;
+    int rc;
+    if ((rc = tw_warp_tps_check(ctx, src, dst, f, interp, precision))) return rc;
     if ((rc = tw_bind(ctx))) return rc;
+    return tw_warp_tps_chain(ctx, src, dst, centres, targets, K, f, lambda, interp, precision);
+}
+
+int tw_warp_tps_chain(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres, const float* targets,
+                      int K, int f, double lambda, int interp, int precision) {
+    int rc;
+    const int basis = (interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
+    interp &= ~TW_DF_LINEAR;
     tw_stages_reset(ctx);
     tw_stage_begin(ctx, ST_TOTAL);
 

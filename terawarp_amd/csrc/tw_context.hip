@@ -31,8 +31,53 @@ void tw_host_stage_reset(tw_ctx* ctx) {
     ctx->staging.clear();
 }
 
+namespace {
+__global__ void k_fill_zero_i32(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+}  // namespace
+
+int tw_fill_zero_i32(tw_ctx* ctx, int* p, int n) {
+    if (n <= 0) return TW_OK;
+    hipLaunchKernelGGL(k_fill_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n);
+    TW_LAUNCH_CHECK(ctx);
+    return TW_OK;
+}
+
+int tw_arena_carve(tw_ctx* ctx, size_t bytes, void** out) {
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (need > ctx->arena_cap - ctx->arena_off)
+        return tw_fail(ctx, TW_E_NOMEM,
+                       "the allocation-free entry point's %zu MiB device scratch is full (%zu bytes more needed)",
+                       ctx->arena_cap >> 20, need - (ctx->arena_cap - ctx->arena_off));
+    *out = ctx->arena + ctx->arena_off;
+    ctx->arena_off += need;
+    return TW_OK;
+}
+
+int tw_arena_host(tw_ctx* ctx, size_t bytes, void** out) {
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (need > ctx->arena_host_cap - ctx->arena_host_off)
+        return tw_fail(ctx, TW_E_NOMEM, "the allocation-free entry point's %zu KiB host staging is full",
+                       ctx->arena_host_cap >> 10);
+    *out = ctx->arena_host + ctx->arena_host_off;
+    ctx->arena_host_off += need;
+    return TW_OK;
+}
+
 int tw_ws(tw_ctx* ctx, int slot, size_t bytes, void** out) {
     if (bytes == 0) bytes = 256;
+    if (ctx->arena) {
+        if (!ctx->arena_ws[slot] || ctx->arena_ws_cap[slot] < bytes) {
+            void* p = nullptr;
+            if (int rc = tw_arena_carve(ctx, bytes, &p)) return rc;
+            ctx->arena_ws[slot] = p;
+            ctx->arena_ws_cap[slot] = bytes;
+        }
+        *out = ctx->arena_ws[slot];
+        return TW_OK;
+    }
     if (ctx->ws_cap[slot] < bytes) {
         if (ctx->ws[slot]) {
             TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -55,6 +100,10 @@ int tw_ws(tw_ctx* ctx, int slot, size_t bytes, void** out) {
 }
 
 int tw_pinned(tw_ctx* ctx, size_t bytes, void** out) {
+    // Allocation-free mode: a fresh region of the static host staging per
+    // request (regions are not reused within a call, and the call has
+    // synchronised before the next one refills them).
+    if (ctx->arena) return tw_arena_host(ctx, bytes, out);
     if (ctx->pin_busy) {
         TW_HIP(ctx, hipEventSynchronize(ctx->pin_ev));
         ctx->pin_busy = false;
@@ -78,6 +127,7 @@ int tw_pinned(tw_ctx* ctx, size_t bytes, void** out) {
 }
 
 int tw_pinned_issued(tw_ctx* ctx) {
+    if (ctx->arena) return TW_OK;
     if (!ctx->pin_ev) TW_HIP(ctx, hipEventCreateWithFlags(&ctx->pin_ev, hipEventDisableTiming));
     TW_HIP(ctx, hipEventRecord(ctx->pin_ev, ctx->stream));
     ctx->pin_busy = true;
@@ -86,6 +136,14 @@ int tw_pinned_issued(tw_ctx* ctx) {
 
 int tw_read_small(tw_ctx* ctx, void* out, const void* dev, size_t bytes) {
     if (bytes > 4096) return tw_fail(ctx, TW_E_ARG, "tw_read_small: %zu bytes", bytes);
+    if (ctx->arena) {
+        void* h = nullptr;
+        if (int rc = tw_arena_host(ctx, bytes, &h)) return rc;
+        TW_HIP(ctx, hipMemcpyAsync(h, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        TW_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy(out, h, bytes);
+        return TW_OK;
+    }
     if (!ctx->pin_state) {
         hipError_t e = hipHostMalloc(&ctx->pin_state, 4096, hipHostMallocDefault);
         if (e != hipSuccess) {
@@ -214,6 +272,7 @@ int tw_create(int device, tw_ctx** out) {
     if (device < 0 || device >= n) return tw_fail(nullptr, TW_E_ARG, "device %d out of range [0,%d)", device, n);
     tw_ctx* ctx = new tw_ctx();
     ctx->device = device;
+    ctx->err.reserve(512);  // tw_fail's messages (< 512 bytes) never allocate
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return tw_fail(nullptr, TW_E_DEVICE, "cannot initialise device %d", device);

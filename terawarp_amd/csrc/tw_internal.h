@@ -35,6 +35,7 @@ enum TwSlot {
     WS_BATCH_GRID,  // batch: grids (FP32 fields, FP16 planes, block tables)
     WS_BATCH_GTAB,  // batch: grid job tables
     WS_BATCH_RTAB,  // batch: resample job tables, queues, timestamps
+    WS_SOLVE_FLAGS, // allocation-free mode: the pipelined solve's flags
     WS_COUNT
 };
 
@@ -75,7 +76,25 @@ struct tw_ctx {
     // Host images of a batch's tables, kept until the next batch (the batch
     // synchronises before it returns).
     std::vector<std::vector<uint8_t>> staging;
+    // Allocation-free calls (tw_warp_tps_noalloc): while `arena` is set, every
+    // workspace is carved from it (arena_cap bytes of static device memory,
+    // bump-allocated from arena_off, one region per slot), host staging comes
+    // from arena_host (static host memory) and nothing is allocated.  The
+    // entry point sets and clears these around one call.
+    uint8_t* arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
+    void* arena_ws[WS_COUNT] = {};
+    size_t arena_ws_cap[WS_COUNT] = {};
+    uint8_t* arena_host = nullptr;
+    size_t arena_host_cap = 0, arena_host_off = 0;
 };
+
+// Allocation-free mode: `bytes` of the context's device arena (256-byte
+// aligned) or of its host staging arena; TW_E_NOMEM when the arena is full.
+int tw_arena_carve(tw_ctx* ctx, size_t bytes, void** out);
+int tw_arena_host(tw_ctx* ctx, size_t bytes, void** out);
+// Zero `n` ints of device memory with a kernel on the context's stream.
+int tw_fill_zero_i32(tw_ctx* ctx, int* p, int n);
 
 // A zeroed host buffer of `bytes` that lives until tw_host_stage_reset.
 uint8_t* tw_host_stage(tw_ctx* ctx, size_t bytes);
@@ -162,6 +181,16 @@ int tw_grid_eval_dev(tw_ctx* ctx, const float4* pts, const double* cf, const dou
 // block's offset; null: p + d.
 int tw_resample_tps_run(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const tw_grid* grid, int interp,
                         int precision, const i64* porigin = nullptr);
+// Argument checks of the chained warp (tw_warp_tps and its allocation-free
+// form); interp may carry TW_DF_LINEAR.
+int tw_warp_tps_check(tw_ctx* ctx, const tw_volume* src, const tw_volume* dst, int f, int interp, int precision);
+// The chained warp of tw_warp_tps on validated arguments (the allocation-free
+// entry point runs it with the arena set).
+int tw_warp_tps_chain(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres, const float* targets,
+                      int K, int f, double lambda, int interp, int precision);
+// Does a source window of nx*ny*nz voxels of `bpv` bytes fit the fused
+// resampler's 32-bit tap offsets (else it runs in z-slabs)?
+bool fits_fast(i64 nx, i64 ny, i64 nz, int bpv);
 
 // ---- batches ---------------------------------------------------------------
 // Fits of a batch: systems up to the small-system limit in one launch (one
