@@ -186,7 +186,8 @@ int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, d
  * gpu_computedistance (weiyijisuan.cu:118-219).  grid->gx..gz, f, ox..oz,
  * storage and mem are inputs; grid->data must hold 3*gx*gy*gz values of
  * the storage type.  precision TW_EXACT: FP64 in the reference's order;
- * otherwise FP32 with compensated summation. */
+ * otherwise FP32 node-landmark distances and square roots with the landmark
+ * sum accumulated in FP64. */
 int tw_tps_grid_eval(tw_ctx* ctx, const float* centres, int K, const double* c, const double* d,
                      int precision, tw_grid* grid);
 
