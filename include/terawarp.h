@@ -135,6 +135,9 @@ int         tw_create(int device, tw_ctx** out);
 int         tw_destroy(tw_ctx* ctx);
 /* Use an existing hipStream_t (as void*); NULL restores the context's own. */
 int         tw_set_stream(tw_ctx* ctx, void* hip_stream);
+/* The hipStream_t (as void*) the context launches on, so that a caller can
+ * order its own streams against it (NULL for a null context). */
+void*       tw_get_stream(tw_ctx* ctx);
 int         tw_sync(tw_ctx* ctx);
 /* 1: record per-stage HIP events on every call (tw_get_timings). */
 int         tw_set_timing(tw_ctx* ctx, int enable);
@@ -267,6 +270,19 @@ typedef struct {
  * kernel to the end of the item's last output tile. */
 int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int f, double lambda, int interp,
                   int precision, int32_t* status, double* done_ms);
+
+/* Many output bricks of one volume against one grid in one persistent launch.
+ * Replaces the tera-block drivers' per-block gpu_interpolation calls
+ * (chazhi.cu:557-663, called once per block from
+ * q_imgwarp_tps_quicksmallmemory.cpp:1309-1848).  srcs[i] is brick i's source
+ * window (origin x0/y0/z0 and full_* the whole volume, which sets the bounds
+ * test), dsts[i] its output box (global origin); every volume in device
+ * memory.  The grid (device, F32 or F16 with TW_MIXED) covers every output
+ * box.  Each brick's output equals tw_resample_tps on the same window, box and
+ * grid.  precision: TW_PLAIN or TW_MIXED.  Asynchronous on the context's
+ * stream. */
+int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const tw_volume* dsts, int n, const tw_grid* grid,
+                           int interp, int precision);
 
 /* ---- streaming --------------------------------------------------------- */
 /* A volume larger than HBM (or than host memory), warped brick by brick.

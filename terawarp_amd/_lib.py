@@ -38,10 +38,10 @@ TW_WARP_AFFINE = 1
 
 # Every entry point include/terawarp.h declares.
 EXPORTS = (
-    "tw_create", "tw_destroy", "tw_set_stream", "tw_sync", "tw_set_timing", "tw_get_timings", "tw_last_error",
+    "tw_create", "tw_destroy", "tw_set_stream", "tw_get_stream", "tw_sync", "tw_set_timing", "tw_get_timings", "tw_last_error",
     "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
     "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_warp_tps_noalloc", "tw_warp_batch",
-    "tw_warp_stream",
+    "tw_resample_tps_bricks", "tw_warp_stream",
     "tw_synth_fill", "tw_synth_read", "tw_checksum64", "tw_checksum_write", "tw_grid_dim",
 )
 # Every entry point include/terawarp_host.h declares.
@@ -148,6 +148,7 @@ def load():
         "tw_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
         "tw_destroy": (ctypes.c_int, [_P]),
         "tw_set_stream": (ctypes.c_int, [_P, _P]),
+        "tw_get_stream": (_P, [_P]),
         "tw_sync": (ctypes.c_int, [_P]),
         "tw_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
         "tw_get_timings": (ctypes.c_int, [_P, T]),
@@ -164,6 +165,7 @@ def load():
                                           ctypes.c_int]),
         "tw_warp_tps": (ctypes.c_int, [_P, V, V, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_int]),
+        "tw_resample_tps_bricks": (ctypes.c_int, [_P, V, V, ctypes.c_int, G, ctypes.c_int, ctypes.c_int]),
         "tw_warp_tps_noalloc": (ctypes.c_int, [_P, V, V, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_int, ctypes.c_int]),
         "tw_warp_batch": (ctypes.c_int, [_P, ctypes.POINTER(TwWarpItem), ctypes.c_int, ctypes.c_int, ctypes.c_double,

@@ -51,7 +51,7 @@ struct BatchJob {
 // parts: for XCD x at parts + x * (2 * njobs + 1): P[0..njobs] (prefix sums of
 // the part sizes) then S[0..njobs-1] (first tile of each part).  t_end[j]:
 // device clock when the last tile of job j ended.
-template <int MODE, bool F16, int INTERP>
+template <int MODE, bool F16, int INTERP, bool WIN = false>
 __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restrict__ jobs_g,
                                                        const int* __restrict__ parts_g, int njobs, int nwg_x,
                                                        unsigned long long* __restrict__ t_end) {
@@ -77,9 +77,9 @@ __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restric
             const void* src = (const uint8_t*)a.src + ch * J.src_cstride;
             void* dst = (uint8_t*)a.dst + ch * J.dst_cstride;
             if (J.dtype == TW_U8)
-                tps_tile<uint8_t, MODE, F16, INTERP, false, false>(a, g, t, src, dst, lds, rmax_w);
+                tps_tile<uint8_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
             else
-                tps_tile<uint16_t, MODE, F16, INTERP, false, false>(a, g, t, src, dst, lds, rmax_w);
+                tps_tile<uint16_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
         } else {
             const AffineArgs& af = *(const AffineArgs*)&J.af;
             const AffGeom& ag = *(const AffGeom*)&J.ag;
@@ -364,4 +364,146 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
             done_ms[i] = khz > 0 && ts[1 + i] >= ts[0] ? (double)(ts[1 + i] - ts[0]) / (double)khz : 0.0;
     }
     return first;
+}
+
+// ---- bricks of one volume against one grid ----------------------------------
+// The tera-block drivers warp one block per call (q_imgwarp_tps_quicksmallmemory.cpp:
+// 1309-1848: per block gpu_interpolation, chazhi.cu:557-663).  Here every brick's
+// output tiles go through one persistent launch of the batch kernel, each job a
+// source window (global origin, bounds tested against the whole volume) and an
+// output brick, all sharing the caller's grid.
+extern "C" int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const tw_volume* dsts, int n,
+                                      const tw_grid* grid, int interp, int precision) {
+    if (!ctx) return TW_E_ARG;
+    if (!srcs || !dsts || n < 1) return tw_fail(ctx, TW_E_ARG, "no bricks");
+    if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
+        return tw_fail(ctx, TW_E_ARG, "interp must be TW_INTERP_LINEAR or TW_INTERP_NN");
+    if (precision != TW_PLAIN && precision != TW_MIXED)
+        return tw_fail(ctx, TW_E_ARG, "bricks run TW_PLAIN or TW_MIXED (TW_EXACT: tw_resample_tps per brick)");
+    if (!grid || !grid->data || grid->mem != TW_MEM_DEVICE || grid->f < 1 || grid->f > 64)
+        return tw_fail(ctx, TW_E_ARG, "the grid must be device-resident with spacing 1..64");
+    const bool f16 = grid->storage == TW_DF_F16;
+    if (f16 && (!grid->qtab || precision != TW_MIXED))
+        return tw_fail(ctx, TW_E_ARG, "an FP16 grid needs its block table and TW_MIXED");
+    int rc;
+    for (int i = 0; i < n; i++) {
+        const tw_volume &s = srcs[i], &d = dsts[i];
+        if ((rc = tw_check_volume(ctx, &s, "src")) || (rc = tw_check_volume(ctx, &d, "dst")))
+            return tw_fail(ctx, rc, "brick %d: %s", i, tw_last_error(ctx));
+        if (s.mem != TW_MEM_DEVICE || d.mem != TW_MEM_DEVICE)
+            return tw_fail(ctx, TW_E_ARG, "brick %d: volumes must be in device memory", i);
+        if (s.dtype != d.dtype || s.nc != d.nc)
+            return tw_fail(ctx, TW_E_ARG, "brick %d: src and dst must share dtype and channel count", i);
+        const i64 f = grid->f;
+        if (d.x0 < grid->ox || d.y0 < grid->oy || d.z0 < grid->oz ||
+            (d.x0 + d.nx - 1 - grid->ox) / f + 3 >= grid->gx || (d.y0 + d.ny - 1 - grid->oy) / f + 3 >= grid->gy ||
+            (d.z0 + d.nz - 1 - grid->oz) / f + 3 >= grid->gz)
+            return tw_fail(ctx, TW_E_ARG, "brick %d: the grid does not cover its output box", i);
+    }
+    if ((rc = tw_bind(ctx))) return rc;
+    tw_host_stage_reset(ctx);
+    tw_stages_reset(ctx);
+    tw_stage_begin(ctx, ST_TOTAL);
+    const size_t jobs_b = up256((size_t)n * sizeof(BatchJob));
+    const size_t parts_b = up256((size_t)8 * (2 * n + 1) * sizeof(int));
+    const size_t ts_b = up256((size_t)(n + 1) * sizeof(unsigned long long));
+    void* rbase = nullptr;
+    if ((rc = tw_ws(ctx, WS_BATCH_RTAB, jobs_b + parts_b + ts_b, &rbase))) return rc;
+    uint8_t* rdev = (uint8_t*)rbase;
+    uint8_t* host = tw_host_stage(ctx, jobs_b + parts_b);
+    BatchJob* hj = (BatchJob*)host;
+    int* parts = (int*)(host + jobs_b);
+    std::vector<i64> tiles((size_t)n);
+    size_t lds = 0;
+    const int tz = pick_tz(grid->f);
+    for (int i = 0; i < n; i++) {
+        const tw_volume &s = srcs[i], &d = dsts[i];
+        BatchJob& J = hj[i];
+        memset(&J, 0, sizeof J);
+        J.kind = BK_TPS;
+        J.dtype = s.dtype;
+        const int bpv = dtype_size(s.dtype);
+        J.src_cstride = s.nx * s.ny * s.nz * (i64)bpv;
+        J.dst_cstride = d.nx * d.ny * d.nz * (i64)bpv;
+        TpsArgs& a = J.t;
+        a.src = s.data;
+        a.dst = d.data;
+        a.snx = s.nx; a.sny = s.ny; a.snz = s.nz;
+        a.sx0 = s.x0; a.sy0 = s.y0; a.sz0 = s.z0;
+        a.fnx = s.full_nx ? s.full_nx : s.x0 + s.nx;
+        a.fny = s.full_ny ? s.full_ny : s.y0 + s.ny;
+        a.fnz = s.full_nz ? s.full_nz : s.z0 + s.nz;
+        a.dnx = d.nx; a.dny = d.ny; a.dnz = d.nz;
+        a.dx0 = d.x0; a.dy0 = d.y0; a.dz0 = d.z0;
+        a.nc = s.nc;
+        a.src_cstride = J.src_cstride;
+        a.dst_cstride = J.dst_cstride;
+        a.grid = grid->data;
+        a.gx = grid->gx; a.gy = grid->gy; a.gz = grid->gz;
+        a.ox = grid->ox; a.oy = grid->oy; a.oz = grid->oz;
+        a.f = grid->f;
+        a.basis = grid->basis;
+        a.tz = tz;
+        a.qtab = grid->qtab;
+        a.qbx = (grid->gx + 7) / 8;
+        a.qby = (grid->gy + 7) / 8;
+        if (s.nx < 2) return tw_fail(ctx, TW_E_UNSUPPORTED, "brick %d: a source window needs two columns", i);
+        size_t l = 0;
+        const char* why = s.dtype == TW_U8 ? fused_geom<uint8_t, false>(a, J.g, l) : fused_geom<uint16_t, false>(a, J.g, l);
+        if (why) return tw_fail(ctx, TW_E_UNSUPPORTED, "brick %d: %s", i, why);
+        lds = std::max(lds, l);
+        if ((i64)J.g.ntiles * s.nc >= (1ll << 30)) return tw_fail(ctx, TW_E_UNSUPPORTED, "brick %d: too many tiles", i);
+        J.ntile = J.g.ntiles;
+        tiles[i] = (i64)J.g.ntiles * s.nc;
+    }
+    for (int x = 0; x < 8; x++) {
+        int* P = parts + x * (2 * n + 1);
+        int* S = P + n + 1;
+        i64 acc = 0;
+        P[0] = 0;
+        for (int i = 0; i < n; i++) {
+            const i64 a = tiles[i] * x / 8, b = tiles[i] * (x + 1) / 8;
+            S[i] = (int)a;
+            acc += b - a;
+            if (acc >= (1ll << 31)) return tw_fail(ctx, TW_E_UNSUPPORTED, "too many tiles for one launch");
+            P[i + 1] = (int)acc;
+        }
+    }
+    TW_HIP(ctx, hipMemcpyAsync(rdev, host, jobs_b + parts_b, hipMemcpyHostToDevice, ctx->stream));
+    unsigned long long* t0 = (unsigned long long*)(rdev + jobs_b + parts_b);
+    unsigned long long* t_end = t0 + 1;
+    hipLaunchKernelGGL(k_batch_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, t0, t_end, n);
+    TW_LAUNCH_CHECK(ctx);
+    const void* kern;
+    if (interp == TW_INTERP_NN)
+        kern = f16 ? (const void*)k_resample_batch<TW_PLAIN, true, TW_INTERP_NN, true>
+                   : (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_NN, true>;
+    else if (precision == TW_MIXED)
+        kern = f16 ? (const void*)k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR, true>
+                   : (const void*)k_resample_batch<TW_MIXED, false, TW_INTERP_LINEAR, true>;
+    else
+        kern = (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR, true>;
+    int per_cu = 0;
+    TW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, FT, lds));
+    int nwg = std::max(1, per_cu) * ctx->num_cus;
+    nwg = std::max(8, nwg - nwg % 8);
+    const int nwg_x = nwg / 8;
+    const BatchJob* djobs = (const BatchJob*)rdev;
+    const int* dparts = (const int*)(rdev + jobs_b);
+    tw_stage_begin(ctx, ST_RESAMPLE);
+    const dim3 lg((unsigned)nwg), lb(TX, FTY, 1);
+#define TW_BRICKS(M, H, I) \
+    hipLaunchKernelGGL((k_resample_batch<M, H, I, true>), lg, lb, lds, ctx->stream, djobs, dparts, n, nwg_x, t_end)
+    if (interp == TW_INTERP_NN) {
+        if (f16) TW_BRICKS(TW_PLAIN, true, TW_INTERP_NN); else TW_BRICKS(TW_PLAIN, false, TW_INTERP_NN);
+    } else if (precision == TW_MIXED) {
+        if (f16) TW_BRICKS(TW_MIXED, true, TW_INTERP_LINEAR); else TW_BRICKS(TW_MIXED, false, TW_INTERP_LINEAR);
+    } else {
+        TW_BRICKS(TW_PLAIN, false, TW_INTERP_LINEAR);
+    }
+#undef TW_BRICKS
+    TW_LAUNCH_CHECK(ctx);
+    tw_stage_end(ctx, ST_RESAMPLE);
+    tw_stage_end(ctx, ST_TOTAL);
+    return TW_OK;
 }

@@ -316,6 +316,7 @@ int tw_set_stream(tw_ctx* ctx, void* hip_stream) {
     return TW_OK;
 }
 
+void* tw_get_stream(tw_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 int tw_sync(tw_ctx* ctx) {
     if (!ctx) return TW_E_ARG;
     TW_HIP(ctx, hipStreamSynchronize(ctx->stream));

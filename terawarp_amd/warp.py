@@ -167,8 +167,25 @@ class Warper:
         if rc != L.TW_OK:
             raise TeraWarpError(rc, "tw_create failed (no usable HIP device?)")
         self.ctx = ctx
+        self._ext = None
         if timing:
             self.lib.tw_set_timing(self.ctx, 1)
+
+    def _ordered(self, fn, *args):
+        """Call the library with its stream ordered after torch's current
+        stream (so torch's writes to the inputs and outputs land first) and
+        torch's current stream ordered after the call's work: event waits on
+        the device, no host synchronisation."""
+        torch = _torch()
+        if not torch.cuda.is_available():
+            return fn(*args)
+        if self._ext is None:
+            self._ext = torch.cuda.ExternalStream(self.lib.tw_get_stream(self.ctx))
+        cur = torch.cuda.current_stream()
+        self._ext.wait_stream(cur)
+        rc = fn(*args)
+        cur.wait_stream(self._ext)
+        return rc
 
     def close(self):
         if self.ctx:
@@ -195,6 +212,7 @@ class Warper:
         """Launch on a torch.cuda.Stream (or a raw hipStream_t int); None = own stream."""
         ptr = None if stream is None else (stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
         self._check(self.lib.tw_set_stream(self.ctx, ptr))
+        self._ext = None
 
     def set_timing(self, on=True):
         self._check(self.lib.tw_set_timing(self.ctx, 1 if on else 0))
@@ -220,7 +238,7 @@ class Warper:
             if src.ndim == 3:
                 dst = dst[0]
         vs, vd = _vol(src, src_origin, full), _vol(dst, dst_origin)
-        self._check(self.lib.tw_resample_affine(self.ctx, ctypes.byref(vs), ctypes.byref(vd), _dptr(M), interp))
+        self._check(self._ordered(self.lib.tw_resample_affine, self.ctx, ctypes.byref(vs), ctypes.byref(vd), _dptr(M), interp))
         return dst
 
     # ---- TPS -------------------------------------------------------------
@@ -250,7 +268,7 @@ class Warper:
         ce = _pts(centres)
         c = np.ascontiguousarray(c, np.float64)
         d = np.ascontiguousarray(d, np.float64)
-        self._check(self.lib.tw_tps_grid_eval(self.ctx, _fptr(ce), ce.shape[0], _dptr(c), _dptr(d), precision,
+        self._check(self._ordered(self.lib.tw_tps_grid_eval, self.ctx, _fptr(ce), ce.shape[0], _dptr(c), _dptr(d), precision,
                                               ctypes.byref(grid.g)))
         return grid
 
@@ -266,14 +284,27 @@ class Warper:
                 raise ValueError("pass dst for device volumes")
             dst = np.zeros_like(src)
         vs, vd = _vol(src, src_origin, full), _vol(dst, dst_origin)
-        self._check(self.lib.tw_resample_tps(self.ctx, ctypes.byref(vs), ctypes.byref(vd), ctypes.byref(grid.g),
+        self._check(self._ordered(self.lib.tw_resample_tps, self.ctx, ctypes.byref(vs), ctypes.byref(vd), ctypes.byref(grid.g),
                                              interp, precision))
         return dst
+
+    def resample_bricks(self, srcs, dsts, grid, src_origins, dst_origins, full, interp=LINEAR, precision=MIXED):
+        """tw_resample_tps_bricks: device source windows srcs[i] (global origin
+        src_origins[i], whole-volume extent full) resampled into the device
+        output boxes dsts[i] (global origin dst_origins[i]) against one device
+        grid, in one launch.  Asynchronous on the context's stream."""
+        n = len(srcs)
+        if len(dsts) != n or len(src_origins) != n or len(dst_origins) != n:
+            raise ValueError("srcs, dsts and origins differ in length")
+        vs = (L.TwVolume * n)(*[_vol(s, o, full) for s, o in zip(srcs, src_origins)])
+        vd = (L.TwVolume * n)(*[_vol(d, o) for d, o in zip(dsts, dst_origins)])
+        self._check(self._ordered(self.lib.tw_resample_tps_bricks, self.ctx, vs, vd, n, ctypes.byref(grid.g), interp, precision))
+        return dsts
 
     def upsample_df(self, grid, size_xyz, origin=(0, 0, 0), precision=PLAIN):
         nx, ny, nz = size_xyz
         out = np.zeros((3, nz, ny, nx), np.float32)
-        self._check(self.lib.tw_upsample_df(self.ctx, ctypes.byref(grid.g), nx, ny, nz, origin[0], origin[1],
+        self._check(self._ordered(self.lib.tw_upsample_df, self.ctx, ctypes.byref(grid.g), nx, ny, nz, origin[0], origin[1],
                                             origin[2], precision, _fptr(out), L.TW_MEM_HOST))
         return out
 
@@ -284,7 +315,7 @@ class Warper:
                 raise ValueError("pass dst for device volumes")
             dst = np.zeros_like(src)
         vs, vd = _vol(src), _vol(dst)
-        self._check(self.lib.tw_warp_tps(self.ctx, ctypes.byref(vs), ctypes.byref(vd), _fptr(ce), _fptr(ta),
+        self._check(self._ordered(self.lib.tw_warp_tps, self.ctx, ctypes.byref(vs), ctypes.byref(vd), _fptr(ce), _fptr(ta),
                                          ce.shape[0], f, lam, interp, precision))
         return dst
 
@@ -322,7 +353,8 @@ class Warper:
                 w.targets = _fptr(ta)
         status = (ctypes.c_int32 * n)()
         done = np.zeros(n, np.float64)
-        rc = self.lib.tw_warp_batch(self.ctx, arr, n, f, lam, interp, precision, status, _dptr(done))
+        rc = self._ordered(self.lib.tw_warp_batch, self.ctx, arr, n, f, lam, interp, precision, status,
+                           _dptr(done))
         self._check(rc)
         return [int(x) for x in status], done
 
@@ -390,7 +422,7 @@ class Warper:
             d.write = ctypes.cast(self.lib.tw_checksum_write, ctypes.c_void_p)
             d.write_user = ctypes.cast(ctypes.byref(sink), ctypes.c_void_p)
         st = L.TwStreamStats()
-        self._check(self.lib.tw_warp_stream(self.ctx, ctypes.byref(d), ctypes.byref(st)))
+        self._check(self._ordered(self.lib.tw_warp_stream, self.ctx, ctypes.byref(d), ctypes.byref(st)))
         return sums, kept, {k: getattr(st, k) for k, _ in L.TwStreamStats._fields_}
 
 

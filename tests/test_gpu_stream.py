@@ -143,3 +143,47 @@ def test_stream_to_block_files_and_slices(warper, tmp_path):
         out = np.zeros((size[1], size[0]), np.uint8)
         assert lib.tw_v3draw_read(tera_paths(base, z=z)[1].encode(), out.ctypes.data, out.nbytes) == 0
         assert np.array_equal(out, ref[z]), z
+
+
+@pytest.mark.parametrize("precision,f16", [(warp.PLAIN, False), (warp.MIXED, False), (warp.MIXED, True)])
+def test_resample_bricks_equals_single_calls(warper, precision, f16):
+    """tw_resample_tps_bricks: every brick of one launch equals
+    tw_resample_tps on the same source window, output box and grid, and the
+    bricks assemble the whole-volume resample (uneven bricks, windows from
+    the field's bound)."""
+    import torch
+    size, brick, g = (200, 150, 90), 64, 8
+    s = Synth(size, np.uint16, seed=6, cell=64, blobs_per_cell=4)
+    vol = s.fill(0, 0, 0, *size)
+    sub, tar = synth.landmarks(size, 40, 6.0, seed=6)
+    c, d = warper.tps_fit(tar, sub, precision=warp.PLAIN)
+    gsh = tuple(warp.grid_dim(n, g) for n in size)
+    grid = warper.new_grid(gsh, g, storage=warp.F16 if f16 else warp.F32, device=True)
+    warper.grid_eval(tar, c, d, grid, precision=precision)
+    m = 6 + 4  # field bound plus the rounding margin
+    srcs, dsts, so, do, boxes_ = [], [], [], [], []
+    for b, box in boxes(size, brick):
+        o = (box[2].start, box[1].start, box[0].start)
+        n = (box[2].stop - o[0], box[1].stop - o[1], box[0].stop - o[2])
+        lo = [max(0, o[a] - m) for a in range(3)]
+        hi = [min(size[a], o[a] + n[a] + m) for a in range(3)]
+        win = np.ascontiguousarray(vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]])
+        srcs.append(torch.from_numpy(win.view(np.int16)).cuda())
+        dsts.append(torch.zeros((n[2], n[1], n[0]), dtype=torch.int16, device="cuda"))
+        so.append(tuple(lo))
+        do.append(o)
+        boxes_.append(box)
+    warper.resample_bricks(srcs, dsts, grid, so, do, size, precision=precision)
+    # No explicit synchronisation: the wrapper orders the library's stream
+    # after torch's (the zero fills) and torch's after the library's.
+    whole = torch.zeros((size[2], size[1], size[0]), dtype=torch.int16, device="cuda")
+    warper.resample_tps(torch.from_numpy(vol.view(np.int16)).cuda(), grid, dst=whole, precision=precision)
+    whole = whole.cpu().numpy().view(np.uint16)
+    for i in range(len(srcs)):
+        one = torch.zeros_like(dsts[i])
+        warper.resample_tps(srcs[i], grid, dst=one, precision=precision, src_origin=so[i], dst_origin=do[i],
+                            full=size)
+        got = dsts[i].cpu().numpy().view(np.uint16)
+        assert np.array_equal(got, one.cpu().numpy().view(np.uint16)), i
+        assert np.array_equal(got, whole[boxes_[i]]), i
+    grid.release()
