@@ -1,24 +1,24 @@
 """Benchmark of the TPS warp path on one MI355X.
 
-Workload (the plain-GPU configuration): a 1024 x 1024 x 256 uint16 volume
-(200 seeded Gaussian blobs, amplitude 100-4000, Poisson-like noise), 512
-landmark pairs with |d| <= 20 voxels, grid spacing 8, cubic-B-spline
-upsample + trilinear resample, output the same size.  The volume is
-resident in HBM before the timed region.
+Headline workload (the batched streaming configuration): an 8192 x 8192 x
+2048 uint16 volume (256 GiB; blob + noise field hashed from the seed and the
+voxel coordinate), 2048 landmark pairs with |d| <= 40 voxels, grid spacing 8,
+512^3 output bricks, cubic-B-spline upsample + trilinear resample.  Its input
+and output cannot both sit in one GPU's HBM, so one step is one layer of its
+bricks (16 x 16 bricks at z in [1024, 1536), 34.4 G voxels; four layers make
+the job): the TPS fit, the grid over the layer's nodes and every brick's
+fused upsample + resample in one persistent launch (tw_resample_tps_bricks),
+with the bricks' exact source windows resident in HBM before the timed
+region.  Two precision forms run side by side:
+  plain : FP64 solve, FP32 grid field, FP32 upsample, FP64 blend
+  mixed : FP64 solve, FP16 offset-scaled field, FP32 upsample and blend
+`value` is the mixed form's Gvox/s; `roofline` is the brick launch's
+algorithmic bytes over its device-event time; the oracle checks sampled
+bricks; `plain_gpu_config` adds the 1024 x 1024 x 256 uint16, K = 512 chain
+(tw_warp_tps, both forms) and its affine and rigid modes.
 
-One step = one whole warp through the library's chained entry point
-(tw_warp_tps): TPS fit, grid displacement evaluation and fused upsample +
-resample, all on the GPU with one synchronisation.  Two precision forms run
-side by side:
-  plain : FP64 null-space Cholesky solve, FP32 grid field, FP64 blend
-          (reference order)
-  mixed : the same FP64 solve at this size (FP32 Cholesky + FP64 refinement
-          above 4096 unknowns), FP16 offset-scaled field, FP32 blend
-`value` is the mixed form's end-to-end Gvox/s; both forms, the per-stage
-kernel times (HIP events on the library's stream) and the displacement
-error against the FP64 oracle are in the same JSON line.
-
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--no-traffic]
+    python bench.py --precision | --batch | --stream      (the other configurations)
 """
 import argparse
 import json
@@ -510,11 +510,234 @@ def stream_run(args):
     print(json.dumps(line))
 
 
-def measure_traffic(vol):
-    """HBM bytes per launch of the mixed resample kernel from two rocprofv3
-    counter passes of this script (--profile), FETCH_SIZE then WRITE_SIZE, each
+# ---- headline: one brick layer of the streaming configuration, in HBM ------------
+# The streaming configuration's volume (8192 x 8192 x 2048 uint16, 256 GiB,
+# K = 2048 landmarks, |d| <= 40, grid 8, 512^3 bricks) cannot have its input
+# and output resident at once on one GPU, so a step is one layer of its bricks:
+# 16 x 16 bricks of 512^3 at z in [1024, 1536), each brick's exact source
+# window (from the displacement's bounds over the brick's nodes) resident in
+# HBM before the timed region.  One step: the TPS fit, the grid over the
+# layer's nodes (FP16 offset-scaled for the mixed form), and every brick's
+# fused upsample + resample in one persistent launch (tw_resample_tps_bricks).
+# Four such steps make the whole 256 GiB job.
+LAYER_Z0 = 1024
+BRICK_KERNEL = "k_resample_batch<1, true, 0, true>"
+
+
+def layer_setup(torch, w, threads):
+    """Untimed setup: landmarks, the layer's exact brick windows (generated on
+    the host cores by the library's procedural source, copied to HBM) and the
+    output bricks."""
+    from terawarp_amd import synth, warp
+    from terawarp_amd.warp import Synth
+    size, zl = SIZE_S, LAYER_Z0
+    src = Synth(size, np.uint16, seed=SEED_S, cell=256, blobs_per_cell=16)
+    sub, tar = synth.landmarks(size, K_S, DMAX_S, seed=SEED_S)
+    nbx, nby = size[0] // BRICK_S, size[1] // BRICK_S
+    gsh = (warp.grid_dim(size[0], G_S), warp.grid_dim(size[1], G_S), warp.grid_dim(BRICK_S, G_S))
+    c, d = w.tps_fit(tar, sub, precision=warp.PLAIN)
+    g32 = w.new_grid(gsh, G_S, origin=(0, 0, zl), device=True)
+    w.grid_eval(tar, c, d, g32, precision=warp.PLAIN)
+    nb = BRICK_S // G_S
+    bricks = []
+    for by in range(nby):
+        for bx in range(nbx):
+            # Nodes bx*nb .. bx*nb + nb + 2 reach the brick's voxels.
+            blk = g32.data[:, :, by * nb:by * nb + nb + 3, bx * nb:bx * nb + nb + 3].reshape(3, -1)
+            dmin, dmax = blk.amin(dim=1).tolist(), blk.amax(dim=1).tolist()
+            o = (bx * BRICK_S, by * BRICK_S, zl)
+            lo = tuple(max(0, int(np.floor(o[a] + dmin[a])) - 2) for a in range(3))
+            hi = tuple(min(size[a], int(np.ceil(o[a] + BRICK_S - 1 + dmax[a])) + 3) for a in range(3))
+            bricks.append((o, lo, hi))
+    del g32
+    counts = [(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) for _, lo, hi in bricks]
+    log("layer: %d bricks, windows %.1f GB in HBM, generating on %d host threads" %
+        (len(bricks), 2 * sum(counts) / 1e9, threads))
+    pins = [torch.empty(max(counts), dtype=torch.int16, pin_memory=True) for _ in range(2)]
+    evs = [None, None]
+    srcs, so, do = [], [], []
+    for i, (o, lo, hi) in enumerate(bricks):
+        n = tuple(hi[a] - lo[a] for a in range(3))
+        k = i % 2
+        if evs[k] is not None:
+            evs[k].synchronize()
+        host = pins[k][:counts[i]]
+        src.fill_into(host.numpy().view(np.uint16).reshape(n[2], n[1], n[0]), lo[0], lo[1], lo[2], n[0], n[1], n[2],
+                      threads)
+        dev = torch.empty((n[2], n[1], n[0]), dtype=torch.int16, device="cuda")
+        dev.copy_(host.view(n[2], n[1], n[0]), non_blocking=True)
+        evs[k] = torch.cuda.Event()
+        evs[k].record()
+        srcs.append(dev)
+        so.append(lo)
+        do.append(o)
+    dsts = [torch.empty((BRICK_S, BRICK_S, BRICK_S), dtype=torch.int16, device="cuda") for _ in bricks]
+    torch.cuda.synchronize()
+    del pins
+    return dict(src=src, sub=sub, tar=tar, bricks=bricks, srcs=srcs, dsts=dsts, so=so, do=do, gsh=gsh,
+                prepared=w.prepare_bricks(srcs, dsts, so, do, size), nbx=nbx, nby=nby)
+
+
+def layer_form(w, wp, torch, L, precision, steps, warmup):
+    """Time `steps` layer steps on the production context `wp` (wall clock),
+    then the same number on the timing context `w` for the stage times."""
+    from terawarp_amd import warp
+    storage = warp.F16 if precision == warp.MIXED else warp.F32
+    grid = w.new_grid(L["gsh"], G_S, origin=(0, 0, LAYER_Z0), storage=storage, device=True)
+    rec = {k: [] for k in ("fit_ms", "grid_ms", "quant_ms", "resample_ms")}
+
+    def step(ctx, record):
+        c, d = ctx.tps_fit(L["tar"], L["sub"], precision=precision)
+        if record:
+            rec["fit_ms"].append(ctx.timings()["fit_ms"])
+        ctx.grid_eval(L["tar"], c, d, grid, precision=precision)
+        if record:
+            t = ctx.timings()
+            rec["grid_ms"].append(t["grid_ms"])
+            rec["quant_ms"].append(t["quant_ms"])
+        ctx.resample_bricks(None, None, grid, None, None, None, precision=precision, prepared=L["prepared"])
+        if record:
+            rec["resample_ms"].append(ctx.timings()["resample_ms"])
+
+    for _ in range(warmup):
+        step(wp, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(wp, False)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for _ in range(max(1, warmup)):
+        step(w, False)
+    for _ in range(steps):
+        step(w, True)
+    torch.cuda.synchronize()
+    vox = float(len(L["bricks"])) * BRICK_S ** 3
+    res = float(np.mean(rec["resample_ms"]))
+    out = {"gvox_s": vox * steps / wall / 1e9, "ms_per_step": wall / steps * 1e3, "resample_ms": res,
+           "resample_gvox_s": vox / (res * 1e-3) / 1e9}
+    for k in ("fit_ms", "grid_ms", "quant_ms"):
+        out[k] = float(np.mean(rec[k]))
+    grid.release()
+    return out
+
+
+def face_band(dense, size_xyz, origin, eps=1e-2):
+    """Voxels whose FP64 sample position lies within eps of a face of the
+    volume: a displacement error of eps can move such a sample across the
+    strict bounds test (0 or the full value)."""
+    band = np.zeros(dense.shape[1:], bool)
+    for a, shape in ((0, (1, 1, -1)), (1, (1, -1, 1)), (2, (-1, 1, 1))):
+        n = dense.shape[3 - a]
+        p = (np.arange(n, dtype=np.float32) + origin[a]).reshape(shape) + dense[a]
+        band |= (np.abs(p) <= eps) | (np.abs(p - (size_xyz[a] - 1)) <= eps)
+    return band
+
+
+def layer_oracle(L, threads, nz_sample=16):
+    """Accuracy of the mixed output on sampled bricks (interior, a layer
+    corner, the far corner) against the FP64 oracle's window form on their
+    first nz_sample slices, and the CPU baseline: the oracle's fit (prorated
+    by the sample's share of the 256 GiB job) plus the grid and resample of
+    the whole interior brick."""
+    from oracle import oracle
+    from terawarp_amd import warp
+    size, src, tar, sub = SIZE_S, L["src"], L["tar"], L["sub"]
+    t0 = time.perf_counter()
+    c, d = oracle.stps_fit(tar, sub, nthreads=threads, bordered=True)
+    t_fit = time.perf_counter() - t0
+
+    def oracle_box(i, nz):
+        o, lo, hi = L["bricks"][i]
+        box = (BRICK_S, BRICK_S, nz)
+        zhi = min(hi[2], o[2] + nz + int(DMAX_S) + 8)
+        win = src.fill_into(np.empty((zhi - lo[2], hi[1] - lo[1], hi[0] - lo[0]), np.uint16), lo[0], lo[1], lo[2],
+                            hi[0] - lo[0], hi[1] - lo[1], zhi - lo[2], threads)
+        df = oracle.grid_eval(tar, c, d, tuple(warp.grid_dim(v, G_S) for v in box), G_S, origin=o, nthreads=threads)
+        ref = oracle.tps_resample_window(win, lo, size, df, G_S, o, box, o, nthreads=threads)
+        return ref, df, box, o
+
+    acc = []
+    nbx, nby = L["nbx"], L["nby"]
+    for bx, by in ((nbx // 2, nby // 2), (0, 0), (nbx - 1, nby - 1)):
+        i = by * nbx + bx
+        ref, df, box, o = oracle_box(i, nz_sample)
+        got = L["dsts"][i][:nz_sample].cpu().numpy().view(np.uint16)
+        diff = np.abs(got.astype(np.int64) - ref.astype(np.int64))
+        band = face_band(oracle.dense_df(box, df, G_S, nthreads=threads), size, o)
+        off = diff[~band]
+        acc.append({"brick_xyz": [bx, by, LAYER_Z0 // BRICK_S], "slices": nz_sample,
+                    "lsb_exact_frac": float(np.mean(diff == 0)), "lsb_le1_frac": float(np.mean(diff <= 1)),
+                    "lsb_max_off_face_band": int(off.max()) if off.size else 0, "face_band_voxels": int(band.sum()),
+                    "face_band_flips": int(np.count_nonzero(diff[band] > 1))})
+    i = (nby // 2) * nbx + nbx // 2
+    t1 = time.perf_counter()
+    oracle_box(i, BRICK_S)
+    t_s = time.perf_counter() - t1
+    vox_s, vox_job = float(BRICK_S) ** 3, float(size[0]) * size[1] * size[2]
+    cpu = {"value": vox_s / (t_fit * vox_s / vox_job + t_s) / 1e9, "unit": "Gvox/s", "cores": threads,
+           "kind": "port",
+           "sample": f"FP64 oracle: full K={K_S} fit ({t_fit:.1f} s, prorated by the sample's share of the "
+                     f"256 GiB job) + grid and resample of one interior 512^3 brick ({t_s:.1f} s, "
+                     f"including its window's generation), {threads} OpenMP threads"}
+    return acc, cpu
+
+
+def config2_legs(args, torch, w, wp):
+    """The plain-GPU configuration (1024x1024x256 uint16, K=512, |d|<=20,
+    grid 8, in-memory chain) in both forms, and its affine and rigid modes on
+    the same volume (volume resident in HBM)."""
+    from terawarp_amd import synth, warp
+    vol = synth.blob_volume(SIZE[::-1], np.uint16, n_blobs=200, amp=(100, 4000), sigma=(2, 8), seed=SEED)
+    sub, tar = synth.landmarks(SIZE, K, DMAX, seed=SEED)
+    src = torch.from_numpy(vol.view(np.int16)).cuda()
+    dst = torch.empty_like(src)
+    out = {"workload": "plain-GPU configuration: 1024x1024x256 uint16, K=512 |d|<=20, grid 8, in-memory chain "
+                       "(tw_warp_tps); affine (random 3x4, det in [0.8, 1.2]) and rigid (rotation 10/-7/15 deg "
+                       "about the centre, translation 3/-2/5) on the same volume"}
+    steps, warmup = max(5, args.steps // 2), max(1, args.warmup)
+    out["plain"] = run_form(w, wp, torch, src, dst, tar, sub, warp.PLAIN, steps, warmup)
+    out["mixed"] = run_form(w, wp, torch, src, dst, tar, sub, warp.MIXED, steps, warmup)
+    out["mixed_over_plain_end_to_end"] = out["mixed"]["gvox_s"] / out["plain"]["gvox_s"]
+    vox = float(SIZE[0] * SIZE[1] * SIZE[2])
+    nodes = 1
+    for s_ in SIZE:
+        nodes *= warp.grid_dim(s_, G)
+    out["mixed"]["resample_frac_of_8tbs"] = b_alg_per_voxel(2, 2, nodes, vox) * vox / (
+        out["mixed"]["resample_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    Ma = synth.random_affine(SIZE, seed=SEED)
+    R = warp.mode_matrix(warp.ROTATION, (10.0, -7.0, 15.0), SIZE)
+    t = np.array([3.0, -2.0, 5.0])
+    Mr = R.copy()
+    Mr[:3, 3] = R[:3, 3] - R[:3, :3] @ t  # subject(R^T(p - t - c) + c): rotation after the translation
+    for name, M in (("affine", Ma), ("rigid", Mr)):
+        for _ in range(warmup):
+            wp.resample_affine(src, M, dst=dst)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            wp.resample_affine(src, M, dst=dst)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = []
+        for _ in range(steps):
+            w.resample_affine(src, M, dst=dst)
+            ms.append(w.timings()["resample_ms"])
+        r = float(np.mean(ms))
+        out[name] = {"gvox_s": vox * steps / wall / 1e9, "ms_per_step": wall / steps * 1e3, "resample_ms": r,
+                     "roofline": {"bound": "hbm", "achieved": 4 * vox / (r * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": 4 * vox / (r * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                  "b_alg_per_voxel": 4.0}}
+    del src, dst
+    torch.cuda.empty_cache()
+    return out
+
+
+def measure_traffic(profile_args, kernel, files=None):
+    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes of
+    this script run with `profile_args` (FETCH_SIZE, then WRITE_SIZE), each
     scaled by what the same counter reads for the 1 GiB device copy that the
-    profile run performs after the warps."""
+    profile run performs after its warps."""
     import csv
     import shutil
     import subprocess
@@ -523,32 +746,43 @@ def measure_traffic(vol):
     if not rocprof:
         return None
     tmp = tempfile.mkdtemp(prefix="tw_pmc_")
-    vpath = os.path.join(tmp, "vol.npy")
-    np.save(vpath, vol)
+    args = list(profile_args)
+    for name, arr in (files or {}).items():
+        path = os.path.join(tmp, name + ".npy")
+        np.save(path, arr)
+        args += ["--" + name, path]
     out = {}
     try:
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             d = os.path.join(tmp, ctr)
             cmd = [rocprof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--",
-                   sys.executable, os.path.abspath(__file__), "--profile", "--steps", "1", "--warmup", "0",
-                   "--volume", vpath]
+                   sys.executable, os.path.abspath(__file__)] + args
             log("counter pass", ctr)
-            subprocess.run(cmd, check=True, timeout=240, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+            subprocess.run(cmd, check=True, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
                            cwd=tmp, env=dict(os.environ, TMPDIR=tmp))
             path = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")][0]
             rows = list(csv.DictReader(open(path)))
-            fused = [r for r in rows if MIXED_KERNEL in r["Kernel_Name"]]
-            last = max(int(r["Dispatch_Id"]) for r in fused)
+            hits = [r for r in rows if kernel in r["Kernel_Name"]]
+            last = max(int(r["Dispatch_Id"]) for r in hits)
             copy = [r for r in rows if "copyBuffer" in r["Kernel_Name"] and int(r["Dispatch_Id"]) > last]
-            kib = float(np.mean([float(r["Counter_Value"]) for r in fused]))
+            kib = float(np.mean([float(r["Counter_Value"]) for r in hits]))
             copy_kib = sum(float(r["Counter_Value"]) for r in copy)
             scale = (1 << 30) / (copy_kib * 1024.0)
             out[ctr] = {"kernel_kib": kib, "copy_kib": copy_kib, "scale": scale, "bytes": kib * 1024.0 * scale}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     total = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
-    return {"bytes_per_launch": total, "counters": out, "kernel": MIXED_KERNEL,
+    return {"bytes_per_launch": total, "counters": out, "kernel": kernel,
             "calibration": "1 GiB device copy in the same pass"}
+
+
+def calibration_copy(torch):
+    """A device copy of known size (1 GiB read + 1 GiB written) for the
+    counter passes' calibration."""
+    a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -570,6 +804,9 @@ def main():
                     help="the streaming configuration (8192x8192x2048 uint16, generated per brick on the host)")
     ap.add_argument("--stream-size", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
                     help="volume size of --stream (default 8192 8192 2048)")
+    ap.add_argument("--profile-layer", action="store_true",
+                    help="profiler run of the headline: layer setup, one mixed step, a 1 GiB device copy")
+    ap.add_argument("--no-config2", action="store_true", help="skip the plain-GPU configuration legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -587,92 +824,75 @@ def main():
         stream_run(args)
         return
     args.steps = args.steps or 10
-
     import torch
-    from terawarp_amd import synth, warp
+    from terawarp_amd import warp
     from terawarp_amd.warp import Warper
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
 
-    if args.volume:
-        vol = np.load(args.volume)
-    else:
-        log("generating workload", SIZE, "uint16, K =", K)
-        vol = synth.blob_volume(SIZE[::-1], np.uint16, n_blobs=200, amp=(100, 4000), sigma=(2, 8), seed=SEED)
-    sub, tar = synth.landmarks(SIZE, K, DMAX, seed=SEED)
-    src = torch.from_numpy(vol.view(np.int16)).cuda()
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-
-    w = Warper(0, timing=True)
-    wp = Warper(0)
-    if args.profile:
+    if args.profile:  # the plain-GPU configuration's chain under a profiler
+        from terawarp_amd import synth
+        vol = np.load(args.volume) if args.volume else synth.blob_volume(
+            SIZE[::-1], np.uint16, n_blobs=200, amp=(100, 4000), sigma=(2, 8), seed=SEED)
+        sub, tar = synth.landmarks(SIZE, K, DMAX, seed=SEED)
+        src = torch.from_numpy(vol.view(np.int16)).cuda()
+        dst = torch.empty_like(src)
+        w, wp = Warper(0, timing=True), Warper(0)
         run_form(w, wp, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
         run_form(w, wp, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
-        # Calibration copy of a known size: 1 GiB read + 1 GiB written.
-        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
+        calibration_copy(torch)
         print(json.dumps({"profile": "done", "copy_bytes": 1 << 30}))
-        w.close()
-        wp.close()
+        return
+
+    w, wp = Warper(0, timing=True), Warper(0)
+    log("headline: one brick layer of", SIZE_S, "uint16, K =", K_S)
+    L = layer_setup(torch, wp, threads)
+    if args.profile_layer:
+        for _ in range(max(1, args.steps)):
+            c, d = wp.tps_fit(L["tar"], L["sub"], precision=warp.MIXED)
+            grid = wp.new_grid(L["gsh"], G_S, origin=(0, 0, LAYER_Z0), storage=warp.F16, device=True)
+            wp.grid_eval(L["tar"], c, d, grid, precision=warp.MIXED)
+            wp.resample_bricks(None, None, grid, None, None, None, precision=warp.MIXED, prepared=L["prepared"])
+            torch.cuda.synchronize()
+            grid.release()
+        calibration_copy(torch)
+        print(json.dumps({"profile": "done", "copy_bytes": 1 << 30}))
         return
     log("plain form")
-    plain = run_form(w, wp, torch, src, dst, tar, sub, warp.PLAIN, args.steps, args.warmup)
-    out_plain = dst.cpu().numpy().view(np.uint16)
+    plain = layer_form(w, wp, torch, L, warp.PLAIN, args.steps, args.warmup)
     log("mixed form")
-    mixed = run_form(w, wp, torch, src, dst, tar, sub, warp.MIXED, args.steps, args.warmup)
-    out_mixed = dst.cpu().numpy().view(np.uint16)
-    gp = accuracy_grid(w, tar, sub, warp.PLAIN)
-    gm = accuracy_grid(w, tar, sub, warp.MIXED)
-
-    vox = SIZE[0] * SIZE[1] * SIZE[2]
-    nodes = 1
-    for s in SIZE:
-        nodes *= warp.grid_dim(s, G)
+    mixed = layer_form(w, wp, torch, L, warp.MIXED, args.steps, args.warmup)
+    vox = float(len(L["bricks"])) * BRICK_S ** 3
+    nodes = float(np.prod(L["gsh"]))
     b_mixed = b_alg_per_voxel(2, 2, nodes, vox)
     b_plain = b_alg_per_voxel(2, 4, nodes, vox)
     achieved = b_mixed * vox / (mixed["resample_ms"] * 1e-3) / 1e9
-    plain["resample_gbs"] = b_plain * vox / (plain["resample_ms"] * 1e-3) / 1e9
     mixed["resample_gbs"] = achieved
-
-    # Accuracy of both forms against the FP64 oracle on a sampled slab.
-    cpu = None
-    acc = {}
+    plain["resample_gbs"] = b_plain * vox / (plain["resample_ms"] * 1e-3) / 1e9
+    acc, cpu = [], None
     try:
-        from oracle import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        zb0, zb1 = 10, 12
-        c_ref, d_ref = oracle.stps_fit(tar, sub, nthreads=threads)
-        gsh = oracle.grid_shape(SIZE, G)
-        df_sl = oracle.grid_eval(tar, c_ref, d_ref, (gsh[0], gsh[1], zb1 - zb0 + 3), G, origin=(0, 0, zb0 * G),
-                                 nthreads=threads)
-        box = (SIZE[0], SIZE[1], (zb1 - zb0) * G)
-        dense_ref = oracle.dense_df(box, df_sl, G, nthreads=threads)
-        for name, grid in (("plain", gp), ("mixed", gm)):
-            prec = warp.PLAIN if name == "plain" else warp.MIXED
-            dense = w.upsample_df(grid, box, origin=(0, 0, zb0 * G), precision=prec)
-            acc[name + "_df_err_vox"] = float(np.abs(dense - dense_ref).max())
-        ref_out = oracle_slab_resample(oracle, vol, df_sl, zb0, zb1, threads)
-        sl = slice(zb0 * G, zb1 * G)
-        for name, o in (("plain", out_plain), ("mixed", out_mixed)):
-            diff = np.abs(o[sl].astype(np.int64) - ref_out[sl].astype(np.int64))
-            acc[name + "_lsb_exact_frac"] = float(np.mean(diff == 0))
-            acc[name + "_lsb_le1_frac"] = float(np.mean(diff <= 1))
-            acc[name + "_lsb_max"] = int(diff.max())
-        if not args.no_cpu:
-            cpu, cpu_out = cpu_baseline(vol, tar, sub, threads)
-            cpu["one_thread"] = cpu_faithful(vol, tar, sub, cpu_out)
-    except Exception as e:  # the accuracy leg must not hide the GPU numbers
+        log("oracle checks and CPU baseline")
+        acc, cpu = layer_oracle(L, threads)
+    except Exception as e:  # the oracle leg must not hide the GPU numbers
         log("oracle leg failed:", repr(e))
-        acc["error"] = repr(e)
-
+        acc = [{"error": repr(e)}]
+    if args.no_cpu:
+        cpu = None
+    window_gb = 2.0 * sum((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) for _, lo, hi in L["bricks"]) / 1e9
+    del L
+    torch.cuda.empty_cache()
+    c2 = None
+    if not args.no_config2:
+        log("plain-GPU configuration legs")
+        c2 = config2_legs(args, torch, w, wp)
+    w.close()
+    wp.close()
     traffic = None
     if not args.no_traffic:
         try:
-            traffic = measure_traffic(vol)
+            traffic = measure_traffic(["--profile-layer", "--steps", "1", "--warmup", "0", "--no-config2"],
+                                      BRICK_KERNEL)
         except Exception as e:  # counters are optional evidence, never fatal
             log("traffic passes failed:", repr(e))
-
     line = {
         "metric": "TPS warp Gvox/s + resample % HBM peak; mixed vs plain at <=1e-2 vox err",
         "value": mixed["gvox_s"],
@@ -684,25 +904,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16",
-        "data": "synthetic (seeded blob volume + smooth-field landmarks)",
-        "config": {"workload": "plain-GPU TPS warp 1024x1024x256 uint16, K=512 landmarks |d|<=20, grid 8, "
-                               "B-spline upsample + trilinear",
-                   "size_xyz": list(SIZE), "landmarks": K, "grid_spacing": G, "forms": "mixed (value) and plain"},
+        "dtype": "f32",
+        "data": "synthetic (procedural blob+noise volume hashed from seed and coordinate; smooth-field landmarks)",
+        "config": {"workload": "batched streaming configuration, one brick layer per step: 8192x8192x2048 uint16 "
+                               "(256 GiB) volume, K=2048 landmarks |d|<=40, grid 8, 512^3 bricks; a step is the "
+                               "TPS fit, the grid of the layer's nodes and the fused B-spline upsample + trilinear "
+                               "resample of 16x16 bricks (z in [1024,1536)) in one persistent launch, their exact "
+                               "source windows (%.0f GB) resident in HBM; four steps make the job" % window_gb,
+                   "size_xyz": list(SIZE_S), "landmarks": K_S, "grid_spacing": G_S, "brick": BRICK_S,
+                   "layer_voxels": vox, "arithmetic": "FP64 fit; FP32 grid distances with FP64 accumulation, "
+                   "FP16 offset-scaled field; FP32 upsample and blend on uint16 voxels",
+                   "forms": "mixed (value) and plain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "k_tps_fused<uint16, MIXED, F16, LINEAR>", "b_alg_per_voxel": b_mixed},
+                     "kernel": BRICK_KERNEL + " (k_resample_batch<MIXED, F16, LINEAR, windows>)",
+                     "b_alg_per_voxel": b_mixed, "voxels_per_launch": vox},
         "cpu_baseline": cpu,
         "plain": plain,
         "mixed": mixed,
         "mixed_over_plain_end_to_end": mixed["gvox_s"] / plain["gvox_s"],
         "accuracy": acc,
+        "plain_gpu_config": c2,
         "traffic_detail": traffic,
     }
     print(json.dumps(line))
-    w.close()
-    wp.close()
 
 
 if __name__ == "__main__":

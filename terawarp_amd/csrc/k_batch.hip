@@ -56,7 +56,7 @@ __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restric
                                                        const int* __restrict__ parts_g, int njobs, int nwg_x,
                                                        unsigned long long* __restrict__ t_end) {
     extern __shared__ float lds[];
-    __shared__ float rmax_w[FTY];
+    __shared__ float rmax_w[FTY * 6];
     const TW_CONST BatchJob* jobs = (const TW_CONST BatchJob*)jobs_g;
     const TW_CONST int* P = (const TW_CONST int*)parts_g + (blockIdx.x & 7) * (2 * njobs + 1);
     const TW_CONST int* S = P + njobs + 1;

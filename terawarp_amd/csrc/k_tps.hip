@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_tps_exact(const TpsArgs a) {
 template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF, bool WIN>
 __global__ __launch_bounds__(FT) void k_tps_fused(const TpsArgs a, const FastGeom g) {
     extern __shared__ float lds[];
-    __shared__ float rmax_w[FTY];
+    __shared__ float rmax_w[FTY * 6];
     // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD
     // a contiguous run of tiles (x fastest, then y, then z) so that tiles
     // sharing source rows share that XCD's L2.

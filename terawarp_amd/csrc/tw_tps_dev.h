@@ -360,7 +360,7 @@ __device__ __forceinline__ float wave_max(float v) {
 
 // One 64 x 4 x tz tile (index `tile` of the launch's x-fastest tile grid) of
 // the fused kernel; src / dst: the channel's volumes.  lds: the node and
-// x-collapse arrays (fused_lds_bytes), rmax_w: FTY floats.  Every thread of
+// x-collapse arrays (fused_lds_bytes), rmax_w: 6 * FTY floats.  Every thread of
 // the work-group calls it (two work-group barriers inside).
 template <typename T, int MODE, bool F16, int INTERP, bool WRITE_DF, bool WIN>
 __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, int tile, const void* src, void* dst,
@@ -388,7 +388,9 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
     const uint32_t plane = (uint32_t)(a.gx * a.gy * a.gz);
     const int NP = NZM * NYM * NXM;
     const int ntile = nxn * nyn * nzn;
-    float rm = 0.f;  // largest |displacement| among this thread's nodes
+    // Per component c: ext[c] = the largest -d_c, ext[3 + c] = the largest d_c
+    // among this thread's nodes.
+    float ext[6] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (int i = tid; i < ntile; i += FT) {
         const int q = div_f(i, ivx, 0.5f * ivx), iz = div_f(q, ivy, 0.5f * ivy);
         const int ix = i - q * nxn, iy = q - iz * nyn;
@@ -413,10 +415,18 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
         nodes[li] = v0;
         nodes[NP + li] = v1;
         nodes[2 * NP + li] = v2;
-        rm = fmaxf(rm, fmaxf(fabsf(v0), fmaxf(fabsf(v1), fabsf(v2))));
+        ext[0] = fmaxf(ext[0], -v0);
+        ext[1] = fmaxf(ext[1], -v1);
+        ext[2] = fmaxf(ext[2], -v2);
+        ext[3] = fmaxf(ext[3], v0);
+        ext[4] = fmaxf(ext[4], v1);
+        ext[5] = fmaxf(ext[5], v2);
     }
-    rm = wave_max(rm);
-    if (lxi == 0) rmax_w[lyi] = rm;
+    #pragma unroll
+    for (int k = 0; k < 6; k++) {
+        ext[k] = wave_max(ext[k]);
+        if (lxi == 0) rmax_w[lyi * 6 + k] = ext[k];
+    }
     // Node index and phase of a column: base / f for the tile (scalar) plus
     // (base % f + lane) / f < 2 + 64 / f for the lane.
     auto split = [&](int base, int lane, int n0, int& b, int& ph) {
@@ -509,21 +519,32 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
         }
         return;
     }
-    // Samples of this tile lie within rmax (+ one voxel for rounding) of
-    // their voxel, rmax bounding the field (a convex combination of the
-    // staged nodes).  Cells whose samples all stay a voxel inside the
-    // volume and the source window skip the per-voxel bounds test.
-    // (Integer margin R = ceil(rmax) + 1, clamped past any coordinate, so
-    // the tests below are scalar.)
-    float rmx = rmax_w[0];
+    // Along axis a the samples of this tile lie in [X + dmin_a, X + dmax_a]
+    // (+ one voxel for rounding), the tile's nodes bounding the field (a
+    // convex combination of them).  Cells whose samples all stay a voxel
+    // inside the volume and the source window skip the per-voxel bounds
+    // test.  (Integer margins Rlo = ceil(-dmin) + 1 below and Rhi =
+    // ceil(dmax) + 1 above the voxel, clamped past any coordinate, so the
+    // tests below are scalar.)
+    int Rlo[3], Rhi[3];
     #pragma unroll
-    for (int w = 1; w < FTY; w++) rmx = fmaxf(rmx, rmax_w[w]);
-    const int R = __builtin_amdgcn_readfirstlane((int)ceilf(fminf(rmx, 16777216.f))) + 1;
+    for (int c = 0; c < 3; c++) {
+        float lo = rmax_w[c], hi = rmax_w[3 + c];
+        #pragma unroll
+        for (int w = 1; w < FTY; w++) {
+            lo = fmaxf(lo, rmax_w[w * 6 + c]);
+            hi = fmaxf(hi, rmax_w[w * 6 + 3 + c]);
+        }
+        Rlo[c] = __builtin_amdgcn_readfirstlane((int)ceilf(fminf(fmaxf(lo, -16777216.f), 16777216.f))) + 1;
+        Rhi[c] = __builtin_amdgcn_readfirstlane((int)ceilf(fminf(fmaxf(hi, -16777216.f), 16777216.f))) + 1;
+    }
     auto lo_bound = [](int s0) { return max(0, s0); };
     auto hi_bound = [](int im, int s0, int sn) { return min(im, s0 + sn - 1) - 1; };
     const int loz = lo_bound(g.sz0), hiz = hi_bound(g.imz, g.sz0, g.snz);
-    const bool fast_xy = (int)a.dx0 + tx0 - R >= lo_bound(g.sx0) && (int)a.dx0 + tx1 + R <= hi_bound(g.imx, g.sx0, g.snx) &&
-                         (int)a.dy0 + ty0 - R >= lo_bound(g.sy0) && (int)a.dy0 + ty1 + R <= hi_bound(g.imy, g.sy0, g.sny);
+    const bool fast_xy = (int)a.dx0 + tx0 - Rlo[0] >= lo_bound(g.sx0) &&
+                         (int)a.dx0 + tx1 + Rhi[0] <= hi_bound(g.imx, g.sx0, g.snx) &&
+                         (int)a.dy0 + ty0 - Rlo[1] >= lo_bound(g.sy0) &&
+                         (int)a.dy0 + ty1 + Rhi[1] <= hi_bound(g.imy, g.sy0, g.sny);
     // One channel per launch (the host offsets src / dst per channel).
     const SrcRs rs = make_src_rs(src, g);
     const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
@@ -568,7 +589,7 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
     while (true) {
         const int cnt = min(f - tzp, nzv - lz);
         const int zc = (int)a.dz0 + tz0 + lz;  // first slice of the cell
-        if (fast_xy && zc - R >= loz && zc + cnt - 1 + R <= hiz)
+        if (fast_xy && zc - Rlo[2] >= loz && zc + cnt - 1 + Rhi[2] <= hiz)
             cell(std::false_type{}, cnt);
         else
             cell(std::true_type{}, cnt);

@@ -288,17 +288,26 @@ class Warper:
                                              interp, precision))
         return dst
 
-    def resample_bricks(self, srcs, dsts, grid, src_origins, dst_origins, full, interp=LINEAR, precision=MIXED):
-        """tw_resample_tps_bricks: device source windows srcs[i] (global origin
-        src_origins[i], whole-volume extent full) resampled into the device
-        output boxes dsts[i] (global origin dst_origins[i]) against one device
-        grid, in one launch.  Asynchronous on the context's stream."""
+    @staticmethod
+    def prepare_bricks(srcs, dsts, src_origins, dst_origins, full):
+        """The descriptor arrays of resample_bricks, built once for repeated
+        calls on the same buffers."""
         n = len(srcs)
         if len(dsts) != n or len(src_origins) != n or len(dst_origins) != n:
             raise ValueError("srcs, dsts and origins differ in length")
         vs = (L.TwVolume * n)(*[_vol(s, o, full) for s, o in zip(srcs, src_origins)])
         vd = (L.TwVolume * n)(*[_vol(d, o) for d, o in zip(dsts, dst_origins)])
-        self._check(self._ordered(self.lib.tw_resample_tps_bricks, self.ctx, vs, vd, n, ctypes.byref(grid.g), interp, precision))
+        return vs, vd, n
+
+    def resample_bricks(self, srcs, dsts, grid, src_origins, dst_origins, full, interp=LINEAR, precision=MIXED,
+                        prepared=None):
+        """tw_resample_tps_bricks: device source windows srcs[i] (global origin
+        src_origins[i], whole-volume extent full) resampled into the device
+        output boxes dsts[i] (global origin dst_origins[i]) against one device
+        grid, in one launch.  Asynchronous on the context's stream."""
+        vs, vd, n = prepared or self.prepare_bricks(srcs, dsts, src_origins, dst_origins, full)
+        self._check(self._ordered(self.lib.tw_resample_tps_bricks, self.ctx, vs, vd, n, ctypes.byref(grid.g),
+                                  interp, precision))
         return dsts
 
     def upsample_df(self, grid, size_xyz, origin=(0, 0, 0), precision=PLAIN):
@@ -445,6 +454,30 @@ class Synth:
         rc = L.load().tw_synth_fill(ctypes.byref(self.s), x0, y0, z0, nx, ny, nz, out.ctypes.data)
         if rc != L.TW_OK:
             raise TeraWarpError(rc, "tw_synth_fill: box outside the volume")
+        return out
+
+    def fill_into(self, out, x0, y0, z0, nx, ny, nz, threads=16):
+        """Fill the box into `out` (a C-contiguous (nz, ny, nx) array of this
+        dtype, e.g. a pinned buffer's view) with `threads` host threads over
+        z-ranges (the library call releases the interpreter lock)."""
+        from concurrent.futures import ThreadPoolExecutor
+        if out.shape != (nz, ny, nx) or out.dtype != self.dtype or not out.flags["C_CONTIGUOUS"]:
+            raise ValueError("out must be a C-contiguous (nz, ny, nx) array of the volume's dtype")
+        lib = L.load()
+        plane = nx * ny * self.dtype.itemsize
+        nt = max(1, min(threads, nz))
+
+        def part(t):
+            za, zb = nz * t // nt, nz * (t + 1) // nt
+            if zb > za:
+                return lib.tw_synth_fill(ctypes.byref(self.s), x0, y0, z0 + za, nx, ny, zb - za,
+                                         out.ctypes.data + za * plane)
+            return L.TW_OK
+
+        with ThreadPoolExecutor(nt) as ex:
+            rcs = list(ex.map(part, range(nt)))
+        if any(rc != L.TW_OK for rc in rcs):
+            raise TeraWarpError(min(rcs), "tw_synth_fill: box outside the volume")
         return out
 
 
