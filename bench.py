@@ -270,56 +270,13 @@ N_B, SEED_B, G_B, DMAX_B = 256, 5, 8, 8.0
 
 
 def batch_specs(seed=SEED_B, n=N_B):
-    """The uneven-instance batch: axes U[64, 512], uint8 / uint16 50/50, and
-    60 % TPS (K U[16, 256], |d| <= DMAX_B), 25 % affine (random 3x4, det in
-    [0.8, 1.2]), 15 % rigid (rotation U[-180, 180] deg per axis, integer
-    translation U[-32, 32])."""
-    from terawarp_amd import synth, warp
-    rng = np.random.default_rng(seed)
-    specs = []
-    for i in range(n):
-        size = tuple(int(v) for v in rng.integers(64, 513, 3))
-        dtype = np.uint8 if rng.random() < 0.5 else np.uint16
-        u = rng.random()
-        sp = {"size": size, "dtype": dtype, "seed": seed * 100000 + i}
-        if u < 0.6:
-            K = int(rng.integers(16, 257))
-            sub, tar = synth.landmarks(size, K, DMAX_B, seed=sp["seed"])
-            sp.update(kind="tps", centres=tar, targets=sub)
-        elif u < 0.85:
-            sp.update(kind="affine", M=synth.random_affine(size, seed=sp["seed"]))
-        else:
-            ang = rng.uniform(-180.0, 180.0, 3)
-            tr = rng.integers(-32, 33, 3).astype(np.float64)
-            sp.update(kind="rigid", M=warp.mode_matrix(warp.ROTATION, ang, size) @
-                      warp.mode_matrix(warp.TRANSLATION, tr, size))
-        specs.append(sp)
-    return specs
+    from terawarp_amd import synth
+    return synth.batch_instances(seed=seed, n=n, dmax=DMAX_B)
 
 
 def gpu_blob_volume(torch, size, dtype, seed, n_blobs=24):
-    """Seeded blob volume generated on the device (separable Gaussian blobs,
-    sigma 2-8 % of the smallest axis, plus uniform noise)."""
-    g = np.random.default_rng(seed)
-    nx, ny, nz = size
-    hi = 255.0 if dtype == np.uint8 else 4095.0
-    amp = (20.0, 200.0) if dtype == np.uint8 else (100.0, 4000.0)
-    ax = [torch.arange(n, device="cuda", dtype=torch.float32) for n in (nz, ny, nx)]
-    vol = torch.zeros((nz, ny, nx), device="cuda", dtype=torch.float32)
-    smin = min(size)
-    for _ in range(n_blobs):
-        c = g.uniform(0, 1, 3) * np.array([nz, ny, nx])
-        sg = g.uniform(0.02, 0.08) * smin
-        a = g.uniform(*amp)
-        e = [torch.exp(-((ax[k] - float(c[k])) ** 2) / (2 * sg * sg)) for k in range(3)]
-        vol += a * e[0][:, None, None] * e[1][None, :, None] * e[2][None, None, :]
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(int(seed))
-    vol += (torch.rand(vol.shape, generator=gen, device="cuda") - 0.5) * 10.0
-    vol.clamp_(0, hi)
-    if dtype == np.uint8:
-        return vol.to(torch.uint8)
-    return vol.to(torch.int32).to(torch.int16)
+    from terawarp_amd import synth
+    return synth.blob_volume_device(size, dtype, seed, n_blobs=n_blobs)
 
 
 def batch_run(args):
