@@ -478,7 +478,10 @@ def stream_run(args):
 # fused upsample + resample in one persistent launch (tw_resample_tps_bricks).
 # Four such steps make the whole 256 GiB job.
 LAYER_Z0 = 1024
-BRICK_KERNEL = "k_resample_batch<1, true, 0, true>"
+# Brick kernel per tile: k_resample_batch<MIXED, F16, LINEAR, windows, LDS tiles>.
+BRICK_KERNELS = {"auto": "k_resample_batch<1, true, 0, true, false>",
+                 "gather": "k_resample_batch<1, true, 0, true, false>",
+                 "lds": "k_resample_batch<1, true, 0, true, true>"}
 
 
 def layer_setup(torch, w, threads):
@@ -764,6 +767,8 @@ def main():
     ap.add_argument("--profile-layer", action="store_true",
                     help="profiler run of the headline: layer setup, one mixed step, a 1 GiB device copy")
     ap.add_argument("--no-config2", action="store_true", help="skip the plain-GPU configuration legs")
+    ap.add_argument("--tile", choices=["auto", "gather", "lds"], default="auto",
+                    help="fused resample tile: auto (gather), per-tap gathers, or LDS-streamed columns")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -801,7 +806,10 @@ def main():
         return
 
     w, wp = Warper(0, timing=True), Warper(0)
-    log("headline: one brick layer of", SIZE_S, "uint16, K =", K_S)
+    w.set_tile(args.tile)
+    wp.set_tile(args.tile)
+    brick_kernel = BRICK_KERNELS[args.tile]
+    log("headline: one brick layer of", SIZE_S, "uint16, K =", K_S, "tile", args.tile)
     L = layer_setup(torch, wp, threads)
     if args.profile_layer:
         for _ in range(max(1, args.steps)):
@@ -846,8 +854,8 @@ def main():
     traffic = None
     if not args.no_traffic:
         try:
-            traffic = measure_traffic(["--profile-layer", "--steps", "1", "--warmup", "0", "--no-config2"],
-                                      BRICK_KERNEL)
+            traffic = measure_traffic(["--profile-layer", "--steps", "1", "--warmup", "0", "--no-config2",
+                                       "--tile", args.tile], brick_kernel)
         except Exception as e:  # counters are optional evidence, never fatal
             log("traffic passes failed:", repr(e))
     line = {
@@ -875,7 +883,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": BRICK_KERNEL + " (k_resample_batch<MIXED, F16, LINEAR, windows>)",
+                     "kernel": brick_kernel + " (k_resample_batch<MIXED, F16, LINEAR, windows, LDS tiles>)",
                      "b_alg_per_voxel": b_mixed, "voxels_per_launch": vox},
         "cpu_baseline": cpu,
         "plain": plain,

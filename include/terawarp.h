@@ -141,6 +141,16 @@ void*       tw_get_stream(tw_ctx* ctx);
 int         tw_sync(tw_ctx* ctx);
 /* 1: record per-stage HIP events on every call (tw_get_timings). */
 int         tw_set_timing(tw_ctx* ctx, int enable);
+/* Output tile of the fused TPS resample (TW_PLAIN / TW_MIXED, linear
+ * interpolation): TW_TILE_GATHER reads every tap from global memory (64x4
+ * tiles); TW_TILE_LDS streams each 16x16 column's source box through an LDS
+ * ring of planes; TW_TILE_AUTO (default) is TW_TILE_GATHER, the faster on
+ * MI355X as measured.  Outputs are identical; only speed differs.  No
+ * reference counterpart (the reference has one CPU loop). */
+#define TW_TILE_AUTO    0
+#define TW_TILE_GATHER  1
+#define TW_TILE_LDS     2
+int         tw_set_tile(tw_ctx* ctx, int tile);
 int         tw_get_timings(tw_ctx* ctx, tw_timings* out);
 const char* tw_last_error(tw_ctx* ctx);
 const char* tw_version(void);

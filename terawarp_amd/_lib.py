@@ -35,10 +35,13 @@ TW_DF_F32 = 0
 TW_DF_F16 = 1
 TW_WARP_TPS = 0
 TW_WARP_AFFINE = 1
+TW_TILE_AUTO = 0
+TW_TILE_GATHER = 1
+TW_TILE_LDS = 2
 
 # Every entry point include/terawarp.h declares.
 EXPORTS = (
-    "tw_create", "tw_destroy", "tw_set_stream", "tw_get_stream", "tw_sync", "tw_set_timing", "tw_get_timings", "tw_last_error",
+    "tw_create", "tw_destroy", "tw_set_stream", "tw_get_stream", "tw_sync", "tw_set_timing", "tw_set_tile", "tw_get_timings", "tw_last_error",
     "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
     "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_warp_tps_noalloc", "tw_warp_batch",
     "tw_resample_tps_bricks", "tw_warp_stream",
@@ -151,6 +154,7 @@ def load():
         "tw_get_stream": (_P, [_P]),
         "tw_sync": (ctypes.c_int, [_P]),
         "tw_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+        "tw_set_tile": (ctypes.c_int, [_P, ctypes.c_int]),
         "tw_get_timings": (ctypes.c_int, [_P, T]),
         "tw_last_error": (ctypes.c_char_p, [_P]),
         "tw_version": (ctypes.c_char_p, []),

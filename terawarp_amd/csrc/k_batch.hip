@@ -30,6 +30,7 @@
 
 #include "tw_affine_dev.h"
 #include "tw_tps_dev.h"
+#include "tw_tps_lds.h"
 
 namespace {
 
@@ -51,11 +52,14 @@ struct BatchJob {
 // parts: for XCD x at parts + x * (2 * njobs + 1): P[0..njobs] (prefix sums of
 // the part sizes) then S[0..njobs-1] (first tile of each part).  t_end[j]:
 // device clock when the last tile of job j ended.
-template <int MODE, bool F16, int INTERP, bool WIN = false>
+// LDS: TPS tiles are tps_column_lds's 16 x 16 x SEG columns (linear
+// interpolation; the jobs' geometry from fused_geom_lds), else tps_tile's.
+template <int MODE, bool F16, int INTERP, bool WIN = false, bool LDS = false>
 __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restrict__ jobs_g,
                                                        const int* __restrict__ parts_g, int njobs, int nwg_x,
                                                        unsigned long long* __restrict__ t_end) {
-    extern __shared__ float lds[];
+    static_assert(FT == 256 && (!LDS || INTERP == TW_INTERP_LINEAR), "LDS tiles: 64 x 4 threads, linear");
+    extern __shared__ __align__(16) float lds[];
     __shared__ float rmax_w[FTY * 6];
     const TW_CONST BatchJob* jobs = (const TW_CONST BatchJob*)jobs_g;
     const TW_CONST int* P = (const TW_CONST int*)parts_g + (blockIdx.x & 7) * (2 * njobs + 1);
@@ -76,10 +80,17 @@ __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restric
             const FastGeom& g = *(const FastGeom*)&J.g;
             const void* src = (const uint8_t*)a.src + ch * J.src_cstride;
             void* dst = (uint8_t*)a.dst + ch * J.dst_cstride;
-            if (J.dtype == TW_U8)
-                tps_tile<uint8_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
-            else
-                tps_tile<uint16_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
+            if constexpr (LDS) {
+                if (J.dtype == TW_U8)
+                    tps_column_lds<uint8_t, MODE, F16, WIN>(a, g, t, src, dst, lds, rmax_w);
+                else
+                    tps_column_lds<uint16_t, MODE, F16, WIN>(a, g, t, src, dst, lds, rmax_w);
+            } else {
+                if (J.dtype == TW_U8)
+                    tps_tile<uint8_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
+                else
+                    tps_tile<uint16_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
+            }
         } else {
             const AffineArgs& af = *(const AffineArgs*)&J.af;
             const AffGeom& ag = *(const AffGeom*)&J.ag;
@@ -99,6 +110,13 @@ __global__ void k_batch_init(unsigned long long* __restrict__ t0, unsigned long 
 }
 
 inline size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Geometry of a TPS job for the tile the launch runs (the column kernel sets
+// a.tz to its column depth).
+const char* tps_job_geom(TpsArgs& a, int dtype, bool lds_tile, FastGeom& g, size_t& lds) {
+    if (lds_tile) return dtype == TW_U8 ? fused_geom_lds<uint8_t>(a, g, lds) : fused_geom_lds<uint16_t>(a, g, lds);
+    return dtype == TW_U8 ? fused_geom<uint8_t, false>(a, g, lds) : fused_geom<uint16_t, false>(a, g, lds);
+}
 
 int check_item(tw_ctx* ctx, const tw_warp_item& it, int i, int f) {
     int rc;
@@ -227,6 +245,7 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
     std::vector<i64> tiles((size_t)n);
     size_t lds = 0;
     const int tz = pick_tz(f);
+    const bool lds_tile = tw_lds_tile(ctx, interp);
     for (int i = 0; i < n; i++) {
         const tw_warp_item& it = items[i];
         BatchJob& J = hj[i];
@@ -259,8 +278,7 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
             a.qbx = (g.gx + 7) / 8;
             a.qby = (g.gy + 7) / 8;
             size_t l = 0;
-            const char* why = it.src.dtype == TW_U8 ? fused_geom<uint8_t, false>(a, J.g, l)
-                                                    : fused_geom<uint16_t, false>(a, J.g, l);
+            const char* why = tps_job_geom(a, it.src.dtype, lds_tile, J.g, l);
             if (why) return tw_fail(ctx, TW_E_UNSUPPORTED, "item %d: %s", i, why);
             lds = std::max(lds, l);
             per_ch = J.g.ntiles;
@@ -303,6 +321,9 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
     if (interp == TW_INTERP_NN)
         kern = f16 ? (const void*)k_resample_batch<TW_PLAIN, true, TW_INTERP_NN>
                    : (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_NN>;
+    else if (lds_tile)
+        kern = f16 ? (const void*)k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR, false, true>
+                   : (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR, false, true>;
     else
         kern = f16 ? (const void*)k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR>
                    : (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR>;
@@ -315,21 +336,17 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
     const int* dparts = (const int*)(rdev + jobs_b);
     tw_stage_begin(ctx, ST_RESAMPLE);
     const dim3 grid((unsigned)nwg), block(TX, FTY, 1);
+#define TW_BATCH(M, H, I, L) \
+    hipLaunchKernelGGL((k_resample_batch<M, H, I, false, L>), grid, block, lds, ctx->stream, djobs, dparts, n, nwg_x, \
+                       t_end)
     if (interp == TW_INTERP_NN) {
-        if (f16)
-            hipLaunchKernelGGL((k_resample_batch<TW_PLAIN, true, TW_INTERP_NN>), grid, block, lds, ctx->stream, djobs,
-                               dparts, n, nwg_x, t_end);
-        else
-            hipLaunchKernelGGL((k_resample_batch<TW_PLAIN, false, TW_INTERP_NN>), grid, block, lds, ctx->stream, djobs,
-                               dparts, n, nwg_x, t_end);
+        if (f16) TW_BATCH(TW_PLAIN, true, TW_INTERP_NN, false); else TW_BATCH(TW_PLAIN, false, TW_INTERP_NN, false);
+    } else if (lds_tile) {
+        if (f16) TW_BATCH(TW_MIXED, true, TW_INTERP_LINEAR, true); else TW_BATCH(TW_PLAIN, false, TW_INTERP_LINEAR, true);
     } else {
-        if (f16)
-            hipLaunchKernelGGL((k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR>), grid, block, lds, ctx->stream,
-                               djobs, dparts, n, nwg_x, t_end);
-        else
-            hipLaunchKernelGGL((k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR>), grid, block, lds, ctx->stream,
-                               djobs, dparts, n, nwg_x, t_end);
+        if (f16) TW_BATCH(TW_MIXED, true, TW_INTERP_LINEAR, false); else TW_BATCH(TW_PLAIN, false, TW_INTERP_LINEAR, false);
     }
+#undef TW_BATCH
     TW_LAUNCH_CHECK(ctx);
     tw_stage_end(ctx, ST_RESAMPLE);
     tw_stage_end(ctx, ST_TOTAL);
@@ -416,6 +433,7 @@ extern "C" int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const 
     std::vector<i64> tiles((size_t)n);
     size_t lds = 0;
     const int tz = pick_tz(grid->f);
+    const bool lds_tile = tw_lds_tile(ctx, interp);
     for (int i = 0; i < n; i++) {
         const tw_volume &s = srcs[i], &d = dsts[i];
         BatchJob& J = hj[i];
@@ -449,7 +467,7 @@ extern "C" int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const 
         a.qby = (grid->gy + 7) / 8;
         if (s.nx < 2) return tw_fail(ctx, TW_E_UNSUPPORTED, "brick %d: a source window needs two columns", i);
         size_t l = 0;
-        const char* why = s.dtype == TW_U8 ? fused_geom<uint8_t, false>(a, J.g, l) : fused_geom<uint16_t, false>(a, J.g, l);
+        const char* why = tps_job_geom(a, s.dtype, lds_tile, J.g, l);
         if (why) return tw_fail(ctx, TW_E_UNSUPPORTED, "brick %d: %s", i, why);
         lds = std::max(lds, l);
         if ((i64)J.g.ntiles * s.nc >= (1ll << 30)) return tw_fail(ctx, TW_E_UNSUPPORTED, "brick %d: too many tiles", i);
@@ -479,10 +497,13 @@ extern "C" int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const 
         kern = f16 ? (const void*)k_resample_batch<TW_PLAIN, true, TW_INTERP_NN, true>
                    : (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_NN, true>;
     else if (precision == TW_MIXED)
-        kern = f16 ? (const void*)k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR, true>
-                   : (const void*)k_resample_batch<TW_MIXED, false, TW_INTERP_LINEAR, true>;
+        kern = lds_tile ? (f16 ? (const void*)k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR, true, true>
+                               : (const void*)k_resample_batch<TW_MIXED, false, TW_INTERP_LINEAR, true, true>)
+                        : (f16 ? (const void*)k_resample_batch<TW_MIXED, true, TW_INTERP_LINEAR, true>
+                               : (const void*)k_resample_batch<TW_MIXED, false, TW_INTERP_LINEAR, true>);
     else
-        kern = (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR, true>;
+        kern = lds_tile ? (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR, true, true>
+                        : (const void*)k_resample_batch<TW_PLAIN, false, TW_INTERP_LINEAR, true>;
     int per_cu = 0;
     TW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, FT, lds));
     int nwg = std::max(1, per_cu) * ctx->num_cus;
@@ -492,14 +513,18 @@ extern "C" int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const 
     const int* dparts = (const int*)(rdev + jobs_b);
     tw_stage_begin(ctx, ST_RESAMPLE);
     const dim3 lg((unsigned)nwg), lb(TX, FTY, 1);
-#define TW_BRICKS(M, H, I) \
-    hipLaunchKernelGGL((k_resample_batch<M, H, I, true>), lg, lb, lds, ctx->stream, djobs, dparts, n, nwg_x, t_end)
+#define TW_BRICKS(M, H, I, L) \
+    hipLaunchKernelGGL((k_resample_batch<M, H, I, true, L>), lg, lb, lds, ctx->stream, djobs, dparts, n, nwg_x, t_end)
     if (interp == TW_INTERP_NN) {
-        if (f16) TW_BRICKS(TW_PLAIN, true, TW_INTERP_NN); else TW_BRICKS(TW_PLAIN, false, TW_INTERP_NN);
+        if (f16) TW_BRICKS(TW_PLAIN, true, TW_INTERP_NN, false); else TW_BRICKS(TW_PLAIN, false, TW_INTERP_NN, false);
     } else if (precision == TW_MIXED) {
-        if (f16) TW_BRICKS(TW_MIXED, true, TW_INTERP_LINEAR); else TW_BRICKS(TW_MIXED, false, TW_INTERP_LINEAR);
+        if (lds_tile) {
+            if (f16) TW_BRICKS(TW_MIXED, true, TW_INTERP_LINEAR, true); else TW_BRICKS(TW_MIXED, false, TW_INTERP_LINEAR, true);
+        } else {
+            if (f16) TW_BRICKS(TW_MIXED, true, TW_INTERP_LINEAR, false); else TW_BRICKS(TW_MIXED, false, TW_INTERP_LINEAR, false);
+        }
     } else {
-        TW_BRICKS(TW_PLAIN, false, TW_INTERP_LINEAR);
+        if (lds_tile) TW_BRICKS(TW_PLAIN, false, TW_INTERP_LINEAR, true); else TW_BRICKS(TW_PLAIN, false, TW_INTERP_LINEAR, false);
     }
 #undef TW_BRICKS
     TW_LAUNCH_CHECK(ctx);

@@ -43,6 +43,7 @@
 #include <type_traits>
 
 #include "tw_tps_dev.h"
+#include "tw_tps_lds.h"
 
 namespace {
 
@@ -98,6 +99,50 @@ __global__ __launch_bounds__(FT) void k_tps_fused(const TpsArgs a, const FastGeo
     tps_tile<T, MODE, F16, INTERP, WRITE_DF, WIN>(a, g, tile, a.src, a.dst, lds, rmax_w);
 }
 
+// The fused kernel with the source streamed through LDS (linear PLAIN /
+// MIXED): one 16 x 16 x SEG column per work-group, XCD-major as above.
+template <typename T, int MODE, bool F16, bool WIN>
+__global__ __launch_bounds__(256) void k_tps_lds(const TpsArgs a, const FastGeom g) {
+    extern __shared__ __align__(16) float lds16[];
+    __shared__ float ext_w[4 * 6];
+    const int wg = blockIdx.x;
+    const int tile = (wg & 7) * g.per_xcd + (wg >> 3);
+    if (tile >= g.ntiles) return;
+    tps_column_lds<T, MODE, F16, WIN>(a, g, tile, a.src, a.dst, lds16, ext_w);
+}
+
+template <typename T>
+int launch_tps_lds(tw_ctx* ctx, TpsArgs a, int precision, bool f16) {
+    FastGeom g;
+    size_t lds = 0;
+    if (const char* why = fused_geom_lds<T>(a, g, lds)) return tw_fail(ctx, TW_E_UNSUPPORTED, "%s", why);
+    const bool win = !(a.sx0 == 0 && a.sy0 == 0 && a.sz0 == 0 && a.snx == a.fnx && a.sny == a.fny && a.snz == a.fnz);
+    const dim3 grid((unsigned)(8 * g.per_xcd), 1, 1), block(TX, 4, 1);
+    const void* src0 = a.src;
+    void* dst0 = a.dst;
+    for (int ch = 0; ch < (int)a.nc; ch++) {
+        a.src = (const void*)((const uint8_t*)src0 + (size_t)ch * a.src_cstride);
+        a.dst = (void*)((uint8_t*)dst0 + (size_t)ch * a.dst_cstride);
+#define TW_LDS(M, H, W) hipLaunchKernelGGL((k_tps_lds<T, M, H, W>), grid, block, lds, ctx->stream, a, g)
+        if (precision == TW_PLAIN) {
+            if (f16) {
+                if (win) TW_LDS(TW_PLAIN, true, true); else TW_LDS(TW_PLAIN, true, false);
+            } else {
+                if (win) TW_LDS(TW_PLAIN, false, true); else TW_LDS(TW_PLAIN, false, false);
+            }
+        } else {
+            if (f16) {
+                if (win) TW_LDS(TW_MIXED, true, true); else TW_LDS(TW_MIXED, true, false);
+            } else {
+                if (win) TW_LDS(TW_MIXED, false, true); else TW_LDS(TW_MIXED, false, false);
+            }
+        }
+#undef TW_LDS
+        TW_LAUNCH_CHECK(ctx);
+    }
+    return TW_OK;
+}
+
 template <typename T, bool WRITE_DF>
 int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     const dim3 block(TX, TY, 1);
@@ -118,6 +163,8 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
         return TW_OK;
     }
     if (a.f > 64) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d above 64 is not supported", a.f);
+    if (!WRITE_DF && tw_lds_tile(ctx, interp))
+        return launch_tps_lds<T>(ctx, a, precision, f16);
     a.tz = pick_tz(a.f);
     FastGeom g;
     size_t lds = 0;

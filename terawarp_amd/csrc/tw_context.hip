@@ -329,6 +329,14 @@ int tw_set_timing(tw_ctx* ctx, int enable) {
     return TW_OK;
 }
 
+int tw_set_tile(tw_ctx* ctx, int tile) {
+    if (!ctx) return TW_E_ARG;
+    if (tile != TW_TILE_AUTO && tile != TW_TILE_GATHER && tile != TW_TILE_LDS)
+        return tw_fail(ctx, TW_E_ARG, "tile must be TW_TILE_AUTO, TW_TILE_GATHER or TW_TILE_LDS");
+    ctx->tile = tile;
+    return TW_OK;
+}
+
 int tw_get_timings(tw_ctx* ctx, tw_timings* out) {
     if (!ctx || !out) return TW_E_ARG;
     memset(out, 0, sizeof *out);

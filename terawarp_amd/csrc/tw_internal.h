@@ -46,6 +46,7 @@ struct tw_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     bool timing = false;
+    int tile = TW_TILE_AUTO;  // fused TPS resample tile (tw_set_tile)
     std::string err;
     void* ws[WS_COUNT] = {};
     size_t ws_cap[WS_COUNT] = {};
@@ -111,6 +112,13 @@ int tw_pinned_issued(tw_ctx* ctx);
 int tw_read_small(tw_ctx* ctx, void* out, const void* dev, size_t bytes);
 
 // Set the context error string and return code.
+// Does a fused TPS resample run the LDS-streamed column tile?  TW_TILE_AUTO
+// takes the per-tap gather tile, the faster of the two as measured on MI355X
+// for the mixed form (the column tile ties it on the plain form).
+inline bool tw_lds_tile(const tw_ctx* ctx, int interp) {
+    return interp == TW_INTERP_LINEAR && ctx->tile == TW_TILE_LDS;
+}
+
 int tw_fail(tw_ctx* ctx, int code, const char* fmt, ...);
 
 #define TW_HIP(ctx, call)                                                              \

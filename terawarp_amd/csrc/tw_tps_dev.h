@@ -262,13 +262,11 @@ __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, 
 
 // Trilinear blend of one voxel from its four (x, x+1) pairs: p00 / p10 the
 // lower plane's rows y / y+1, p01 / p11 the upper plane's.
-template <typename T, int MODE>
-__device__ __forceinline__ uint32_t blend_pairs(uint32_t p00, uint32_t p10, uint32_t p01, uint32_t p11,
-                                                const Tap& t) {
-    const uint32_t u000 = pair_lo<T>(p00), u100 = pair_hi<T>(p00);
-    const uint32_t u010 = pair_lo<T>(p10), u110 = pair_hi<T>(p10);
-    const uint32_t u001 = pair_lo<T>(p01), u101 = pair_hi<T>(p01);
-    const uint32_t u011 = pair_lo<T>(p11), u111 = pair_hi<T>(p11);
+// Trilinear blend of the eight taps u[xyz] (x, y, z = 0 / 1).
+template <int MODE>
+__device__ __forceinline__ uint32_t blend8(uint32_t u000, uint32_t u100, uint32_t u010, uint32_t u110,
+                                           uint32_t u001, uint32_t u101, uint32_t u011, uint32_t u111,
+                                           const Tap& t) {
     if (MODE == TW_PLAIN) {
         // The reference's FP64 blend (q_imgwarp_tps_quicksmallmemory.cpp:141-160);
         // 1 - (pos - floor(pos)) is exact, so the weights equal the reference's.
@@ -288,6 +286,13 @@ __device__ __forceinline__ uint32_t blend_pairs(uint32_t p00, uint32_t p10, uint
     const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
     const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
     return cvt_rpi(__builtin_fmaf(t.wz, b1 - b0, b0));
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ uint32_t blend_pairs(uint32_t p00, uint32_t p10, uint32_t p01, uint32_t p11,
+                                                const Tap& t) {
+    return blend8<MODE>(pair_lo<T>(p00), pair_hi<T>(p00), pair_lo<T>(p10), pair_hi<T>(p10), pair_lo<T>(p01),
+                        pair_hi<T>(p01), pair_lo<T>(p11), pair_hi<T>(p11), t);
 }
 
 // The eight taps of a voxel as four (x, x+1) pairs: y+1 / z+1 through the

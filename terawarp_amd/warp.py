@@ -217,6 +217,12 @@ class Warper:
     def set_timing(self, on=True):
         self._check(self.lib.tw_set_timing(self.ctx, 1 if on else 0))
 
+    def set_tile(self, tile):
+        """Fused TPS resample tile: "auto" (the gather tile), "gather" or
+        "lds" (source streamed through LDS); outputs are identical."""
+        code = {"auto": L.TW_TILE_AUTO, "gather": L.TW_TILE_GATHER, "lds": L.TW_TILE_LDS}[tile]
+        self._check(self.lib.tw_set_tile(self.ctx, code))
+
     def sync(self):
         self._check(self.lib.tw_sync(self.ctx))
 
