@@ -506,9 +506,13 @@ def layer_setup(torch, w, threads):
             blk = g32.data[:, :, by * nb:by * nb + nb + 3, bx * nb:bx * nb + nb + 3].reshape(3, -1)
             dmin, dmax = blk.amin(dim=1).tolist(), blk.amax(dim=1).tolist()
             o = (bx * BRICK_S, by * BRICK_S, zl)
-            lo = tuple(max(0, int(np.floor(o[a] + dmin[a])) - 2) for a in range(3))
-            hi = tuple(min(size[a], int(np.ceil(o[a] + BRICK_S - 1 + dmax[a])) + 3) for a in range(3))
-            bricks.append((o, lo, hi))
+            lo = [max(0, int(np.floor(o[a] + dmin[a])) - 2) for a in range(3)]
+            hi = [min(size[a], int(np.ceil(o[a] + BRICK_S - 1 + dmax[a])) + 3) for a in range(3)]
+            # Rows of whole 16-byte pieces (x rounded out to 8 voxels), so that
+            # the resampler's row loads are aligned.
+            lo[0] -= lo[0] % 8
+            hi[0] = min(size[0], hi[0] + (-(hi[0] - lo[0])) % 8)
+            bricks.append((o, tuple(lo), tuple(hi)))
     del g32
     counts = [(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) for _, lo, hi in bricks]
     log("layer: %d bricks, windows %.1f GB in HBM, generating on %d host threads" %

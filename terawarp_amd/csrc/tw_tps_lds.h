@@ -38,7 +38,7 @@ struct BoxDims<uint16_t> {
 };
 template <>
 struct BoxDims<uint8_t> {
-    static constexpr int X = 32, Y = 32, CHK = 2, RPI = 32;
+    static constexpr int X = 48, Y = 32, CHK = 3, RPI = 21;
 };
 
 // LDS bytes of the ring: RING planes plus a copy of slot 0 after the last,
@@ -60,11 +60,14 @@ constexpr uint32_t ring_max_off() {
 constexpr float LDS_DMAX = 65536.f;
 constexpr int LDS_CMAX = 1 << 22;
 
-// The voxel at byte offset `off` of the ring.  (Element-aligned reads: a
-// misaligned wider read of a tap pair stalls the LDS many times over.)
+// The voxel at byte offset `off` of the ring.  Element-aligned reads: a
+// misaligned wider read of a tap pair stalls the LDS many times over, so the
+// read is volatile, which keeps the compiler from merging neighbouring taps
+// into one (it takes unaligned LDS access as supported).
 template <typename T>
 __device__ __forceinline__ uint32_t lds_at(const uint8_t* box, uint32_t off) {
-    return *(const T*)(box + off);
+    typedef const volatile __attribute__((address_space(3))) T lds_t;
+    return *(lds_t*)(box + off);
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -89,13 +92,23 @@ __device__ __noinline__ u32x4 load16_tail(__amdgpu_buffer_rsrc_t r, uint32_t at)
 __host__ __device__ inline int lds_xc_floats(int nym, int nzm) { return (3 * nzm * nym * LTX + 3) & ~3; }
 
 // LDS bytes per work-group: the x-collapse array, the z weights (4 x 64
-// floats: read from LDS, their loads stay in order with the tap reads), then
-// the ring, which the column's nodes occupy before the ring is first filled.
+// floats: read from LDS, their loads stay in order with the tap reads), the
+// output staging, then the ring and its trash area, which the column's nodes
+// occupy before the ring is first filled.
 constexpr int LDS_WT_FLOATS = 4 * 64;
+// Staging writes of lanes without a chunk land in LDS_TRASH bytes after the
+// ring (one 16-byte chunk per lane).
+constexpr uint32_t LDS_TRASH = 64 * 16;
+// Output staging: a chunk's LCH slices of the column's 16 x 16 voxels, [z][y][x].
+template <typename T>
+constexpr uint32_t obuf_bytes() {
+    return (uint32_t)(LCH * LTY * LTX * sizeof(T));
+}
 template <typename T>
 inline size_t lds_column_bytes(int nxm, int nym, int nzm) {
     const size_t nodes = (size_t)3 * nzm * nym * nxm * sizeof(float);
-    return (size_t)(lds_xc_floats(nym, nzm) + LDS_WT_FLOATS) * sizeof(float) + std::max<size_t>(nodes, ring_bytes<T>());
+    return (size_t)(lds_xc_floats(nym, nzm) + LDS_WT_FLOATS) * sizeof(float) + obuf_bytes<T>() +
+           std::max<size_t>(nodes, ring_bytes<T>() + LDS_TRASH);
 }
 
 // tap_setup<T, TW_INTERP_LINEAR, true, CHECK> with the tap's byte offset in
@@ -178,7 +191,8 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
     const int NXM = g.nxm, NYM = g.nym, NZM = g.nzm;
     float* xc = lds;                                               // [3][NZM][NYM][LTX]
     float* wts = lds + lds_xc_floats(NYM, NZM);                    // [f][4]
-    uint8_t* box = (uint8_t*)(wts + LDS_WT_FLOATS);                // ring [RING + 1][Y][X] of T
+    T* obuf = (T*)(wts + LDS_WT_FLOATS);                           // [LCH][LTY][LTX] outputs of a chunk
+    uint8_t* box = (uint8_t*)obuf + obuf_bytes<T>();               // ring [RING + 1][Y][X] of T
     float* nodes = (float*)box;                                    // [3][NZM][NYM][NXM] until the ring fills
 
     const int tid = threadIdx.y * TX + threadIdx.x, lane = threadIdx.x;
@@ -274,7 +288,11 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
     auto reach_hi = [&](int c, int u1) {
         return min(__builtin_amdgcn_readfirstlane((int)floorf((float)u1 + dhi[c])) + 2, whi[c]);
     };
-    const int bx0 = reach_lo(0, tl[0]), by0 = reach_lo(1, tl[1]);
+    // The box's x origin sits a multiple of 16 bytes from the window's, so
+    // that the staging loads are 16-byte aligned where the window's rows are
+    // (a misaligned 16-byte load costs the address unit several times over).
+    const int bx0 = g.sx0 + (int)((uint32_t)((reach_lo(0, tl[0]) - g.sx0) * BPV) & ~15u) / BPV;
+    const int by0 = reach_lo(1, tl[1]);
     const int bnx = reach_hi(0, th[0]) - bx0 + 1, bny = reach_hi(1, th[1]) - by0 + 1;
     // A chunk spans LCH slices plus the z range and four planes of margin.
     const int zspan = __builtin_amdgcn_readfirstlane((int)ceilf(dlo[2] + dhi[2])) + LCH + 5;
@@ -364,8 +382,11 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
                          (int)a.dy0 + ty1 + Rhi[1] <= hi_bound(g.imy, g.sy0, g.sny);
     const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
 
-    // Four slices per group keep the LDS reads of four voxels in flight (a
-    // short last group repeats its last slice and stores only the real ones).
+    int lz = 0;  // slices of the column walked so far
+    // LDS taps, four slices per group: the taps of all four first, then their
+    // 32 reads, so that one LDS round trip serves the group (a short last
+    // group repeats its last slice).  Outputs go to the chunk's staging at
+    // slot = slice within the chunk.
     auto group_lds = [&](auto check, auto full, int nh) {
         constexpr bool CHECK = decltype(check)::value, FULL = decltype(full)::value;
         Tap tp[4];
@@ -378,7 +399,6 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
             const float zj = FULL ? Zf + (float)j : Zf + (float)jj;
             tp[j] = tap_ring<T, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2], bxy, ok[j]);
         }
-        // All 32 reads first, so that they are in flight together.
         uint32_t u[4][8];
         #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -392,21 +412,37 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
             u[j][6] = lds_at<T>(box, q + PB + RB);
             u[j][7] = lds_at<T>(box, q + PB + RB + BPV);
         }
-        // Keep the scheduler from sinking each voxel's reads to their use
-        // (one LDS round trip per voxel instead of one per group).
+        // Keep the scheduler from sinking each voxel's reads to their use.
         __builtin_amdgcn_sched_barrier(0);
-        uint32_t v[4];
+        const int slot = lz % LCH;
         #pragma unroll
         for (int j = 0; j < 4; j++) {
-            v[j] = blend8<MODE>(u[j][0], u[j][1], u[j][2], u[j][3], u[j][4], u[j][5], u[j][6], u[j][7], tp[j]);
-            if (CHECK) v[j] = ok[j] ? v[j] : 0u;
+            uint32_t v = blend8<MODE>(u[j][0], u[j][1], u[j][2], u[j][3], u[j][4], u[j][5], u[j][6], u[j][7], tp[j]);
+            if (CHECK) v = ok[j] ? v : 0u;
+            if (FULL || j < nh) obuf[((slot + j) * LTY + lyi) * LTX + lxi] = (T)v;
         }
-        #pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (live && (FULL || j < nh)) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)BPV, v[j]);
         tzp += nh;
-        o += (uint32_t)nh * ostep;
+        lz += nh;
         Zf += (float)nh;
+    };
+    // Slices [lz, lz_end) by LDS taps, cell by cell (a cell: the slices
+    // sharing one node-layer window; the fast form where every sample of the
+    // cell stays a voxel inside the volume and the window).
+    auto walk_lds = [&](int lz_end) {
+        while (lz < lz_end) {
+            const int cnt = min(f - tzp, lz_end - lz);
+            const int zc = (int)a.dz0 + tz0 + lz;
+            const bool fast = fast_xy && zc - Rlo[2] >= loz && zc + cnt - 1 + Rhi[2] <= hiz;
+            int k = 0;
+            if (fast) {
+                for (; k + 4 <= cnt; k += 4) group_lds(std::false_type{}, std::true_type{}, 4);
+                if (k < cnt) group_lds(std::false_type{}, std::false_type{}, cnt - k);
+            } else {
+                for (; k + 4 <= cnt; k += 4) group_lds(std::true_type{}, std::true_type{}, 4);
+                if (k < cnt) group_lds(std::true_type{}, std::false_type{}, cnt - k);
+            }
+            if (tzp == f && lz < nzv) next_cell();
+        }
     };
     const SrcRs rs = make_src_rs(src, g);
     auto group_gather = [&](auto check, auto full, int nh) {
@@ -428,38 +464,28 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
         o += (uint32_t)nh * ostep;
         Zf += (float)nh;
     };
-    // Slices [lz, lz_end) of the column, cell by cell (a cell: the slices
-    // sharing one node-layer window; the fast form where every sample of the
-    // cell stays a voxel inside the volume and the window).
-    int lz = 0;
-    auto walk = [&](auto lds_taps, int lz_end) {
-        constexpr bool LDS = decltype(lds_taps)::value;
-        while (lz < lz_end) {
-            const int cnt = min(f - tzp, lz_end - lz);
+    // Gather taps: the column cell by cell (a cell: the slices sharing one
+    // node-layer window; the fast form where every sample of the cell stays a
+    // voxel inside the volume and the window).
+    auto walk = [&]() {
+        while (lz < nzv) {
+            const int cnt = min(f - tzp, nzv - lz);
             const int zc = (int)a.dz0 + tz0 + lz;
             const bool fast = fast_xy && zc - Rlo[2] >= loz && zc + cnt - 1 + Rhi[2] <= hiz;
             int k = 0;
             if (fast) {
-                for (; k + 4 <= cnt; k += 4)
-                    LDS ? group_lds(std::false_type{}, std::true_type{}, 4)
-                        : group_gather(std::false_type{}, std::true_type{}, 4);
-                if (k < cnt)
-                    LDS ? group_lds(std::false_type{}, std::false_type{}, cnt - k)
-                        : group_gather(std::false_type{}, std::false_type{}, cnt - k);
+                for (; k + 4 <= cnt; k += 4) group_gather(std::false_type{}, std::true_type{}, 4);
+                if (k < cnt) group_gather(std::false_type{}, std::false_type{}, cnt - k);
             } else {
-                for (; k + 4 <= cnt; k += 4)
-                    LDS ? group_lds(std::true_type{}, std::true_type{}, 4)
-                        : group_gather(std::true_type{}, std::true_type{}, 4);
-                if (k < cnt)
-                    LDS ? group_lds(std::true_type{}, std::false_type{}, cnt - k)
-                        : group_gather(std::true_type{}, std::false_type{}, cnt - k);
+                for (; k + 4 <= cnt; k += 4) group_gather(std::true_type{}, std::true_type{}, 4);
+                if (k < cnt) group_gather(std::true_type{}, std::false_type{}, cnt - k);
             }
             lz += cnt;
             if (tzp == f && lz < nzv) next_cell();
         }
     };
     if (!use_lds) {
-        walk(std::false_type{}, nzv);
+        walk();
         return;
     }
 
@@ -472,72 +498,138 @@ __device__ __forceinline__ void tps_column_lds(const TpsArgs& a, const FastGeom&
     const int rsub = lane / B::CHK, chunk = lane - rsub * B::CHK;
     const uint32_t voff = (uint32_t)rsub * g.row_b + (uint32_t)chunk * 16u;
     const uint32_t wxy = (uint32_t)(by0 - g.sy0) * g.row_b + (uint32_t)(bx0 - g.sx0) * BPV;
-    auto st_load = [&](int i, int pa, int np, bool& on) -> u32x4 {
-        const int pz = i / H, hh = i - pz * H;
-        on = pz < np && hh * B::RPI + rsub < bny && rsub < B::RPI;
-        const uint32_t soff = wxy + (uint32_t)(pa + pz - g.sz0) * g.plane_b + (uint32_t)(hh * B::RPI) * g.row_b;
-        u32x4 v = {0u, 0u, 0u, 0u};
-        if (on) {
-            if (voff + soff + 16u <= g.chan_b) v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)voff, (int)soff, 0);
-            else v = load16_tail(rsrc, voff + soff);
-        }
-        return v;
-    };
+    constexpr uint32_t OOB_V = 0x7FFFFFF0u;  // a vector offset past every range: loads read 0, stores drop
+    // Every lane writes (lanes without a piece write to the trash area): a
+    // write under a lane mask would leave its load counted as pending on the
+    // path that skips it, and the compiler would then wait for every vector
+    // memory operation there.
+    uint8_t* trash = box + ring_bytes<T>();
     auto st_store = [&](int i, int pa, bool on, const u32x4& v) {
         const int pz = i / H, hh = i - pz * H;
         const int slot = (pa + pz) & (RING - 1);
-        if (on) {
-            uint8_t* p = box + hh * B::RPI * RB + lane * 16;
-            *(u32x4*)(p + slot * PB) = v;
-            if (slot == 0) *(u32x4*)(p + RING * PB) = v;
+        uint8_t* p = box + hh * B::RPI * RB + lane * 16;
+        *(u32x4*)(on ? p + slot * PB : trash + lane * 16) = v;
+        if (slot == 0) *(u32x4*)(on ? p + RING * PB : trash + lane * 16) = v;
+    };
+    // Staging load i of planes [pa, pa + np).  EXACT: rows of whole aligned
+    // 16-byte pieces (no piece crosses the channel's end), one load per lane
+    // whatever `on` (idle lanes read past the range); otherwise a piece that
+    // crosses the channel's end is read byte by byte.
+    auto st_load = [&](auto exact, int i, int pa, int np, bool& on) -> u32x4 {
+        const int pz = i / H, hh = i - pz * H;
+        on = pz < np && hh * B::RPI + rsub < bny && rsub < B::RPI;
+        // (Wave-uniform; said so, or the compiler wraps the load in a loop
+        // over the lanes' values.)
+        const uint32_t soff = __builtin_amdgcn_readfirstlane(wxy + (uint32_t)(pa + pz - g.sz0) * g.plane_b +
+                                                             (uint32_t)(hh * B::RPI) * g.row_b);
+        if constexpr (decltype(exact)::value) {
+            return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(on ? voff : OOB_V), (int)soff, 0);
+        } else {
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (on) {
+                if (voff + soff + 16u <= g.chan_b) v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)voff, (int)soff, 0);
+                else v = load16_tail(rsrc, voff + soff);
+            }
+            return v;
         }
     };
-    // Synchronous staging of [pa, pa + np).
-    auto st_sync = [&](int pa, int np) {
-        for (int i0 = 0; i0 < np * H; i0 += NW * KPF) {
+    // Planes chunk c (slices [c * LCH, c * LCH + LCH) of the column) reaches:
+    // the first chunk's reach shifted by c * LCH (exact integers), so that a
+    // chunk adds at most LCH planes to its predecessor's.  (The last chunk
+    // may reach past its slices: at most the extra planes of a full chunk.)
+    const int zl0 = __builtin_amdgcn_readfirstlane((int)floorf((float)tl[2] - dlo[2])) - 1;
+    const int zh0 = __builtin_amdgcn_readfirstlane((int)floorf((float)(tl[2] + LCH - 1) + dhi[2])) + 2;
+    auto chunk_planes = [&](int c, int& pl, int& ph) {
+        pl = max(zl0 + c * LCH, wlo[2]);
+        ph = min(zh0 + c * LCH, whi[2]);
+    };
+    constexpr int EPC = 16 / BPV, CPR = LTX / EPC;  // voxels per 16-byte piece, pieces per row
+    const bool full_x = tx0 + LTX <= dnx;
+    const bool rows16 = (g.row_b & 15u) == 0 && ((uintptr_t)src & 15u) == 0;
+    const int nch = (nzv + LCH - 1) / LCH;
+    // Planes in flight for a chunk: loaded into v (planes [na, na + min(nn,
+    // LCH))), written to the ring at the end of the chunk before it.
+    struct Inflight {
+        int na, nn;
+        u32x4 v[KPF];
+        bool on[KPF];
+    };
+    // The chunk loop.  EXACT (aligned rows, a full 16-wide column): every
+    // chunk issues the same vector memory operations in the same order, so
+    // that the compiler counts exactly and the refill waits only for the
+    // planes it writes; loads then run two chunks ahead of their use, since
+    // one chunk's resampling is shorter than a loaded HBM round trip.
+    auto ring_loop = [&](auto exact) {
+        constexpr bool EXACT = decltype(exact)::value;
+        // Chunk cc's planes after `after` (the highest plane staged or in
+        // flight), the first LCH of them loaded now.
+        auto issue = [&](int cc, int& after, Inflight& fl) {
+            int npl, nph;
+            chunk_planes(cc, npl, nph);
+            fl.na = max(after + 1, npl);
+            fl.nn = cc < nch ? nph - fl.na + 1 : 0;
+            #pragma unroll
+            for (int j = 0; j < KPF; j++) fl.v[j] = st_load(exact, wave + NW * j, fl.na, min(fl.nn, LCH), fl.on[j]);
+            if (fl.nn > 0) after = nph;
+        };
+        // Chunk c's staged outputs in 16-byte row pieces (EXACT: one store per
+        // thread, idle threads' past the range); otherwise voxel by voxel
+        // where the volume's x edge cuts the column.
+        auto store_chunk = [&](int c, int cnt) {
+            const int k = tid / (LTY * CPR), y = (tid / CPR) % LTY, h = tid % CPR;
+            const bool ok = k < cnt && ty0 + y <= ty1;
+            const uint32_t e0 = ((uint32_t)(tz0 + c * LCH + k) * (uint32_t)dny + (uint32_t)(ty0 + y)) * (uint32_t)dnx +
+                                (uint32_t)(tx0 + h * EPC);
+            const T* src_row = obuf + (min(k, LCH - 1) * LTY + y) * LTX + h * EPC;
+            if constexpr (EXACT) {
+                __builtin_amdgcn_raw_buffer_store_b128(*(const u32x4*)src_row, rd, (int)(ok ? e0 * BPV : OOB_V), 0, 0);
+            } else if (ok) {
+                if (full_x) {
+                    __builtin_amdgcn_raw_buffer_store_b128(*(const u32x4*)src_row, rd, (int)(e0 * BPV), 0, 0);
+                } else {
+                    #pragma unroll
+                    for (int e = 0; e < EPC; e++)
+                        if (tx0 + h * EPC + e < dnx) bst<T>(rd, (e0 + (uint32_t)e) * BPV, src_row[e]);
+                }
+            }
+        };
+        int pl, ph;
+        chunk_planes(0, pl, ph);
+        // The first chunk's planes, synchronously.
+        for (int i0 = 0; i0 < (ph - pl + 1) * H; i0 += NW * KPF) {
             u32x4 v[KPF];
             bool on[KPF];
             #pragma unroll
-            for (int j = 0; j < KPF; j++) v[j] = st_load(i0 + wave + NW * j, pa, np, on[j]);
+            for (int j = 0; j < KPF; j++) v[j] = st_load(exact, i0 + wave + NW * j, pl, ph - pl + 1, on[j]);
             #pragma unroll
-            for (int j = 0; j < KPF; j++) st_store(i0 + wave + NW * j, pa, on[j], v[j]);
+            for (int j = 0; j < KPF; j++) st_store(i0 + wave + NW * j, pl, on[j], v[j]);
         }
-    };
-    // Planes chunk c (slices [c * LCH, c * LCH + LCH) of the column) reaches.
-    auto chunk_planes = [&](int c, int& pl, int& ph) {
-        const int z0 = (int)a.dz0 + tz0 + c * LCH, z1 = min(z0 + LCH - 1, (int)a.dz0 + tz1);
-        pl = reach_lo(2, z0);
-        ph = reach_hi(2, z1);
-    };
-    const int nch = (nzv + LCH - 1) / LCH;
-    int pl, ph;
-    chunk_planes(0, pl, ph);
-    if (ph >= pl) st_sync(pl, ph - pl + 1);
-    int top = max(ph, pl - 1);  // the ring holds planes up to `top`
-    __syncthreads();
-    for (int c = 0; c < nch; c++) {
-        // Prefetch the first LCH planes the next chunk adds.
-        int npl = 0, nph = -1, na = 0, nn = 0;
-        u32x4 pf[KPF];
-        bool pon[KPF];
-        if (c + 1 < nch) {
-            chunk_planes(c + 1, npl, nph);
-            na = max(top + 1, npl);
-            nn = nph - na + 1;
-            #pragma unroll
-            for (int j = 0; j < KPF; j++) pf[j] = st_load(wave + NW * j, na, min(nn, LCH), pon[j]);
-        }
-        walk(std::true_type{}, min(nzv, (c + 1) * LCH));
-        if (c + 1 == nch) break;
-        __syncthreads();  // chunk c done: its planes may be overwritten
-        if (nn > 0) {
-            #pragma unroll
-            for (int j = 0; j < KPF; j++) st_store(wave + NW * j, na, pon[j], pf[j]);
-            if (nn > LCH) st_sync(na + LCH, nn - LCH);
-            top = nph;
-        }
+        int top = max(ph, pl - 1);  // the highest plane staged or in flight
+        Inflight fa, fb;
+        issue(1, top, fa);
         __syncthreads();
-    }
+        // Chunk c with `cur` holding chunk c + 1's planes; loads chunk c + 2's
+        // into `nxt`.  Two calls per iteration swap the roles, so that no
+        // in-flight register is copied (a copy would wait for its load).
+        auto step = [&](int c, Inflight& cur, Inflight& nxt) {
+            issue(c + 2, top, nxt);
+            const int cnt = min(LCH, nzv - c * LCH);
+            walk_lds(c * LCH + cnt);
+            __syncthreads();  // chunk c done: its planes may be overwritten, its outputs are staged
+            if (EXACT || cur.nn > 0) {
+                #pragma unroll
+                for (int j = 0; j < KPF; j++) st_store(wave + NW * j, cur.na, cur.on[j], cur.v[j]);
+            }
+            store_chunk(c, cnt);
+            __syncthreads();
+        };
+        for (int c = 0; c < nch; c += 2) {
+            step(c, fa, fb);
+            if (c + 1 < nch) step(c + 1, fb, fa);
+        }
+    };
+    if (rows16 && full_x) ring_loop(std::true_type{});
+    else ring_loop(std::false_type{});
 }
 
 }  // namespace
