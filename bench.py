@@ -771,6 +771,8 @@ def main():
     ap.add_argument("--profile-layer", action="store_true",
                     help="profiler run of the headline: layer setup, one mixed step, a 1 GiB device copy")
     ap.add_argument("--no-config2", action="store_true", help="skip the plain-GPU configuration legs")
+    ap.add_argument("--form", choices=["mixed", "plain"], default="mixed",
+                    help="form of the --profile-layer run")
     ap.add_argument("--tile", choices=["auto", "gather", "lds"], default="auto",
                     help="fused resample tile: auto (gather), per-tap gathers, or LDS-streamed columns")
     args = ap.parse_args()
@@ -816,11 +818,13 @@ def main():
     log("headline: one brick layer of", SIZE_S, "uint16, K =", K_S, "tile", args.tile)
     L = layer_setup(torch, wp, threads)
     if args.profile_layer:
+        prec = warp.PLAIN if args.form == "plain" else warp.MIXED
         for _ in range(max(1, args.steps)):
-            c, d = wp.tps_fit(L["tar"], L["sub"], precision=warp.MIXED)
-            grid = wp.new_grid(L["gsh"], G_S, origin=(0, 0, LAYER_Z0), storage=warp.F16, device=True)
-            wp.grid_eval(L["tar"], c, d, grid, precision=warp.MIXED)
-            wp.resample_bricks(None, None, grid, None, None, None, precision=warp.MIXED, prepared=L["prepared"])
+            c, d = wp.tps_fit(L["tar"], L["sub"], precision=prec)
+            grid = wp.new_grid(L["gsh"], G_S, origin=(0, 0, LAYER_Z0),
+                               storage=warp.F32 if prec == warp.PLAIN else warp.F16, device=True)
+            wp.grid_eval(L["tar"], c, d, grid, precision=prec)
+            wp.resample_bricks(None, None, grid, None, None, None, precision=prec, prepared=L["prepared"])
             torch.cuda.synchronize()
             grid.release()
         calibration_copy(torch)
