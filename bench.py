@@ -184,6 +184,32 @@ SIZE_P = (2048, 2048, 512)
 K_P, DMAX_P, G_P, SEED_P = 4096, 40.0, 8, 3
 
 
+def fit_sweep(args):
+    """Fit time of the FP64 factor (TW_PLAIN) and of the FP32 factor with FP64
+    refinement (TW_MIXED forced onto it) at each landmark count of
+    args.fit_sweep (landmarks in the 2048x2048x512 precision volume); one JSON
+    line per K.  The default TW_TUNE_FP32_FACTOR_MIN comes from this sweep."""
+    from terawarp_amd import synth, warp
+    from terawarp_amd.warp import Warper
+    w = Warper(0, timing=True)
+    for K in [int(k) for k in args.fit_sweep.split(",")]:
+        sub, tar = synth.landmarks((2048, 2048, 512), K, 40.0, seed=3)
+        out = {"K": K}
+        for name, prec, thr in (("fp64", warp.PLAIN, None), ("fp32_ir", warp.MIXED, 1)):
+            if thr is not None:
+                w.set_tune("fp32_factor_min", thr)
+            ms = []
+            for _ in range(args.warmup + args.steps):
+                w.tps_fit(tar, sub, precision=prec)
+                ms.append(w.timings()["fit_ms"])
+            t = w.timings()
+            out[name] = {"fit_ms": float(np.median(ms[args.warmup:])), "refine_iters": t["refine_iters"],
+                         "residual": t["fit_residual"]}
+        w.set_tune("fp32_factor_min", 14000)
+        print(json.dumps(out), flush=True)
+    w.close()
+
+
 def precision_run(args):
     """Per-stage error of both forms against the FP64 oracle at the precision
     configuration, plus the chain's timing there.  Oracle work is bounded:
@@ -771,6 +797,8 @@ def main():
     ap.add_argument("--profile-layer", action="store_true",
                     help="profiler run of the headline: layer setup, one mixed step, a 1 GiB device copy")
     ap.add_argument("--no-config2", action="store_true", help="skip the plain-GPU configuration legs")
+    ap.add_argument("--fit-sweep", default=None, metavar="K1,K2,...",
+                    help="fit times of the FP64 and the FP32+refinement factor at these landmark counts")
     ap.add_argument("--form", choices=["mixed", "plain"], default="mixed",
                     help="form of the --profile-layer run")
     ap.add_argument("--tile", choices=["auto", "gather", "lds"], default="auto",
@@ -781,6 +809,10 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.fit_sweep:
+        args.steps = args.steps or 4
+        fit_sweep(args)
+        return
     if args.precision:
         precision_run(args)
         return

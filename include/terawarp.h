@@ -151,6 +151,23 @@ int         tw_set_timing(tw_ctx* ctx, int enable);
 #define TW_TILE_GATHER  1
 #define TW_TILE_LDS     2
 int         tw_set_tile(tw_ctx* ctx, int tile);
+/* Tunables of the hot paths, per context.  None changes a TW_PLAIN or
+ * TW_EXACT result; TW_TUNE_FP32_FACTOR_MIN moves TW_MIXED fits between two
+ * factorisations that both meet its 1e-8 bar.  The reference hard-codes its
+ * counterparts (threads per block ele.cuh:4, block size and read halo
+ * q_imgwarp_tps_quicksmallmemory.cpp:1264,1279); terawarp_amd.tune measures
+ * and persists these per device.
+ *   TW_TUNE_TILE             TW_TILE_* (as tw_set_tile)
+ *   TW_TUNE_TZ               gather tile depth in voxels along z, 1..256;
+ *                            0: 4 grid cells clamped to [16, 64] (default)
+ *   TW_TUNE_FP32_FACTOR_MIN  TW_MIXED: smallest projected system order
+ *                            (landmarks - 4) factored in FP32 with FP64
+ *                            refinement; smaller ones factor in FP64 */
+#define TW_TUNE_TILE             1
+#define TW_TUNE_TZ               2
+#define TW_TUNE_FP32_FACTOR_MIN  3
+int         tw_set_tune(tw_ctx* ctx, int key, int64_t value);
+int         tw_get_tune(tw_ctx* ctx, int key, int64_t* value);
 int         tw_get_timings(tw_ctx* ctx, tw_timings* out);
 const char* tw_last_error(tw_ctx* ctx);
 const char* tw_version(void);

@@ -38,10 +38,13 @@ TW_WARP_AFFINE = 1
 TW_TILE_AUTO = 0
 TW_TILE_GATHER = 1
 TW_TILE_LDS = 2
+TW_TUNE_TILE = 1
+TW_TUNE_TZ = 2
+TW_TUNE_FP32_FACTOR_MIN = 3
 
 # Every entry point include/terawarp.h declares.
 EXPORTS = (
-    "tw_create", "tw_destroy", "tw_set_stream", "tw_get_stream", "tw_sync", "tw_set_timing", "tw_set_tile", "tw_get_timings", "tw_last_error",
+    "tw_create", "tw_destroy", "tw_set_stream", "tw_get_stream", "tw_sync", "tw_set_timing", "tw_set_tile", "tw_set_tune", "tw_get_tune", "tw_get_timings", "tw_last_error",
     "tw_version", "tw_affine_fit", "tw_mode_matrix", "tw_resample_affine", "tw_tps_fit", "tw_tps_grid_eval",
     "tw_grid_release", "tw_resample_tps", "tw_upsample_df", "tw_warp_tps", "tw_warp_tps_noalloc", "tw_warp_batch",
     "tw_resample_tps_bricks", "tw_warp_stream",
@@ -155,6 +158,8 @@ def load():
         "tw_sync": (ctypes.c_int, [_P]),
         "tw_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
         "tw_set_tile": (ctypes.c_int, [_P, ctypes.c_int]),
+        "tw_set_tune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
+        "tw_get_tune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
         "tw_get_timings": (ctypes.c_int, [_P, T]),
         "tw_last_error": (ctypes.c_char_p, [_P]),
         "tw_version": (ctypes.c_char_p, []),

@@ -46,7 +46,6 @@
 namespace {
 
 constexpr int NB = 64;  // Cholesky block size
-constexpr int MIXED_FP32_MIN = 4097;  // TW_MIXED: smallest projected order factored in FP32
 
 struct SysDesc {
     const float4* pts;   // N centres (x, y, z, 0), original coordinates
@@ -892,11 +891,87 @@ __device__ __forceinline__ void syrk_body(T* __restrict__ B, int n, int j0, int 
         }
 }
 
+// D += A B for a 16 x 4 A and a 4 x 16 B on one wave's matrix core, FP64 or
+// FP32: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; item m of
+// the result holds D[mma_row<T>(m, l / 16)][l % 16] (FP64: 4m + l/16; FP32:
+// 4 (l/16) + m, as measured on gfx950).
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+    typedef v4d V;
+    static __device__ __forceinline__ V op(double a, double b, V c) { return mfma16x4(a, b, c); }
+    static __device__ __forceinline__ int row(int m, int lk) { return 4 * m + lk; }
+};
+template <>
+struct Mma<float> {
+    typedef v4f V;
+    static __device__ __forceinline__ V op(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int m, int lk) { return 4 * lk + m; }
+};
+
+// The trailing update of syrk_body on the matrix cores (FP64 or FP32 with
+// FP32 accumulation): tile (bx, by) of B_RC -= L_R L_C', formed transposed
+// (D = L_C L_R', wave w owning columns 16w .. 16w + 15 of the tile), so that
+// a lane's results run down B's columns: the tile's loads and stores take
+// 16 consecutive elements per column.
+template <typename T>
+__device__ __forceinline__ void syrk_mfma_body(T* __restrict__ B, int n, int j0, int jb, int bx, int by,
+                                               SyrkLds<T>& S) {
+    typedef typename Mma<T>::V V;
+    auto& As = S.As;
+    auto& Bs = S.Bs;
+    const int tid = threadIdx.x, lr = tid & 15, lk = (tid >> 4) & 3;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r0 = j0 + jb;
+    const int R = r0 + bx * NB, C = r0 + by * NB;
+    T va[NB * NB / 256], vb[NB * NB / 256], old[4][4];
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) {
+        const int e = tid + 256 * u, r = e & (NB - 1), k = e / NB;
+        va[u] = (k < jb && R + r < n) ? B[(i64)(j0 + k) * n + R + r] : (T)0;
+        vb[u] = (k < jb && C + r < n) ? B[(i64)(j0 + k) * n + C + r] : (T)0;
+    }
+    #pragma unroll
+    for (int J = 0; J < 4; J++)
+        #pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int c = C + 16 * w + Mma<T>::row(m, lk), r = R + 16 * J + lr;
+            old[J][m] = (r < n && c < n && c <= r) ? B[(i64)c * n + r] : (T)0;
+        }
+    #pragma unroll
+    for (int u = 0; u < NB * NB / 256; u++) {
+        const int e = tid + 256 * u;
+        As[e / NB][e & (NB - 1)] = va[u];
+        Bs[e / NB][e & (NB - 1)] = vb[u];
+    }
+    __syncthreads();
+    V acc[4];
+    #pragma unroll
+    for (int J = 0; J < 4; J++) acc[J] = V{0, 0, 0, 0};
+    #pragma unroll
+    for (int s = 0; s < NB / 4; s++) {
+        const T a = Bs[4 * s + lk][16 * w + lr];
+        #pragma unroll
+        for (int J = 0; J < 4; J++) acc[J] = Mma<T>::op(a, As[4 * s + lk][16 * J + lr], acc[J]);
+    }
+    #pragma unroll
+    for (int J = 0; J < 4; J++)
+        #pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int c = C + 16 * w + Mma<T>::row(m, lk), r = R + 16 * J + lr;
+            if (r < n && c < n && c <= r) B[(i64)c * n + r] = old[J][m] - acc[J][m];
+        }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, int jb) {
     if (blockIdx.y > blockIdx.x) return;
     __shared__ SyrkLds<T> S;
-    syrk_body<T>(B, n, j0, jb, blockIdx.x, blockIdx.y, S);
+    syrk_mfma_body<T>(B, n, j0, jb, blockIdx.x, blockIdx.y, S);
 }
 
 // One right-looking step of the blocked Cholesky in one launch, for small
@@ -2410,10 +2485,11 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
 
     // Workspaces: B (n x n, FP64 or FP32); vectors in one FP64 arena;
     // state, FP32 vectors and the inverted diagonal blocks in aux.
-    // MI355X runs FP64 vector FMAs at the FP32 rate, so below MIXED_FP32_MIN
-    // the FP64 factor (one solve) beats the FP32 factor plus refinement
-    // sweeps (several solves and residuals); TW_MIXED then takes it too.
-    const bool mixed = precision == TW_MIXED && n >= MIXED_FP32_MIN;
+    // The block steps and triangular solves are latency-bound at these
+    // orders, so below ctx->fp32_factor_min the FP64 factor (one solve) beats
+    // the FP32 factor plus refinement sweeps (several solves and residuals);
+    // TW_MIXED then takes it too.
+    const bool mixed = precision == TW_MIXED && n >= ctx->fp32_factor_min;
     void *Bw = nullptr, *arena = nullptr, *aux = nullptr;
     if ((rc = tw_ws(ctx, WS_SOLVE_A, (size_t)n * n * (mixed ? sizeof(float) : sizeof(double)), &Bw))) return rc;
     const size_t NV = (size_t)N * 4;

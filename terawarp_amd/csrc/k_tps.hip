@@ -165,7 +165,7 @@ int launch_tps(tw_ctx* ctx, TpsArgs a, int interp, int precision, bool f16) {
     if (a.f > 64) return tw_fail(ctx, TW_E_UNSUPPORTED, "grid spacing %d above 64 is not supported", a.f);
     if (!WRITE_DF && tw_lds_tile(ctx, interp))
         return launch_tps_lds<T>(ctx, a, precision, f16);
-    a.tz = pick_tz(a.f);
+    a.tz = pick_tz(a.f, ctx->tune_tz);
     FastGeom g;
     size_t lds = 0;
     if (const char* why = fused_geom<T, WRITE_DF>(a, g, lds)) return tw_fail(ctx, TW_E_UNSUPPORTED, "%s", why);

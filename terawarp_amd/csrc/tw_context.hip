@@ -337,6 +337,35 @@ int tw_set_tile(tw_ctx* ctx, int tile) {
     return TW_OK;
 }
 
+int tw_set_tune(tw_ctx* ctx, int key, int64_t value) {
+    if (!ctx) return TW_E_ARG;
+    switch (key) {
+        case TW_TUNE_TILE:
+            return tw_set_tile(ctx, (int)value);
+        case TW_TUNE_TZ:
+            if (value < 0 || value > 256) return tw_fail(ctx, TW_E_ARG, "TW_TUNE_TZ takes 0 .. 256 (got %lld)", (long long)value);
+            ctx->tune_tz = (int)value;
+            return TW_OK;
+        case TW_TUNE_FP32_FACTOR_MIN:
+            if (value < 1 || value > (1ll << 30))
+                return tw_fail(ctx, TW_E_ARG, "TW_TUNE_FP32_FACTOR_MIN takes 1 .. 2^30 (got %lld)", (long long)value);
+            ctx->fp32_factor_min = (int)value;
+            return TW_OK;
+        default:
+            return tw_fail(ctx, TW_E_ARG, "unknown tunable %d", key);
+    }
+}
+
+int tw_get_tune(tw_ctx* ctx, int key, int64_t* value) {
+    if (!ctx || !value) return TW_E_ARG;
+    switch (key) {
+        case TW_TUNE_TILE: *value = ctx->tile; return TW_OK;
+        case TW_TUNE_TZ: *value = ctx->tune_tz; return TW_OK;
+        case TW_TUNE_FP32_FACTOR_MIN: *value = ctx->fp32_factor_min; return TW_OK;
+        default: return tw_fail(ctx, TW_E_ARG, "unknown tunable %d", key);
+    }
+}
+
 int tw_get_timings(tw_ctx* ctx, tw_timings* out) {
     if (!ctx || !out) return TW_E_ARG;
     memset(out, 0, sizeof *out);

@@ -41,12 +41,21 @@ enum TwSlot {
 
 enum TwStage { ST_FIT = 0, ST_GRID, ST_QUANT, ST_RESAMPLE, ST_H2D, ST_D2H, ST_TOTAL, ST_COUNT };
 
+// TW_MIXED fits factor projected systems of at least this order in FP32
+// (MFMA trailing updates, FP64 refinement); smaller ones in FP64.  Measured
+// on MI355X: the FP64 factor is faster up to 12000 landmarks (38.8 vs 42.7
+// ms), the FP32 one from 16000 (72.0 vs 78.0 ms); below, the block steps and
+// the refinement's extra triangular solves are latency-bound.
+constexpr int TW_FP32_FACTOR_MIN_DEFAULT = 14000;
+
 struct tw_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     bool timing = false;
     int tile = TW_TILE_AUTO;  // fused TPS resample tile (tw_set_tile)
+    int tune_tz = 0;          // gather tile depth (0: from the grid spacing)
+    int fp32_factor_min = TW_FP32_FACTOR_MIN_DEFAULT;  // TW_MIXED: FP32 factor from this order
     std::string err;
     void* ws[WS_COUNT] = {};
     size_t ws_cap[WS_COUNT] = {};

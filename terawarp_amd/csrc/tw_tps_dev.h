@@ -630,13 +630,16 @@ inline int max_node_span(i64 r0, int len, i64 n, int f) {
     return best;
 }
 
-// Voxels per work-group along z: 4 cells, clamped to [16, 64], halved
-// until the LDS tile fits in 64 KiB.
-inline int pick_tz(int f) {
-    int tz = TW_TZ_CELLS * f;
-    if (tz < 16) tz = 16;
-    if (tz > 64) tz = 64;
-    while (tz > 1 && fused_lds_bytes(f, tz) > 64 * 1024) tz /= 2;
+// Voxels per work-group along z: `tuned` if given (halved until the LDS tile
+// fits in 160 KiB), else 4 cells clamped to [16, 64], halved until the LDS
+// tile fits in 64 KiB.
+inline int pick_tz(int f, int tuned = 0) {
+    int tz = tuned > 0 ? tuned : TW_TZ_CELLS * f;
+    if (tuned <= 0) {
+        if (tz < 16) tz = 16;
+        if (tz > 64) tz = 64;
+    }
+    while (tz > 1 && fused_lds_bytes(f, tz) > (tuned > 0 ? 160 : 64) * 1024) tz /= 2;
     return tz;
 }
 

@@ -217,6 +217,18 @@ class Warper:
     def set_timing(self, on=True):
         self._check(self.lib.tw_set_timing(self.ctx, 1 if on else 0))
 
+    TUNABLES = {"tile": L.TW_TUNE_TILE, "tz": L.TW_TUNE_TZ, "fp32_factor_min": L.TW_TUNE_FP32_FACTOR_MIN}
+
+    def set_tune(self, key, value):
+        """Set a tunable (tw_set_tune): "tile" (0 auto, 1 gather, 2 lds), "tz"
+        (gather tile depth, 0 = from the grid spacing), "fp32_factor_min"."""
+        self._check(self.lib.tw_set_tune(self.ctx, self.TUNABLES[key], int(value)))
+
+    def get_tune(self, key):
+        v = ctypes.c_int64()
+        self._check(self.lib.tw_get_tune(self.ctx, self.TUNABLES[key], ctypes.byref(v)))
+        return v.value
+
     def set_tile(self, tile):
         """Fused TPS resample tile: "auto" (the gather tile), "gather" or
         "lds" (source streamed through LDS); outputs are identical."""

@@ -81,20 +81,47 @@ def test_fit_four_wave_qr_at_its_largest(warper):
     assert np.abs(d2[1:, 1:] - np.eye(3)).max() < 1e-9, d2[1:, 1:]
 
 
-def test_fit_fp32_refinement_large(warper):
-    """Above 4096 projected unknowns TW_MIXED factors in FP32 and refines the
-    solution with FP64 residuals (an FP64 refactor if refinement stalls);
-    either way the coefficients match the FP64 factor's."""
-    sub, tar = synth.landmarks((2048, 2048, 512), 4200, 20.0, seed=7)
+@pytest.mark.parametrize("K", [1029, 2048, 4200])
+def test_fit_fp32_factor_with_refinement(warper, oracle, K):
+    """TW_MIXED's FP32 factor (MFMA trailing updates) with FP64 refinement,
+    forced on below its default threshold: it converges without the FP64
+    refactor (refine_iters > 0) and the coefficients match the oracle's
+    bordered FP64 solve and the FP64 factor's."""
+    sub, tar = synth.landmarks((2048, 2048, 512), K, 20.0, seed=7)
     c64, d64 = warper.tps_fit(tar, sub, precision=warp.PLAIN)
-    r64 = warper.timings()["fit_residual"]
-    c32, d32 = warper.tps_fit(tar, sub, precision=warp.MIXED)
-    t = warper.timings()
-    print("refine_iters", t["refine_iters"], "residual", t["fit_residual"], "fp64 residual", r64)
-    assert t["refine_iters"] != 0
+    default = warper.get_tune("fp32_factor_min")
+    try:
+        warper.set_tune("fp32_factor_min", 1)
+        c32, d32 = warper.tps_fit(tar, sub, precision=warp.MIXED)
+        t = warper.timings()
+    finally:
+        warper.set_tune("fp32_factor_min", default)
+    assert t["refine_iters"] > 0, t["refine_iters"]
     assert rel_err(c32, c64) < 1e-8, rel_err(c32, c64)
     assert rel_err(d32, d64) < 1e-8, rel_err(d32, d64)
     assert t["fit_residual"] < 1e-10
+    if K <= 2048:
+        c_ref, d_ref = oracle.stps_fit(tar, sub, bordered=True)
+        assert rel_err(c32, c_ref) < 1e-8, rel_err(c32, c_ref)
+        assert np.abs(d32 - d_ref).max() < 1e-8
+
+
+def test_tunables_round_trip(warper):
+    """tw_set_tune / tw_get_tune: values stick, bad keys and values are
+    rejected with TW_E_ARG, and the default threshold keeps config-size
+    mixed fits on the FP64 factor (refine_iters == 0)."""
+    for key, val in (("tile", 1), ("tz", 32), ("fp32_factor_min", 5000)):
+        old = warper.get_tune(key)
+        warper.set_tune(key, val)
+        assert warper.get_tune(key) == val
+        warper.set_tune(key, old)
+    with pytest.raises(warp.TeraWarpError):
+        warper.set_tune("tz", 999)
+    with pytest.raises(warp.TeraWarpError):
+        warper.set_tune("tile", 7)
+    sub, tar = synth.landmarks((1024, 1024, 256), 2048, 20.0, seed=9)
+    warper.tps_fit(tar, sub, precision=warp.MIXED)
+    assert warper.timings()["refine_iters"] == 0
 
 
 @pytest.mark.parametrize("K", [16, 64, 512])
