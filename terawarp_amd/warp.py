@@ -160,7 +160,9 @@ class Grid:
 class Warper:
     """One library context (device, HIP stream, cached workspaces)."""
 
-    def __init__(self, device=0, timing=False):
+    def __init__(self, device=0, timing=False, tuned=True):
+        """tuned: apply the device's stored tunables (terawarp_amd.tune);
+        False keeps the library defaults."""
         self.lib = L.load()
         ctx = ctypes.c_void_p()
         rc = self.lib.tw_create(device, ctypes.byref(ctx))
@@ -168,8 +170,12 @@ class Warper:
             raise TeraWarpError(rc, "tw_create failed (no usable HIP device?)")
         self.ctx = ctx
         self._ext = None
+        self.stream_brick = 512
         if timing:
             self.lib.tw_set_timing(self.ctx, 1)
+        if tuned:
+            from . import tune
+            tune.apply(self, tune.load_settings(tune.device_arch(device)))
 
     def _ordered(self, fn, *args):
         """Call the library with its stream ordered after torch's current
@@ -386,7 +392,7 @@ class Warper:
         return [int(x) for x in status], done
 
     # ---- streaming ----------------------------------------------------------
-    def warp_stream(self, size_xyz, dtype, centres, targets, source, f=8, brick=512, lam=0.2, interp=LINEAR,
+    def warp_stream(self, size_xyz, dtype, centres, targets, source, f=8, brick=None, lam=0.2, interp=LINEAR,
                     precision=MIXED, threads=16, keep=(), block_base=None, block_origin=False):
         """Warp a volume brick by brick (tw_warp_stream).
 
@@ -400,6 +406,7 @@ class Warper:
         block_origin: TW_STREAM_BLOCK_ORIGIN (EXACT positions in the tera
         block warp's block-local order, the bricks being its blocks).
         """
+        brick = brick or self.stream_brick  # the tuned brick edge unless given
         ce, ta = _pts(centres), _pts(targets)
         nx, ny, nz = size_xyz
         dt = L.TW_U8 if np.dtype(dtype) == np.uint8 else L.TW_U16
