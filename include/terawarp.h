@@ -158,14 +158,18 @@ int         tw_set_tile(tw_ctx* ctx, int tile);
  * q_imgwarp_tps_quicksmallmemory.cpp:1264,1279); terawarp_amd.tune measures
  * and persists these per device.
  *   TW_TUNE_TILE             TW_TILE_* (as tw_set_tile)
- *   TW_TUNE_TZ               gather tile depth in voxels along z, 1..256;
- *                            0: 4 grid cells clamped to [16, 64] (default)
+ *   TW_TUNE_TZ               gather tile depth in voxels along z of
+ *                            single-volume launches, 1..256; 0: 4 grid cells
+ *                            clamped to [16, 64] (default)
+ *   TW_TUNE_TZ_BATCH         the same for the persistent launches
+ *                            (tw_warp_batch, tw_resample_tps_bricks)
  *   TW_TUNE_FP32_FACTOR_MIN  TW_MIXED: smallest projected system order
  *                            (landmarks - 4) factored in FP32 with FP64
  *                            refinement; smaller ones factor in FP64 */
 #define TW_TUNE_TILE             1
 #define TW_TUNE_TZ               2
 #define TW_TUNE_FP32_FACTOR_MIN  3
+#define TW_TUNE_TZ_BATCH         4
 int         tw_set_tune(tw_ctx* ctx, int key, int64_t value);
 int         tw_get_tune(tw_ctx* ctx, int key, int64_t* value);
 int         tw_get_timings(tw_ctx* ctx, tw_timings* out);

@@ -41,6 +41,7 @@ TW_TILE_LDS = 2
 TW_TUNE_TILE = 1
 TW_TUNE_TZ = 2
 TW_TUNE_FP32_FACTOR_MIN = 3
+TW_TUNE_TZ_BATCH = 4
 
 # Every entry point include/terawarp.h declares.
 EXPORTS = (

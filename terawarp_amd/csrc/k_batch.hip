@@ -244,7 +244,7 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
     int* parts = (int*)(host + jobs_b);
     std::vector<i64> tiles((size_t)n);
     size_t lds = 0;
-    const int tz = pick_tz(f, ctx->tune_tz);
+    const int tz = pick_tz(f, ctx->tune_tz_batch);
     const bool lds_tile = tw_lds_tile(ctx, interp);
     for (int i = 0; i < n; i++) {
         const tw_warp_item& it = items[i];
@@ -432,7 +432,7 @@ extern "C" int tw_resample_tps_bricks(tw_ctx* ctx, const tw_volume* srcs, const 
     int* parts = (int*)(host + jobs_b);
     std::vector<i64> tiles((size_t)n);
     size_t lds = 0;
-    const int tz = pick_tz(grid->f, ctx->tune_tz);
+    const int tz = pick_tz(grid->f, ctx->tune_tz_batch);
     const bool lds_tile = tw_lds_tile(ctx, interp);
     for (int i = 0; i < n; i++) {
         const tw_volume &s = srcs[i], &d = dsts[i];

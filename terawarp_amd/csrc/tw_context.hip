@@ -343,8 +343,10 @@ int tw_set_tune(tw_ctx* ctx, int key, int64_t value) {
         case TW_TUNE_TILE:
             return tw_set_tile(ctx, (int)value);
         case TW_TUNE_TZ:
-            if (value < 0 || value > 256) return tw_fail(ctx, TW_E_ARG, "TW_TUNE_TZ takes 0 .. 256 (got %lld)", (long long)value);
-            ctx->tune_tz = (int)value;
+        case TW_TUNE_TZ_BATCH:
+            if (value < 0 || value > 256)
+                return tw_fail(ctx, TW_E_ARG, "tile depth takes 0 .. 256 (got %lld)", (long long)value);
+            (key == TW_TUNE_TZ ? ctx->tune_tz : ctx->tune_tz_batch) = (int)value;
             return TW_OK;
         case TW_TUNE_FP32_FACTOR_MIN:
             if (value < 1 || value > (1ll << 30))
@@ -361,6 +363,7 @@ int tw_get_tune(tw_ctx* ctx, int key, int64_t* value) {
     switch (key) {
         case TW_TUNE_TILE: *value = ctx->tile; return TW_OK;
         case TW_TUNE_TZ: *value = ctx->tune_tz; return TW_OK;
+        case TW_TUNE_TZ_BATCH: *value = ctx->tune_tz_batch; return TW_OK;
         case TW_TUNE_FP32_FACTOR_MIN: *value = ctx->fp32_factor_min; return TW_OK;
         default: return tw_fail(ctx, TW_E_ARG, "unknown tunable %d", key);
     }

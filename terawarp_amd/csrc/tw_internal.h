@@ -54,7 +54,8 @@ struct tw_ctx {
     hipStream_t stream = nullptr;
     bool timing = false;
     int tile = TW_TILE_AUTO;  // fused TPS resample tile (tw_set_tile)
-    int tune_tz = 0;          // gather tile depth (0: from the grid spacing)
+    int tune_tz = 0;          // gather tile depth, single-volume launches (0: from the grid spacing)
+    int tune_tz_batch = 0;    // the same, persistent launches
     int fp32_factor_min = TW_FP32_FACTOR_MIN_DEFAULT;  // TW_MIXED: FP32 factor from this order
     std::string err;
     void* ws[WS_COUNT] = {};

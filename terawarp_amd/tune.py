@@ -7,7 +7,9 @@ workloads they serve and persisted per device:
 
   tile   TW_TUNE_TILE: the fused resample's gather tile (64x4xTZ) or LDS
          column tile (16x16x64), bit-identical outputs
-  tz     TW_TUNE_TZ: the gather tile's depth along z (0: 4 grid cells)
+  tz     TW_TUNE_TZ / TW_TUNE_TZ_BATCH: the gather tile's depth along z in
+         single-volume and in persistent (brick, batch) launches (0: 4 grid
+         cells)
   brick  the brick edge of Warper.warp_stream (outputs identical: positions
          are global)
 
@@ -31,7 +33,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PACKAGE_DIR = os.path.join(HERE, "tuning")
-DEFAULTS = {"tile": 0, "tz": 0, "brick": 512}
+DEFAULTS = {"tile": 0, "tz": 0, "tz_batch": 0, "brick": 512}
 TILES = {0: "auto", 1: "gather", 2: "lds"}
 
 
@@ -66,6 +68,7 @@ def apply(warper, settings):
     """Set the context tunables of a Warper from stored settings."""
     warper.set_tune("tile", int(settings.get("tile", 0)))
     warper.set_tune("tz", int(settings.get("tz", 0)))
+    warper.set_tune("tz_batch", int(settings.get("tz_batch", 0)))
     warper.stream_brick = int(settings.get("brick", 512))
 
 
@@ -147,37 +150,69 @@ def batch_workload(warper):
     return run, items
 
 
+def warp_workload(warper):
+    """The plain-GPU configuration's chain: 1024x1024x256 uint16, K = 512
+    landmarks, |d| <= 20, grid 8, one tw_warp_tps call on device buffers."""
+    import torch
+    from . import synth, warp
+    size = (1024, 1024, 256)
+    src = synth.blob_volume_device(size, np.uint16, 2)
+    dst = torch.empty_like(src)
+    sub, tar = synth.landmarks(size, 512, 20.0, seed=2)
+
+    def run():
+        warper.warp_tps(src, tar, sub, f=8, lam=0.2, precision=warp.MIXED, dst=dst)
+
+    return run, (src, dst)
+
+
 def tune(warper, reps=3, tz_candidates=(16, 24, 32, 48, 64, 96, 128), bricks=4, stream=True, log=print):
-    """Measure every (tile, tz) candidate on the brick-layer and batch
-    workloads (resample-stage device time) and the streaming brick edge on a
-    1024x1024x512 volume (wall time); return the evidence and the settings
-    that minimise the summed time relative to the library defaults."""
+    """Measure every (tile, tile depth) candidate on the brick-layer and batch
+    workloads (persistent launches) and on the plain-GPU chain (a
+    single-volume launch) by resample-stage device time, and the streaming
+    brick edge on a 1024x1024x512 volume by wall time.  The depths minimise
+    the summed time relative to the library defaults over their family's
+    workloads; the tile (shared by both families) is the LDS column tile only
+    if it beats the best gather depths summed over all three."""
     import torch
     from . import synth, warp
     from .warp import Synth
     warper.set_timing(True)
-    cands = [(1, tz) for tz in tz_candidates] + [(2, 0)]
+    keys = ["auto/0"] + ["gather/%d" % tz for tz in tz_candidates] + ["lds/0"]
     evidence = {}
-    for kind, make in (("bricks", lambda: bricks_workload(warper, nb=bricks)), ("batch", lambda: batch_workload(warper))):
+    kinds = (("bricks", "tz_batch", lambda: bricks_workload(warper, nb=bricks)),
+             ("batch", "tz_batch", lambda: batch_workload(warper)),
+             ("warp", "tz", lambda: warp_workload(warper)))
+    for kind, knob, make in kinds:
         log("tuning on", kind)
         run, keep = make()
         times = {}
-        for tile, tz in [(0, 0)] + cands:
-            warper.set_tune("tile", tile)
-            warper.set_tune("tz", tz)
-            times["%s/%d" % (TILES[tile], tz)] = _median_ms(run, warper, "resample_ms", reps)
-            log("  ", kind, TILES[tile], tz, "%.3f ms" % times["%s/%d" % (TILES[tile], tz)])
-        evidence[kind] = {"resample_ms": times, "fixed": "auto/0"}
+        for key in keys:
+            tile, tz = key.split("/")
+            warper.set_tune("tile", {v: k for k, v in TILES.items()}[tile])
+            warper.set_tune(knob, int(tz))
+            times[key] = _median_ms(run, warper, "resample_ms", reps)
+            log("  ", kind, key, "%.3f ms" % times[key])
+        warper.set_tune(knob, 0)
+        warper.set_tune("tile", 0)
+        evidence[kind] = {"resample_ms": times, "fixed": "auto/0", "knob": knob}
         del keep, run
         torch.cuda.empty_cache()
-    score = {}
-    for key in evidence["bricks"]["resample_ms"]:
-        score[key] = sum(evidence[k]["resample_ms"][key] / evidence[k]["resample_ms"]["auto/0"] for k in evidence)
-    best = min(score, key=score.get)
-    tile_name, tz = best.split("/")
-    settings = {"tile": {v: k for k, v in TILES.items()}[tile_name], "tz": int(tz), "brick": DEFAULTS["brick"]}
-    warper.set_tune("tile", settings["tile"])
-    warper.set_tune("tz", settings["tz"])
+
+    def rel(kind, key):
+        return evidence[kind]["resample_ms"][key] / evidence[kind]["resample_ms"]["auto/0"]
+
+    gathers = ["gather/%d" % tz for tz in tz_candidates]
+    best_b = min(gathers, key=lambda k: rel("bricks", k) + rel("batch", k))
+    best_w = min(gathers, key=lambda k: rel("warp", k))
+    score = {"gather": rel("bricks", best_b) + rel("batch", best_b) + rel("warp", best_w),
+             "lds": sum(rel(k, "lds/0") for k in ("bricks", "batch", "warp"))}
+    lds = score["lds"] < score["gather"]
+    settings = {"tile": 2 if lds else 1, "tz": int(best_w.split("/")[1]), "tz_batch": int(best_b.split("/")[1]),
+                "brick": DEFAULTS["brick"]}
+    evidence["bricks"]["chosen"] = evidence["batch"]["chosen"] = "lds/0" if lds else best_b
+    evidence["warp"]["chosen"] = "lds/0" if lds else best_w
+    apply(warper, settings)
     if stream:
         log("tuning the streaming brick edge")
         size = (1024, 1024, 512)
@@ -195,6 +230,7 @@ def tune(warper, reps=3, tz_candidates=(16, 24, 32, 48, 64, 96, 128), bricks=4, 
             log("   stream brick", brick, "%.1f ms" % walls[str(brick)])
         evidence["stream"] = {"wall_ms": walls, "fixed": str(DEFAULTS["brick"]), "size_xyz": list(size)}
         settings["brick"] = int(min(walls, key=walls.get))
+        warper.stream_brick = settings["brick"]
     return {"settings": settings, "score": score, "evidence": evidence}
 
 
@@ -202,7 +238,6 @@ def report_md(arch, result):
     """The evidence as a markdown table (tuned vs fixed)."""
     lines = ["| workload | setting | time ms | vs fixed |", "|---|---|---:|---:|"]
     s = result["settings"]
-    chosen = "%s/%d" % (TILES[s["tile"]], s["tz"])
     for kind, ev in result["evidence"].items():
         if kind == "stream":
             fixed = ev["wall_ms"][ev["fixed"]]
@@ -213,7 +248,7 @@ def report_md(arch, result):
             continue
         fixed = ev["resample_ms"][ev["fixed"]]
         for key, t in ev["resample_ms"].items():
-            mark = " (chosen)" if key == chosen else (" (fixed)" if key == ev["fixed"] else "")
+            mark = " (chosen)" if key == ev.get("chosen") else (" (fixed)" if key == ev["fixed"] else "")
             lines.append("| %s | %s%s | %.3f | %.3f |" % (kind, key, mark, t, t / fixed))
     return "\n".join(lines)
 

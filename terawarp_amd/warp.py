@@ -223,11 +223,13 @@ class Warper:
     def set_timing(self, on=True):
         self._check(self.lib.tw_set_timing(self.ctx, 1 if on else 0))
 
-    TUNABLES = {"tile": L.TW_TUNE_TILE, "tz": L.TW_TUNE_TZ, "fp32_factor_min": L.TW_TUNE_FP32_FACTOR_MIN}
+    TUNABLES = {"tile": L.TW_TUNE_TILE, "tz": L.TW_TUNE_TZ, "tz_batch": L.TW_TUNE_TZ_BATCH,
+                "fp32_factor_min": L.TW_TUNE_FP32_FACTOR_MIN}
 
     def set_tune(self, key, value):
         """Set a tunable (tw_set_tune): "tile" (0 auto, 1 gather, 2 lds), "tz"
-        (gather tile depth, 0 = from the grid spacing), "fp32_factor_min"."""
+        / "tz_batch" (gather tile depth of single-volume / persistent
+        launches, 0 = from the grid spacing), "fp32_factor_min"."""
         self._check(self.lib.tw_set_tune(self.ctx, self.TUNABLES[key], int(value)))
 
     def get_tune(self, key):

@@ -11,7 +11,7 @@ def test_user_file_overrides_defaults(tmp_path, monkeypatch):
     path.write_text(json.dumps({"gfx_test": {"settings": {"tz": 32, "brick": 256}}}))
     monkeypatch.setenv("TERAWARP_TUNE_FILE", str(path))
     s = tune.load_settings("gfx_test")
-    assert s == {"tile": 0, "tz": 32, "brick": 256}
+    assert s == {"tile": 0, "tz": 32, "tz_batch": 0, "brick": 256}
     assert tune.load_settings("gfx_other") == tune.DEFAULTS
 
 
@@ -24,7 +24,8 @@ def test_unreadable_user_file_keeps_defaults(tmp_path, monkeypatch):
 
 def test_report_marks_chosen_and_fixed():
     res = {"settings": {"tile": 1, "tz": 32, "brick": 256},
-           "evidence": {"bricks": {"resample_ms": {"auto/0": 10.0, "gather/32": 8.0, "lds/0": 12.0}, "fixed": "auto/0"},
+           "evidence": {"bricks": {"resample_ms": {"auto/0": 10.0, "gather/32": 8.0, "lds/0": 12.0}, "fixed": "auto/0",
+                                   "chosen": "gather/32"},
                         "stream": {"wall_ms": {"256": 90.0, "512": 100.0}, "fixed": "512", "size_xyz": [64, 64, 64]}}}
     md = tune.report_md("gfx_test", res)
     assert "| bricks | gather/32 (chosen) | 8.000 | 0.800 |" in md
