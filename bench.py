@@ -427,32 +427,35 @@ def stream_run(args):
     from terawarp_amd import synth, warp
     from terawarp_amd.warp import Synth, Warper
     size = tuple(args.stream_size) if args.stream_size else SIZE_S
+    brick = args.stream_brick or BRICK_S
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     src = Synth(size, np.uint16, seed=SEED_S, cell=256, blobs_per_cell=16)
     sub, tar = synth.landmarks(size, K_S, DMAX_S, seed=SEED_S)
-    nbx, nby, nbz = (-(-n // BRICK_S) for n in size)
+    nbx, nby, nbz = (-(-n // brick) for n in size)
     corners = [(0, 0, 0), (nbx - 1, nby - 1, nbz - 1), (0, nby - 1, 0), (nbx - 1, 0, nbz - 1),
                (nbx // 2, nby // 2, nbz // 2), (nbx // 3, (2 * nby) // 3, nbz - 1), (nbx - 1, nby // 2, 0),
                (nbx // 2, 0, nbz // 2)]
     sample = sorted({bx + nbx * (by + nby * bz) for bx, by, bz in corners})
     w = Warper(0)
-    log("streaming", size, "uint16 in", BRICK_S, "^3 bricks,", threads, "host threads")
+    log("streaming", size, "uint16 in", brick, "^3 bricks,", threads, "host threads")
     t0 = time.perf_counter()
-    sums, kept, st = w.warp_stream(size, np.uint16, tar, sub, src, f=G_S, brick=BRICK_S, lam=0.2,
+    sums, kept, st = w.warp_stream(size, np.uint16, tar, sub, src, f=G_S, brick=brick, lam=0.2,
                                    precision=warp.MIXED, threads=threads, keep=sample)
     wall = time.perf_counter() - t0
     vox = float(size[0]) * size[1] * size[2]
     log("stream done in %.1f s" % wall)
     checks = []
     try:
+        if args.no_cpu:
+            raise RuntimeError("oracle leg skipped (--no-cpu)")
         from oracle import oracle
         log("oracle fit (bordered FP64 LU)")
         c, d = oracle.stps_fit(tar, sub, nthreads=threads, bordered=True)
         halo = int(DMAX_S) + 8
         for b in sample:
             bx, by, bz = b % nbx, (b // nbx) % nby, b // (nbx * nby)
-            o = (bx * BRICK_S, by * BRICK_S, bz * BRICK_S)
-            n = tuple(min(BRICK_S, size[a] - o[a]) for a in range(3))
+            o = (bx * brick, by * brick, bz * brick)
+            n = tuple(min(brick, size[a] - o[a]) for a in range(3))
             lo = tuple(max(0, o[a] - halo) for a in range(3))
             hi = tuple(min(size[a], o[a] + n[a] + halo) for a in range(3))
             win = src.fill(lo[0], lo[1], lo[2], hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2])
@@ -476,7 +479,7 @@ def stream_run(args):
         "config": {"workload": "streaming: %dx%dx%d uint16 (%.0f GiB) procedural blob+noise volume generated "
                                "per brick window on the host, K=%d |d|<=%g, grid %d, %d^3 bricks, exact "
                                "per-brick source windows, mixed form" % (size + (vox * 2 / 2 ** 30, K_S, DMAX_S,
-                                                                             G_S, BRICK_S)),
+                                                                             G_S, brick)),
                    "host_threads": threads},
         "wall_s": wall, "gvox_s": vox / wall / 1e9,
         "stats": st,
@@ -485,7 +488,10 @@ def stream_run(args):
                          "frac_of_8tbs": b_alg / (st["resample_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "pcie": {"h2d_gb": st["h2d_bytes"] / 1e9, "d2h_gb": st["d2h_bytes"] / 1e9,
                  "h2d_over_volume": st["h2d_bytes"] / (2 * vox)},
-        "host": {"read_s_per_brick": st["read_s"] / st["bricks"], "write_s_per_brick": st["write_s"] / st["bricks"]},
+        "host": {"read_s_per_brick": st["read_s"] / st["bricks"], "write_s_per_brick": st["write_s"] / st["bricks"],
+                 "issue_ms_per_brick": st["issue_s"] / st["bricks"] * 1e3,
+                 "wall_ms_per_brick": wall / st["bricks"] * 1e3,
+                 "resample_ms_per_brick": st["resample_ms"] / st["bricks"]},
         "checksums_sha256": hashlib.sha256(sums.tobytes()).hexdigest(),
         "oracle_checks": checks,
     }
@@ -792,6 +798,7 @@ def main():
                     help="the batched uneven-instance configuration (256 warps in one call)")
     ap.add_argument("--stream", action="store_true",
                     help="the streaming configuration (8192x8192x2048 uint16, generated per brick on the host)")
+    ap.add_argument("--stream-brick", type=int, default=None, help="brick edge of --stream (default 512)")
     ap.add_argument("--stream-size", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
                     help="volume size of --stream (default 8192 8192 2048)")
     ap.add_argument("--profile-layer", action="store_true",

@@ -375,6 +375,9 @@ typedef struct {
     double  read_s, write_s;      /* time spent in the callbacks (summed over bricks) */
     double  h2d_bytes, d2h_bytes; /* bytes copied */
     double  max_margin;           /* largest source margin used (voxels past the brick) */
+    double  issue_s;              /* host time the issuing thread spent enqueuing the bricks'
+                                     copies, kernels and events (its waits on the reader and
+                                     writer excluded) */
 } tw_stream_stats;
 
 int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* desc, tw_stream_stats* stats);

@@ -106,7 +106,8 @@ class TwStreamDesc(ctypes.Structure):
 class TwStreamStats(ctypes.Structure):
     _fields_ = [("bricks", _i64), ("wall_s", ctypes.c_double), ("setup_ms", ctypes.c_double),
                 ("resample_ms", ctypes.c_double), ("read_s", ctypes.c_double), ("write_s", ctypes.c_double),
-                ("h2d_bytes", ctypes.c_double), ("d2h_bytes", ctypes.c_double), ("max_margin", ctypes.c_double)]
+                ("h2d_bytes", ctypes.c_double), ("d2h_bytes", ctypes.c_double), ("max_margin", ctypes.c_double),
+                ("issue_s", ctypes.c_double)]
 
 
 class TwSynth(ctypes.Structure):

@@ -343,12 +343,20 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
         }
     });
     // Issuing (this thread): upload, resample, download.
-    double h2d_bytes = 0, d2h_bytes = 0;
+    double h2d_bytes = 0, d2h_bytes = 0, wait_s = 0;
+    // The issuing thread's waits on the reader and writer (excluded from issue_s).
+    auto wait_for = [&](auto pred) {
+        const double t0 = now_s();
+        const bool ok = S.wait(pred);
+        wait_s += now_s() - t0;
+        return ok;
+    };
+    const double t_issue = now_s();
     auto issue = [&]() -> int {
         for (i64 b = 0; b < nb; b++) {
             const int s = (int)(b % NSLOT);
             const Brick& k = bricks[(size_t)b];
-            if (!S.wait([&] { return S.ready >= b; })) return S.err;
+            if (!wait_for([&] { return S.ready >= b; })) return S.err;
             const size_t sb = (size_t)(k.snx * k.sny * k.snz * bpv), db = (size_t)(k.nx * k.ny * k.nz * bpv);
             // The device window of slot s is free once brick b - NSLOT's resample ended.
             if (b >= NSLOT) TW_HIP(ctx, hipStreamWaitEvent(P.h2d, P.ev_end[(b - NSLOT) % NEV], 0));
@@ -368,7 +376,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
             if (r != TW_OK) return r;
             TW_HIP(ctx, hipEventRecord(P.ev_end[b % NEV], ctx->stream));
             // The pinned brick of slot s is free once the writer took brick b - NSLOT.
-            if (b >= NSLOT && !S.wait([&] { return S.written >= b - NSLOT; })) return S.err;
+            if (b >= NSLOT && !wait_for([&] { return S.written >= b - NSLOT; })) return S.err;
             TW_HIP(ctx, hipStreamWaitEvent(P.d2h, P.ev_end[b % NEV], 0));
             TW_HIP(ctx, hipMemcpyAsync(P.hdst[s], P.ddst[s], db, hipMemcpyDeviceToHost, P.d2h));
             TW_HIP(ctx, hipEventRecord(P.ev_d2h[s], P.d2h));
@@ -378,6 +386,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
         return TW_OK;
     };
     rc = issue();
+    const double issue_s = now_s() - t_issue - wait_s;
     if (rc != TW_OK && S.err == TW_OK) S.fail(rc, ctx->err);
     reader.join();
     writer.join();
@@ -392,6 +401,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
         stats->h2d_bytes = h2d_bytes;
         stats->d2h_bytes = d2h_bytes;
         stats->max_margin = max_margin;
+        stats->issue_s = issue_s;
     }
     return TW_OK;
 }

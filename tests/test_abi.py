@@ -39,6 +39,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(L.TwVolume) == 8 + 10 * 8 + 2 * 4
     assert L.TwGrid.qtab.offset == 8 + 3 * 8 + 2 * 4 + 3 * 8 + 2 * 4
     assert ctypes.sizeof(L.TwTimings) == 7 * 8 + 2 * 4 + 8
+    assert ctypes.sizeof(L.TwStreamStats) == 8 + 9 * 8 and L.TwStreamStats.issue_s.offset == 8 + 8 * 8
 
 
 def test_version_and_grid_dim():

@@ -53,6 +53,8 @@ def test_stream_equals_in_memory_warp(warper, precision, interp):
         assert int(sums[b]) == checksum64(ref[box])
     # Exact windows: no wider than the field's reach plus the rounding margin.
     assert 0 < st["max_margin"] <= 6 + 4
+    # The issuing thread's own enqueue time lies inside the run's wall time.
+    assert 0 < st["issue_s"] < st["wall_s"]
 
 
 def test_stream_exact_checksums_match_oracle(warper, oracle):
