@@ -249,8 +249,14 @@ def precision_run(args):
                                   "|d|<=%g, grid %d" % (SIZE_P + (K_P, DMAX_P, G_P)),
                       "oracle_sample": "full fit; grid nodes and resample of z-slices [%d,%d)" % (zb0 * G_P, zb1 * G_P)},
            "oracle_fit_s": t_fit}
-    for name, prec in (("plain", warp.PLAIN), ("mixed", warp.MIXED)):
+    fp32_min = w.get_tune("fp32_factor_min")
+    # The third form is the mixed one with its FP32 factor (matrix-core
+    # trailing updates, FP64 refinement) forced at this order: below the
+    # measured crossover the mixed form factors in FP64.
+    for name, prec, fmin in (("plain", warp.PLAIN, fp32_min), ("mixed", warp.MIXED, fp32_min),
+                             ("mixed_fp32_factor", warp.MIXED, 1)):
         log(name, "form")
+        w.set_tune("fp32_factor_min", fmin)
         keys = ("fit_ms", "grid_ms", "quant_ms", "resample_ms", "total_ms", "refine_iters")
         rec = {k: [] for k in keys}
         w.warp_tps(src, tar, sub, f=G_P, lam=0.2, precision=prec, dst=dst)   # warm-up
@@ -288,6 +294,7 @@ def precision_run(args):
             "lsb_max": int(diff.max()),
         }
         del grid, dense
+    w.set_tune("fp32_factor_min", fp32_min)
     w.close()
     print(json.dumps(out))
 
@@ -351,6 +358,25 @@ def batch_run(args):
     st = {k: float(np.median([t[k] for t in stages])) for k in ("fit_ms", "grid_ms", "quant_ms", "resample_ms",
                                                                    "total_ms")}
     out_batch = [d.clone() for d in dsts]
+    # Each kind's items as a batch of their own: the resample launch's rate
+    # per kind (the same tiles as in the mixed batch).
+    per_kind = {}
+    for kind in ("tps", "affine", "rigid"):
+        sel = [i for i, sp in enumerate(specs) if sp["kind"] == kind]
+        if not sel:
+            continue
+        sub = [items[i] for i in sel]
+        w.warp_batch(sub, f=G_B, lam=0.2, precision=prec)
+        ms = []
+        for _ in range(args.steps):
+            w.warp_batch(sub, f=G_B, lam=0.2, precision=prec)
+            ms.append(w.timings()["resample_ms"])
+        r = float(np.median(ms))
+        b = float((2 * bpv[sel] * vox[sel]).sum() + (6 * nodes[sel]).sum())
+        per_kind[kind] = {"items": len(sel), "voxels": float(vox[sel].sum()), "resample_ms": r,
+                          "gvox_s": float(vox[sel].sum()) / (r * 1e-3) / 1e9, "b_alg_bytes": b,
+                          "gbs": b / (r * 1e-3) / 1e9, "frac_of_8tbs": b / (r * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log("kind", kind, per_kind[kind]["gbs"], "GB/s")
     # The same warps as single-volume calls, one after another.
     log("sequential single-volume calls")
 
@@ -403,6 +429,7 @@ def batch_run(args):
         "stages_ms": st,
         "resample": {"b_alg_bytes": b_alg, "gbs": b_alg / (st["resample_ms"] * 1e-3) / 1e9,
                      "frac_of_8tbs": b_alg / (st["resample_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "resample_per_kind": per_kind,
         "latency_ms": {"p50": float(np.percentile(done, 50)), "p99": float(np.percentile(done, 99)),
                        "max": float(np.max(done)), "from": "the batch's first kernel (device clock)"},
         "sequential": {"gvox_s": total_vox / seq_wall / 1e9, "ms": seq_wall * 1e3},
@@ -463,8 +490,12 @@ def stream_run(args):
             df = oracle.grid_eval(tar, c, d, gsh, G_S, origin=o, nthreads=threads)
             ref = oracle.tps_resample_window(win, lo, size, df, G_S, o, n, o, nthreads=threads)
             dd = np.abs(kept[b].astype(np.int64) - ref.astype(np.int64))
+            band = face_band(oracle.dense_df(n, df, G_S, nthreads=threads), size, o)
+            off = dd[~band]
             checks.append({"brick": b, "origin_xyz": list(o), "lsb_exact_frac": float(np.mean(dd == 0)),
                            "lsb_le1_frac": float(np.mean(dd <= 1)), "lsb_max": int(dd.max()),
+                           "lsb_max_off_face_band": int(off.max()) if off.size else 0,
+                           "face_band_voxels": int(band.sum()), "face_band_flips": int(np.count_nonzero(dd[band] > 1)),
                            "checksum_equal": bool(int(sums[b]) == warp.checksum64(ref))})
             log("brick", b, checks[-1]["lsb_le1_frac"], checks[-1]["lsb_max"])
             del win, df, ref
