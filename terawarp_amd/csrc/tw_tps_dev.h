@@ -44,7 +44,11 @@ constexpr int TX = 64, TY = 4;  // reference-order kernel block
 #ifndef TW_TZ_CELLS
 #define TW_TZ_CELLS 4
 #endif
-constexpr int FTY = TW_FUSED_TY, FT = TX * FTY;
+#ifndef TW_FUSED_RY
+#define TW_FUSED_RY 2
+#endif
+// FRY columns per thread: a tile is TX x FTH voxels in (x, y).
+constexpr int FTY = TW_FUSED_TY, FT = TX * FTY, FRY = TW_FUSED_RY, FTH = FTY * FRY;
 
 // ---------------------------------------------------------------------------
 // Shared per-voxel resample tail.
@@ -260,8 +264,45 @@ __device__ __forceinline__ Tap tap_setup(const FastGeom& g, float p0, float p1, 
     return t;
 }
 
-// Trilinear blend of one voxel from its four (x, x+1) pairs: p00 / p10 the
-// lower plane's rows y / y+1, p01 / p11 the upper plane's.
+// TW_MIXED blend.  The x step of each (x, x+1) pair is exact integer
+// arithmetic on the matrix-free dot unit: the pair as two 16-bit lanes
+// (v(x), v(x+1)) against the weights (2^15 - q, q), q = round(wx * 2^15),
+// in one v_dot2_u32_u16 (below 65535 * 2^15, so no overflow).  The y and z
+// steps are FP32 lerps of those sums, and the result is scaled back by 2^-15
+// and rounded half up.  q carries the x fraction to 2^-16 voxel, far inside
+// the FP16 field's own error.
+typedef unsigned short tw_u16x2 __attribute__((ext_vector_type(2)));
+
+// (2^15 - q) | q << 16: fma(wx, 2^15, 2^23) holds q in its low 24 bits for
+// wx in [0, 1] (round to nearest even), which the 24-bit multiply reads.
+__device__ __forceinline__ uint32_t wx_pair(float wx) {
+    const uint32_t q = __float_as_uint(__builtin_fmaf(wx, 32768.0f, 8388608.0f)) & 0xFFFFFFu;
+    return q * 0xFFFFu + 0x8000u;
+}
+
+// A pair as two 16-bit lanes: uint16 pairs already are; a uint8 pair (bits
+// 0-7, 8-15) is spread to bits 0-7 and 16-23.
+template <typename T>
+__device__ __forceinline__ uint32_t pair16(uint32_t p) {
+    if (sizeof(T) == 1) return __builtin_amdgcn_perm(0u, p, 0x0c010c00u);
+    return p;
+}
+
+__device__ __forceinline__ float xlerp_q15(uint32_t p16, uint32_t w) {
+    return (float)__builtin_amdgcn_udot2(__builtin_bit_cast(tw_u16x2, p16), __builtin_bit_cast(tw_u16x2, w), 0u,
+                                         false);
+}
+
+__device__ __forceinline__ uint32_t blend_mixed(uint32_t q00, uint32_t q10, uint32_t q01, uint32_t q11,
+                                                const Tap& t) {
+    const uint32_t w = wx_pair(t.wx);
+    const float a00 = xlerp_q15(q00, w), a10 = xlerp_q15(q10, w);
+    const float a01 = xlerp_q15(q01, w), a11 = xlerp_q15(q11, w);
+    const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
+    const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
+    return cvt_rpi(__builtin_fmaf(t.wz, b1 - b0, b0) * (1.0f / 32768.0f));
+}
+
 // Trilinear blend of the eight taps u[xyz] (x, y, z = 0 / 1).
 template <int MODE>
 __device__ __forceinline__ uint32_t blend8(uint32_t u000, uint32_t u100, uint32_t u010, uint32_t u110,
@@ -277,22 +318,19 @@ __device__ __forceinline__ uint32_t blend8(uint32_t u000, uint32_t u100, uint32_
         const double lo = tw * (lw * (double)u001 + rw * (double)u101) + bw * (lw * (double)u011 + rw * (double)u111);
         return (uint32_t)(uw * hi + dw * lo + 0.5);
     }
-    const float v000 = (float)u000, v100 = (float)u100, v010 = (float)u010, v110 = (float)u110;
-    const float v001 = (float)u001, v101 = (float)u101, v011 = (float)u011, v111 = (float)u111;
-    const float a00 = __builtin_fmaf(t.wx, v100 - v000, v000);
-    const float a10 = __builtin_fmaf(t.wx, v110 - v010, v010);
-    const float a01 = __builtin_fmaf(t.wx, v101 - v001, v001);
-    const float a11 = __builtin_fmaf(t.wx, v111 - v011, v011);
-    const float b0 = __builtin_fmaf(t.wy, a10 - a00, a00);
-    const float b1 = __builtin_fmaf(t.wy, a11 - a01, a01);
-    return cvt_rpi(__builtin_fmaf(t.wz, b1 - b0, b0));
+    return blend_mixed(u000 | u100 << 16, u010 | u110 << 16, u001 | u101 << 16, u011 | u111 << 16, t);
 }
 
+// Trilinear blend of one voxel from its four (x, x+1) pairs: p00 / p10 the
+// lower plane's rows y / y+1, p01 / p11 the upper plane's.  Equal to blend8
+// on the unpacked taps.
 template <typename T, int MODE>
 __device__ __forceinline__ uint32_t blend_pairs(uint32_t p00, uint32_t p10, uint32_t p01, uint32_t p11,
                                                 const Tap& t) {
-    return blend8<MODE>(pair_lo<T>(p00), pair_hi<T>(p00), pair_lo<T>(p10), pair_hi<T>(p10), pair_lo<T>(p01),
-                        pair_hi<T>(p01), pair_lo<T>(p11), pair_hi<T>(p11), t);
+    if (MODE == TW_PLAIN)
+        return blend8<MODE>(pair_lo<T>(p00), pair_hi<T>(p00), pair_lo<T>(p10), pair_hi<T>(p10), pair_lo<T>(p01),
+                            pair_hi<T>(p01), pair_lo<T>(p11), pair_hi<T>(p11), t);
+    return blend_mixed(pair16<T>(p00), pair16<T>(p10), pair16<T>(p01), pair16<T>(p11), t);
 }
 
 // The eight taps of a voxel as four (x, x+1) pairs: y+1 / z+1 through the
@@ -310,9 +348,8 @@ __device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
 // Four consecutive slices of one column (trilinear, every tap in bounds):
 // where slice j's offset is slice j-1's plus one plane, its lower plane is
 // slice j-1's upper plane (the same bytes), so those two pairs are not
-// loaded again; the other lanes load them under an exec mask.  (have, poff,
-// pu0, pu1) carry the previous group's last slice the same way.  Results
-// are those of blend.
+// fetched again.  (have, poff, pu0, pu1) carry the previous group's last
+// slice the same way.  Results are those of blend.
 template <typename T, int MODE>
 __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v,
                                               bool& have, uint32_t& poff, uint32_t& pu0, uint32_t& pu1) {
@@ -323,24 +360,26 @@ __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, ui
         up0[j] = bld2<T>(rs.r10, tp[j].off);
         up1[j] = bld2<T>(rs.r11, tp[j].off);
     }
-    {
-        uint32_t a = pu0, b = pu1;
-        if (!(have && tp[0].off == poff + plane_b)) {
-            a = bld2<T>(rs.r00, tp[0].off);
-            b = bld2<T>(rs.r01, tp[0].off);
-        }
-        lo0[0] = a;
-        lo1[0] = b;
+    // The lower-plane pairs are loaded by every lane, at a past-the-end
+    // offset (reads 0) where the lane takes them from the slice before, and
+    // selected after the loads: so no load waits for an earlier one.
+    bool need[4];
+    uint32_t la[4], lb[4];
+    need[0] = !(have && tp[0].off == poff + plane_b);
+    #pragma unroll
+    for (int j = 1; j < 4; j++) need[j] = tp[j].off != tp[j - 1].off + plane_b;
+    #pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t off = need[j] ? tp[j].off : OOB_OFF;
+        la[j] = bld2<T>(rs.r00, off);
+        lb[j] = bld2<T>(rs.r01, off);
     }
+    lo0[0] = need[0] ? la[0] : pu0;
+    lo1[0] = need[0] ? lb[0] : pu1;
     #pragma unroll
     for (int j = 1; j < 4; j++) {
-        uint32_t a = up0[j - 1], b = up1[j - 1];
-        if (tp[j].off != tp[j - 1].off + plane_b) {
-            a = bld2<T>(rs.r00, tp[j].off);
-            b = bld2<T>(rs.r01, tp[j].off);
-        }
-        lo0[j] = a;
-        lo1[j] = b;
+        lo0[j] = need[j] ? la[j] : up0[j - 1];
+        lo1[j] = need[j] ? lb[j] : up1[j - 1];
     }
     have = true;
     poff = tp[3].off;
@@ -378,8 +417,8 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
     const int lxi = threadIdx.x, lyi = threadIdx.y, tid = lyi * TX + lxi;
     const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
     const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
-    const int tx0 = tix * TX, ty0 = tiy * FTY, tz0 = tiz * a.tz;
-    const int tx1 = min(tx0 + TX, dnx) - 1, ty1 = min(ty0 + FTY, dny) - 1, tz1 = min(tz0 + a.tz, dnz) - 1;
+    const int tx0 = tix * TX, ty0 = tiy * FTH, tz0 = tiz * a.tz;
+    const int tx1 = min(tx0 + TX, dnx) - 1, ty1 = min(ty0 + FTH, dny) - 1, tz1 = min(tz0 + a.tz, dnz) - 1;
     // Window-relative coordinates of the grid (r = global - origin >= 0).
     const int rx0 = (int)(a.dx0 - a.ox), ry0 = (int)(a.dy0 - a.oy), rz0 = (int)(a.dz0 - a.oz);
     const int nx0 = (rx0 + tx0) / f, ny0 = (ry0 + ty0) / f, nz0 = (rz0 + tz0) / f;
@@ -466,64 +505,8 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
         }
     }
     __syncthreads();
-    if (tx0 + lxi > tx1 || ty0 + lyi > ty1) return;
+    if (tx0 + lxi > tx1) return;
 
-    const int lx = tx0 + lxi, ly = ty0 + lyi;
-    const float Xf = (float)((int)a.dx0 + lx), Yf = (float)((int)a.dy0 + ly);
-    int byl, typ;
-    split(ry0 + ty0, lyi, ny0, byl, typ);
-    const float wy0 = g.wt[typ * 4 + 0], wy1 = g.wt[typ * 4 + 1], wy2 = g.wt[typ * 4 + 2], wy3 = g.wt[typ * 4 + 3];
-    const uint32_t dvol = (uint32_t)dnx * (uint32_t)dny * (uint32_t)dnz;
-
-    // Layer k of the thread's column: xcol + k * lstep (+ c * cstep).
-    const float* xcol = xc + byl * TX + lxi;
-    const int lstep = NYM * TX, cstep = NZM * NYM * TX;
-    auto layer = [&](int iz, float* L) {
-        const float* col = xcol + iz * lstep;
-        #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float* cc = col + c * cstep;
-            L[c] = __builtin_fmaf(wy3, cc[3 * TX], __builtin_fmaf(wy2, cc[2 * TX], __builtin_fmaf(wy1, cc[TX], wy0 * cc[0])));
-        }
-    };
-    // z walk: the phase advances by one per slice, the node window by one
-    // per f slices (wave-uniform).
-    int rz = rz0 + tz0;
-    int bzl = rz / f - nz0;
-    int tzp = rz - (rz / f) * f;
-    float L0[3], L1[3], L2[3], L3[3];
-    layer(bzl, L0); layer(bzl + 1, L1); layer(bzl + 2, L2); layer(bzl + 3, L3);
-    const int nzv = tz1 - tz0 + 1;
-    const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
-    uint32_t o = (uint32_t)tz0 * ostep + (uint32_t)ly * (uint32_t)dnx + (uint32_t)lx;
-    float Zf = (float)((int)a.dz0 + tz0);
-
-    auto df_ph = [&](float* d, int ph) {
-        const float wz0 = g.wt[ph * 4 + 0], wz1 = g.wt[ph * 4 + 1], wz2 = g.wt[ph * 4 + 2], wz3 = g.wt[ph * 4 + 3];
-        #pragma unroll
-        for (int c = 0; c < 3; c++)
-            d[c] = __builtin_fmaf(wz3, L3[c], __builtin_fmaf(wz2, L2[c], __builtin_fmaf(wz1, L1[c], wz0 * L0[c])));
-    };
-    auto next_cell = [&]() {
-        tzp = 0;
-        bzl++;
-        #pragma unroll
-        for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
-        layer(bzl + 3, L3);
-    };
-
-    if (WRITE_DF) {
-        for (int lz = 0; lz < nzv; lz++) {
-            float d[3];
-            df_ph(d, tzp);
-            a.dfout[o] = d[0];
-            a.dfout[dvol + o] = d[1];
-            a.dfout[2 * dvol + o] = d[2];
-            o += ostep;
-            if (++tzp == f && lz + 1 < nzv) next_cell();
-        }
-        return;
-    }
     // Along axis a the samples of this tile lie in [X + dmin_a, X + dmax_a]
     // (+ one voxel for rounding), the tile's nodes bounding the field (a
     // convex combination of them).  Cells whose samples all stay a voxel
@@ -553,61 +536,128 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
     // One channel per launch (the host offsets src / dst per channel).
     const SrcRs rs = make_src_rs(src, g);
     const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (int)g.dchan_b, BUF_FLAGS);
-    // Cell by cell: the node-layer window moves once per cell; inside a
-    // cell four slices per iteration keep 32 taps in flight (a short last
-    // group repeats its last slice and stores only the real ones).  The
-    // last slice's upper-plane pairs carry to the next group.
-    bool have = false;
-    uint32_t poff = 0, pu0 = 0, pu1 = 0;
-    auto group = [&](auto check, auto full, int nh) {
-        constexpr bool CHECK = decltype(check)::value, FULL = decltype(full)::value;
-        Tap tp[4];
-        #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int jj = FULL ? j : min(j, nh - 1);
-            float d[3];
-            df_ph(d, tzp + jj);
-            const float zj = FULL ? Zf + (float)j : Zf + (float)jj;
-            tp[j] = tap_setup<T, INTERP, WIN, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2]);
-        }
-        uint32_t v[4];
-        if constexpr (!CHECK && INTERP != TW_INTERP_NN) {
-            blend4_shared<T, MODE>(rs, tp, g.plane_b, v, have, poff, pu0, pu1);
-        } else {
+    const int lx = tx0 + lxi;
+    const float Xf = (float)((int)a.dx0 + lx);
+    const uint32_t dvol = (uint32_t)dnx * (uint32_t)dny * (uint32_t)dnz;
+    const int nzv = tz1 - tz0 + 1;
+    const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
+    const float* xlane = xc + lxi;
+    const int lstep = NYM * TX, cstep = NZM * NYM * TX;
+
+    // The thread's FRY columns, rows lyi, lyi + FTY, ... of the tile, one
+    // after the other; every column walks the tile's depth.
+    for (int ry = lyi; ry < FTH; ry += FTY) {
+        const int ly = ty0 + ry;
+        if (ly > ty1) break;
+        const float Yf = (float)((int)a.dy0 + ly);
+        int byl, typ;
+        split(ry0 + ty0, ry, ny0, byl, typ);
+        const float wy0 = g.wt[typ * 4 + 0], wy1 = g.wt[typ * 4 + 1], wy2 = g.wt[typ * 4 + 2], wy3 = g.wt[typ * 4 + 3];
+
+        // Layer k of the column: xcol + k * lstep (+ c * cstep).
+        const float* xcol = xlane + byl * TX;
+        auto layer = [&](int iz, float* L) {
+            const float* col = xcol + iz * lstep;
             #pragma unroll
-            for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
-            have = false;
+            for (int c = 0; c < 3; c++) {
+                const float* cc = col + c * cstep;
+                L[c] = __builtin_fmaf(wy3, cc[3 * TX],
+                                      __builtin_fmaf(wy2, cc[2 * TX], __builtin_fmaf(wy1, cc[TX], wy0 * cc[0])));
+            }
+        };
+        // z walk: the phase advances by one per slice, the node window by one
+        // per f slices (wave-uniform).
+        const int rz = rz0 + tz0;
+        int bzl = rz / f - nz0;
+        int tzp = rz - (rz / f) * f;
+        float L0[3], L1[3], L2[3], L3[3];
+        layer(bzl, L0); layer(bzl + 1, L1); layer(bzl + 2, L2); layer(bzl + 3, L3);
+        uint32_t o = (uint32_t)tz0 * ostep + (uint32_t)ly * (uint32_t)dnx + (uint32_t)lx;
+        float Zf = (float)((int)a.dz0 + tz0);
+
+        auto df_ph = [&](float* d, int ph) {
+            const float wz0 = g.wt[ph * 4 + 0], wz1 = g.wt[ph * 4 + 1], wz2 = g.wt[ph * 4 + 2],
+                        wz3 = g.wt[ph * 4 + 3];
+            #pragma unroll
+            for (int c = 0; c < 3; c++)
+                d[c] = __builtin_fmaf(wz3, L3[c], __builtin_fmaf(wz2, L2[c], __builtin_fmaf(wz1, L1[c], wz0 * L0[c])));
+        };
+        auto next_cell = [&]() {
+            tzp = 0;
+            bzl++;
+            #pragma unroll
+            for (int c = 0; c < 3; c++) { L0[c] = L1[c]; L1[c] = L2[c]; L2[c] = L3[c]; }
+            layer(bzl + 3, L3);
+        };
+
+        if (WRITE_DF) {
+            for (int lz = 0; lz < nzv; lz++) {
+                float d[3];
+                df_ph(d, tzp);
+                a.dfout[o] = d[0];
+                a.dfout[dvol + o] = d[1];
+                a.dfout[2 * dvol + o] = d[2];
+                o += ostep;
+                if (++tzp == f && lz + 1 < nzv) next_cell();
+            }
+            continue;
         }
-        #pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (FULL || j < nh) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
-        tzp += nh;
-        o += (uint32_t)nh * ostep;
-        Zf += (float)nh;
-    };
-    auto cell = [&](auto check, int cnt) {
-        int k = 0;
-        for (; k + 4 <= cnt; k += 4) group(check, std::true_type{}, 4);
-        if (k < cnt) group(check, std::false_type{}, cnt - k);
-    };
-    int lz = 0;
-    while (true) {
-        const int cnt = min(f - tzp, nzv - lz);
-        const int zc = (int)a.dz0 + tz0 + lz;  // first slice of the cell
-        if (fast_xy && zc - Rlo[2] >= loz && zc + cnt - 1 + Rhi[2] <= hiz)
-            cell(std::false_type{}, cnt);
-        else
-            cell(std::true_type{}, cnt);
-        lz += cnt;
-        if (lz >= nzv) break;
-        next_cell();
+        // Cell by cell: the node-layer window moves once per cell; inside a
+        // cell four slices per iteration keep 32 taps in flight (a short last
+        // group repeats its last slice and stores only the real ones).  The
+        // last slice's upper-plane pairs carry to the next group.
+        bool have = false;
+        uint32_t poff = 0, pu0 = 0, pu1 = 0;
+        auto group = [&](auto check, auto full, int nh) {
+            constexpr bool CHECK = decltype(check)::value, FULL = decltype(full)::value;
+            Tap tp[4];
+            #pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int jj = FULL ? j : min(j, nh - 1);
+                float d[3];
+                df_ph(d, tzp + jj);
+                const float zj = FULL ? Zf + (float)j : Zf + (float)jj;
+                tp[j] = tap_setup<T, INTERP, WIN, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2]);
+            }
+            uint32_t v[4];
+            if constexpr (!CHECK && INTERP != TW_INTERP_NN) {
+                blend4_shared<T, MODE>(rs, tp, g.plane_b, v, have, poff, pu0, pu1);
+            } else {
+                #pragma unroll
+                for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
+                have = false;
+            }
+            #pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (FULL || j < nh) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
+            tzp += nh;
+            o += (uint32_t)nh * ostep;
+            Zf += (float)nh;
+        };
+        auto cell = [&](auto check, int cnt) {
+            int k = 0;
+            for (; k + 4 <= cnt; k += 4) group(check, std::true_type{}, 4);
+            if (k < cnt) group(check, std::false_type{}, cnt - k);
+        };
+        int lz = 0;
+        while (true) {
+            const int cnt = min(f - tzp, nzv - lz);
+            const int zc = (int)a.dz0 + tz0 + lz;  // first slice of the cell
+            if (fast_xy && zc - Rlo[2] >= loz && zc + cnt - 1 + Rhi[2] <= hiz)
+                cell(std::false_type{}, cnt);
+            else
+                cell(std::true_type{}, cnt);
+            lz += cnt;
+            if (lz >= nzv) break;
+            next_cell();
+        }
     }
 }
 
 // ---- launch shaping (host) ----------------------------------------------------
 
 inline size_t fused_lds_bytes(int f, int tz) {
-    const int nxm = node_span(TX, f), nym = node_span(FTY, f), nzm = node_span(tz, f);
+    const int nxm = node_span(TX, f), nym = node_span(FTH, f), nzm = node_span(tz, f);
     return sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
 }
 
@@ -653,15 +703,16 @@ inline void fill_weights(FastGeom& g, int f, int basis) {
 // Geometry block of the fused kernel for the arguments a (a.tz set): node-tile
 // extents, LDS bytes per work-group, strides, tile grid.  Returns null, or
 // the limit the arguments break.
+// (gather_lds = false: the caller runs another tile and sizes its own LDS.)
 template <typename T, bool WRITE_DF>
-const char* fused_geom(const TpsArgs& a, FastGeom& g, size_t& lds) {
+const char* fused_geom(const TpsArgs& a, FastGeom& g, size_t& lds, bool gather_lds = true) {
     // Exact node-tile extents of this launch (tiles aligned to the grid need
     // fewer nodes than the worst case), which sets the LDS per work-group.
     const int nxm = max_node_span(a.dx0 - a.ox, TX, a.dnx, a.f);
-    const int nym = max_node_span(a.dy0 - a.oy, FTY, a.dny, a.f);
+    const int nym = max_node_span(a.dy0 - a.oy, FTH, a.dny, a.f);
     const int nzm = max_node_span(a.dz0 - a.oz, a.tz, a.dnz, a.f);
     lds = sizeof(float) * (size_t)(3 * nzm * nym * nxm + 3 * nzm * nym * TX);
-    if (lds > 160 * 1024) return "the grid spacing needs more than 160 KiB of LDS per work-group";
+    if (gather_lds && lds > 160 * 1024) return "the grid spacing needs more than 160 KiB of LDS per work-group";
     const i64 nbz = (a.dnz + a.tz - 1) / a.tz;
     // 32-bit addressing limits of the fused kernel: byte offsets of every tap
     // below 2 GiB (so the x+1 taps fold into the immediate offset), 24-bit
@@ -690,7 +741,7 @@ const char* fused_geom(const TpsArgs& a, FastGeom& g, size_t& lds) {
     g.dchan_b = (uint32_t)(a.dnx * a.dny * a.dnz * sizeof(T));
     fill_weights(g, a.f, a.basis);
     g.tiles_x = (int)((a.dnx + TX - 1) / TX);
-    g.tiles_y = (int)((a.dny + FTY - 1) / FTY);
+    g.tiles_y = (int)((a.dny + FTH - 1) / FTH);
     const i64 ntiles = (i64)g.tiles_x * g.tiles_y * nbz;
     if (ntiles > (1ll << 30)) return "too many tiles for one launch";
     g.ntiles = (int)ntiles;

@@ -161,7 +161,7 @@ const char* fused_geom_lds(TpsArgs& a, FastGeom& g, size_t& lds, size_t lds_budg
     }
     a.tz = seg;
     size_t unused = 0;
-    if (const char* why = fused_geom<T, false>(a, g, unused)) return why;
+    if (const char* why = fused_geom<T, false>(a, g, unused, false)) return why;
     g.nxm = max_node_span(a.dx0 - a.ox, LTX, a.dnx, a.f);
     g.nym = max_node_span(a.dy0 - a.oy, LTY, a.dny, a.f);
     g.nzm = max_node_span(a.dz0 - a.oz, seg, a.dnz, a.f);
