@@ -961,7 +961,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": brick_kernel + " (k_resample_batch<MIXED, F16, LINEAR, windows, LDS tiles>)",
+                     "kernel": brick_kernel + " (template arguments: MIXED, FP16 grid, LINEAR, windows, "
+                               + ("LDS column tile)" if args.tile == "lds" else "gather tile)"),
                      "b_alg_per_voxel": b_mixed, "voxels_per_launch": vox},
         "cpu_baseline": cpu,
         "plain": plain,

@@ -1472,11 +1472,31 @@ constexpr int SOLVE_MULTI_MIN = 1025;
 // PIPE_MAX_BLOCKS blocks the pipelined kernel runs (one launch); above, one
 // launch per block.
 // fwd_done: the factor steps ran the forward substitution (pipelined path only).
+// The pipelined solve needs all its nb work-groups resident at once (work-
+// group j waits on blocks dispatched after it); pipe_fits says whether the
+// device holds that many (once per context and type).
+template <typename T>
+bool pipe_fits(tw_ctx* ctx, int nb) {
+    int& cap = sizeof(T) == 8 ? ctx->pipe_cap64 : ctx->pipe_cap32;
+    if (cap < 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solve_pipe<T>, PIPE_THREADS, 0) != hipSuccess)
+            per_cu = 0;
+        cap = per_cu * ctx->num_cus;
+    }
+    return nb <= cap;
+}
+
 template <typename T>
 int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const int* skip, FitState* st,
                int fwd_done = 0) {
     const int nb = (n + NB - 1) / NB;
-    if (nb <= PIPE_MAX_BLOCKS && ctx->arena) {
+    // fwd_done (at most 16 blocks, from the fused factor steps) leaves only
+    // the pipelined kernel's backward pass: the fallbacks run both passes.
+    const bool pipe = nb <= PIPE_MAX_BLOCKS && pipe_fits<T>(ctx, nb);
+    if (fwd_done && !pipe)
+        return tw_fail(ctx, TW_E_DEVICE, "the device cannot hold the %d work-groups of the pipelined solve", nb);
+    if (pipe && ctx->arena) {
         // Allocation-free mode: the flags come from the arena and are cleared
         // by a kernel before each solve (the arena keeps earlier calls' data).
         void* p = nullptr;
@@ -1485,7 +1505,7 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
         if ((rc = tw_fill_zero_i32(ctx, (int*)p, 2 * PIPE_MAX_BLOCKS))) return rc;
         hipLaunchKernelGGL(k_solve_pipe<T>, dim3(nb), dim3(PIPE_THREADS), 0, ctx->stream, L, Dinv, n, b, y, (int*)p,
                            1, st, skip, fwd_done);
-    } else if (nb <= PIPE_MAX_BLOCKS) {
+    } else if (pipe) {
         if (!ctx->solve_flags) {
             void* p = nullptr;
             TW_HIP(ctx, hipMalloc(&p, 2 * PIPE_MAX_BLOCKS * sizeof(int)));

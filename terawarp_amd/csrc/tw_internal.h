@@ -67,6 +67,7 @@ struct tw_ctx {
     double fit_residual = 0.0;
     // Device properties used for launch shaping.
     int num_cus = 256;
+    int pipe_cap64 = -1, pipe_cap32 = -1;  // resident work-groups of the pipelined solve (-1: not asked yet)
     // B-spline weight tables on the device (WS_WEIGHTS) are for spacing
     // weights_f (0: none yet); their host image must outlive the copy.
     int weights_f = 0;
