@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "tw_tps_dev.h"  // buffer-load helpers and the tile shape
 
@@ -148,12 +149,37 @@ struct ATap {
     double lw, rw, tw, bw, uw, dw;
 };
 
-template <typename T, int INTERP>
+// CHECK = false: the caller has shown that every sample of the tile lies a
+// voxel inside the volume and the window (affine_tile_interior), so the
+// bounds tests, the window test and the last-column step are left out; the
+// arithmetic of the samples is the same.
+template <typename T, int INTERP, bool CHECK = true>
 __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g, double gx, double gy, double gz) {
     ATap t;
     const double p0 = a.m[0] * gx + a.m[1] * gy + a.m[2] * gz + a.m[3] * 1.0;
     const double p1 = a.m[4] * gx + a.m[5] * gy + a.m[6] * gz + a.m[7] * 1.0;
     const double p2 = a.m[8] * gx + a.m[9] * gy + a.m[10] * gz + a.m[11] * 1.0;
+    if (!CHECK) {
+        t.ok = true;
+        t.shift = false;
+        if (INTERP == TW_INTERP_NN) {
+            const int qx = (int)(p0 + 0.5) - g.sx0, qy = (int)(p1 + 0.5) - g.sy0, qz = (int)(p2 + 0.5) - g.sz0;
+            t.off = (__umul24((uint32_t)(qz * g.sny + qy), (uint32_t)g.snx) + (uint32_t)qx) * (uint32_t)sizeof(T);
+            t.lw = t.rw = t.tw = t.bw = t.uw = t.dw = 0.0;
+            return t;
+        }
+        // floor(p) as a double equals the reference's (double)(long)floor(p).
+        const double fx = floor(p0), fy = floor(p1), fz = floor(p2);
+        t.lw = 1.0 - (p0 - fx);
+        t.rw = 1.0 - t.lw;
+        t.tw = 1.0 - (p1 - fy);
+        t.bw = 1.0 - t.tw;
+        t.uw = 1.0 - (p2 - fz);
+        t.dw = 1.0 - t.uw;
+        const int lxs = (int)fx - g.sx0, lys = (int)fy - g.sy0, lzs = (int)fz - g.sz0;
+        t.off = (__umul24((uint32_t)(lzs * g.sny + lys), (uint32_t)g.snx) + (uint32_t)lxs) * (uint32_t)sizeof(T);
+        return t;
+    }
     const bool in = !(p0 < 0 || p0 > g.fmx || p1 < 0 || p1 > g.fmy || p2 < 0 || p2 > g.fmz);
     // Outside: evaluate at 0 (keeps the conversions in range); the result is 0.
     const double q0 = in ? p0 : 0.0, q1 = in ? p1 : 0.0, q2 = in ? p2 : 0.0;
@@ -190,20 +216,47 @@ __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g
 }
 
 // The reference's blend of a tap (q_warp_affine_tps.cpp:1418-1433), truncated.
-template <typename T, int INTERP>
+template <typename T, int INTERP, bool CHECK = true>
 __device__ __forceinline__ uint32_t affine_blend(const SrcRs& rs, const ATap& t) {
     const uint32_t off = t.off;
     __builtin_assume(off <= OOB_OFF);
-    if (INTERP == TW_INTERP_NN) return t.ok ? bld<T>(rs.r00, off) : 0u;
+    if (INTERP == TW_INTERP_NN) return (!CHECK || t.ok) ? bld<T>(rs.r00, off) : 0u;
     const uint32_t p00 = bld2<T>(rs.r00, off), p10 = bld2<T>(rs.r01, off);
     const uint32_t p01 = bld2<T>(rs.r10, off), p11 = bld2<T>(rs.r11, off);
-    const double v000 = (double)(t.shift ? pair_hi<T>(p00) : pair_lo<T>(p00)), v100 = (double)pair_hi<T>(p00);
-    const double v010 = (double)(t.shift ? pair_hi<T>(p10) : pair_lo<T>(p10)), v110 = (double)pair_hi<T>(p10);
-    const double v001 = (double)(t.shift ? pair_hi<T>(p01) : pair_lo<T>(p01)), v101 = (double)pair_hi<T>(p01);
-    const double v011 = (double)(t.shift ? pair_hi<T>(p11) : pair_lo<T>(p11)), v111 = (double)pair_hi<T>(p11);
+    const bool sh = CHECK && t.shift;
+    const double v000 = (double)(sh ? pair_hi<T>(p00) : pair_lo<T>(p00)), v100 = (double)pair_hi<T>(p00);
+    const double v010 = (double)(sh ? pair_hi<T>(p10) : pair_lo<T>(p10)), v110 = (double)pair_hi<T>(p10);
+    const double v001 = (double)(sh ? pair_hi<T>(p01) : pair_lo<T>(p01)), v101 = (double)pair_hi<T>(p01);
+    const double v011 = (double)(sh ? pair_hi<T>(p11) : pair_lo<T>(p11)), v111 = (double)pair_hi<T>(p11);
     const double hi = t.tw * (t.lw * v000 + t.rw * v100) + t.bw * (t.lw * v010 + t.rw * v110);
     const double lo = t.tw * (t.lw * v001 + t.rw * v101) + t.bw * (t.lw * v011 + t.rw * v111);
-    return t.ok ? (uint32_t)(T)(t.uw * hi + t.dw * lo) : 0u;
+    return (!CHECK || t.ok) ? (uint32_t)(T)(t.uw * hi + t.dw * lo) : 0u;
+}
+
+// Whether every sample of the output box [lo, hi] (destination window
+// coordinates) lies a voxel inside the volume and the source window, with
+// the last column excluded: the map is affine, so each component's extremes
+// over the box sit at its corners; the one-voxel margin covers the rounding
+// of the per-voxel evaluation.
+__device__ __forceinline__ bool affine_box_interior(const AffineArgs& a, const AffGeom& g, const int* lo,
+                                                    const int* hi) {
+    const double c0[3] = {(double)(a.dx0 + lo[0]), (double)(a.dy0 + lo[1]), (double)(a.dz0 + lo[2])};
+    const double c1[3] = {(double)(a.dx0 + hi[0]), (double)(a.dy0 + hi[1]), (double)(a.dz0 + hi[2])};
+    const int s0[3] = {g.sx0, g.sy0, g.sz0}, sn[3] = {g.snx, g.sny, g.snz};
+    const double fm[3] = {g.fmx, g.fmy, g.fmz};
+    bool ok = true;
+    #pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double pl = a.m[4 * c + 3], ph = a.m[4 * c + 3];
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double u = a.m[4 * c + k] * c0[k], v = a.m[4 * c + k] * c1[k];
+            pl += fmin(u, v);
+            ph += fmax(u, v);
+        }
+        ok = ok && pl - 1.0 >= fmax(0.0, (double)s0[c]) && ph + 1.0 <= fmin(fm[c], (double)(s0[c] + sn[c] - 2));
+    }
+    return ok;
 }
 
 __device__ __forceinline__ SrcRs affine_src_rs(const void* src, const AffGeom& g) {
@@ -225,27 +278,36 @@ __device__ __forceinline__ void affine_tile_fast(const AffineArgs& a, const AffG
     const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
     if (x >= dnx || y >= dny) return;
     const int z0 = tiz * AFF_TZ, z1 = min(z0 + AFF_TZ, dnz);
+    // The tile's box (clipped to the destination): wave-uniform.
+    const int blo[3] = {tix * TX, tiy * FTY, z0};
+    const int bhi[3] = {min(blo[0] + TX, dnx) - 1, min(blo[1] + FTY, dny) - 1, z1 - 1};
     const double gx = (double)(a.dx0 + x), gy = (double)(a.dy0 + y);
     const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
-    for (int z = z0; z < z1; z += 4) {
-        const int nz = min(4, z1 - z);
-        ATap t[4];
-        #pragma unroll
-        for (int j = 0; j < 4; j++) t[j] = affine_tap<T, INTERP>(a, g, gx, gy, (double)(a.dz0 + z + min(j, nz - 1)));
-        const uint32_t o = (uint32_t)z * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x;
-        for (int c = c0; c < c1; c++) {
-            const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
-            const __amdgpu_buffer_rsrc_t rd =
-                __builtin_amdgcn_make_buffer_rsrc((uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b,
-                                                  BUF_FLAGS);
-            uint32_t v[4];
-            #pragma unroll
-            for (int j = 0; j < 4; j++) v[j] = affine_blend<T, INTERP>(rs, t[j]);
+    auto walk = [&](auto check) {
+        constexpr bool CHECK = decltype(check)::value;
+        for (int z = z0; z < z1; z += 4) {
+            const int nz = min(4, z1 - z);
+            ATap t[4];
             #pragma unroll
             for (int j = 0; j < 4; j++)
-                if (j < nz) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
+                t[j] = affine_tap<T, INTERP, CHECK>(a, g, gx, gy, (double)(a.dz0 + z + min(j, nz - 1)));
+            const uint32_t o = (uint32_t)z * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x;
+            for (int c = c0; c < c1; c++) {
+                const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
+                const __amdgpu_buffer_rsrc_t rd =
+                    __builtin_amdgcn_make_buffer_rsrc((uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0,
+                                                      (int)g.dchan_b, BUF_FLAGS);
+                uint32_t v[4];
+                #pragma unroll
+                for (int j = 0; j < 4; j++) v[j] = affine_blend<T, INTERP, CHECK>(rs, t[j]);
+                #pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (j < nz) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
+            }
         }
-    }
+    };
+    if (affine_box_interior(a, g, blo, bhi)) walk(std::false_type{});
+    else walk(std::true_type{});
 }
 
 }  // namespace
