@@ -45,15 +45,32 @@ __global__ __launch_bounds__(FT) void k_affine_fast(const AffineArgs a, const Af
     affine_tile_fast<T, INTERP>(a, g, tile, 0, (int)a.nc);
 }
 
+template <typename T, int INTERP>
+__global__ __launch_bounds__(FT) void k_affine_lds(const AffineArgs a, const AffGeom g) {
+    extern __shared__ __align__(16) uint8_t lds_box[];
+    const int wg = blockIdx.x;
+    const int tile = (wg & 7) * g.per_xcd + (wg >> 3);
+    if (tile >= g.ntiles) return;
+    affine_tile_lds<T, INTERP>(a, g, tile, 0, (int)a.nc, lds_box);
+}
+
 template <typename T>
 int launch_affine(tw_ctx* ctx, const AffineArgs& a, int interp) {
     AffGeom g;
-    if (a.nc <= 1024 && !affine_geom<T>(a, g)) {
+    const bool lds = affine_prefers_lds(a.m);
+    if (a.nc <= 1024 && !affine_geom<T>(a, g, lds)) {
         const dim3 grid((unsigned)(8 * g.per_xcd)), block(TX, FTY, 1);
-        if (interp == TW_INTERP_NN)
+        if (lds) {
+            const size_t sh = (size_t)g.lds_bytes;
+            if (interp == TW_INTERP_NN)
+                hipLaunchKernelGGL((k_affine_lds<T, TW_INTERP_NN>), grid, block, sh, ctx->stream, a, g);
+            else
+                hipLaunchKernelGGL((k_affine_lds<T, TW_INTERP_LINEAR>), grid, block, sh, ctx->stream, a, g);
+        } else if (interp == TW_INTERP_NN) {
             hipLaunchKernelGGL((k_affine_fast<T, TW_INTERP_NN>), grid, block, 0, ctx->stream, a, g);
-        else
+        } else {
             hipLaunchKernelGGL((k_affine_fast<T, TW_INTERP_LINEAR>), grid, block, 0, ctx->stream, a, g);
+        }
         TW_LAUNCH_CHECK(ctx);
         return TW_OK;
     }

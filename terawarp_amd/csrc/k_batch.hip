@@ -94,8 +94,14 @@ __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restric
         } else {
             const AffineArgs& af = *(const AffineArgs*)&J.af;
             const AffGeom& ag = *(const AffGeom*)&J.ag;
-            if (J.dtype == TW_U8) affine_tile_fast<uint8_t, INTERP>(af, ag, t, ch, ch + 1);
-            else affine_tile_fast<uint16_t, INTERP>(af, ag, t, ch, ch + 1);
+            if (ag.lds) {
+                if (J.dtype == TW_U8) affine_tile_lds<uint8_t, INTERP>(af, ag, t, ch, ch + 1, (uint8_t*)lds);
+                else affine_tile_lds<uint16_t, INTERP>(af, ag, t, ch, ch + 1, (uint8_t*)lds);
+            } else if (J.dtype == TW_U8) {
+                affine_tile_fast<uint8_t, INTERP>(af, ag, t, ch, ch + 1);
+            } else {
+                affine_tile_fast<uint16_t, INTERP>(af, ag, t, ch, ch + 1);
+            }
         }
     }
     __syncthreads();
@@ -246,6 +252,7 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
     size_t lds = 0;
     const int tz = pick_tz(f, ctx->tune_tz_batch);
     const bool lds_tile = tw_lds_tile(ctx, interp);
+    bool any_lds_affine = false;
     for (int i = 0; i < n; i++) {
         const tw_warp_item& it = items[i];
         BatchJob& J = hj[i];
@@ -293,13 +300,23 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
             a.dx0 = it.dst.x0; a.dy0 = it.dst.y0; a.dz0 = it.dst.z0;
             a.nc = it.src.nc;
             for (int k = 0; k < 12; k++) a.m[k] = it.M[k];
-            const char* why = it.src.dtype == TW_U8 ? affine_geom<uint8_t>(a, J.ag) : affine_geom<uint16_t>(a, J.ag);
+            // The box budget is the launch's LDS, set after every job (below).
+            const bool al = affine_prefers_lds(a.m);
+            const char* why = it.src.dtype == TW_U8 ? affine_geom<uint8_t>(a, J.ag, al) : affine_geom<uint16_t>(a, J.ag, al);
             if (why) return tw_fail(ctx, TW_E_UNSUPPORTED, "item %d: %s", i, why);
+            any_lds_affine = any_lds_affine || al;
             per_ch = J.ag.ntiles;
         }
         if (per_ch * it.src.nc >= (1ll << 30)) return tw_fail(ctx, TW_E_UNSUPPORTED, "item %d: too many tiles", i);
         J.ntile = (int)per_ch;
         tiles[i] = per_ch * it.src.nc;
+    }
+    // LDS-staged affine tiles get the launch's LDS (at least their own
+    // budget) for their source boxes.
+    if (any_lds_affine) {
+        lds = std::max(lds, (size_t)AFF_LDS_BYTES);
+        for (int i = 0; i < n; i++)
+            if (hj[i].kind == BK_AFFINE && hj[i].ag.lds) hj[i].ag.lds_bytes = (int)lds;
     }
     for (int x = 0; x < 8; x++) {
         int* P = parts + x * (2 * n + 1);

@@ -105,14 +105,30 @@ struct AffGeom {
     int tiles_x, tiles_y, ntiles, per_xcd;
     double fmx, fmy, fmz;               // full extent - 1 (bounds test)
     int fnx, fny, fnz;                  // full extent
+    int lds;                            // 1: LDS-staged 16 x 16 x 8 tiles (affine_tile_lds), 0: 64 x 4 x 16
+    int lds_bytes;                      // LDS per work-group for the staged box
 };
 
 constexpr int AFF_TZ = 16;  // tile depth (voxels along z)
+// LDS-staged tiles: 16 x 16 x 8 output voxels; a work-group of 256 threads,
+// each walking one (x, y) column of 8 slices.
+constexpr int ALX = 16, ALY = 16, ALZ = 8;
+constexpr int AFF_LDS_BYTES = 32 * 1024;  // box budget of the standalone kernel
+
+// Whether the map's gathers scatter: along a row of output x (a gather's
+// lanes) the sample moves by the matrix's first column; when its y and z
+// parts move it to another source row at least every second lane, the
+// gathers touch a cache line per lane or two and the LDS-staged tile pays
+// for its staging (measured: near-identity maps and rotations up to 15
+// degrees run faster on the 64 x 4 gather tile, large rotations on LDS).
+inline bool affine_prefers_lds(const double* m) {
+    return std::fabs(m[4]) + std::fabs(m[8]) > 0.5;
+}
 
 // Geometry of the fast path, or null if the window exceeds its limits (the
 // caller then takes affine_voxel).
 template <typename T>
-const char* affine_geom(const AffineArgs& a, AffGeom& g) {
+const char* affine_geom(const AffineArgs& a, AffGeom& g, bool lds = false, int lds_bytes = AFF_LDS_BYTES) {
     const uint64_t schan = (uint64_t)a.snx * a.sny * a.snz * sizeof(T);
     const uint64_t taps = schan + (uint64_t)a.snx * (a.sny + 1) * sizeof(T) + 16;
     const uint64_t dchan = (uint64_t)a.dnx * a.dny * a.dnz * sizeof(T);
@@ -131,9 +147,12 @@ const char* affine_geom(const AffineArgs& a, AffGeom& g) {
     g.nrec10 = past(g.plane_b);
     g.nrec11 = past((uint64_t)g.row_b + g.plane_b);
     g.dchan_b = (uint32_t)dchan;
-    g.tiles_x = (int)((a.dnx + TX - 1) / TX);
-    g.tiles_y = (int)((a.dny + FTY - 1) / FTY);
-    const i64 nt = (i64)g.tiles_x * g.tiles_y * ((a.dnz + AFF_TZ - 1) / AFF_TZ);
+    g.lds = lds ? 1 : 0;
+    g.lds_bytes = lds_bytes;
+    const int tx = lds ? ALX : TX, ty = lds ? ALY : FTY, tz = lds ? ALZ : AFF_TZ;
+    g.tiles_x = (int)((a.dnx + tx - 1) / tx);
+    g.tiles_y = (int)((a.dny + ty - 1) / ty);
+    const i64 nt = (i64)g.tiles_x * g.tiles_y * ((a.dnz + tz - 1) / tz);
     if (nt > (1ll << 30)) return "too many tiles for one launch";
     g.ntiles = (int)nt;
     g.per_xcd = (int)((nt + 7) / 8);
@@ -146,6 +165,7 @@ struct ATap {
     uint32_t off;   // byte offset of (x_s, y_s, z_s) (or of x_s - 1: shift), OOB_OFF when outside
     bool ok;        // inside the volume and the window
     bool shift;     // last column: the pair is (x_s - 1, x_s)
+    int ex, ey, ez; // CHECK = false: window indices of (x_s, y_s, z_s)
     double lw, rw, tw, bw, uw, dw;
 };
 
@@ -164,6 +184,9 @@ __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g
         t.shift = false;
         if (INTERP == TW_INTERP_NN) {
             const int qx = (int)(p0 + 0.5) - g.sx0, qy = (int)(p1 + 0.5) - g.sy0, qz = (int)(p2 + 0.5) - g.sz0;
+            t.ex = qx;
+            t.ey = qy;
+            t.ez = qz;
             t.off = (__umul24((uint32_t)(qz * g.sny + qy), (uint32_t)g.snx) + (uint32_t)qx) * (uint32_t)sizeof(T);
             t.lw = t.rw = t.tw = t.bw = t.uw = t.dw = 0.0;
             return t;
@@ -177,6 +200,9 @@ __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g
         t.uw = 1.0 - (p2 - fz);
         t.dw = 1.0 - t.uw;
         const int lxs = (int)fx - g.sx0, lys = (int)fy - g.sy0, lzs = (int)fz - g.sz0;
+        t.ex = lxs;
+        t.ey = lys;
+        t.ez = lzs;
         t.off = (__umul24((uint32_t)(lzs * g.sny + lys), (uint32_t)g.snx) + (uint32_t)lxs) * (uint32_t)sizeof(T);
         return t;
     }
@@ -308,6 +334,129 @@ __device__ __forceinline__ void affine_tile_fast(const AffineArgs& a, const AffG
     };
     if (affine_box_interior(a, g, blo, bhi)) walk(std::false_type{});
     else walk(std::true_type{});
+}
+
+// The source box [blo, bhi] (window coordinates, inclusive) that every
+// sample of the output box [lo, hi] reaches with its +1 taps, and whether
+// every sample lies a voxel inside the volume and the window (as
+// affine_box_interior; then the box lies inside the window).
+__device__ __forceinline__ bool affine_box_reach(const AffineArgs& a, const AffGeom& g, const int* lo, const int* hi,
+                                                 int* blo, int* bhi) {
+    const double c0[3] = {(double)(a.dx0 + lo[0]), (double)(a.dy0 + lo[1]), (double)(a.dz0 + lo[2])};
+    const double c1[3] = {(double)(a.dx0 + hi[0]), (double)(a.dy0 + hi[1]), (double)(a.dz0 + hi[2])};
+    const int s0[3] = {g.sx0, g.sy0, g.sz0}, sn[3] = {g.snx, g.sny, g.snz};
+    const double fm[3] = {g.fmx, g.fmy, g.fmz};
+    bool ok = true;
+    #pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double pl = a.m[4 * c + 3], ph = a.m[4 * c + 3];
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double u = a.m[4 * c + k] * c0[k], v = a.m[4 * c + k] * c1[k];
+            pl += fmin(u, v);
+            ph += fmax(u, v);
+        }
+        ok = ok && pl - 1.0 >= fmax(0.0, (double)s0[c]) && ph + 1.0 <= fmin(fm[c], (double)(s0[c] + sn[c] - 2));
+        // (Only used when ok: then both bounds are inside the window.)
+        blo[c] = ok ? (int)floor(pl - 1.0) - s0[c] : 0;
+        bhi[c] = ok ? (int)floor(ph + 1.0) + 1 - s0[c] : 0;
+    }
+    return ok;
+}
+
+// Division by d for 0 <= i < 2^20 and d >= 1, as div_f (exact).
+__device__ __forceinline__ int aff_div(int i, float inv) { return (int)__builtin_fmaf((float)i, inv, 0.5f * inv); }
+
+// One 16 x 16 x 8 tile of the output (x-fastest tile index), channels
+// [c0, c1), through LDS: the source box the tile's samples reach is staged by
+// 16-byte row pieces (the box's x start aligned down to a piece of the
+// window's row), then every thread walks its (x, y) column reading the eight
+// taps from LDS, with affine_tap / affine_blend's arithmetic.  A tile whose
+// samples leave the volume or the window, or whose box exceeds g.lds_bytes,
+// takes global gathers instead.  lds: g.lds_bytes, 16-byte aligned.  Every
+// thread of the work-group calls it (one barrier inside for the staged path).
+template <typename T, int INTERP>
+__device__ __forceinline__ void affine_tile_lds(const AffineArgs& a, const AffGeom& g, int tile, int c0, int c1,
+                                                uint8_t* lds) {
+    typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+    constexpr int EPP = 16 / (int)sizeof(T);  // elements per 16-byte piece
+    const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
+    const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
+    const int tid = (int)(threadIdx.y * TX + threadIdx.x);
+    const int lo[3] = {tix * ALX, tiy * ALY, tiz * ALZ};
+    const int hi[3] = {min(lo[0] + ALX, dnx) - 1, min(lo[1] + ALY, dny) - 1, min(lo[2] + ALZ, dnz) - 1};
+    const int x = lo[0] + (tid % ALX), y = lo[1] + tid / ALX;
+    const bool live = x <= hi[0] && y <= hi[1];
+    const double gx = (double)(a.dx0 + x), gy = (double)(a.dy0 + y);
+    const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
+    const uint32_t o0 = (uint32_t)lo[2] * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x;
+    int blo[3], bhi[3];
+    const bool inside = affine_box_reach(a, g, lo, hi, blo, bhi);
+    // Box rows: the x range aligned down to a piece of the window row.
+    const int bx0 = blo[0] & ~(EPP - 1);
+    const int ppr = (bhi[0] - bx0) / EPP + 1, bny = bhi[1] - blo[1] + 1, bnz = bhi[2] - blo[2] + 1;
+    const int RB = ppr * 16, PB = bny * RB;  // LDS row / plane bytes
+    // (A piece that crosses the channel's end would read as 0 whole: the
+    // window's last row stays on the gather path when its last piece would.)
+    const bool tail = bhi[2] == g.snz - 1 && bhi[1] == g.sny - 1 && bx0 + ppr * EPP > g.snx;
+    const bool staged = inside && !tail && (i64)bnz * PB <= g.lds_bytes;
+    if (!staged) {
+        // Global gathers (the 64 x 4 tile's arithmetic, this tile's voxels).
+        if (!live) return;
+        for (int z = lo[2]; z <= hi[2]; z++) {
+            const ATap t = affine_tap<T, INTERP>(a, g, gx, gy, (double)(a.dz0 + z));
+            const uint32_t o = o0 + (uint32_t)(z - lo[2]) * ostep;
+            for (int c = c0; c < c1; c++) {
+                const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
+                const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+                    (uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b, BUF_FLAGS);
+                bst<T>(rd, o * (uint32_t)sizeof(T), affine_blend<T, INTERP>(rs, t));
+            }
+        }
+        return;
+    }
+    const int npiece = bnz * bny * ppr;
+    const float inv_ppr = 1.0f / (float)ppr, inv_bny = 1.0f / (float)bny;
+    for (int c = c0; c < c1; c++) {
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)((const uint8_t*)a.src + (size_t)c * g.chan_b), (short)0, (int)g.chan_b, BUF_FLAGS);
+        const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((uint8_t*)a.dst + (size_t)c * g.dchan_b,
+                                                                            (short)0, (int)g.dchan_b, BUF_FLAGS);
+        if (c > c0) __syncthreads();  // the previous channel's taps are read
+        // Stage: piece i = (row, k), row = (z, y) of the box.  A piece past
+        // the row's end reads the bytes after it (0 past the channel) into
+        // LDS bytes no tap reads.
+        for (int i = tid; i < npiece; i += 256) {
+            const int row = aff_div(i, inv_ppr), k = i - row * ppr;
+            const int zz = aff_div(row, inv_bny), yy = row - zz * bny;
+            const uint32_t e = __umul24((uint32_t)((blo[2] + zz) * g.sny + blo[1] + yy), (uint32_t)g.snx) +
+                               (uint32_t)(bx0 + k * EPP);
+            const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(e * (uint32_t)sizeof(T)), 0, 0);
+            *(u32x4_t*)(lds + i * 16) = v;
+        }
+        __syncthreads();
+        if (live) {
+            for (int z = lo[2]; z <= hi[2]; z++) {
+                const ATap t = affine_tap<T, INTERP, false>(a, g, gx, gy, (double)(a.dz0 + z));
+                const uint8_t* q = lds + (t.ez - blo[2]) * PB + (t.ey - blo[1]) * RB + (t.ex - bx0) * (int)sizeof(T);
+                auto at = [&](int off) -> uint32_t { return sizeof(T) == 1 ? (uint32_t)q[off] : (uint32_t)*(const uint16_t*)(q + off); };
+                uint32_t v;
+                if (INTERP == TW_INTERP_NN) {
+                    v = at(0);
+                } else {
+                    constexpr int X1 = (int)sizeof(T);
+                    const double v000 = (double)at(0), v100 = (double)at(X1);
+                    const double v010 = (double)at(RB), v110 = (double)at(RB + X1);
+                    const double v001 = (double)at(PB), v101 = (double)at(PB + X1);
+                    const double v011 = (double)at(PB + RB), v111 = (double)at(PB + RB + X1);
+                    const double hi_ = t.tw * (t.lw * v000 + t.rw * v100) + t.bw * (t.lw * v010 + t.rw * v110);
+                    const double lo_ = t.tw * (t.lw * v001 + t.rw * v101) + t.bw * (t.lw * v011 + t.rw * v111);
+                    v = (uint32_t)(T)(t.uw * hi_ + t.dw * lo_);
+                }
+                bst<T>(rd, (o0 + (uint32_t)(z - lo[2]) * ostep) * (uint32_t)sizeof(T), v);
+            }
+        }
+    }
 }
 
 }  // namespace
