@@ -91,7 +91,7 @@ __global__ __launch_bounds__(FT) void k_resample_batch(const BatchJob* __restric
                 else
                     tps_tile<uint16_t, MODE, F16, INTERP, false, WIN>(a, g, t, src, dst, lds, rmax_w);
             }
-        } else {
+        } else if constexpr (!WIN) {  // (windowed launches are brick layers: TPS items only)
             const AffineArgs& af = *(const AffineArgs*)&J.af;
             const AffGeom& ag = *(const AffGeom*)&J.ag;
             if (ag.lds) {

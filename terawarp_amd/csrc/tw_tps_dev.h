@@ -49,6 +49,12 @@ constexpr int TX = 64, TY = 4;  // reference-order kernel block
 #endif
 // FRY columns per thread: a tile is TX x FTH voxels in (x, y).
 constexpr int FTY = TW_FUSED_TY, FT = TX * FTY, FRY = TW_FUSED_RY, FTH = FTY * FRY;
+// Slices per group of the fused tile's z walk: the group's gathers are all
+// issued before its blends.
+#ifndef TW_FUSED_TG
+#define TW_FUSED_TG 4
+#endif
+constexpr int TG = TW_FUSED_TG;
 
 // ---------------------------------------------------------------------------
 // Shared per-voxel resample tail.
@@ -345,17 +351,17 @@ __device__ __forceinline__ uint32_t blend(const SrcRs& rs, const Tap& t) {
                                 bld2<T>(rs.r11, off), t);
 }
 
-// Four consecutive slices of one column (trilinear, every tap in bounds):
+// TG consecutive slices of one column (trilinear, every tap in bounds):
 // where slice j's offset is slice j-1's plus one plane, its lower plane is
 // slice j-1's upper plane (the same bytes), so those two pairs are not
 // fetched again.  (have, poff, pu0, pu1) carry the previous group's last
 // slice the same way.  Results are those of blend.
 template <typename T, int MODE>
-__device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v,
+__device__ __forceinline__ void blend_shared(const SrcRs& rs, const Tap* tp, uint32_t plane_b, uint32_t* v,
                                               bool& have, uint32_t& poff, uint32_t& pu0, uint32_t& pu1) {
-    uint32_t up0[4], up1[4], lo0[4], lo1[4];
+    uint32_t up0[TG], up1[TG], lo0[TG], lo1[TG];
     #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < TG; j++) {
         __builtin_assume(tp[j].off <= OOB_OFF);
         up0[j] = bld2<T>(rs.r10, tp[j].off);
         up1[j] = bld2<T>(rs.r11, tp[j].off);
@@ -363,13 +369,13 @@ __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, ui
     // The lower-plane pairs are loaded by every lane, at a past-the-end
     // offset (reads 0) where the lane takes them from the slice before, and
     // selected after the loads: so no load waits for an earlier one.
-    bool need[4];
-    uint32_t la[4], lb[4];
+    bool need[TG];
+    uint32_t la[TG], lb[TG];
     need[0] = !(have && tp[0].off == poff + plane_b);
     #pragma unroll
-    for (int j = 1; j < 4; j++) need[j] = tp[j].off != tp[j - 1].off + plane_b;
+    for (int j = 1; j < TG; j++) need[j] = tp[j].off != tp[j - 1].off + plane_b;
     #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < TG; j++) {
         const uint32_t off = need[j] ? tp[j].off : OOB_OFF;
         la[j] = bld2<T>(rs.r00, off);
         lb[j] = bld2<T>(rs.r01, off);
@@ -377,16 +383,16 @@ __device__ __forceinline__ void blend4_shared(const SrcRs& rs, const Tap* tp, ui
     lo0[0] = need[0] ? la[0] : pu0;
     lo1[0] = need[0] ? lb[0] : pu1;
     #pragma unroll
-    for (int j = 1; j < 4; j++) {
+    for (int j = 1; j < TG; j++) {
         lo0[j] = need[j] ? la[j] : up0[j - 1];
         lo1[j] = need[j] ? lb[j] : up1[j - 1];
     }
     have = true;
-    poff = tp[3].off;
-    pu0 = up0[3];
-    pu1 = up1[3];
+    poff = tp[TG - 1].off;
+    pu0 = up0[TG - 1];
+    pu1 = up1[TG - 1];
     #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = blend_pairs<T, MODE>(lo0[j], lo1[j], up0[j], up1[j], tp[j]);
+    for (int j = 0; j < TG; j++) v[j] = blend_pairs<T, MODE>(lo0[j], lo1[j], up0[j], up1[j], tp[j]);
 }
 
 // floor(i / d) for 0 <= i < 2^21 as trunc((i + 1/2) / d) in FP32, with
@@ -603,32 +609,32 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
             continue;
         }
         // Cell by cell: the node-layer window moves once per cell; inside a
-        // cell four slices per iteration keep 32 taps in flight (a short last
+        // cell TG slices per iteration keep 8 TG taps in flight (a short last
         // group repeats its last slice and stores only the real ones).  The
         // last slice's upper-plane pairs carry to the next group.
         bool have = false;
         uint32_t poff = 0, pu0 = 0, pu1 = 0;
         auto group = [&](auto check, auto full, int nh) {
             constexpr bool CHECK = decltype(check)::value, FULL = decltype(full)::value;
-            Tap tp[4];
+            Tap tp[TG];
             #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < TG; j++) {
                 const int jj = FULL ? j : min(j, nh - 1);
                 float d[3];
                 df_ph(d, tzp + jj);
                 const float zj = FULL ? Zf + (float)j : Zf + (float)jj;
                 tp[j] = tap_setup<T, INTERP, WIN, CHECK>(g, Xf + d[0], Yf + d[1], zj + d[2]);
             }
-            uint32_t v[4];
+            uint32_t v[TG];
             if constexpr (!CHECK && INTERP != TW_INTERP_NN) {
-                blend4_shared<T, MODE>(rs, tp, g.plane_b, v, have, poff, pu0, pu1);
+                blend_shared<T, MODE>(rs, tp, g.plane_b, v, have, poff, pu0, pu1);
             } else {
                 #pragma unroll
-                for (int j = 0; j < 4; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
+                for (int j = 0; j < TG; j++) v[j] = blend<T, MODE, INTERP>(rs, tp[j]);
                 have = false;
             }
             #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < TG; j++)
                 if (FULL || j < nh) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
             tzp += nh;
             o += (uint32_t)nh * ostep;
@@ -636,7 +642,7 @@ __device__ __forceinline__ void tps_tile(const TpsArgs& a, const FastGeom& g, in
         };
         auto cell = [&](auto check, int cnt) {
             int k = 0;
-            for (; k + 4 <= cnt; k += 4) group(check, std::true_type{}, 4);
+            for (; k + TG <= cnt; k += TG) group(check, std::true_type{}, TG);
             if (k < cnt) group(check, std::false_type{}, cnt - k);
         };
         int lz = 0;
