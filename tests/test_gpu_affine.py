@@ -102,3 +102,65 @@ def test_affine_identity_reproduces_every_voxel(warper, rng, dtype, interp):
     v = _vol(rng, (2, 13, 7, 33), dtype)
     I = np.eye(4)
     assert np.array_equal(warper.resample_affine(v, I, interp=interp), v)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+@pytest.mark.parametrize("case", ["rot_2ch", "rot_nn", "rot_scaled", "rot_window", "rot_edges"])
+def test_affine_large_rotation_tiles(warper, oracle, rng, dtype, case):
+    """Maps whose x step crosses source rows every lane or two run the
+    LDS-staged tile (source box staged per 16 x 16 x 8 output tile); its
+    fallbacks (a box over the LDS budget, the window's last row, samples
+    leaving the volume) gather.  Every case is bit-identical to the oracle."""
+    size = (83, 71, 45)
+    amp = (20, 200) if dtype == np.uint8 else (100, 4000)
+    v = synth.blob_volume(size[::-1], dtype, n_blobs=40, amp=amp, seed=13)
+    R = warp.mode_matrix(warp.ROTATION, (37.0, -71.0, 113.0), size)
+    if case == "rot_2ch":
+        v2 = np.stack([v, v[::-1].copy()])
+        got = warper.resample_affine(v2, R)
+        assert np.array_equal(got, oracle.affine_resample(v2, R))
+        return
+    if case == "rot_nn":
+        got = warper.resample_affine(v, R, interp=warp.NN)
+        assert np.array_equal(got, _affine_nn(v, R))
+        return
+    if case == "rot_scaled":
+        # Rotation then 5x minification: a tile's source box is far over the
+        # LDS budget, so those tiles gather.
+        M = R @ warp.mode_matrix(warp.SCALING, (0.2, 0.2, 0.2), size)
+        got = warper.resample_affine(v, M)
+        assert np.array_equal(got, oracle.affine_resample(v, M))
+        return
+    if case == "rot_window":
+        # Output brick at an offset, output larger than the source.
+        ref = oracle.affine_resample(v, R, out_size_xyz=(96, 80, 50))
+        sub = np.zeros((20, 33, 40), dtype)
+        warper.resample_affine(v, R, dst=sub, dst_origin=(31, 17, 9))
+        assert np.array_equal(sub, ref[9:29, 17:50, 31:71])
+        return
+    # rot_edges: a 90-degree turn about z with a shift, so that tiles' boxes
+    # reach the volume's last rows and planes exactly.
+    M = warp.mode_matrix(warp.ROTATION, (0.0, 0.0, 90.0), size) @ warp.mode_matrix(warp.TRANSLATION, (1, 2, 0), size)
+    got = warper.resample_affine(v, M)
+    assert np.array_equal(got, oracle.affine_resample(v, M))
+
+
+def _affine_nn(v, M):
+    """Nearest-neighbour affine resample with the linear path's position and
+    bounds (q_warp_affine_tps.cpp:1383-1400: p = M [x y z 1] in FP64, summed
+    left to right, strict bounds against the extent) and the TPS resampler's
+    nearest-neighbour rule ((long)(p + 0.5), clamped to the extent;
+    q_imgwarp_tps_quicksmallmemory.cpp:119-129)."""
+    nz, ny, nx = v.shape
+    gz, gy, gx = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                             np.arange(nx, dtype=np.float64), indexing="ij")
+    p = [M[c, 0] * gx + M[c, 1] * gy + M[c, 2] * gz + M[c, 3] * 1.0 for c in range(3)]
+    ext = (nx, ny, nz)
+    inside = np.ones(v.shape, bool)
+    q = []
+    for c in range(3):
+        inside &= (p[c] >= 0) & (p[c] <= ext[c] - 1)
+        qc = np.floor(np.where(inside, p[c], 0.0) + 0.5).astype(np.int64)
+        q.append(np.minimum(qc, ext[c] - 1))
+    out = v[q[2], q[1], q[0]]
+    return np.where(inside, out, 0).astype(v.dtype)
