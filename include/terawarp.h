@@ -210,9 +210,11 @@ int tw_resample_affine(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const 
  * c: K x 4 row-major, d: 4 x 4 row-major, host memory (the reference's
  * xnx4_c, x4x4_d).  Computed by the reference's null-space method
  * (Householder QR of X, Cholesky of q2'(K + lambda I)q2); precision:
- * TW_PLAIN/TW_EXACT (FP64 factor) or TW_MIXED (the FP64 factor up to 4096
- * projected unknowns; above, FP32 factor + FP64 iterative refinement, FP64
- * refactor if the refinement stalls).  K <= 16384. */
+ * TW_PLAIN/TW_EXACT (FP64 factor) or TW_MIXED (the FP64 factor below
+ * TW_TUNE_FP32_FACTOR_MIN projected unknowns, 14000 by default, the measured
+ * crossover; from there the FP32 factor with matrix-core updates + FP64
+ * iterative refinement, an FP64 refactor if the refinement stalls).
+ * K <= 16384. */
 int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* targets, int K, double lambda,
                int precision, double* c, double* d);
 
