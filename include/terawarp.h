@@ -165,11 +165,24 @@ int         tw_set_tile(tw_ctx* ctx, int tile);
  *                            (tw_warp_batch, tw_resample_tps_bricks)
  *   TW_TUNE_FP32_FACTOR_MIN  TW_MIXED: smallest projected system order
  *                            (landmarks - 4) factored in FP32 with FP64
- *                            refinement; smaller ones factor in FP64 */
+ *                            refinement; smaller ones factor in FP64
+ *   TW_TUNE_SOLVE_WAIT_US    bound of each wait between the work-groups of
+ *                            the pipelined triangular solve, 0..10^7 µs
+ *                            (20000 by default).  A wait that runs out
+ *                            (other work held the CUs) turns the pipelined
+ *                            solve off for the context and the fit is redone
+ *                            with one launch per block, same result to
+ *                            rounding; setting this tunable turns it back
+ *                            on.
+ *   TW_TUNE_SOLVE_PIPELINED  read only: 1 while the context uses the
+ *                            pipelined solve, 0 after one of its waits ran
+ *                            out. */
 #define TW_TUNE_TILE             1
 #define TW_TUNE_TZ               2
 #define TW_TUNE_FP32_FACTOR_MIN  3
 #define TW_TUNE_TZ_BATCH         4
+#define TW_TUNE_SOLVE_WAIT_US    5
+#define TW_TUNE_SOLVE_PIPELINED  6
 int         tw_set_tune(tw_ctx* ctx, int key, int64_t value);
 int         tw_get_tune(tw_ctx* ctx, int key, int64_t* value);
 int         tw_get_timings(tw_ctx* ctx, tw_timings* out);

@@ -42,6 +42,8 @@ TW_TUNE_TILE = 1
 TW_TUNE_TZ = 2
 TW_TUNE_FP32_FACTOR_MIN = 3
 TW_TUNE_TZ_BATCH = 4
+TW_TUNE_SOLVE_WAIT_US = 5
+TW_TUNE_SOLVE_PIPELINED = 6
 
 # Every entry point include/terawarp.h declares.
 EXPORTS = (

@@ -156,6 +156,7 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
                              int precision, int32_t* status, double* done_ms) {
     if (!ctx) return TW_E_ARG;
     if (!items || n < 1) return tw_fail(ctx, TW_E_ARG, "empty batch");
+    const int interp_arg = interp, pipe_was_off = ctx->pipe_off;
     const int basis = (interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
     interp &= ~TW_DF_LINEAR;
     if (interp != TW_INTERP_LINEAR && interp != TW_INTERP_NN)
@@ -397,6 +398,9 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
         for (int i = 0; i < n; i++)
             done_ms[i] = khz > 0 && ts[1 + i] >= ts[0] ? (double)(ts[1 + i] - ts[0]) / (double)khz : 0.0;
     }
+    // A pipelined solve whose wait ran out: once more, per-block launches.
+    if (first != TW_OK && ctx->pipe_off && !pipe_was_off)
+        return tw_warp_batch(ctx, items, n, f, lambda, interp_arg, precision, status, done_ms);
     return first;
 }
 

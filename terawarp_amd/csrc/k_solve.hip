@@ -1347,14 +1347,16 @@ __global__ __launch_bounds__(STEP_THREADS) void k_solve_step(const T* __restrict
 // the wait for the vector it multiplies.  The critical path is one block
 // product and one flag hand-over per block instead of the single work-group's
 // stream of the whole factor through one CU.  Flags hold the call's epoch
-// (no reset between calls).  Every wait is bounded: one that runs out marks
-// the fit state (reported as a failed fit) and stops waiting.
+// (no reset between calls).  Every wait is bounded (wait_ticks of the
+// 100 MHz device clock): one that runs out marks the fit state and stops
+// waiting, and the host redoes the fit with one launch per block.
 constexpr int PIPE_THREADS = 256, PIPE_MAX_BLOCKS = 128;
 
 template <typename T>
 __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict__ L, const T* __restrict__ Dinv, int n,
                                                              T* __restrict__ b, T* __restrict__ y, int* flags,
-                                                             int epoch, FitState* st, const int* skip, int fwd_done) {
+                                                             int epoch, FitState* st, const int* skip, int fwd_done,
+                                                             unsigned long long wait_ticks) {
     if (skip && *skip) return;
     __shared__ T vs[NB][4];       // the published block being applied (y_k or x_k)
     __shared__ T part[4][NB][4];  // partial sums of the four quarters
@@ -1370,10 +1372,9 @@ __global__ __launch_bounds__(PIPE_THREADS) void k_solve_pipe(const T* __restrict
     __syncthreads();
     auto wait_for = [&](int* f) {
         if (tid == 0 && !timed_out) {
-            // Bounded by the device clock (100 MHz): 20 ms per wait.
             const unsigned long long t0 = wall_clock64();
             while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                if (wall_clock64() - t0 > 2000000ull) {
+                if (wall_clock64() - t0 >= wait_ticks) {
                     timed_out = 1;
                     st->sync_timeout = 1;
                     break;
@@ -1474,9 +1475,11 @@ constexpr int SOLVE_MULTI_MIN = 1025;
 // fwd_done: the factor steps ran the forward substitution (pipelined path only).
 // The pipelined solve needs all its nb work-groups resident at once (work-
 // group j waits on blocks dispatched after it); pipe_fits says whether the
-// device holds that many (once per context and type).
+// device holds that many (once per context and type), and not after one of
+// its waits ran out on this context.
 template <typename T>
 bool pipe_fits(tw_ctx* ctx, int nb) {
+    if (ctx->pipe_off) return false;
     int& cap = sizeof(T) == 8 ? ctx->pipe_cap64 : ctx->pipe_cap32;
     if (cap < 0) {
         int per_cu = 0;
@@ -1496,6 +1499,7 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
     const bool pipe = nb <= PIPE_MAX_BLOCKS && pipe_fits<T>(ctx, nb);
     if (fwd_done && !pipe)
         return tw_fail(ctx, TW_E_DEVICE, "the device cannot hold the %d work-groups of the pipelined solve", nb);
+    const unsigned long long wait_ticks = 100ull * (unsigned long long)ctx->solve_wait_us;  // 100 MHz clock
     if (pipe && ctx->arena) {
         // Allocation-free mode: the flags come from the arena and are cleared
         // by a kernel before each solve (the arena keeps earlier calls' data).
@@ -1504,7 +1508,7 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
         if ((rc = tw_ws(ctx, WS_SOLVE_FLAGS, 2 * PIPE_MAX_BLOCKS * sizeof(int), &p))) return rc;
         if ((rc = tw_fill_zero_i32(ctx, (int*)p, 2 * PIPE_MAX_BLOCKS))) return rc;
         hipLaunchKernelGGL(k_solve_pipe<T>, dim3(nb), dim3(PIPE_THREADS), 0, ctx->stream, L, Dinv, n, b, y, (int*)p,
-                           1, st, skip, fwd_done);
+                           1, st, skip, fwd_done, wait_ticks);
     } else if (pipe) {
         if (!ctx->solve_flags) {
             void* p = nullptr;
@@ -1515,7 +1519,7 @@ int chol_solve(tw_ctx* ctx, const T* L, const T* Dinv, int n, T* b, T* y, const 
         if (++ctx->solve_epoch == 0) ctx->solve_epoch = 1;  // a new value per call, never the initial 0
         const int epoch = (int)ctx->solve_epoch;
         hipLaunchKernelGGL(k_solve_pipe<T>, dim3(nb), dim3(PIPE_THREADS), 0, ctx->stream, L, Dinv, n, b, y,
-                           ctx->solve_flags, epoch, st, skip, fwd_done);
+                           ctx->solve_flags, epoch, st, skip, fwd_done, wait_ticks);
     } else if (n < SOLVE_MULTI_MIN) {
         hipLaunchKernelGGL(k_chol_solve<T>, dim3(1), dim3(SOLVE_THREADS), 0, ctx->stream, L, Dinv, n, b, skip);
     } else {
@@ -2577,17 +2581,22 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
         // substitution.
         if (!qfast || mixed)
             hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
-        const bool fused = (n + NB - 1) / NB <= STEP_MAX_BLOCKS;
+        const int nbk = (n + NB - 1) / NB;
+        const bool fused = nbk <= STEP_MAX_BLOCKS;
+        // The steps' forward substitution leaves the backward pass to the
+        // pipelined solve; without it the solve runs both passes.
+        const bool fwd = fused && pipe_fits<double>(ctx, nbk);
         if (fused) {
             void* lp = nullptr;
             int r2 = tw_ws(ctx, WS_SOLVE_L, (size_t)n * n * sizeof(double), &lp);
             if (r2) return r2;
-            cholesky_steps<double>(ctx, B, (double*)lp, (double*)dinv, n, st, z, (double*)ys);
+            cholesky_steps<double>(ctx, B, (double*)lp, (double*)dinv, n, st, fwd ? z : nullptr,
+                                   fwd ? (double*)ys : nullptr);
             Lf = (const double*)lp;
         } else {
             cholesky<double>(ctx, B, (double*)dinv, n, st);
         }
-        return chol_solve<double>(ctx, Lf, (const double*)dinv, n, z, (double*)ys, nullptr, st, fused ? 1 : 0);
+        return chol_solve<double>(ctx, Lf, (const double*)dinv, n, z, (double*)ys, nullptr, st, fwd ? 1 : 0);
     };
 
     int iters = 0;
@@ -2667,7 +2676,10 @@ int fit_finish(tw_ctx* ctx, const FitState* st, double bmax, int iters) {
     if (hst.info == 1) return tw_fail(ctx, TW_E_SINGULAR, "landmark centres are coplanar (rank-deficient X)");
     if (hst.info >= 2)
         return tw_fail(ctx, TW_E_SINGULAR, "projected TPS system not positive definite (column %d)", hst.info - 2);
-    if (hst.sync_timeout) return tw_fail(ctx, TW_E_DEVICE, "the solve's work-groups did not meet (wait bound reached)");
+    if (hst.sync_timeout) {
+        ctx->pipe_off = 1;  // the callers redo the fit with one launch per block
+        return tw_fail(ctx, TW_E_DEVICE, "the solve's work-groups did not meet (wait bound reached)");
+    }
     ctx->fit_residual = hst.fit_rmax / bmax;
     ctx->refine_iters = iters;
     if (!std::isfinite(ctx->fit_residual)) return tw_fail(ctx, TW_E_SINGULAR, "non-finite TPS coefficients");
@@ -2683,6 +2695,7 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     int rc;
     if ((rc = check_fit_args(ctx, centres, targets, K, lambda, precision))) return rc;
     if ((rc = tw_bind(ctx))) return rc;
+    const int pipe_was_off = ctx->pipe_off;
     tw_stages_reset(ctx);
     tw_stage_begin(ctx, ST_TOTAL);
     tw_stage_begin(ctx, ST_FIT);
@@ -2694,7 +2707,11 @@ extern "C" int tw_tps_fit(tw_ctx* ctx, const float* centres, const float* target
     TW_HIP(ctx, hipMemcpyAsync(hx.data(), r.x, hx.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     tw_stage_end(ctx, ST_FIT);
     tw_stage_end(ctx, ST_TOTAL);
-    if ((rc = fit_finish(ctx, r.st, r.bmax, r.iters))) return rc;
+    if ((rc = fit_finish(ctx, r.st, r.bmax, r.iters))) {
+        // A pipelined solve whose wait ran out: once more, per-block launches.
+        if (ctx->pipe_off && !pipe_was_off) return tw_tps_fit(ctx, centres, targets, K, lambda, precision, c, d);
+        return rc;
+    }
     // Undo the scaling: c = c'/s, d = T^-1 d'.
     const double s = r.s;
     for (int i = 0; i < N; i++)
@@ -2838,6 +2855,10 @@ int tw_fit_batch_check(tw_ctx* ctx, const TwFitItem* items, int n, int* status) 
         if (hst.info == 1) {
             code = TW_E_SINGULAR;
             snprintf(msg, sizeof msg, "item %d: landmark centres are coplanar (rank-deficient X)", i);
+        } else if (hst.sync_timeout) {
+            code = TW_E_DEVICE;
+            ctx->pipe_off = 1;  // the caller redoes the fits with one launch per block
+            snprintf(msg, sizeof msg, "item %d: the solve's work-groups did not meet (wait bound reached)", i);
         } else if (hst.info >= 2) {
             code = TW_E_SINGULAR;
             snprintf(msg, sizeof msg, "item %d: projected TPS system not positive definite (column %d)", i,

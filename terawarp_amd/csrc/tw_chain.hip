@@ -42,6 +42,7 @@ extern "C" int tw_warp_tps(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, co
 int tw_warp_tps_chain(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const float* centres, const float* targets,
                       int K, int f, double lambda, int interp, int precision) {
     int rc;
+    const int interp_arg = interp, pipe_was_off = ctx->pipe_off;
     const int basis = (interp & TW_DF_LINEAR) ? TW_BASIS_LINEAR : TW_BASIS_BSPLINE;
     interp &= ~TW_DF_LINEAR;
     tw_stages_reset(ctx);
@@ -79,5 +80,9 @@ int tw_warp_tps_chain(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const f
     if ((rc = tw_resample_tps_run(ctx, src, dst, &g, interp, precision))) return rc;
     tw_stage_end(ctx, ST_TOTAL);
     // One synchronisation: the fit's status (a failed fit leaves dst unspecified).
-    return tw_tps_fit_check(ctx, &fit);
+    rc = tw_tps_fit_check(ctx, &fit);
+    // A pipelined solve whose wait ran out: once more, per-block launches.
+    if (rc && ctx->pipe_off && !pipe_was_off)
+        return tw_warp_tps_chain(ctx, src, dst, centres, targets, K, f, lambda, interp_arg, precision);
+    return rc;
 }

@@ -353,6 +353,14 @@ int tw_set_tune(tw_ctx* ctx, int key, int64_t value) {
                 return tw_fail(ctx, TW_E_ARG, "TW_TUNE_FP32_FACTOR_MIN takes 1 .. 2^30 (got %lld)", (long long)value);
             ctx->fp32_factor_min = (int)value;
             return TW_OK;
+        case TW_TUNE_SOLVE_WAIT_US:
+            if (value < 0 || value > 10000000)
+                return tw_fail(ctx, TW_E_ARG, "TW_TUNE_SOLVE_WAIT_US takes 0 .. 10^7 (got %lld)", (long long)value);
+            ctx->solve_wait_us = (int)value;
+            ctx->pipe_off = 0;
+            return TW_OK;
+        case TW_TUNE_SOLVE_PIPELINED:
+            return tw_fail(ctx, TW_E_ARG, "TW_TUNE_SOLVE_PIPELINED is read only (set TW_TUNE_SOLVE_WAIT_US)");
         default:
             return tw_fail(ctx, TW_E_ARG, "unknown tunable %d", key);
     }
@@ -365,6 +373,8 @@ int tw_get_tune(tw_ctx* ctx, int key, int64_t* value) {
         case TW_TUNE_TZ: *value = ctx->tune_tz; return TW_OK;
         case TW_TUNE_TZ_BATCH: *value = ctx->tune_tz_batch; return TW_OK;
         case TW_TUNE_FP32_FACTOR_MIN: *value = ctx->fp32_factor_min; return TW_OK;
+        case TW_TUNE_SOLVE_WAIT_US: *value = ctx->solve_wait_us; return TW_OK;
+        case TW_TUNE_SOLVE_PIPELINED: *value = ctx->pipe_off ? 0 : 1; return TW_OK;
         default: return tw_fail(ctx, TW_E_ARG, "unknown tunable %d", key);
     }
 }

@@ -48,6 +48,12 @@ enum TwStage { ST_FIT = 0, ST_GRID, ST_QUANT, ST_RESAMPLE, ST_H2D, ST_D2H, ST_TO
 // the refinement's extra triangular solves are latency-bound.
 constexpr int TW_FP32_FACTOR_MIN_DEFAULT = 14000;
 
+// Bound of each wait between the pipelined solve's work-groups (µs).  A wait
+// that runs out means other work held the CUs the solve's work-groups needed:
+// the context then turns the pipelined solve off and the fit is redone with
+// one launch per block.
+constexpr int TW_SOLVE_WAIT_US_DEFAULT = 20000;
+
 struct tw_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -68,6 +74,8 @@ struct tw_ctx {
     // Device properties used for launch shaping.
     int num_cus = 256;
     int pipe_cap64 = -1, pipe_cap32 = -1;  // resident work-groups of the pipelined solve (-1: not asked yet)
+    int solve_wait_us = TW_SOLVE_WAIT_US_DEFAULT;  // bound of each wait of the pipelined solve
+    int pipe_off = 0;  // a pipelined solve's wait ran out: solves take the per-block launches
     // B-spline weight tables on the device (WS_WEIGHTS) are for spacing
     // weights_f (0: none yet); their host image must outlive the copy.
     int weights_f = 0;

@@ -185,6 +185,7 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
     const int threads = d->threads < 1 ? 1 : (d->threads > 256 ? 256 : d->threads);
     int rc;
     if ((rc = tw_bind(ctx))) return rc;
+    const int pipe_was_off = ctx->pipe_off;
     const double t_start = now_s();
     tw_stages_reset(ctx);
     tw_host_stage_reset(ctx);
@@ -226,7 +227,12 @@ extern "C" int tw_warp_stream(tw_ctx* ctx, const tw_stream_desc* d, tw_stream_st
     TW_LAUNCH_CHECK(ctx);
     std::vector<float> bounds((size_t)nb * 6);
     TW_HIP(ctx, hipMemcpyAsync(bounds.data(), bp, bounds.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-    if ((rc = tw_tps_fit_check(ctx, &fit))) return rc;  // synchronises
+    if ((rc = tw_tps_fit_check(ctx, &fit))) {  // synchronises
+        // A pipelined solve whose wait ran out: once more, per-block launches
+        // (no brick has been read or written yet).
+        if (ctx->pipe_off && !pipe_was_off) return tw_warp_stream(ctx, d, stats);
+        return rc;
+    }
     const int bpv = dtype_size(d->dtype);
     std::vector<Brick> bricks((size_t)nb);
     size_t max_src = 0, max_dst = 0;

@@ -224,12 +224,16 @@ class Warper:
         self._check(self.lib.tw_set_timing(self.ctx, 1 if on else 0))
 
     TUNABLES = {"tile": L.TW_TUNE_TILE, "tz": L.TW_TUNE_TZ, "tz_batch": L.TW_TUNE_TZ_BATCH,
-                "fp32_factor_min": L.TW_TUNE_FP32_FACTOR_MIN}
+                "fp32_factor_min": L.TW_TUNE_FP32_FACTOR_MIN, "solve_wait_us": L.TW_TUNE_SOLVE_WAIT_US,
+                "solve_pipelined": L.TW_TUNE_SOLVE_PIPELINED}
 
     def set_tune(self, key, value):
         """Set a tunable (tw_set_tune): "tile" (0 auto, 1 gather, 2 lds), "tz"
         / "tz_batch" (gather tile depth of single-volume / persistent
-        launches, 0 = from the grid spacing), "fp32_factor_min"."""
+        launches, 0 = from the grid spacing), "fp32_factor_min",
+        "solve_wait_us" (bound of the pipelined solve's waits; setting it
+        turns that solve back on after a wait ran out); "solve_pipelined" is
+        read only (0 once a wait ran out)."""
         self._check(self.lib.tw_set_tune(self.ctx, self.TUNABLES[key], int(value)))
 
     def get_tune(self, key):

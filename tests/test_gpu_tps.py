@@ -124,6 +124,63 @@ def test_tunables_round_trip(warper):
     assert warper.timings()["refine_iters"] == 0
 
 
+@pytest.mark.parametrize("K", [600, 1500])
+def test_solve_wait_bound_falls_back_to_block_launches(warper, oracle, K):
+    """The pipelined solve's waits between work-groups are bounded.  With the
+    bound at 0 every wait runs out at once (each work-group stops waiting and
+    the kernel ends normally), the fit state reports it, and the library
+    turns the pipelined solve off for the context and redoes the fit with one
+    launch per block.  K = 600: fused factor steps, whose forward substitution
+    the redo leaves to the per-block solve; K = 1500: per-column factor.  Each
+    entry point that fits (tw_tps_fit, the warp chain, the batch) is forced
+    through the fallback on its own (setting the bound turns the pipelined
+    solve back on) and must match the oracle, or the default context's
+    output."""
+    import torch
+    sub, tar = synth.landmarks((512, 512, 128), K, 10.0, seed=K + 3)
+    c_ref, d_ref = oracle.stps_fit(tar, sub, bordered=True)
+    size = (96, 80, 40)
+    vol = synth.blob_volume(size[::-1], np.uint16, n_blobs=40, amp=(100, 4000), seed=K)
+    lm_t, lm_s = tar * 0.15, sub * 0.15  # landmarks inside the small volume
+    out_ref = warper.warp_tps(vol, lm_t, lm_s, f=4, precision=warp.PLAIN)
+    src = torch.from_numpy(vol.view(np.int16)).cuda()
+    dst_ref = torch.empty_like(src)
+    warper.warp_batch([{"src": src, "dst": dst_ref, "centres": lm_t, "targets": lm_s}], f=4,
+                      precision=warp.PLAIN)
+    torch.cuda.synchronize()
+    w = warp.Warper(0)
+    try:
+        assert w.get_tune("solve_wait_us") == 20000
+        assert w.get_tune("solve_pipelined") == 1
+        with pytest.raises(warp.TeraWarpError):
+            w.set_tune("solve_wait_us", -1)
+        with pytest.raises(warp.TeraWarpError):
+            w.set_tune("solve_pipelined", 1)
+        w.set_tune("solve_wait_us", 0)
+        c, d = w.tps_fit(tar, sub, precision=warp.PLAIN)
+        assert w.get_tune("solve_pipelined") == 0  # the wait ran out; the fit was redone
+        assert rel_err(c, c_ref) < 1e-8, rel_err(c, c_ref)
+        assert np.abs(d - d_ref).max() < 1e-8
+        w.set_tune("solve_wait_us", 0)
+        out = w.warp_tps(vol, lm_t, lm_s, f=4, precision=warp.PLAIN)
+        assert w.get_tune("solve_pipelined") == 0
+        diff = np.abs(out.astype(np.int64) - out_ref.astype(np.int64))
+        assert diff.max() <= 1 and np.mean(diff == 0) >= 0.9999, (diff.max(), np.mean(diff == 0))
+        w.set_tune("solve_wait_us", 0)
+        dst = torch.empty_like(src)
+        status, _ = w.warp_batch([{"src": src, "dst": dst, "centres": lm_t, "targets": lm_s}], f=4,
+                                 precision=warp.PLAIN)
+        assert status == [0] and w.get_tune("solve_pipelined") == 0
+        torch.cuda.synchronize()
+        bd = (dst.cpu().numpy().view(np.uint16).astype(np.int64) - dst_ref.cpu().numpy().view(np.uint16))
+        assert np.abs(bd).max() <= 1 and np.mean(bd == 0) >= 0.9999
+        w.set_tune("solve_wait_us", 20000)
+        c2, _ = w.tps_fit(tar, sub, precision=warp.PLAIN)  # pipelined again
+        assert rel_err(c2, c_ref) < 1e-8 and w.get_tune("solve_pipelined") == 1
+    finally:
+        w.close()
+
+
 @pytest.mark.parametrize("K", [16, 64, 512])
 @pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
 def test_fit_matches_oracle(warper, oracle, K, precision):
