@@ -107,7 +107,14 @@ struct AffGeom {
     int fnx, fny, fnz;                  // full extent
     int lds;                            // 1: LDS-staged 16 x 16 x 8 tiles (affine_tile_lds), 0: 64 x 4 x 16
     int lds_bytes;                      // LDS per work-group for the staged box
+    int whole;                          // the source window is the whole volume
 };
+
+// Sample checks of the fast path: a tile's samples take the cheapest level
+// its box allows.
+constexpr int AFF_INTERIOR = 0;  // every sample a voxel inside the volume and the window: no tests
+constexpr int AFF_BOUNDS = 1;    // whole-volume window: the volume bounds test only
+constexpr int AFF_FULL = 2;      // bounds and window tests
 
 constexpr int AFF_TZ = 16;  // tile depth (voxels along z)
 // LDS-staged tiles: 16 x 16 x 8 output voxels; a work-group of 256 threads,
@@ -133,8 +140,9 @@ const char* affine_geom(const AffineArgs& a, AffGeom& g, bool lds = false, int l
     const uint64_t taps = schan + (uint64_t)a.snx * (a.sny + 1) * sizeof(T) + 16;
     const uint64_t dchan = (uint64_t)a.dnx * a.dny * a.dnz * sizeof(T);
     const i64 cmax = std::max({a.dx0 + a.dnx, a.dy0 + a.dny, a.dz0 + a.dnz, a.fnx, a.fny, a.fnz});
+    const i64 cmin = std::min({a.dx0, a.dy0, a.dz0, a.sx0, a.sy0, a.sz0});
     if (a.snx < 2 || taps > 0x7FFF0000ull || a.sny * a.snz >= (1ll << 24) || dchan > 0xF0000000ull ||
-        cmax > (1ll << 24))
+        cmax > (1ll << 24) || cmin < -(1ll << 24))
         return "window exceeds the fast affine path's 32-bit addressing";
     memset(&g, 0, sizeof g);
     g.snx = (int)a.snx; g.sny = (int)a.sny; g.snz = (int)a.snz;
@@ -158,6 +166,7 @@ const char* affine_geom(const AffineArgs& a, AffGeom& g, bool lds = false, int l
     g.per_xcd = (int)((nt + 7) / 8);
     g.fmx = (double)(a.fnx - 1); g.fmy = (double)(a.fny - 1); g.fmz = (double)(a.fnz - 1);
     g.fnx = (int)a.fnx; g.fny = (int)a.fny; g.fnz = (int)a.fnz;
+    g.whole = a.sx0 == 0 && a.sy0 == 0 && a.sz0 == 0 && a.snx == a.fnx && a.sny == a.fny && a.snz == a.fnz;
     return nullptr;
 }
 
@@ -165,21 +174,53 @@ struct ATap {
     uint32_t off;   // byte offset of (x_s, y_s, z_s) (or of x_s - 1: shift), OOB_OFF when outside
     bool ok;        // inside the volume and the window
     bool shift;     // last column: the pair is (x_s - 1, x_s)
-    int ex, ey, ez; // CHECK = false: window indices of (x_s, y_s, z_s)
-    double lw, rw, tw, bw, uw, dw;
+    int ex, ey, ez; // AFF_INTERIOR / AFF_BOUNDS (when ok): window indices of (x_s, y_s, z_s)
+    double lw, tw, uw;  // rw = 1 - lw, bw = 1 - tw, dw = 1 - uw: formed in the blend
 };
 
-// CHECK = false: the caller has shown that every sample of the tile lies a
-// voxel inside the volume and the window (affine_tile_interior), so the
-// bounds tests, the window test and the last-column step are left out; the
-// arithmetic of the samples is the same.
-template <typename T, int INTERP, bool CHECK = true>
+// CHK = AFF_INTERIOR: the caller has shown that every sample of the tile
+// lies a voxel inside the volume and the window (affine_box_interior), so the
+// bounds tests, the window test and the last-column step are left out.
+// CHK = AFF_BOUNDS: the window is the whole volume, so a sample inside the
+// volume has its taps inside it (x_b = ceil(p) <= n - 1) and only the bounds
+// test and the last-column step remain; outside, the offset is OOB_OFF and
+// the result 0 (the floor and conversions of such a sample are not used).
+// The arithmetic of the samples is the same at every level.
+template <typename T, int INTERP, int CHK = AFF_FULL>
 __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g, double gx, double gy, double gz) {
     ATap t;
     const double p0 = a.m[0] * gx + a.m[1] * gy + a.m[2] * gz + a.m[3] * 1.0;
     const double p1 = a.m[4] * gx + a.m[5] * gy + a.m[6] * gz + a.m[7] * 1.0;
     const double p2 = a.m[8] * gx + a.m[9] * gy + a.m[10] * gz + a.m[11] * 1.0;
-    if (!CHECK) {
+    if (CHK == AFF_BOUNDS) {
+        const bool in = !(p0 < 0 || p0 > g.fmx || p1 < 0 || p1 > g.fmy || p2 < 0 || p2 > g.fmz);
+        t.ok = in;
+        if (INTERP == TW_INTERP_NN) {
+            // Inside, p + 0.5 < n: the reference's clamp to n - 1 is a no-op.
+            const uint32_t qx = (uint32_t)(int)(p0 + 0.5), qy = (uint32_t)(int)(p1 + 0.5),
+                           qz = (uint32_t)(int)(p2 + 0.5);
+            t.ex = (int)qx;
+            t.ey = (int)qy;
+            t.ez = (int)qz;
+            t.shift = false;
+            t.off = in ? (__umul24(qz * (uint32_t)g.sny + qy, (uint32_t)g.snx) + qx) * (uint32_t)sizeof(T) : OOB_OFF;
+            t.lw = t.tw = t.uw = 0.0;
+            return t;
+        }
+        const double fx = floor(p0), fy = floor(p1), fz = floor(p2);
+        t.lw = 1.0 - (p0 - fx);
+        t.tw = 1.0 - (p1 - fy);
+        t.uw = 1.0 - (p2 - fz);
+        const uint32_t xs = (uint32_t)(int)fx, ys = (uint32_t)(int)fy, zs = (uint32_t)(int)fz;
+        t.ex = (int)xs;
+        t.ey = (int)ys;
+        t.ez = (int)zs;
+        t.shift = xs == (uint32_t)g.snx - 1u;
+        const uint32_t e = __umul24(zs * (uint32_t)g.sny + ys, (uint32_t)g.snx) + xs - (t.shift ? 1u : 0u);
+        t.off = in ? e * (uint32_t)sizeof(T) : OOB_OFF;
+        return t;
+    }
+    if (CHK == AFF_INTERIOR) {
         t.ok = true;
         t.shift = false;
         if (INTERP == TW_INTERP_NN) {
@@ -188,17 +229,14 @@ __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g
             t.ey = qy;
             t.ez = qz;
             t.off = (__umul24((uint32_t)(qz * g.sny + qy), (uint32_t)g.snx) + (uint32_t)qx) * (uint32_t)sizeof(T);
-            t.lw = t.rw = t.tw = t.bw = t.uw = t.dw = 0.0;
+            t.lw = t.tw = t.uw = 0.0;
             return t;
         }
         // floor(p) as a double equals the reference's (double)(long)floor(p).
         const double fx = floor(p0), fy = floor(p1), fz = floor(p2);
         t.lw = 1.0 - (p0 - fx);
-        t.rw = 1.0 - t.lw;
         t.tw = 1.0 - (p1 - fy);
-        t.bw = 1.0 - t.tw;
         t.uw = 1.0 - (p2 - fz);
-        t.dw = 1.0 - t.uw;
         const int lxs = (int)fx - g.sx0, lys = (int)fy - g.sy0, lzs = (int)fz - g.sz0;
         t.ex = lxs;
         t.ey = lys;
@@ -219,18 +257,15 @@ __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g
         t.ok = in && w;
         t.off = t.ok ? (__umul24((uint32_t)(qz * g.sny + qy), (uint32_t)g.snx) + (uint32_t)qx) * (uint32_t)sizeof(T)
                      : OOB_OFF;
-        t.lw = t.rw = t.tw = t.bw = t.uw = t.dw = 0.0;
+        t.lw = t.tw = t.uw = 0.0;
         return t;
     }
     const int xs = (int)floor(q0), xb = (int)ceil(q0);
     const int ys = (int)floor(q1), yb = (int)ceil(q1);
     const int zs = (int)floor(q2), zb = (int)ceil(q2);
     t.lw = 1.0 - (q0 - (double)xs);
-    t.rw = 1.0 - t.lw;
     t.tw = 1.0 - (q1 - (double)ys);
-    t.bw = 1.0 - t.tw;
     t.uw = 1.0 - (q2 - (double)zs);
-    t.dw = 1.0 - t.uw;
     const int lxs = xs - g.sx0, lxb = xb - g.sx0, lys = ys - g.sy0, lyb = yb - g.sy0;
     const int lzs = zs - g.sz0, lzb = zb - g.sz0;
     const bool w = !(lxs < 0 || lxb >= g.snx || lys < 0 || lyb >= g.sny || lzs < 0 || lzb >= g.snz);
@@ -242,21 +277,46 @@ __device__ __forceinline__ ATap affine_tap(const AffineArgs& a, const AffGeom& g
 }
 
 // The reference's blend of a tap (q_warp_affine_tps.cpp:1418-1433), truncated.
-template <typename T, int INTERP, bool CHECK = true>
+template <typename T, int INTERP, int CHK = AFF_FULL>
 __device__ __forceinline__ uint32_t affine_blend(const SrcRs& rs, const ATap& t) {
     const uint32_t off = t.off;
     __builtin_assume(off <= OOB_OFF);
-    if (INTERP == TW_INTERP_NN) return (!CHECK || t.ok) ? bld<T>(rs.r00, off) : 0u;
+    if (INTERP == TW_INTERP_NN) return (CHK == AFF_INTERIOR || t.ok) ? bld<T>(rs.r00, off) : 0u;
     const uint32_t p00 = bld2<T>(rs.r00, off), p10 = bld2<T>(rs.r01, off);
     const uint32_t p01 = bld2<T>(rs.r10, off), p11 = bld2<T>(rs.r11, off);
-    const bool sh = CHECK && t.shift;
+    const bool sh = CHK != AFF_INTERIOR && t.shift;
     const double v000 = (double)(sh ? pair_hi<T>(p00) : pair_lo<T>(p00)), v100 = (double)pair_hi<T>(p00);
     const double v010 = (double)(sh ? pair_hi<T>(p10) : pair_lo<T>(p10)), v110 = (double)pair_hi<T>(p10);
     const double v001 = (double)(sh ? pair_hi<T>(p01) : pair_lo<T>(p01)), v101 = (double)pair_hi<T>(p01);
     const double v011 = (double)(sh ? pair_hi<T>(p11) : pair_lo<T>(p11)), v111 = (double)pair_hi<T>(p11);
-    const double hi = t.tw * (t.lw * v000 + t.rw * v100) + t.bw * (t.lw * v010 + t.rw * v110);
-    const double lo = t.tw * (t.lw * v001 + t.rw * v101) + t.bw * (t.lw * v011 + t.rw * v111);
-    return (!CHECK || t.ok) ? (uint32_t)(T)(t.uw * hi + t.dw * lo) : 0u;
+    const double rw = 1.0 - t.lw, bw = 1.0 - t.tw, dw = 1.0 - t.uw;
+    const double hi = t.tw * (t.lw * v000 + rw * v100) + bw * (t.lw * v010 + rw * v110);
+    const double lo = t.tw * (t.lw * v001 + rw * v101) + bw * (t.lw * v011 + rw * v111);
+    return (CHK == AFF_INTERIOR || t.ok) ? (uint32_t)(T)(t.uw * hi + dw * lo) : 0u;
+}
+
+// Whether no sample of the output box [lo, hi] lies inside the volume: along
+// some axis the box's image (its corners' extremes) is more than half a voxel
+// outside [0, n - 1], far beyond the rounding of the per-voxel evaluation.
+// Every output voxel of such a box is 0.
+__device__ __forceinline__ bool affine_box_outside(const AffineArgs& a, const AffGeom& g, const int* lo,
+                                                   const int* hi) {
+    const double c0[3] = {(double)(a.dx0 + lo[0]), (double)(a.dy0 + lo[1]), (double)(a.dz0 + lo[2])};
+    const double c1[3] = {(double)(a.dx0 + hi[0]), (double)(a.dy0 + hi[1]), (double)(a.dz0 + hi[2])};
+    const double fm[3] = {g.fmx, g.fmy, g.fmz};
+    bool out = false;
+    #pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double pl = a.m[4 * c + 3], ph = a.m[4 * c + 3];
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double u = a.m[4 * c + k] * c0[k], v = a.m[4 * c + k] * c1[k];
+            pl += fmin(u, v);
+            ph += fmax(u, v);
+        }
+        out = out || ph < -0.5 || pl > fm[c] + 0.5;
+    }
+    return out;
 }
 
 // Whether every sample of the output box [lo, hi] (destination window
@@ -307,16 +367,27 @@ __device__ __forceinline__ void affine_tile_fast(const AffineArgs& a, const AffG
     // The tile's box (clipped to the destination): wave-uniform.
     const int blo[3] = {tix * TX, tiy * FTY, z0};
     const int bhi[3] = {min(blo[0] + TX, dnx) - 1, min(blo[1] + FTY, dny) - 1, z1 - 1};
-    const double gx = (double)(a.dx0 + x), gy = (double)(a.dy0 + y);
     const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
+    if (affine_box_outside(a, g, blo, bhi)) {
+        for (int c = c0; c < c1; c++) {
+            const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+                (uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b, BUF_FLAGS);
+            for (int z = z0; z < z1; z++)
+                bst<T>(rd, ((uint32_t)z * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x) * (uint32_t)sizeof(T), 0u);
+        }
+        return;
+    }
+    // Coordinates are below 2^24 (affine_geom): 32-bit conversions are exact.
+    const double gx = (double)(int)(a.dx0 + x), gy = (double)(int)(a.dy0 + y);
+    const int gz0 = (int)a.dz0;
     auto walk = [&](auto check) {
-        constexpr bool CHECK = decltype(check)::value;
+        constexpr int CHK = decltype(check)::value;
         for (int z = z0; z < z1; z += 4) {
             const int nz = min(4, z1 - z);
             ATap t[4];
             #pragma unroll
             for (int j = 0; j < 4; j++)
-                t[j] = affine_tap<T, INTERP, CHECK>(a, g, gx, gy, (double)(a.dz0 + z + min(j, nz - 1)));
+                t[j] = affine_tap<T, INTERP, CHK>(a, g, gx, gy, (double)(gz0 + z + min(j, nz - 1)));
             const uint32_t o = (uint32_t)z * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x;
             for (int c = c0; c < c1; c++) {
                 const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
@@ -325,15 +396,16 @@ __device__ __forceinline__ void affine_tile_fast(const AffineArgs& a, const AffG
                                                       (int)g.dchan_b, BUF_FLAGS);
                 uint32_t v[4];
                 #pragma unroll
-                for (int j = 0; j < 4; j++) v[j] = affine_blend<T, INTERP, CHECK>(rs, t[j]);
+                for (int j = 0; j < 4; j++) v[j] = affine_blend<T, INTERP, CHK>(rs, t[j]);
                 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if (j < nz) bst<T>(rd, (o + (uint32_t)j * ostep) * (uint32_t)sizeof(T), v[j]);
             }
         }
     };
-    if (affine_box_interior(a, g, blo, bhi)) walk(std::false_type{});
-    else walk(std::true_type{});
+    if (affine_box_interior(a, g, blo, bhi)) walk(std::integral_constant<int, AFF_INTERIOR>{});
+    else if (g.whole) walk(std::integral_constant<int, AFF_BOUNDS>{});
+    else walk(std::integral_constant<int, AFF_FULL>{});
 }
 
 // The source box [blo, bhi] (window coordinates, inclusive) that every
@@ -364,6 +436,30 @@ __device__ __forceinline__ bool affine_box_reach(const AffineArgs& a, const AffG
     return ok;
 }
 
+// The source box [blo, bhi] of the output box [lo, hi] clipped to the volume
+// (whole-volume window; the box is not affine_box_outside): every sample
+// inside the volume has x_s in [blo, bhi] and, where x_s < bhi, x_s + 1
+// (likewise y and z).  A sample with x_s = n - 1 lies on the last column
+// exactly (p = n - 1), where the reference's x_b = ceil(p) = x_s.
+__device__ __forceinline__ void affine_box_clip(const AffineArgs& a, const AffGeom& g, const int* lo, const int* hi,
+                                                int* blo, int* bhi) {
+    const double c0[3] = {(double)(a.dx0 + lo[0]), (double)(a.dy0 + lo[1]), (double)(a.dz0 + lo[2])};
+    const double c1[3] = {(double)(a.dx0 + hi[0]), (double)(a.dy0 + hi[1]), (double)(a.dz0 + hi[2])};
+    const int sn[3] = {g.snx, g.sny, g.snz};
+    #pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double pl = a.m[4 * c + 3], ph = a.m[4 * c + 3];
+        #pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double u = a.m[4 * c + k] * c0[k], v = a.m[4 * c + k] * c1[k];
+            pl += fmin(u, v);
+            ph += fmax(u, v);
+        }
+        blo[c] = (int)fmax(0.0, floor(pl - 1.0));
+        bhi[c] = (int)fmin((double)(sn[c] - 1), floor(ph + 1.0) + 1.0);
+    }
+}
+
 // Division by d for 0 <= i < 2^20 and d >= 1, as div_f (exact).
 __device__ __forceinline__ int aff_div(int i, float inv) { return (int)__builtin_fmaf((float)i, inv, 0.5f * inv); }
 
@@ -372,13 +468,15 @@ __device__ __forceinline__ int aff_div(int i, float inv) { return (int)__builtin
 // 16-byte row pieces (the box's x start aligned down to a piece of the
 // window's row), then every thread walks its (x, y) column reading the eight
 // taps from LDS, with affine_tap / affine_blend's arithmetic.  A tile whose
-// samples leave the volume or the window, or whose box exceeds g.lds_bytes,
-// takes global gathers instead.  lds: g.lds_bytes, 16-byte aligned.  Every
-// thread of the work-group calls it (one barrier inside for the staged path).
+// samples all lie outside the volume stores zeros; one whose samples partly
+// leave a whole-volume source stages the box clipped to the volume and tests
+// each sample's bounds.  A tile whose samples leave a partial window, or
+// whose box exceeds g.lds_bytes, takes global gathers instead.  lds:
+// g.lds_bytes, 16-byte aligned.  Every thread of the work-group calls it (one
+// barrier inside for the staged path).
 template <typename T, int INTERP>
 __device__ __forceinline__ void affine_tile_lds(const AffineArgs& a, const AffGeom& g, int tile, int c0, int c1,
                                                 uint8_t* lds) {
-    typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
     constexpr int EPP = 16 / (int)sizeof(T);  // elements per 16-byte piece
     const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
     const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
@@ -387,32 +485,49 @@ __device__ __forceinline__ void affine_tile_lds(const AffineArgs& a, const AffGe
     const int hi[3] = {min(lo[0] + ALX, dnx) - 1, min(lo[1] + ALY, dny) - 1, min(lo[2] + ALZ, dnz) - 1};
     const int x = lo[0] + (tid % ALX), y = lo[1] + tid / ALX;
     const bool live = x <= hi[0] && y <= hi[1];
-    const double gx = (double)(a.dx0 + x), gy = (double)(a.dy0 + y);
     const uint32_t ostep = (uint32_t)dnx * (uint32_t)dny;
     const uint32_t o0 = (uint32_t)lo[2] * ostep + (uint32_t)y * (uint32_t)dnx + (uint32_t)x;
+    if (affine_box_outside(a, g, lo, hi)) {  // (work-group uniform: no barrier skipped)
+        if (!live) return;
+        for (int c = c0; c < c1; c++) {
+            const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+                (uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b, BUF_FLAGS);
+            for (int z = lo[2]; z <= hi[2]; z++)
+                bst<T>(rd, (o0 + (uint32_t)(z - lo[2]) * ostep) * (uint32_t)sizeof(T), 0u);
+        }
+        return;
+    }
+    // Coordinates are below 2^24 (affine_geom): 32-bit conversions are exact.
+    const double gx = (double)(int)(a.dx0 + x), gy = (double)(int)(a.dy0 + y);
+    const int gz0 = (int)a.dz0;
     int blo[3], bhi[3];
     const bool inside = affine_box_reach(a, g, lo, hi, blo, bhi);
+    const bool clip = !inside && g.whole;
+    if (clip) affine_box_clip(a, g, lo, hi, blo, bhi);
     // Box rows: the x range aligned down to a piece of the window row.
     const int bx0 = blo[0] & ~(EPP - 1);
     const int ppr = (bhi[0] - bx0) / EPP + 1, bny = bhi[1] - blo[1] + 1, bnz = bhi[2] - blo[2] + 1;
     const int RB = ppr * 16, PB = bny * RB;  // LDS row / plane bytes
-    // (A piece that crosses the channel's end would read as 0 whole: the
-    // window's last row stays on the gather path when its last piece would.)
-    const bool tail = bhi[2] == g.snz - 1 && bhi[1] == g.sny - 1 && bx0 + ppr * EPP > g.snx;
-    const bool staged = inside && !tail && (i64)bnz * PB <= g.lds_bytes;
+    const bool staged = (inside || clip) && (i64)bnz * PB <= g.lds_bytes;
     if (!staged) {
         // Global gathers (the 64 x 4 tile's arithmetic, this tile's voxels).
         if (!live) return;
-        for (int z = lo[2]; z <= hi[2]; z++) {
-            const ATap t = affine_tap<T, INTERP>(a, g, gx, gy, (double)(a.dz0 + z));
-            const uint32_t o = o0 + (uint32_t)(z - lo[2]) * ostep;
-            for (int c = c0; c < c1; c++) {
-                const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
-                const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
-                    (uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b, BUF_FLAGS);
-                bst<T>(rd, o * (uint32_t)sizeof(T), affine_blend<T, INTERP>(rs, t));
+        auto walk = [&](auto check) {
+            constexpr int CHK = decltype(check)::value;
+            for (int z = lo[2]; z <= hi[2]; z++) {
+                const ATap t = affine_tap<T, INTERP, CHK>(a, g, gx, gy, (double)(gz0 + z));
+                const uint32_t o = o0 + (uint32_t)(z - lo[2]) * ostep;
+                for (int c = c0; c < c1; c++) {
+                    const SrcRs rs = affine_src_rs((const uint8_t*)a.src + (size_t)c * g.chan_b, g);
+                    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+                        (uint8_t*)a.dst + (size_t)c * g.dchan_b, (short)0, (int)g.dchan_b, BUF_FLAGS);
+                    bst<T>(rd, o * (uint32_t)sizeof(T), affine_blend<T, INTERP, CHK>(rs, t));
+                }
             }
-        }
+        };
+        if (inside) walk(std::integral_constant<int, AFF_INTERIOR>{});
+        else if (g.whole) walk(std::integral_constant<int, AFF_BOUNDS>{});
+        else walk(std::integral_constant<int, AFF_FULL>{});
         return;
     }
     const int npiece = bnz * bny * ppr;
@@ -424,38 +539,58 @@ __device__ __forceinline__ void affine_tile_lds(const AffineArgs& a, const AffGe
                                                                             (short)0, (int)g.dchan_b, BUF_FLAGS);
         if (c > c0) __syncthreads();  // the previous channel's taps are read
         // Stage: piece i = (row, k), row = (z, y) of the box.  A piece past
-        // the row's end reads the bytes after it (0 past the channel) into
-        // LDS bytes no tap reads.
+        // the row's end reads the bytes after it into LDS bytes no tap
+        // reads; one that crosses the channel's end is read byte by byte.
         for (int i = tid; i < npiece; i += 256) {
             const int row = aff_div(i, inv_ppr), k = i - row * ppr;
             const int zz = aff_div(row, inv_bny), yy = row - zz * bny;
             const uint32_t e = __umul24((uint32_t)((blo[2] + zz) * g.sny + blo[1] + yy), (uint32_t)g.snx) +
                                (uint32_t)(bx0 + k * EPP);
-            const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(e * (uint32_t)sizeof(T)), 0, 0);
-            *(u32x4_t*)(lds + i * 16) = v;
+            const uint32_t at = e * (uint32_t)sizeof(T);
+            const u32x4 v = at + 16u <= g.chan_b ? __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)at, 0, 0)
+                                                 : load16_tail(rsrc, at);
+            *(u32x4*)(lds + i * 16) = v;
         }
         __syncthreads();
-        if (live) {
+        if (!live) continue;
+        // Taps from LDS.  AFF_BOUNDS: a sample outside the volume is 0 and
+        // reads LDS byte 0; the +1 neighbour of a tap on the box's last
+        // column / row / plane is the tap itself (that sample lies on the
+        // volume's last one, where the reference's ceil(p) = floor(p)).
+        auto walk = [&](auto lvl) {
+            constexpr int CHK = decltype(lvl)::value;
+            constexpr int X1 = (int)sizeof(T);
             for (int z = lo[2]; z <= hi[2]; z++) {
-                const ATap t = affine_tap<T, INTERP, false>(a, g, gx, gy, (double)(a.dz0 + z));
-                const uint8_t* q = lds + (t.ez - blo[2]) * PB + (t.ey - blo[1]) * RB + (t.ex - bx0) * (int)sizeof(T);
+                const ATap t = affine_tap<T, INTERP, CHK>(a, g, gx, gy, (double)(gz0 + z));
+                int idx = (t.ez - blo[2]) * PB + (t.ey - blo[1]) * RB + (t.ex - bx0) * X1;
+                int dx = X1, dy = RB, dz = PB;
+                if (CHK != AFF_INTERIOR) {
+                    idx = t.ok ? idx : 0;
+                    dx = t.ok && t.ex < bhi[0] ? X1 : 0;
+                    dy = t.ok && t.ey < bhi[1] ? RB : 0;
+                    dz = t.ok && t.ez < bhi[2] ? PB : 0;
+                }
+                const uint8_t* q = lds + idx;
                 auto at = [&](int off) -> uint32_t { return sizeof(T) == 1 ? (uint32_t)q[off] : (uint32_t)*(const uint16_t*)(q + off); };
                 uint32_t v;
                 if (INTERP == TW_INTERP_NN) {
                     v = at(0);
                 } else {
-                    constexpr int X1 = (int)sizeof(T);
-                    const double v000 = (double)at(0), v100 = (double)at(X1);
-                    const double v010 = (double)at(RB), v110 = (double)at(RB + X1);
-                    const double v001 = (double)at(PB), v101 = (double)at(PB + X1);
-                    const double v011 = (double)at(PB + RB), v111 = (double)at(PB + RB + X1);
-                    const double hi_ = t.tw * (t.lw * v000 + t.rw * v100) + t.bw * (t.lw * v010 + t.rw * v110);
-                    const double lo_ = t.tw * (t.lw * v001 + t.rw * v101) + t.bw * (t.lw * v011 + t.rw * v111);
-                    v = (uint32_t)(T)(t.uw * hi_ + t.dw * lo_);
+                    const double v000 = (double)at(0), v100 = (double)at(dx);
+                    const double v010 = (double)at(dy), v110 = (double)at(dy + dx);
+                    const double v001 = (double)at(dz), v101 = (double)at(dz + dx);
+                    const double v011 = (double)at(dz + dy), v111 = (double)at(dz + dy + dx);
+                    const double rw = 1.0 - t.lw, bw = 1.0 - t.tw, dw = 1.0 - t.uw;
+                    const double hi_ = t.tw * (t.lw * v000 + rw * v100) + bw * (t.lw * v010 + rw * v110);
+                    const double lo_ = t.tw * (t.lw * v001 + rw * v101) + bw * (t.lw * v011 + rw * v111);
+                    v = (uint32_t)(T)(t.uw * hi_ + dw * lo_);
                 }
+                if (CHK != AFF_INTERIOR) v = t.ok ? v : 0u;
                 bst<T>(rd, (o0 + (uint32_t)(z - lo[2]) * ostep) * (uint32_t)sizeof(T), v);
             }
-        }
+        };
+        if (inside) walk(std::integral_constant<int, AFF_INTERIOR>{});
+        else walk(std::integral_constant<int, AFF_BOUNDS>{});
     }
 }
 

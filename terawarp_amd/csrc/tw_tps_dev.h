@@ -158,6 +158,23 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t off, uint
     else __builtin_amdgcn_raw_buffer_store_b16((uint16_t)v, r, (int)off, 0, 0);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 16 bytes at byte offset `at` of a buffer whose range ends inside them,
+// byte by byte: a wider load that crosses the range's end returns 0 for the
+// whole dword that crosses it, which would drop the channel's last bytes.
+__device__ __noinline__ u32x4 load16_tail(__amdgpu_buffer_rsrc_t r, uint32_t at) {
+    uint32_t w[4];
+    #pragma unroll
+    for (int q = 0; q < 4; q++) {
+        w[q] = 0;
+        #pragma unroll
+        for (int b = 0; b < 4; b++)
+            w[q] |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)(at + 4 * q + b), 0, 0) << (8 * b);
+    }
+    return (u32x4){w[0], w[1], w[2], w[3]};
+}
+
 struct FastGeom {
     int snx, sny, snz, sx0, sy0, sz0;   // source window
     uint32_t row_b, plane_b, chan_b;    // source strides in bytes

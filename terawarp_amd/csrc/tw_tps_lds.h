@@ -70,22 +70,6 @@ __device__ __forceinline__ uint32_t lds_at(const uint8_t* box, uint32_t off) {
     return *(lds_t*)(box + off);
 }
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// 16 bytes at byte offset `at` of a buffer whose range ends inside them,
-// byte by byte: a wider load that crosses the range's end returns 0 for the
-// whole dword that crosses it, which would drop the channel's last bytes.
-__device__ __noinline__ u32x4 load16_tail(__amdgpu_buffer_rsrc_t r, uint32_t at) {
-    uint32_t w[4];
-    #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        w[q] = 0;
-        #pragma unroll
-        for (int b = 0; b < 4; b++)
-            w[q] |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, (int)(at + 4 * q + b), 0, 0) << (8 * b);
-    }
-    return (u32x4){w[0], w[1], w[2], w[3]};
-}
 
 // Floats of the x-collapse array (a multiple of 4, so that what follows it
 // is 16-byte aligned).

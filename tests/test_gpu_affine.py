@@ -105,12 +105,14 @@ def test_affine_identity_reproduces_every_voxel(warper, rng, dtype, interp):
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
-@pytest.mark.parametrize("case", ["rot_2ch", "rot_nn", "rot_scaled", "rot_window", "rot_edges"])
+@pytest.mark.parametrize("case", ["rot_2ch", "rot_nn", "rot_scaled", "rot_window", "rot_edges", "rot_srcwin"])
 def test_affine_large_rotation_tiles(warper, oracle, rng, dtype, case):
     """Maps whose x step crosses source rows every lane or two run the
-    LDS-staged tile (source box staged per 16 x 16 x 8 output tile); its
-    fallbacks (a box over the LDS budget, the window's last row, samples
-    leaving the volume) gather.  Every case is bit-identical to the oracle."""
+    LDS-staged tile (source box staged per 16 x 16 x 8 output tile).  Tiles
+    whose samples all leave the volume store zeros; tiles whose samples
+    partly leave a whole-volume source stage the box clipped to the volume;
+    a box over the LDS budget, or samples leaving a partial source window,
+    gather.  Every case is bit-identical to the oracle."""
     size = (83, 71, 45)
     amp = (20, 200) if dtype == np.uint8 else (100, 4000)
     v = synth.blob_volume(size[::-1], dtype, n_blobs=40, amp=amp, seed=13)
@@ -138,8 +140,26 @@ def test_affine_large_rotation_tiles(warper, oracle, rng, dtype, case):
         warper.resample_affine(v, R, dst=sub, dst_origin=(31, 17, 9))
         assert np.array_equal(sub, ref[9:29, 17:50, 31:71])
         return
+    if case == "rot_srcwin":
+        # A partial source window of the volume: a sample whose taps leave
+        # the window is 0 (q_warp_affine_tps.cpp's block form reads only the
+        # block it loaded), the others equal the whole-volume result.
+        full = oracle.affine_resample(v, R)
+        w0, wn = (10, 5, 3), (60, 55, 37)
+        win = v[w0[2]:w0[2] + wn[2], w0[1]:w0[1] + wn[1], w0[0]:w0[0] + wn[0]].copy()
+        got = warper.resample_affine(win, R, out_size_xyz=size, src_origin=w0, full=size)
+        z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in v.shape), indexing="ij")
+        keep = np.ones(v.shape, bool)
+        for c, coord in enumerate((x, y, z)):
+            p = R[c, 0] * x + R[c, 1] * y + R[c, 2] * z + R[c, 3] * 1.0
+            keep &= (np.floor(p) >= w0[c]) & (np.ceil(p) <= w0[c] + wn[c] - 1)
+        exp = np.where(keep, full, 0).astype(v.dtype)
+        assert keep.any() and (~keep & (full != 0)).any()
+        assert np.array_equal(got, exp)
+        return
     # rot_edges: a 90-degree turn about z with a shift, so that tiles' boxes
-    # reach the volume's last rows and planes exactly.
+    # reach the volume's last rows and planes exactly (the channel's last
+    # 16-byte piece is read byte by byte).
     M = warp.mode_matrix(warp.ROTATION, (0.0, 0.0, 90.0), size) @ warp.mode_matrix(warp.TRANSLATION, (1, 2, 0), size)
     got = warper.resample_affine(v, M)
     assert np.array_equal(got, oracle.affine_resample(v, M))
