@@ -303,7 +303,8 @@ extern "C" int tw_warp_batch(tw_ctx* ctx, const tw_warp_item* items, int n, int 
             for (int k = 0; k < 12; k++) a.m[k] = it.M[k];
             // The box budget is the launch's LDS, set after every job (below).
             const bool al = affine_prefers_lds(a.m);
-            const char* why = it.src.dtype == TW_U8 ? affine_geom<uint8_t>(a, J.ag, al) : affine_geom<uint16_t>(a, J.ag, al);
+            const char* why = it.src.dtype == TW_U8 ? affine_geom<uint8_t>(a, J.ag, al, AFF_LDS_BYTES, true)
+                                                    : affine_geom<uint16_t>(a, J.ag, al, AFF_LDS_BYTES, true);
             if (why) return tw_fail(ctx, TW_E_UNSUPPORTED, "item %d: %s", i, why);
             any_lds_affine = any_lds_affine || al;
             per_ch = J.ag.ntiles;

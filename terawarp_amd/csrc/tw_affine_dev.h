@@ -108,6 +108,8 @@ struct AffGeom {
     int lds;                            // 1: LDS-staged 16 x 16 x 8 tiles (affine_tile_lds), 0: 64 x 4 x 16
     int lds_bytes;                      // LDS per work-group for the staged box
     int whole;                          // the source window is the whole volume
+    int ord0, ord1, ord2;               // LDS tiles: output axes from fastest to slowest in the tile order
+    int tn[3];                          // tiles along output x, y, z
 };
 
 // Sample checks of the fast path: a tile's samples take the cheapest level
@@ -134,8 +136,11 @@ inline bool affine_prefers_lds(const double* m) {
 
 // Geometry of the fast path, or null if the window exceeds its limits (the
 // caller then takes affine_voxel).
+// aligned_order (LDS tiles of persistent batch launches): the tile order
+// follows the map (below); single-volume launches keep x fastest.
 template <typename T>
-const char* affine_geom(const AffineArgs& a, AffGeom& g, bool lds = false, int lds_bytes = AFF_LDS_BYTES) {
+const char* affine_geom(const AffineArgs& a, AffGeom& g, bool lds = false, int lds_bytes = AFF_LDS_BYTES,
+                        bool aligned_order = false) {
     const uint64_t schan = (uint64_t)a.snx * a.sny * a.snz * sizeof(T);
     const uint64_t taps = schan + (uint64_t)a.snx * (a.sny + 1) * sizeof(T) + 16;
     const uint64_t dchan = (uint64_t)a.dnx * a.dny * a.dnz * sizeof(T);
@@ -161,6 +166,31 @@ const char* affine_geom(const AffineArgs& a, AffGeom& g, bool lds = false, int l
     g.tiles_x = (int)((a.dnx + tx - 1) / tx);
     g.tiles_y = (int)((a.dny + ty - 1) / ty);
     const i64 nt = (i64)g.tiles_x * g.tiles_y * ((a.dnz + tz - 1) / tz);
+    g.tn[0] = g.tiles_x;
+    g.tn[1] = g.tiles_y;
+    g.tn[2] = (int)((a.dnz + tz - 1) / tz);
+    // Aligned order: the output axis whose step moves the sample furthest
+    // along source x fastest, then the one furthest along source y, so that
+    // tiles that follow each other stage boxes along the same source rows,
+    // whose cache lines they share while the lines are in the XCD's L2.
+    // Each XCD takes a contiguous part of an item's tiles, so for one
+    // volume this order leaves the XCDs whose slabs lie mostly outside the
+    // source idle (a 1024 x 1024 x 256 rotation: 0.54 -> 0.94 ms); a batch's
+    // XCDs walk their parts of every item in turn, which evens that out
+    // (rigid items of the 256-item batch: 5.95 -> 5.2-5.6 ms).
+    g.ord0 = 0;
+    g.ord1 = 1;
+    g.ord2 = 2;
+    if (lds && aligned_order) {
+        int f = 0;
+        for (int j = 1; j < 3; j++)
+            if (std::fabs(a.m[j]) > std::fabs(a.m[f])) f = j;
+        const int r0 = f == 0 ? 1 : 0, r1 = f == 2 ? 1 : 2;
+        const bool m0 = std::fabs(a.m[4 + r0]) >= std::fabs(a.m[4 + r1]);
+        g.ord0 = f;
+        g.ord1 = m0 ? r0 : r1;
+        g.ord2 = m0 ? r1 : r0;
+    }
     if (nt > (1ll << 30)) return "too many tiles for one launch";
     g.ntiles = (int)nt;
     g.per_xcd = (int)((nt + 7) / 8);
@@ -478,7 +508,13 @@ template <typename T, int INTERP>
 __device__ __forceinline__ void affine_tile_lds(const AffineArgs& a, const AffGeom& g, int tile, int c0, int c1,
                                                 uint8_t* lds) {
     constexpr int EPP = 16 / (int)sizeof(T);  // elements per 16-byte piece
-    const int tix = tile % g.tiles_x, tiy = (tile / g.tiles_x) % g.tiles_y, tiz = tile / (g.tiles_x * g.tiles_y);
+    // Tile coordinates in the geometry's order (ord0 fastest).
+    const int n0 = g.ord0 == 0 ? g.tn[0] : (g.ord0 == 1 ? g.tn[1] : g.tn[2]);
+    const int n1 = g.ord1 == 0 ? g.tn[0] : (g.ord1 == 1 ? g.tn[1] : g.tn[2]);
+    const int i0 = tile % n0, i1 = (tile / n0) % n1, i2 = tile / (n0 * n1);
+    const int tix = g.ord0 == 0 ? i0 : (g.ord1 == 0 ? i1 : i2);
+    const int tiy = g.ord0 == 1 ? i0 : (g.ord1 == 1 ? i1 : i2);
+    const int tiz = g.ord0 == 2 ? i0 : (g.ord1 == 2 ? i1 : i2);
     const int dnx = (int)a.dnx, dny = (int)a.dny, dnz = (int)a.dnz;
     const int tid = (int)(threadIdx.y * TX + threadIdx.x);
     const int lo[3] = {tix * ALX, tiy * ALY, tiz * ALZ};
