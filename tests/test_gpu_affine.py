@@ -117,26 +117,30 @@ def test_affine_large_rotation_tiles(warper, oracle, rng, dtype, case):
     amp = (20, 200) if dtype == np.uint8 else (100, 4000)
     v = synth.blob_volume(size[::-1], dtype, n_blobs=40, amp=amp, seed=13)
     R = warp.mode_matrix(warp.ROTATION, (37.0, -71.0, 113.0), size)
+    # Outputs start as 0xA5 bytes: a voxel left unwritten (an all-outside
+    # tile's zeros included) differs from the oracle.
+    def filled(a):
+        return np.full_like(a, 0xA5A5 if a.dtype == np.uint16 else 0xA5)
     if case == "rot_2ch":
         v2 = np.stack([v, v[::-1].copy()])
-        got = warper.resample_affine(v2, R)
+        got = warper.resample_affine(v2, R, dst=filled(v2))
         assert np.array_equal(got, oracle.affine_resample(v2, R))
         return
     if case == "rot_nn":
-        got = warper.resample_affine(v, R, interp=warp.NN)
+        got = warper.resample_affine(v, R, interp=warp.NN, dst=filled(v))
         assert np.array_equal(got, _affine_nn(v, R))
         return
     if case == "rot_scaled":
         # Rotation then 5x minification: a tile's source box is far over the
         # LDS budget, so those tiles gather.
         M = R @ warp.mode_matrix(warp.SCALING, (0.2, 0.2, 0.2), size)
-        got = warper.resample_affine(v, M)
+        got = warper.resample_affine(v, M, dst=filled(v))
         assert np.array_equal(got, oracle.affine_resample(v, M))
         return
     if case == "rot_window":
         # Output brick at an offset, output larger than the source.
         ref = oracle.affine_resample(v, R, out_size_xyz=(96, 80, 50))
-        sub = np.zeros((20, 33, 40), dtype)
+        sub = filled(np.zeros((20, 33, 40), dtype))
         warper.resample_affine(v, R, dst=sub, dst_origin=(31, 17, 9))
         assert np.array_equal(sub, ref[9:29, 17:50, 31:71])
         return
@@ -147,7 +151,7 @@ def test_affine_large_rotation_tiles(warper, oracle, rng, dtype, case):
         full = oracle.affine_resample(v, R)
         w0, wn = (10, 5, 3), (60, 55, 37)
         win = v[w0[2]:w0[2] + wn[2], w0[1]:w0[1] + wn[1], w0[0]:w0[0] + wn[0]].copy()
-        got = warper.resample_affine(win, R, out_size_xyz=size, src_origin=w0, full=size)
+        got = warper.resample_affine(win, R, dst=filled(v), src_origin=w0, full=size)
         z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in v.shape), indexing="ij")
         keep = np.ones(v.shape, bool)
         for c, coord in enumerate((x, y, z)):
@@ -161,7 +165,7 @@ def test_affine_large_rotation_tiles(warper, oracle, rng, dtype, case):
     # reach the volume's last rows and planes exactly (the channel's last
     # 16-byte piece is read byte by byte).
     M = warp.mode_matrix(warp.ROTATION, (0.0, 0.0, 90.0), size) @ warp.mode_matrix(warp.TRANSLATION, (1, 2, 0), size)
-    got = warper.resample_affine(v, M)
+    got = warper.resample_affine(v, M, dst=filled(v))
     assert np.array_equal(got, oracle.affine_resample(v, M))
 
 

@@ -29,6 +29,14 @@ def _host(t, dtype):
     return a.view(np.uint16) if dtype == np.uint16 else a
 
 
+def _filled(src, byte):
+    """An output tensor like src with every byte set to `byte`: a voxel a
+    launch leaves unwritten keeps it (different bytes for the batch and the
+    single calls, so that such a voxel differs between the two)."""
+    import torch
+    return torch.full_like(src, byte if src.dtype == torch.uint8 else (byte << 8 | byte) - (1 << 16 if byte >= 128 else 0))
+
+
 def make_items(seed, n, size_lo=24, size_hi=80, k_lo=8, k_hi=48, nc2=True):
     """A seeded mix of TPS (60 %), affine (25 %) and rigid (15 %) items."""
     rng = np.random.default_rng(seed)
@@ -61,7 +69,7 @@ def make_items(seed, n, size_lo=24, size_hi=80, k_lo=8, k_hi=48, nc2=True):
 def run_batch(warper, items, f, precision, interp=warp.LINEAR):
     import torch
     srcs = [_tensor(it["vol"]) for it in items]
-    dsts = [torch.zeros_like(s) for s in srcs]
+    dsts = [_filled(s, 0xA5) for s in srcs]
     spec = []
     for it, s, d in zip(items, srcs, dsts):
         e = {"src": s, "dst": d}
@@ -77,7 +85,7 @@ def run_batch(warper, items, f, precision, interp=warp.LINEAR):
 
 def single(warper, it, src, f, precision, interp=warp.LINEAR):
     import torch
-    dst = torch.zeros_like(src)
+    dst = _filled(src, 0x5A)
     if it["kind"] == "tps":
         warper.warp_tps(src, it["centres"], it["targets"], f=f, lam=0.2, interp=interp, precision=precision, dst=dst)
     else:

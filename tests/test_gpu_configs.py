@@ -109,7 +109,11 @@ def test_batch_configuration(warper, oracle):
     import torch
     specs = synth.batch_instances(seed=5, n=256, dmax=8.0)
     srcs = [synth.blob_volume_device(sp["size"], sp["dtype"], sp["seed"]) for sp in specs]
-    dsts = [torch.zeros_like(s) for s in srcs]
+    # Outputs start as different byte patterns for the batch and the single
+    # calls: a voxel either leaves unwritten differs between the two.
+    def filled(s, byte):
+        return torch.full_like(s, byte if s.dtype == torch.uint8 else (byte << 8 | byte) - (1 << 16 if byte >= 128 else 0))
+    dsts = [filled(s, 0xA5) for s in srcs]
     items = []
     for sp, s, d in zip(specs, srcs, dsts):
         it = {"src": s, "dst": d}
@@ -120,7 +124,7 @@ def test_batch_configuration(warper, oracle):
         items.append(it)
     status, _ = warper.warp_batch(items, f=8, lam=0.2, precision=warp.MIXED)
     assert all(s == 0 for s in status)
-    one = [torch.zeros_like(s) for s in srcs]
+    one = [filled(s, 0x5A) for s in srcs]
     for sp, s, o in zip(specs, srcs, one):
         if sp["kind"] == "tps":
             warper.warp_tps(s, sp["centres"], sp["targets"], f=8, lam=0.2, precision=warp.MIXED, dst=o)

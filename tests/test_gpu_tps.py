@@ -181,7 +181,7 @@ def test_solve_wait_bound_falls_back_to_block_launches(warper, oracle, K):
         w.close()
 
 
-@pytest.mark.parametrize("K", [16, 64, 512])
+@pytest.mark.parametrize("K", [16, 17, 64, 100, 256, 512])
 @pytest.mark.parametrize("precision", [warp.PLAIN, warp.MIXED])
 def test_fit_matches_oracle(warper, oracle, K, precision):
     size = (256, 256, 64)
