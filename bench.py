@@ -192,6 +192,7 @@ def fit_sweep(args):
     from terawarp_amd import synth, warp
     from terawarp_amd.warp import Warper
     w = Warper(0, timing=True)
+    default_min = w.get_tune("fp32_factor_min")
     for K in [int(k) for k in args.fit_sweep.split(",")]:
         sub, tar = synth.landmarks((2048, 2048, 512), K, 40.0, seed=3)
         out = {"K": K}
@@ -205,7 +206,7 @@ def fit_sweep(args):
             t = w.timings()
             out[name] = {"fit_ms": float(np.median(ms[args.warmup:])), "refine_iters": t["refine_iters"],
                          "residual": t["fit_residual"]}
-        w.set_tune("fp32_factor_min", 14000)
+        w.set_tune("fp32_factor_min", default_min)
         print(json.dumps(out), flush=True)
     w.close()
 

@@ -224,7 +224,7 @@ int tw_resample_affine(tw_ctx* ctx, const tw_volume* src, tw_volume* dst, const 
  * xnx4_c, x4x4_d).  Computed by the reference's null-space method
  * (Householder QR of X, Cholesky of q2'(K + lambda I)q2); precision:
  * TW_PLAIN/TW_EXACT (FP64 factor) or TW_MIXED (the FP64 factor below
- * TW_TUNE_FP32_FACTOR_MIN projected unknowns, 14000 by default, the measured
+ * TW_TUNE_FP32_FACTOR_MIN projected unknowns, 15000 by default, the measured
  * crossover; from there the FP32 factor with matrix-core updates + FP64
  * iterative refinement, an FP64 refactor if the refinement stalls).
  * K <= 16384. */

@@ -786,9 +786,10 @@ struct TrsmLds {
     T Xs[NB][NB];
 };
 
-template <typename T>
+// acc: the lane's row of L21, columns 4t + q (for the forward substitution).
+template <typename T, typename Lds>
 __device__ __forceinline__ void trsm_body(T* __restrict__ B, const T* __restrict__ Dinv, int n, int j0, int jb,
-                                          int bx, TrsmLds<T>& S) {
+                                          int bx, Lds& S, T (&acc)[NB / 4]) {
     auto& Xs = S.Xs;
     const int lane = threadIdx.x & 63;
     const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -804,7 +805,6 @@ __device__ __forceinline__ void trsm_body(T* __restrict__ B, const T* __restrict
     #pragma unroll
     for (int u = 0; u < NB * NB / 256; u++) (&Xs[0][0])[threadIdx.x + 256 * u] = xv[u];
     __syncthreads();  // also: all waves have read the panel before any writes it
-    T acc[NB / 4];
     #pragma unroll
     for (int t = 0; t < NB / 4; t++) {
         const T* xr = &Xs[4 * t + q][0];
@@ -826,7 +826,58 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_trsm_chol(T* __restrict__ B, const T* __restrict__ Dinv, int n, int j0,
                                                    int jb) {
     __shared__ TrsmLds<T> S;
-    trsm_body<T>(B, Dinv, n, j0, jb, blockIdx.x, S);
+    T acc[NB / 4];
+    trsm_body<T>(B, Dinv, n, j0, jb, blockIdx.x, S, acc);
+}
+
+// The FP64 panel with the forward substitution of step k riding on it
+// (rhs: n x 4, column-major): every work-group forms y_k = X_k r_k from the
+// block's right-hand side (final after the earlier steps; no work-group of
+// this launch writes it), work-group 0 stores it to yout, and each drops
+// L_Rk y_k from its own rows R.  The pipelined solve then runs only the
+// backward pass (and the last block's forward step).  Block k is full: it is
+// not the last.
+struct TrsmFwdLds {
+    double Xs[NB][NB];
+    double rk[4][NB];       // r_k, transposed
+    double yk[NB][4];
+    double part[4][NB][4];  // per wave: the row's sums over its 16 columns
+};
+
+__global__ __launch_bounds__(256) void k_trsm_fwd(double* __restrict__ B, const double* __restrict__ Dinv, int n,
+                                                  int j0, int jb, double* __restrict__ rhs,
+                                                  double* __restrict__ yout) {
+    __shared__ TrsmFwdLds S;
+    const int lane = threadIdx.x & 63;
+    const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = j0 + jb + blockIdx.x * NB + lane;
+    S.rk[q][lane] = rhs[(i64)q * n + j0 + lane];  // visible after trsm_body's barrier
+    const double ri = i < n ? rhs[(i64)q * n + i] : 0.0;
+    double acc[NB / 4];
+    trsm_body<double>(B, Dinv, n, j0, jb, blockIdx.x, S, acc);
+    {
+        // Lane c sums X[c][j] r[j] from j = c on (cyclically): the lanes'
+        // LDS reads fall in different banks.
+        double s = 0;
+        #pragma unroll 8
+        for (int j = 0; j < NB; j++) {
+            const int jj = (j + lane) & (NB - 1);
+            s = fma(S.Xs[lane][jj], S.rk[q][jj], s);
+        }
+        S.yk[lane][q] = s;
+        if (blockIdx.x == 0) yout[(i64)q * n + j0 + lane] = s;
+    }
+    __syncthreads();
+    double p[4] = {0, 0, 0, 0};
+    #pragma unroll
+    for (int t = 0; t < NB / 4; t++)
+        #pragma unroll
+        for (int c = 0; c < 4; c++) p[c] = fma(acc[t], S.yk[4 * t + q][c], p[c]);
+    #pragma unroll
+    for (int c = 0; c < 4; c++) S.part[q][lane][c] = p[c];
+    __syncthreads();
+    if (i < n)
+        rhs[(i64)q * n + i] = ri - ((S.part[0][lane][q] + S.part[1][lane][q]) + (S.part[2][lane][q] + S.part[3][lane][q]));
 }
 
 // Lower trailing tiles: B[R, C] -= L21[R] L21[C]' for tile C <= tile R.
@@ -972,6 +1023,28 @@ __global__ __launch_bounds__(256) void k_syrk(T* __restrict__ B, int n, int j0, 
     if (blockIdx.y > blockIdx.x) return;
     __shared__ SyrkLds<T> S;
     syrk_mfma_body<T>(B, n, j0, jb, blockIdx.x, blockIdx.y, S);
+}
+
+// The FP64 trailing update of step k with the factor of the next diagonal
+// block: work-group (0, 0), dispatched first, updates that block's tile and
+// then factors and inverts it (k_potrf_diag's arithmetic on the same stored
+// values) while the other work-groups update theirs, so the diagonal factor
+// leaves the chain of dependent launches.
+union SyrkPotrfLds {
+    SyrkLds<double> syrk;
+    PotrfLds<double> potrf;
+};
+
+__global__ __launch_bounds__(256) void k_syrk_potrf(double* __restrict__ B, double* __restrict__ Dinv, int n, int j0,
+                                                    int jb, FitState* st) {
+    if (blockIdx.y > blockIdx.x) return;
+    __shared__ SyrkPotrfLds S;
+    syrk_mfma_body<double>(B, n, j0, jb, blockIdx.x, blockIdx.y, S.syrk);
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        const int r0 = j0 + jb;
+        __syncthreads();  // the tile's stores are visible to the work-group; its LDS is free
+        potrf_diag_fp64(B, Dinv, n, r0, min(NB, n - r0), st, S.potrf);
+    }
 }
 
 // One right-looking step of the blocked Cholesky in one launch, for small
@@ -2080,16 +2153,31 @@ void cholesky_steps(tw_ctx* ctx, T* B, T* Lp, T* Dinv, int n, FitState* st, T* r
     }
 }
 
+// FP64: each trailing update also factors the next diagonal block
+// (k_syrk_potrf), and with rhs (n x 4) each panel runs the forward
+// substitution (k_trsm_fwd: yout gets y_k for every block but the last, rhs
+// the last block's reduced right-hand side); FP32: one launch per phase.
 template <typename T>
-void cholesky(tw_ctx* ctx, T* B, T* Dinv, int n, FitState* st) {
+void cholesky(tw_ctx* ctx, T* B, T* Dinv, int n, FitState* st, T* rhs = nullptr, T* yout = nullptr) {
+    constexpr bool FUSE = std::is_same<T, double>::value;
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int jb = std::min(NB, n - j0);
-        hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, Dinv, n, j0, jb, st);
+        if (!FUSE || j0 == 0)
+            hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(256), 0, ctx->stream, B, Dinv, n, j0, jb, st);
         const int rest = n - j0 - jb;
         if (rest > 0) {
-            hipLaunchKernelGGL(k_trsm_chol<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, B, Dinv, n, j0, jb);
-            hipLaunchKernelGGL(k_syrk<T>, dim3(nblk(rest, NB), nblk(rest, NB)), dim3(256), 0, ctx->stream, B, n, j0,
-                               jb);
+            if (FUSE && rhs)
+                hipLaunchKernelGGL(k_trsm_fwd, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, (double*)B,
+                                   (const double*)Dinv, n, j0, jb, (double*)rhs, (double*)yout);
+            else
+                hipLaunchKernelGGL(k_trsm_chol<T>, dim3(nblk(rest, NB)), dim3(256), 0, ctx->stream, B, Dinv, n, j0,
+                                   jb);
+            const dim3 tiles(nblk(rest, NB), nblk(rest, NB));
+            if constexpr (FUSE)
+                hipLaunchKernelGGL(k_syrk_potrf, tiles, dim3(256), 0, ctx->stream, (double*)B, (double*)Dinv, n, j0,
+                                   jb, st);
+            else
+                hipLaunchKernelGGL(k_syrk<T>, tiles, dim3(256), 0, ctx->stream, B, n, j0, jb);
         }
     }
 }
@@ -2283,7 +2371,8 @@ __device__ __forceinline__ void fit_small_body(const SmallFitJob& J, SmallFitLds
         __syncthreads();
         const int nt = (n - j0 - jb + NB - 1) / NB;
         for (int bx = 0; bx < nt; bx++) {
-            trsm_body<double>(B, Dinv, n, j0, jb, bx, L.trsm);
+            double acc[NB / 4];
+            trsm_body<double>(B, Dinv, n, j0, jb, bx, L.trsm, acc);
             __syncthreads();
         }
         for (int bx = 0; bx < nt; bx++)
@@ -2583,9 +2672,10 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
             hipLaunchKernelGGL(k_tail<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, U, (double*)nullptr, z, N);
         const int nbk = (n + NB - 1) / NB;
         const bool fused = nbk <= STEP_MAX_BLOCKS;
-        // The steps' forward substitution leaves the backward pass to the
-        // pipelined solve; without it the solve runs both passes.
-        const bool fwd = fused && pipe_fits<double>(ctx, nbk);
+        // The factor's forward substitution (fused steps, or the panels of
+        // the blocked factor) leaves the backward pass to the pipelined
+        // solve; without the pipelined solve the solve runs both passes.
+        const bool fwd = nbk <= PIPE_MAX_BLOCKS && pipe_fits<double>(ctx, nbk);
         if (fused) {
             void* lp = nullptr;
             int r2 = tw_ws(ctx, WS_SOLVE_L, (size_t)n * n * sizeof(double), &lp);
@@ -2594,7 +2684,7 @@ int fit_run(tw_ctx* ctx, const float* centres, const float* targets, int K, doub
                                    fwd ? (double*)ys : nullptr);
             Lf = (const double*)lp;
         } else {
-            cholesky<double>(ctx, B, (double*)dinv, n, st);
+            cholesky<double>(ctx, B, (double*)dinv, n, st, fwd ? z : nullptr, fwd ? (double*)ys : nullptr);
         }
         return chol_solve<double>(ctx, Lf, (const double*)dinv, n, z, (double*)ys, nullptr, st, fwd ? 1 : 0);
     };

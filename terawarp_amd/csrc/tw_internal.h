@@ -43,10 +43,11 @@ enum TwStage { ST_FIT = 0, ST_GRID, ST_QUANT, ST_RESAMPLE, ST_H2D, ST_D2H, ST_TO
 
 // TW_MIXED fits factor projected systems of at least this order in FP32
 // (MFMA trailing updates, FP64 refinement); smaller ones in FP64.  Measured
-// on MI355X: the FP64 factor is faster up to 12000 landmarks (38.8 vs 42.7
-// ms), the FP32 one from 16000 (72.0 vs 78.0 ms); below, the block steps and
-// the refinement's extra triangular solves are latency-bound.
-constexpr int TW_FP32_FACTOR_MIN_DEFAULT = 14000;
+// on MI355X: the FP64 factor is faster up to 14000 landmarks (53.1 vs 55.8
+// ms), even at 15000 (63.3 vs 63.5 ms), the FP32 one from 16000 (71.0 vs
+// 74.5 ms); below, the block steps and the refinement's extra triangular
+// solves are latency-bound.
+constexpr int TW_FP32_FACTOR_MIN_DEFAULT = 15000;
 
 // Bound of each wait between the pipelined solve's work-groups (µs).  A wait
 // that runs out means other work held the CUs the solve's work-groups needed:
