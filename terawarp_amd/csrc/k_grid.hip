@@ -255,11 +255,90 @@ __device__ __forceinline__ void quantize_body(const float* __restrict__ df, __ha
             }
 }
 
-__global__ __launch_bounds__(256) void k_quantize(const float* __restrict__ df, __half* __restrict__ out,
-                                                  float* __restrict__ qtab, i64 gx, i64 gy, i64 gz, i64 qbx,
-                                                  i64 qby) {
-    __shared__ QuantLds S;
-    quantize_body(df, out, qtab, gx, gy, gz, qbx, qby, blockIdx.x, S);
+// One grid's quantisation: a run of QR = 8 blocks along x per work-group,
+// so that a wave reads and writes whole 64-node rows (quantize_body, which
+// the batch runs one block per work-group, covers eight 8-node rows of one
+// block per wave).  Lane l owns node x = 64 rx + l of
+// block 8 rx + l / 8 and the wave's 16 (y, z) rows of the block row; the
+// blocks' extremes and parameters are those of quantize_body (min / max are
+// exact in any order), so the output and the table are bit-identical.
+constexpr int QR = 8;
+
+struct QuantRowLds {
+    float red[2][3][4][QR];  // per wave: min / max per component and block
+    float prm[QR][6];
+};
+
+__global__ __launch_bounds__(256) void k_quantize_rows(const float* __restrict__ df, __half* __restrict__ out,
+                                                       float* __restrict__ qtab, i64 gx, i64 gy, i64 gz, i64 qbx,
+                                                       i64 qby, i64 nrx) {
+    __shared__ QuantRowLds S;
+    const i64 wg = blockIdx.x;
+    const i64 rx = wg % nrx, by = (wg / nrx) % qby, bz = wg / (nrx * qby);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, blk = lane / QB;
+    const i64 plane = gx * gy * gz;
+    const i64 i = rx * QB * QR + lane;
+    constexpr int RPW = QB * QB / 4;  // (y, z) rows per wave
+    float v[RPW][3];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    #pragma unroll
+    for (int r = 0; r < RPW; r++) {
+        const int row = w * RPW + r;
+        const i64 j = by * QB + row % QB, k = bz * QB + row / QB;
+        const bool in = i < gx && j < gy && k < gz;
+        const i64 idx = (k * gy + j) * gx + i;
+        #pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[r][c] = in ? df[c * plane + idx] : 0.f;
+            if (in) {
+                mn[c] = fminf(mn[c], v[r][c]);
+                mx[c] = fmaxf(mx[c], v[r][c]);
+            }
+        }
+    }
+    #pragma unroll
+    for (int c = 0; c < 3; c++) {
+        #pragma unroll
+        for (int o = 1; o < QB; o <<= 1) {  // the 8 lanes of a block
+            mn[c] = fminf(mn[c], __shfl_xor(mn[c], o, 64));
+            mx[c] = fmaxf(mx[c], __shfl_xor(mx[c], o, 64));
+        }
+        if ((lane & (QB - 1)) == 0) {
+            S.red[0][c][w][blk] = mn[c];
+            S.red[1][c][w][blk] = mx[c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * QR) {
+        const int c = threadIdx.x % 3, bk = threadIdx.x / 3;
+        const i64 bx = rx * QR + bk;
+        const float lo = fminf(fminf(S.red[0][c][0][bk], S.red[0][c][1][bk]), fminf(S.red[0][c][2][bk], S.red[0][c][3][bk]));
+        const float hi = fmaxf(fmaxf(S.red[1][c][0][bk], S.red[1][c][1][bk]), fmaxf(S.red[1][c][2][bk], S.red[1][c][3][bk]));
+        const float off = 0.5f * (lo + hi);
+        float sc = fmaxf(hi - off, off - lo);
+        if (!(sc > 0.f)) sc = 1.f;
+        S.prm[bk][c] = off;
+        S.prm[bk][3 + c] = sc;
+        if (bx < qbx) {
+            const i64 b = (bz * qby + by) * qbx + bx;
+            qtab[b * 6 + c] = off;
+            qtab[b * 6 + 3 + c] = sc;
+        }
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int r = 0; r < RPW; r++) {
+        const int row = w * RPW + r;
+        const i64 j = by * QB + row % QB, k = bz * QB + row / QB;
+        if (i < gx && j < gy && k < gz) {
+            const i64 idx = (k * gy + j) * gx + i;
+            #pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float h = (v[r][c] - S.prm[blk][c]) / S.prm[blk][3 + c];
+                out[c * plane + idx] = __float2half_rn(fminf(1.f, fmaxf(-1.f, h)));
+            }
+        }
+    }
 }
 
 // ---- batches: every system's grid in one launch ---------------------------
@@ -314,8 +393,9 @@ int tw_grid_quantize_f16(tw_ctx* ctx, const float* df32, tw_grid* grid) {
         grid->qtab = t;
     }
     tw_stage_begin(ctx, ST_QUANT);
-    hipLaunchKernelGGL(k_quantize, dim3((unsigned)nb), dim3(256), 0, ctx->stream, df32, (__half*)grid->data,
-                       grid->qtab, grid->gx, grid->gy, grid->gz, qbx, qby);
+    const i64 nrx = (qbx + QR - 1) / QR;
+    hipLaunchKernelGGL(k_quantize_rows, dim3((unsigned)(nrx * qby * qbz)), dim3(256), 0, ctx->stream, df32,
+                       (__half*)grid->data, grid->qtab, grid->gx, grid->gy, grid->gz, qbx, qby, nrx);
     TW_LAUNCH_CHECK(ctx);
     tw_stage_end(ctx, ST_QUANT);
     return TW_OK;
