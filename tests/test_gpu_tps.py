@@ -203,6 +203,48 @@ def test_fit_rejects_degenerate(warper):
         warper.tps_fit(P[:4], P[:4])
 
 
+def _quantize_f16(v):
+    """FP16 offset-scaled storage restated in numpy (float32 arithmetic):
+    per 8 x 8 x 8 node block and component, offset = (lo + hi) / 2 and scale =
+    max(hi - offset, offset - lo) (1 where that is 0) over the block's nodes,
+    stored h = (v - offset) / scale clamped to [-1, 1], rounded to nearest-even
+    FP16.  v: (3, gz, gy, gx) float32."""
+    out = np.empty(v.shape, np.float16)
+    _, gz, gy, gx = v.shape
+    half = np.float32(0.5)
+    for z0 in range(0, gz, 8):
+        for y0 in range(0, gy, 8):
+            for x0 in range(0, gx, 8):
+                b = v[:, z0:z0 + 8, y0:y0 + 8, x0:x0 + 8]
+                lo = b.min(axis=(1, 2, 3))
+                hi = b.max(axis=(1, 2, 3))
+                off = half * (lo + hi)
+                sc = np.maximum(hi - off, off - lo)
+                sc = np.where(sc > 0, sc, np.float32(1)).astype(np.float32)
+                h = (b - off[:, None, None, None]) / sc[:, None, None, None]
+                out[:, z0:z0 + 8, y0:y0 + 8, x0:x0 + 8] = np.clip(h, np.float32(-1), np.float32(1)).astype(np.float16)
+    return out
+
+
+def test_fp16_grid_storage_bitwise(warper):
+    """The mixed form's FP16 offset-scaled grid equals the numpy restatement
+    of the quantisation applied to the same fit's FP32 grid, bit for bit;
+    the grid's extents leave partial 8-node blocks on every axis and a
+    partial run of 8 blocks along x."""
+    size = (1190, 550, 150)
+    sub, tar = synth.landmarks(size, 200, 20.0, seed=41)
+    c, d = warper.tps_fit(tar, sub, precision=warp.MIXED)
+    gsh = (150, 70, 20)
+    g32 = warper.new_grid(gsh, 8, device=False)
+    warper.grid_eval(tar, c, d, g32, precision=warp.PLAIN)
+    g16 = warper.new_grid(gsh, 8, storage=warp.F16, device=True)
+    warper.grid_eval(tar, c, d, g16, precision=warp.MIXED)
+    got = g16.data.cpu().numpy()
+    exp = _quantize_f16(g32.data)
+    assert np.array_equal(got.view(np.uint16), exp.view(np.uint16)), np.count_nonzero(got.view(np.uint16) != exp.view(np.uint16))
+    g16.release()
+
+
 @pytest.mark.parametrize("g", [4, 8])
 def test_grid_eval_exact_bitwise(warper, oracle, g):
     size = (200, 150, 60)
